@@ -1,0 +1,94 @@
+// Shared helpers for the finitevolx HIP library (gfx950 / CDNA4).
+//
+// Conventions used by every kernel in this library:
+//  * Fields are row-major [ny][ld] arrays of T (float or double); ``ld`` is the
+//    row pitch in elements (>= nx).  Operators that mirror the reference's
+//    ghost-ring contract take the FULL array (ghost ring included) and write a
+//    zero ghost ring.
+//  * Masks are uint8 arrays (0 = dry, nonzero = wet) with the same pitch as
+//    the field they gate, or nullptr for "all wet".
+//  * Every entry point takes a hipStream_t and only enqueues work; it returns
+//    0 on success and a negative status otherwise (message in fvx_last_error).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+namespace fvx {
+
+enum Status : int {
+  kOk = 0,
+  kInvalidArgument = -1,
+  kHipError = -2,
+  kNotConverged = 1,  // informational; never returned by kernels
+};
+
+// Thread-local last error message (read through fvx_last_error()).
+void set_error(const char* fmt, ...);
+const char* last_error();
+
+#define FVX_CHECK_ARG(cond, ...)              \
+  do {                                        \
+    if (!(cond)) {                            \
+      ::fvx::set_error(__VA_ARGS__);          \
+      return ::fvx::kInvalidArgument;         \
+    }                                         \
+  } while (0)
+
+#define FVX_HIP(call)                                                          \
+  do {                                                                         \
+    hipError_t err_ = (call);                                                  \
+    if (err_ != hipSuccess) {                                                  \
+      ::fvx::set_error("HIP error %s at %s:%d", hipGetErrorString(err_),       \
+                       __FILE__, __LINE__);                                    \
+      return ::fvx::kHipError;                                                 \
+    }                                                                          \
+  } while (0)
+
+#define FVX_LAUNCH_CHECK() FVX_HIP(hipGetLastError())
+
+inline int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+// Map a linear workgroup id so that consecutive tiles land on the same XCD.
+// The hardware dispatches workgroup b to XCD (b % 8); tiles that share halo
+// rows should share an L2, so tile t is placed on XCD t / (n / 8).
+__device__ __forceinline__ int xcd_swizzle(int b, int n) {
+  constexpr int kXcd = 8;
+  if (n < 2 * kXcd) return b;
+  const int per = n / kXcd;       // full tiles per XCD
+  const int full = per * kXcd;
+  if (b >= full) return b;         // ragged tail keeps its id
+  return (b % kXcd) * per + (b / kXcd);
+}
+
+__device__ __forceinline__ float wet(const uint8_t* m, int64_t idx) {
+  return m ? (m[idx] ? 1.0f : 0.0f) : 1.0f;
+}
+
+// Wavefront (64-lane) and workgroup reductions in double.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+template <int kThreads>
+__device__ __forceinline__ double block_sum(double v, double* smem) {
+  const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
+  v = wave_sum(v);
+  if (lane == 0) smem[wid] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int w = 0; w < kThreads / 64; ++w) s += smem[w];
+  }
+  __syncthreads();
+  return s;  // valid in thread 0 only
+}
+
+}  // namespace fvx
