@@ -17,8 +17,10 @@ from finitevolx_amd.grid import CartesianGrid2D
 from finitevolx_amd.mask import Mask2D
 
 # Method and limiter codes (FVX_ADV_* / FVX_LIM_* in include/fvx.h)
+# ("wenoz3", code 5, exists only as the 4-point tier of "wenoz5", as in the
+# reference's method table, advection.py:363-401.)
 _METHODS = {"naive": 0, "upwind1": 1, "upwind2": 2, "upwind3": 3, "weno3": 4,
-            "wenoz3": 5, "weno5": 6, "wenoz5": 7}
+            "weno5": 6, "wenoz5": 7}
 _LIMITERS = {"minmod": 0, "van_leer": 1, "superbee": 2, "mc": 3}
 _MASK_DISPATCHABLE = frozenset({"weno3", "weno5", "wenoz5"}) | frozenset(_LIMITERS)
 _UNSUPPORTED = frozenset({"weno7", "weno9"})

@@ -1,0 +1,1441 @@
+// Geometric multigrid for the Helmholtz operator and multigrid-preconditioned
+// flexible CG, FP64, FP32, and FP32 with FP64 iterative refinement.
+//
+// Reference semantics (finitevolx 0.0.40):
+//  * hierarchy build: build_multigrid_solver (solvers/multigrid.py:905-1062)
+//  * operator:        _apply_operator (:327-405)        kind FVX_MG_NOFLUX
+//                     masked_laplacian (iterative.py:144-191) on a
+//                     zero-padded array                 kind FVX_MG_DIRICHLET
+//  * smoother:        _weighted_jacobi (:408-461)
+//  * transfers:       _restrict (:464-507), _prolongate (:510-603)
+//  * V-cycle:         MultigridSolver.v_cycle (:808-883)
+//  * preconditioner:  make_multigrid_preconditioner (preconditioners.py:207-244)
+//
+// GPU design (gfx950): per fine level one fused DOWN kernel (nu1 Jacobi sweeps
+// temporally blocked in an LDS tile + residual + 2x2 restriction) and one fused
+// UP kernel (prolongation + correction + nu2 sweeps); the coarse levels
+// (<= kCoarseCells cells) run inside a single workgroup; PCG vector updates
+// and dot products are fused kernels whose last workgroup finishes each
+// reduction on the device (no host round trip); every kernel of an iteration
+// exits immediately once the device-side convergence flag is set, so the host
+// can enqueue a predicted number of iterations and synchronise once.
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "fvx_common.hpp"
+
+namespace fvx {
+
+enum MgKind : int { kNoflux = 0, kDirichlet = 1 };
+
+// ---------------------------------------------------------------------------
+// Device-side scalars of one PCG solve.  Layout is part of the C ABI
+// (fvx_cg_info reads iterations / residual from it).
+// ---------------------------------------------------------------------------
+struct PcgScalars {
+  double rz;      // rho_k = r . z
+  double pq;      // p . A p
+  double rr;      // ||r||^2 (recursive residual)
+  double bb;      // ||b||^2
+  double tol2;    // stop when rr <= tol2
+  double alpha, beta;
+  double zq;      // z_{k+1} . q_k  (flexible beta)
+  double out_rr;  // outer (FP64) residual ||b - A x||^2 for refinement
+  double out_tol2;
+  int iter;       // inner iterations done
+  int done;       // 1 once rr <= tol2 (all kernels become no-ops)
+  int first;      // 1 before the first direction is formed
+  int out_done;   // outer refinement converged
+  int outer;      // outer iterations done
+  int pad[3];
+  unsigned int ticket[8];  // last-block counters, one per reduction site
+};
+
+template <typename T>
+struct LevelDev {
+  int ny, nx;
+  T idx2, idy2, lam;
+  T inv_d;                 // Dirichlet scalar 1/diag
+  const uint8_t* mask;     // nullptr = all wet
+  const T* cx;             // noflux face coefficients (nullptr for Dirichlet)
+  const T* cy;
+  const T* inv;            // noflux inverse diagonal
+  T* u;                    // iterate (coarse levels: error)
+  T* b;                    // right-hand side (coarse levels: restricted residual)
+  T* t;                    // scratch (ping-pong for single-workgroup sweeps)
+  T* t2;                   // scratch for tiled sweep chains
+  T* t3;
+};
+
+template <typename T>
+__device__ __forceinline__ T mval(const uint8_t* m, int64_t k) {
+  return m ? (m[k] ? T(1) : T(0)) : T(1);
+}
+
+// ---------------------------------------------------------------------------
+// Reductions finished by the last workgroup (fixed summation order ->
+// deterministic).  Partials: nblk x NV doubles.
+// ---------------------------------------------------------------------------
+template <int NV, int kThreads>
+__device__ bool block_reduce_last(const double (&v)[NV], double* partials,
+                                  unsigned int* ticket, double (&out)[NV]) {
+  __shared__ double red[kThreads / 64][NV];
+  __shared__ bool is_last;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    const double s = wave_sum(v[k]);
+    if (lane == 0) red[wid][k] = s;
+  }
+  __syncthreads();
+  const int nblk = gridDim.x * gridDim.y;
+  const int bid = blockIdx.y * gridDim.x + blockIdx.x;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double s = 0.0;
+      for (int w = 0; w < kThreads / 64; ++w) s += red[w][k];
+      partials[(int64_t)bid * NV + k] = s;
+    }
+    __threadfence();
+    const unsigned int prev = atomicAdd(ticket, 1u);
+    is_last = (prev == (unsigned int)nblk - 1);
+  }
+  __syncthreads();
+  if (!is_last) return false;
+  __threadfence();
+  // Last workgroup: sum the partials in block order (each lane a strided
+  // subset, then a fixed-shape tree).
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += kThreads) {
+      s += __builtin_nontemporal_load(&partials[(int64_t)b * NV + k]);
+    }
+    s = wave_sum(s);
+    if (lane == 0) red[wid][k] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double s = 0.0;
+      for (int w = 0; w < kThreads / 64; ++w) s += red[w][k];
+      out[k] = s;
+    }
+    *ticket = 0u;  // re-arm for the next launch
+  }
+  return true;  // out[] valid in thread 0
+}
+
+// ---------------------------------------------------------------------------
+// Operator at a cell of an LDS tile (u, cx, cy indexed by tile coordinates).
+// ---------------------------------------------------------------------------
+template <typename T, int KIND, int W>
+__device__ __forceinline__ T tile_Au(const T* U, const T* CX, const T* CY, const T* M, int y,
+                                     int x, T idx2, T idy2, T lam) {
+  const int k = y * W + x;
+  const T u = U[k];
+  T lap;
+  if constexpr (KIND == kDirichlet) {
+    lap = (U[k + 1] + U[k - 1] - T(2) * u) * idx2 + (U[k + W] + U[k - W] - T(2) * u) * idy2;
+  } else {
+    lap = (CX[k] * (U[k + 1] - u) - CX[k - 1] * (u - U[k - 1])) * idx2 +
+          (CY[k] * (U[k + W] - u) - CY[k - W] * (u - U[k - W])) * idy2;
+  }
+  return (lap - lam * u) * M[k];
+}
+
+// Operator at a cell of a global level array (zero outside the array).
+template <typename T, int KIND>
+__device__ __forceinline__ T glob_Au(const LevelDev<T>& L, const T* u, int j, int i) {
+  const int64_t k = (int64_t)j * L.nx + i;
+  const T m = mval<T>(L.mask, k);
+  const T c = u[k];
+  const T e = (i + 1 < L.nx) ? u[k + 1] : T(0);
+  const T w = (i > 0) ? u[k - 1] : T(0);
+  const T n = (j + 1 < L.ny) ? u[k + L.nx] : T(0);
+  const T s = (j > 0) ? u[k - L.nx] : T(0);
+  T lap;
+  if constexpr (KIND == kDirichlet) {
+    lap = (e + w - T(2) * c) * L.idx2 + (n + s - T(2) * c) * L.idy2;
+  } else {
+    const T cxe = L.cx[k], cxw = (i > 0) ? L.cx[k - 1] : T(0);
+    const T cyn = L.cy[k], cys = (j > 0) ? L.cy[k - L.nx] : T(0);
+    lap = (cxe * (e - c) - cxw * (c - w)) * L.idx2 + (cyn * (n - c) - cys * (c - s)) * L.idy2;
+  }
+  return (lap - L.lam * c) * m;
+}
+
+template <typename T, int KIND>
+__device__ __forceinline__ T glob_inv(const LevelDev<T>& L, int64_t k) {
+  if constexpr (KIND == kDirichlet) return L.inv_d * mval<T>(L.mask, k);
+  return L.inv[k];
+}
+
+// ---------------------------------------------------------------------------
+// Tiled smoother kernel.  MODE:
+//   0 SMOOTH   : NS sweeps                                  -> u_out
+//   1 DOWN     : NS sweeps, residual, restriction           -> u_out, C.b
+//   2 UP       : u += P(C.u), NS sweeps                     -> u_out
+//   3 UP_DOTS  : as UP, plus z.r and z.w partial dots for flexible PCG
+// ---------------------------------------------------------------------------
+enum TileMode : int { kSmooth = 0, kDown = 1, kUp = 2, kUpDots = 3 };
+
+constexpr int kTx = 64, kTy = 16, kTileThreads = 256;
+
+template <typename T>
+struct TileArgs {
+  LevelDev<T> L, C;
+  const T* u_in;
+  const T* b;
+  T* u_out;
+  const T* w;
+  T omega;
+  double* partials;
+  PcgScalars* sc;
+  int tiles_x, ntiles;
+};
+
+template <typename T, int KIND>
+__device__ __forceinline__ T prolong_at(const LevelDev<T>& C, int j, int i) {
+  // _prolongate (multigrid.py:510-603) for fine cell (j, i), before x mask_f.
+  const int J = j >> 1, I = i >> 1;
+  const int sj = 2 * (j & 1) - 1, si = 2 * (i & 1) - 1;
+  auto vm = [&](int jj, int ii, T& m) -> T {
+    if (jj < 0 || jj >= C.ny || ii < 0 || ii >= C.nx) {
+      m = T(0);
+      return T(0);
+    }
+    const int64_t k = (int64_t)jj * C.nx + ii;
+    m = mval<T>(C.mask, k);
+    return C.u[k] * m;
+  };
+  T mc, mx, my, md;
+  const T vc = vm(J, I, mc), vx = vm(J, I + si, mx), vy = vm(J + sj, I, my),
+          vd = vm(J + sj, I + si, md);
+  const T w = T(9) * vc + T(3) * vx + T(3) * vy + vd;
+  const T d = T(9) * mc + T(3) * mx + T(3) * my + md;
+  return w / (d > T(1) ? d : T(1));
+}
+
+template <typename T, int KIND, int NS, int MODE, bool ZERO>
+__global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
+  if (a.sc && a.sc->done) return;
+  constexpr int H = NS + (MODE == kDown ? 1 : 0);
+  constexpr int W = kTx + 2 * H, Hh = kTy + 2 * H, N = W * Hh;
+  __shared__ T sU[2][N];
+  __shared__ T sB[N];
+  __shared__ T sM[N];
+  __shared__ T sCX[KIND == kNoflux ? N : 1];
+  __shared__ T sCY[KIND == kNoflux ? N : 1];
+  __shared__ T sI[KIND == kNoflux ? N : 1];
+
+  const LevelDev<T>& L = a.L;
+  const int t = xcd_swizzle(blockIdx.x, a.ntiles);
+  const int i0 = (t % a.tiles_x) * kTx, j0 = (t / a.tiles_x) * kTy;
+  const int tid = threadIdx.x;
+
+  for (int k = tid; k < N; k += kTileThreads) {
+    const int y = k / W, x = k % W;
+    const int gj = j0 - H + y, gi = i0 - H + x;
+    const bool in = (gj >= 0 && gj < L.ny && gi >= 0 && gi < L.nx);
+    const int64_t g = (int64_t)gj * L.nx + gi;
+    const T m = in ? mval<T>(L.mask, g) : T(0);
+    sM[k] = m;
+    sB[k] = in ? a.b[g] : T(0);
+    if constexpr (KIND == kNoflux) {
+      sCX[k] = in ? L.cx[g] : T(0);
+      sCY[k] = in ? L.cy[g] : T(0);
+      sI[k] = in ? L.inv[g] : T(0);
+    }
+    T u = T(0);
+    if (in) {
+      if constexpr (!ZERO) u = a.u_in[g];
+      if constexpr (MODE == kUp || MODE == kUpDots) {
+        // u = (u + P e_c) * mask  (multigrid.py:875-876)
+        const T e = prolong_at<T, KIND>(a.C, gj, gi) * m;
+        u = (u + e) * m;
+      }
+    }
+    sU[0][k] = u;
+  }
+  __syncthreads();
+
+  const T om = a.omega;
+  int cur = 0;
+#pragma unroll
+  for (int s = 1; s <= NS; ++s) {
+    // Valid region shrinks by one cell per sweep.
+    const int lo = s, hiy = Hh - s, hix = W - s;
+    const int ny_ = hiy - lo, nx_ = hix - lo;
+    for (int k = tid; k < ny_ * nx_; k += kTileThreads) {
+      const int y = lo + k / nx_, x = lo + k % nx_;
+      const int q = y * W + x;
+      T inv;
+      if constexpr (KIND == kDirichlet) inv = L.inv_d * sM[q];
+      else inv = sI[q];
+      T un;
+      if (ZERO && s == 1 && MODE != kUp && MODE != kUpDots) {
+        un = (om * inv * sB[q]) * sM[q];
+      } else {
+        const T au = tile_Au<T, KIND, W>(sU[cur], sCX, sCY, sM, y, x, L.idx2, L.idy2, L.lam);
+        un = (sU[cur][q] + om * inv * (sB[q] - au)) * sM[q];
+      }
+      sU[cur ^ 1][q] = un;
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+
+  if constexpr (MODE == kDown) {
+    // r = (b - A u) * m on the tile interior, kept in the spare buffer.
+    T* R = sU[cur ^ 1];
+    for (int k = tid; k < kTx * kTy; k += kTileThreads) {
+      const int y = H + k / kTx, x = H + k % kTx;
+      const int q = y * W + x;
+      const T au = tile_Au<T, KIND, W>(sU[cur], sCX, sCY, sM, y, x, L.idx2, L.idy2, L.lam);
+      R[q] = (sB[q] - au) * sM[q];
+    }
+    __syncthreads();
+    // 2x2 mask-weighted restriction (multigrid.py:464-507).
+    const LevelDev<T>& C = a.C;
+    for (int k = tid; k < (kTx / 2) * (kTy / 2); k += kTileThreads) {
+      const int cy_ = k / (kTx / 2), cx_ = k % (kTx / 2);
+      const int J = j0 / 2 + cy_, I = i0 / 2 + cx_;
+      if (J >= C.ny || I >= C.nx) continue;
+      const int y = H + 2 * cy_, x = H + 2 * cx_;
+      const int q00 = y * W + x, q10 = q00 + W, q01 = q00 + 1, q11 = q00 + W + 1;
+      const T s4 = R[q00] * sM[q00] + R[q10] * sM[q10] + R[q01] * sM[q01] + R[q11] * sM[q11];
+      const T cnt = sM[q00] + sM[q10] + sM[q01] + sM[q11];
+      const int64_t g = (int64_t)J * C.nx + I;
+      C.b[g] = (s4 / (cnt > T(1) ? cnt : T(1))) * mval<T>(C.mask, g);
+    }
+  }
+
+  // Write the tile interior.
+  double d0 = 0.0, d1 = 0.0;
+  for (int k = tid; k < kTx * kTy; k += kTileThreads) {
+    const int y = H + k / kTx, x = H + k % kTx;
+    const int gj = j0 + k / kTx, gi = i0 + k % kTx;
+    if (gj < L.ny && gi < L.nx) {
+      const int q = y * W + x;
+      const int64_t g = (int64_t)gj * L.nx + gi;
+      const T z = sU[cur][q];
+      a.u_out[g] = z;
+      if constexpr (MODE == kUpDots) {
+        d0 += (double)z * (double)sB[q];
+        if (!a.sc->first) d1 += (double)z * (double)a.w[g];
+      }
+    }
+  }
+  if constexpr (MODE == kUpDots) {
+    double v[2] = {d0, d1}, out[2];
+    if (block_reduce_last<2, kTileThreads>(v, a.partials, &a.sc->ticket[0], out)) {
+      if (threadIdx.x == 0) {
+        PcgScalars* sc = a.sc;
+        // Flexible (Polak-Ribiere) beta: z_{k+1}.(r_{k+1} - r_k) / rho_k with
+        // r_{k+1} - r_k = -alpha q_k.
+        sc->beta = sc->first ? 0.0 : (-sc->alpha * out[1]) / sc->rz;
+        sc->rz = out[0];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Single-workgroup coarse V-cycle (levels lc .. nL-1) on global arrays that
+// stay resident in L2.  Starts from a zero guess at level lc (preconditioner
+// use) and writes the result to levels[lc].u.
+// ---------------------------------------------------------------------------
+constexpr int kCoarseThreads = 1024;
+
+template <typename T>
+struct CoarseArgs {
+  LevelDev<T> lv[16];
+  int lc, nl, n_pre, n_post, n_coarse;
+  int nonzero_init;  // level lc starts from lv[lc].u instead of zero
+  T omega;
+  const int* done;
+};
+
+template <typename T, int KIND>
+__device__ void cw_sweeps(const LevelDev<T>& L, int n, bool zero, T om) {
+  const int N = L.ny * L.nx;
+  T* src = L.u;
+  T* dst = L.t;
+  for (int s = 0; s < n; ++s) {
+    for (int k = threadIdx.x; k < N; k += kCoarseThreads) {
+      const int j = k / L.nx, i = k % L.nx;
+      const T m = mval<T>(L.mask, k);
+      const T inv = glob_inv<T, KIND>(L, k);
+      T un;
+      if (zero && s == 0) un = (om * inv * L.b[k]) * m;
+      else un = (src[k] + om * inv * (L.b[k] - glob_Au<T, KIND>(L, src, j, i))) * m;
+      dst[k] = un;
+    }
+    __syncthreads();
+    T* tmp = src;
+    src = dst;
+    dst = tmp;
+  }
+  if (src != L.u) {  // odd number of sweeps: copy back
+    for (int k = threadIdx.x; k < N; k += kCoarseThreads) L.u[k] = src[k];
+    __syncthreads();
+  }
+}
+
+template <typename T, int KIND>
+__global__ __launch_bounds__(kCoarseThreads) void mg_coarse_kernel(CoarseArgs<T> a) {
+  if (a.done && *a.done) return;
+  const T om = a.omega;
+  // Down leg.
+  for (int l = a.lc; l < a.nl - 1; ++l) {
+    const LevelDev<T>& L = a.lv[l];
+    const LevelDev<T>& C = a.lv[l + 1];
+    const bool zero = !(a.nonzero_init && l == a.lc);
+    if (a.n_pre > 0) {
+      cw_sweeps<T, KIND>(L, a.n_pre, zero, om);
+    } else if (zero) {
+      for (int k = threadIdx.x; k < L.ny * L.nx; k += kCoarseThreads) L.u[k] = T(0);
+      __syncthreads();
+    }
+    for (int k = threadIdx.x; k < C.ny * C.nx; k += kCoarseThreads) {
+      const int J = k / C.nx, I = k % C.nx;
+      T s4 = T(0), cnt = T(0);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int dj = (q == 1 || q == 3), di = (q >= 2);
+        const int j = 2 * J + dj, i = 2 * I + di;
+        const int64_t f = (int64_t)j * L.nx + i;
+        const T m = mval<T>(L.mask, f);
+        const T r = (L.b[f] - glob_Au<T, KIND>(L, L.u, j, i)) * m;
+        s4 += r * m;
+        cnt += m;
+      }
+      C.b[k] = (s4 / (cnt > T(1) ? cnt : T(1))) * mval<T>(C.mask, k);
+    }
+    __syncthreads();
+  }
+  // Bottom solve: n_coarse sweeps from zero (from lv[lc].u when the whole
+  // hierarchy is one level and a guess was given).
+  {
+    const LevelDev<T>& B = a.lv[a.nl - 1];
+    const bool zero = !(a.nonzero_init && a.lc == a.nl - 1);
+    if (a.n_coarse > 0) {
+      cw_sweeps<T, KIND>(B, a.n_coarse, zero, om);
+    } else if (zero) {
+      for (int k = threadIdx.x; k < B.ny * B.nx; k += kCoarseThreads) B.u[k] = T(0);
+      __syncthreads();
+    }
+  }
+  // Up leg.
+  for (int l = a.nl - 2; l >= a.lc; --l) {
+    const LevelDev<T>& L = a.lv[l];
+    for (int k = threadIdx.x; k < L.ny * L.nx; k += kCoarseThreads) {
+      const int j = k / L.nx, i = k % L.nx;
+      const T m = mval<T>(L.mask, k);
+      L.u[k] = (L.u[k] + prolong_at<T, KIND>(a.lv[l + 1], j, i) * m) * m;
+    }
+    __syncthreads();
+    cw_sweeps<T, KIND>(L, a.n_post, false, om);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Pointwise / matvec kernels on one level (global memory, 64 x 4 blocks).
+// ---------------------------------------------------------------------------
+template <typename T, int KIND>
+__global__ __launch_bounds__(256) void apply_op_kernel(LevelDev<T> L, const T* __restrict__ u,
+                                                       T* __restrict__ out) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+  if (i >= L.nx || j >= L.ny) return;
+  // The reference masks u before the stencil (u_m = u * mask).
+  out[(int64_t)j * L.nx + i] = glob_Au<T, KIND>(L, u, j, i);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void mask_copy_kernel(LevelDev<T> L, const T* __restrict__ in,
+                                                        T* __restrict__ out, int64_t ld_in) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+  if (i >= L.nx || j >= L.ny) return;
+  const int64_t k = (int64_t)j * L.nx + i;
+  out[k] = in ? in[(int64_t)j * ld_in + i] * mval<T>(L.mask, k) : T(0);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void restrict_kernel(LevelDev<T> L, LevelDev<T> C,
+                                                       const T* __restrict__ v,
+                                                       T* __restrict__ out) {
+  const int I = blockIdx.x * 64 + threadIdx.x, J = blockIdx.y * 4 + threadIdx.y;
+  if (I >= C.nx || J >= C.ny) return;
+  T s4 = T(0), cnt = T(0);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int dj = (q == 1 || q == 3), di = (q >= 2);
+    const int64_t f = (int64_t)(2 * J + dj) * L.nx + (2 * I + di);
+    const T m = mval<T>(L.mask, f);
+    s4 += v[f] * m;
+    cnt += m;
+  }
+  const int64_t g = (int64_t)J * C.nx + I;
+  out[g] = (s4 / (cnt > T(1) ? cnt : T(1))) * mval<T>(C.mask, g);
+}
+
+template <typename T, int KIND>
+__global__ __launch_bounds__(256) void prolong_kernel(LevelDev<T> L, LevelDev<T> C,
+                                                      T* __restrict__ out) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+  if (i >= L.nx || j >= L.ny) return;
+  const int64_t k = (int64_t)j * L.nx + i;
+  out[k] = prolong_at<T, KIND>(C, j, i) * mval<T>(L.mask, k);
+}
+
+// PCG: r = (b - A x) m, ||b||^2, ||r||^2; sets the stopping threshold.
+template <typename T, int KIND>
+__global__ __launch_bounds__(256) void pcg_init_kernel(LevelDev<T> L, const T* __restrict__ b,
+                                                       int64_t ld_b, const T* __restrict__ x,
+                                                       T* __restrict__ r, double rtol2,
+                                                       double atol2, double* partials,
+                                                       PcgScalars* sc) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+  double bb = 0.0, rr = 0.0;
+  if (i < L.nx && j < L.ny) {
+    const int64_t k = (int64_t)j * L.nx + i;
+    const T m = mval<T>(L.mask, k);
+    const T bv = b[(int64_t)j * ld_b + i] * m;
+    const T ax = x ? glob_Au<T, KIND>(L, x, j, i) : T(0);
+    const T rv = (bv - ax) * m;
+    r[k] = rv;
+    bb = (double)bv * bv;
+    rr = (double)rv * rv;
+  }
+  double v[2] = {bb, rr}, out[2];
+  if (block_reduce_last<2, 256>(v, partials, &sc->ticket[1], out)) {
+    if (threadIdx.x == 0) {
+      sc->bb = out[0];
+      sc->rr = out[1];
+      sc->tol2 = fmax(rtol2 * out[0], atol2);
+      sc->done = out[1] <= sc->tol2 ? 1 : 0;
+      sc->first = 1;
+      sc->iter = 0;
+      sc->alpha = 0.0;
+      sc->beta = 0.0;
+    }
+  }
+}
+
+// PCG: p_out = z + beta p_in ; q = A p_out ; partial p.q -> alpha = rho / pq.
+template <typename T, int KIND>
+__global__ __launch_bounds__(256) void pcg_dir_kernel(LevelDev<T> L, const T* __restrict__ z,
+                                                      const T* __restrict__ p_in,
+                                                      T* __restrict__ p_out, T* __restrict__ q,
+                                                      double* partials, PcgScalars* sc) {
+  if (sc->done) return;
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+  const bool first = sc->first;
+  const T beta = first ? T(0) : (T)sc->beta;
+  double pq = 0.0;
+  if (i < L.nx && j < L.ny) {
+    const int64_t k = (int64_t)j * L.nx + i;
+    auto P = [&](int64_t kk) { return first ? z[kk] : z[kk] + beta * p_in[kk]; };
+    const T m = mval<T>(L.mask, k);
+    const T c = P(k);
+    const T e = (i + 1 < L.nx) ? P(k + 1) : T(0);
+    const T w = (i > 0) ? P(k - 1) : T(0);
+    const T n = (j + 1 < L.ny) ? P(k + L.nx) : T(0);
+    const T s = (j > 0) ? P(k - L.nx) : T(0);
+    T lap;
+    if constexpr (KIND == kDirichlet) {
+      lap = (e + w - T(2) * c) * L.idx2 + (n + s - T(2) * c) * L.idy2;
+    } else {
+      const T cxe = L.cx[k], cxw = (i > 0) ? L.cx[k - 1] : T(0);
+      const T cyn = L.cy[k], cys = (j > 0) ? L.cy[k - L.nx] : T(0);
+      lap = (cxe * (e - c) - cxw * (c - w)) * L.idx2 + (cyn * (n - c) - cys * (c - s)) * L.idy2;
+    }
+    const T qv = (lap - L.lam * c) * m;
+    p_out[k] = c;
+    q[k] = qv;
+    pq = (double)c * qv;
+  }
+  double v[1] = {pq}, out[1];
+  if (block_reduce_last<1, 256>(v, partials, &sc->ticket[2], out)) {
+    if (threadIdx.x == 0) {
+      sc->pq = out[0];
+      sc->alpha = sc->rz / out[0];
+      sc->first = 0;
+    }
+  }
+}
+
+// PCG: x += alpha p ; r -= alpha q ; ||r||^2 -> convergence flag.
+template <typename T>
+__global__ __launch_bounds__(256) void pcg_update_kernel(int ny, int nx, T* __restrict__ x,
+                                                         const T* __restrict__ p,
+                                                         T* __restrict__ r,
+                                                         const T* __restrict__ q,
+                                                         double* partials, PcgScalars* sc) {
+  if (sc->done) return;
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+  const T alpha = (T)sc->alpha;
+  double rr = 0.0;
+  if (i < nx && j < ny) {
+    const int64_t k = (int64_t)j * nx + i;
+    x[k] += alpha * p[k];
+    const T rv = r[k] - alpha * q[k];
+    r[k] = rv;
+    rr = (double)rv * rv;
+  }
+  double v[1] = {rr}, out[1];
+  if (block_reduce_last<1, 256>(v, partials, &sc->ticket[3], out)) {
+    if (threadIdx.x == 0) {
+      sc->rr = out[0];
+      sc->iter += 1;
+      if (out[0] <= sc->tol2) sc->done = 1;
+    }
+  }
+}
+
+// Mixed precision: r32 = (b64 - A64 x64) m in FP64, ||r||^2 and ||b||^2 in
+// FP64; the inner FP32 solve starts from e = 0 so e is cleared here too.
+template <int KIND>
+__global__ __launch_bounds__(256) void refine_residual_kernel(
+    LevelDev<double> L, const double* __restrict__ b, int64_t ld_b, const double* __restrict__ x,
+    int64_t ld_x, float* __restrict__ r32, float* __restrict__ e32, double rtol2, double atol2,
+    double* partials, PcgScalars* sc) {
+  if (sc->out_done) return;
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+  double bb = 0.0, rr = 0.0;
+  if (i < L.nx && j < L.ny) {
+    const int64_t k = (int64_t)j * L.nx + i;
+    const int64_t kx = (int64_t)j * ld_x + i;
+    const double m = mval<double>(L.mask, k);
+    const double c = x[kx];
+    const double e = (i + 1 < L.nx) ? x[kx + 1] : 0.0;
+    const double w = (i > 0) ? x[kx - 1] : 0.0;
+    const double n = (j + 1 < L.ny) ? x[kx + ld_x] : 0.0;
+    const double s = (j > 0) ? x[kx - ld_x] : 0.0;
+    double lap;
+    if constexpr (KIND == kDirichlet) {
+      lap = (e + w - 2.0 * c) * L.idx2 + (n + s - 2.0 * c) * L.idy2;
+    } else {
+      const double cxe = L.cx[k], cxw = (i > 0) ? L.cx[k - 1] : 0.0;
+      const double cyn = L.cy[k], cys = (j > 0) ? L.cy[k - L.nx] : 0.0;
+      lap = (cxe * (e - c) - cxw * (c - w)) * L.idx2 + (cyn * (n - c) - cys * (c - s)) * L.idy2;
+    }
+    const double bv = b[(int64_t)j * ld_b + i] * m;
+    const double rv = (bv - (lap - L.lam * c) * m) * m;
+    r32[k] = (float)rv;
+    e32[k] = 0.0f;
+    bb = bv * bv;
+    rr = rv * rv;
+  }
+  double v[2] = {bb, rr}, out[2];
+  if (block_reduce_last<2, 256>(v, partials, &sc->ticket[4], out)) {
+    if (threadIdx.x == 0) {
+      sc->bb = out[0];
+      sc->out_rr = out[1];
+      sc->out_tol2 = fmax(rtol2 * out[0], atol2);
+      sc->out_done = out[1] <= sc->out_tol2 ? 1 : 0;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void refine_update_kernel(int ny, int nx, double* __restrict__ x,
+                                                            int64_t ld_x,
+                                                            const float* __restrict__ e,
+                                                            PcgScalars* sc) {
+  if (sc->out_done) return;
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+  if (i < nx && j < ny) x[(int64_t)j * ld_x + i] += (double)e[(int64_t)j * nx + i];
+  if (i == 0 && j == 0) sc->outer += 1;
+}
+
+// Reset an inner solve on an already-formed residual r (mixed precision):
+// tol from ||r0||, flags cleared.  Single thread.
+__global__ void inner_reset_kernel(PcgScalars* inner, const PcgScalars* outer, double inner_rtol) {
+  if (outer->out_done) {
+    inner->done = 1;
+    return;
+  }
+  inner->rr = outer->out_rr;
+  inner->tol2 = inner_rtol * inner_rtol * outer->out_rr;
+  inner->done = 0;
+  inner->first = 1;
+  inner->iter = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Host-side plan
+// ---------------------------------------------------------------------------
+
+template <typename T>
+struct Hier {
+  std::vector<LevelDev<T>> lv;
+  std::vector<void*> allocs;
+  // PCG vectors on level 0
+  T *r = nullptr, *z = nullptr, *p0 = nullptr, *p1 = nullptr, *q = nullptr, *x = nullptr;
+};
+
+struct Plan {
+  int kind, prec, nl, ny, nx;
+  int n_pre, n_post, n_coarse;
+  double omega, lam, dx, dy;
+  int lc;  // first level handled by the single-workgroup coarse kernel
+  bool has64 = false, has32 = false;
+  Hier<double> h64;
+  Hier<float> h32;
+  double* partials = nullptr;
+  PcgScalars* sc = nullptr;   // [0] inner, [1] outer
+  PcgScalars* hsc = nullptr;  // pinned host mirror
+  int last_iters = 4, last_outer = 2, last_inner = 4;
+  std::vector<void*> allocs;
+};
+
+static int compute_n_levels(int ny, int nx, int min_coarse = 8) {  // multigrid.py:50-76
+  int n = 1;
+  while (ny % 2 == 0 && nx % 2 == 0 && ny / 2 >= min_coarse && nx / 2 >= min_coarse) {
+    ny /= 2;
+    nx /= 2;
+    ++n;
+  }
+  return n;
+}
+
+struct HostLevel {
+  int ny, nx;
+  double dx, dy;
+  std::vector<double> mask, coeff, cx, cy, inv;
+  bool all_wet;
+};
+
+static std::vector<HostLevel> build_host_levels(const double* mask, const double* coeff, int ny,
+                                                int nx, double dx, double dy, double lam,
+                                                int nl, int kind) {
+  std::vector<HostLevel> out;
+  HostLevel cur;
+  cur.ny = ny;
+  cur.nx = nx;
+  cur.dx = dx;
+  cur.dy = dy;
+  const size_t n0 = (size_t)ny * nx;
+  cur.mask.assign(mask, mask + n0);
+  cur.coeff.resize(n0);
+  for (size_t k = 0; k < n0; ++k) cur.coeff[k] = (coeff ? coeff[k] : 1.0) * cur.mask[k];
+  for (int l = 0; l < nl; ++l) {
+    const int Ny = cur.ny, Nx = cur.nx;
+    const size_t N = (size_t)Ny * Nx;
+    cur.all_wet = true;
+    for (size_t k = 0; k < N; ++k)
+      if (cur.mask[k] != 1.0) cur.all_wet = false;
+    if (kind == kNoflux) {
+      // _interpolate_coeff_to_faces + _compute_inv_diagonal (multigrid.py:143-279)
+      cur.cx.assign(N, 0.0);
+      cur.cy.assign(N, 0.0);
+      for (int j = 0; j < Ny; ++j)
+        for (int i = 0; i < Nx; ++i) {
+          const size_t k = (size_t)j * Nx + i;
+          if (i + 1 < Nx)
+            cur.cx[k] = 0.5 * (cur.coeff[k] + cur.coeff[k + 1]) * (cur.mask[k] * cur.mask[k + 1]);
+          if (j + 1 < Ny)
+            cur.cy[k] = 0.5 * (cur.coeff[k] + cur.coeff[k + Nx]) * (cur.mask[k] * cur.mask[k + Nx]);
+        }
+      cur.inv.assign(N, 0.0);
+      for (int j = 0; j < Ny; ++j)
+        for (int i = 0; i < Nx; ++i) {
+          const size_t k = (size_t)j * Nx + i;
+          const double cxw = i > 0 ? cur.cx[k - 1] : 0.0;
+          const double cys = j > 0 ? cur.cy[k - Nx] : 0.0;
+          double d = -(cur.cx[k] + cxw) / (cur.dx * cur.dx) - (cur.cy[k] + cys) / (cur.dy * cur.dy) - lam;
+          d *= cur.mask[k];
+          cur.inv[k] = (std::fabs(d) > 1e-30 ? 1.0 / d : 0.0) * cur.mask[k];
+        }
+    }
+    HostLevel next;
+    if (l < nl - 1) {
+      // _restrict_mask / _restrict_coeff (multigrid.py:79-101, 182-218)
+      next.ny = Ny / 2;
+      next.nx = Nx / 2;
+      next.dx = cur.dx * 2.0;
+      next.dy = cur.dy * 2.0;
+      const size_t Nc = (size_t)next.ny * next.nx;
+      next.mask.assign(Nc, 0.0);
+      next.coeff.assign(Nc, 0.0);
+      for (int J = 0; J < next.ny; ++J)
+        for (int I = 0; I < next.nx; ++I) {
+          const size_t a = (size_t)(2 * J) * Nx + 2 * I, b = a + Nx, c = a + 1, d = a + Nx + 1;
+          const double avg = (cur.mask[a] + cur.mask[b] + cur.mask[c] + cur.mask[d]) / 4.0;
+          const double mc = avg >= 0.5 ? 1.0 : 0.0;
+          const double s4 = cur.coeff[a] * cur.mask[a] + cur.coeff[b] * cur.mask[b] +
+                            cur.coeff[c] * cur.mask[c] + cur.coeff[d] * cur.mask[d];
+          const double cnt = cur.mask[a] + cur.mask[b] + cur.mask[c] + cur.mask[d];
+          next.mask[(size_t)J * next.nx + I] = mc;
+          next.coeff[(size_t)J * next.nx + I] = (s4 / std::max(cnt, 1.0)) * mc;
+        }
+    }
+    out.push_back(std::move(cur));
+    if (l < nl - 1) cur = std::move(next);
+  }
+  return out;
+}
+
+template <typename T>
+static int upload_hier(Plan& P, Hier<T>& H, const std::vector<HostLevel>& hl) {
+  auto alloc = [&](size_t bytes, void** p) -> int {
+    FVX_HIP(hipMalloc(p, bytes));
+    H.allocs.push_back(*p);
+    return kOk;
+  };
+  H.lv.resize(hl.size());
+  for (size_t l = 0; l < hl.size(); ++l) {
+    const HostLevel& h = hl[l];
+    LevelDev<T>& L = H.lv[l];
+    std::memset(&L, 0, sizeof(L));
+    L.ny = h.ny;
+    L.nx = h.nx;
+    L.idx2 = (T)(1.0 / (h.dx * h.dx));
+    L.idy2 = (T)(1.0 / (h.dy * h.dy));
+    L.lam = (T)P.lam;
+    L.inv_d = (T)(1.0 / (-2.0 / (h.dx * h.dx) - 2.0 / (h.dy * h.dy) - P.lam));
+    const size_t N = (size_t)h.ny * h.nx;
+    if (!h.all_wet) {
+      std::vector<uint8_t> m(N);
+      for (size_t k = 0; k < N; ++k) m[k] = h.mask[k] != 0.0;
+      void* d;
+      int rc = alloc(N, &d);
+      if (rc) return rc;
+      FVX_HIP(hipMemcpy(d, m.data(), N, hipMemcpyHostToDevice));
+      L.mask = (const uint8_t*)d;
+    }
+    if (P.kind == kNoflux) {
+      std::vector<T> tmp(N);
+      const std::vector<double>* srcs[3] = {&h.cx, &h.cy, &h.inv};
+      const T** dsts[3] = {&L.cx, &L.cy, &L.inv};
+      for (int a = 0; a < 3; ++a) {
+        for (size_t k = 0; k < N; ++k) tmp[k] = (T)(*srcs[a])[k];
+        void* d;
+        int rc = alloc(N * sizeof(T), &d);
+        if (rc) return rc;
+        FVX_HIP(hipMemcpy(d, tmp.data(), N * sizeof(T), hipMemcpyHostToDevice));
+        *dsts[a] = (const T*)d;
+      }
+    }
+    void* w[5];
+    for (int a = 0; a < 5; ++a) {
+      int rc = alloc(N * sizeof(T), &w[a]);
+      if (rc) return rc;
+      FVX_HIP(hipMemset(w[a], 0, N * sizeof(T)));
+    }
+    L.u = (T*)w[0];
+    L.b = (T*)w[1];
+    L.t = (T*)w[2];
+    L.t2 = (T*)w[3];
+    L.t3 = (T*)w[4];
+  }
+  const size_t N0 = (size_t)hl[0].ny * hl[0].nx;
+  void* v[6];
+  for (int k = 0; k < 6; ++k) {
+    int rc = alloc(N0 * sizeof(T), &v[k]);
+    if (rc) return rc;
+    FVX_HIP(hipMemset(v[k], 0, N0 * sizeof(T)));
+  }
+  H.r = (T*)v[0];
+  H.z = (T*)v[1];
+  H.p0 = (T*)v[2];
+  H.p1 = (T*)v[3];
+  H.q = (T*)v[4];
+  H.x = (T*)v[5];
+  return kOk;
+}
+
+template <typename T>
+static void free_hier(Hier<T>& H) {
+  for (void* p : H.allocs) (void)hipFree(p);
+  H.allocs.clear();
+  H.lv.clear();
+}
+
+// Cells at or below which a level is handled by the coarse single-workgroup kernel.
+constexpr int kCoarseCells = 64 * 64;
+
+template <typename T>
+static Hier<T>& hier(Plan& P);
+template <>
+Hier<double>& hier<double>(Plan& P) { return P.h64; }
+template <>
+Hier<float>& hier<float>(Plan& P) { return P.h32; }
+
+template <typename T, int KIND, int NS, int MODE, bool ZERO>
+static void launch_tile(const Plan& P, const LevelDev<T>& L, const LevelDev<T>* C, const T* u_in,
+                        const T* b, T* u_out, const T* w, PcgScalars* sc, hipStream_t s) {
+  TileArgs<T> a;
+  std::memset(&a, 0, sizeof(a));
+  a.L = L;
+  if (C) a.C = *C;
+  a.u_in = u_in;
+  a.b = b;
+  a.u_out = u_out;
+  a.w = w;
+  a.omega = (T)P.omega;
+  a.partials = P.partials;
+  a.sc = sc;
+  a.tiles_x = cdiv(L.nx, kTx);
+  a.ntiles = a.tiles_x * cdiv(L.ny, kTy);
+  hipLaunchKernelGGL((mg_tile_kernel<T, KIND, NS, MODE, ZERO>), dim3(a.ntiles), dim3(kTileThreads),
+                     0, s, a);
+}
+
+// nu Jacobi sweeps on level L, up to 2 per kernel.  Ping-pongs between the
+// scratch arrays s1 / s2 (which must differ from u_in and final_out); the last
+// chunk writes final_out when given.  Returns the array holding the result.
+template <typename T, int KIND>
+static T* smooth_chunks(const Plan& P, const LevelDev<T>& L, const T* u_in, bool zero,
+                        const T* b, int nu, T* s1, T* s2, T* final_out, PcgScalars* sc,
+                        hipStream_t s) {
+  const T* src = u_in;
+  T* dst = nullptr;
+  const int K = (nu + 1) / 2;
+  for (int c = 0; c < K; ++c) {
+    const int ns = (c == K - 1 && (nu & 1)) ? 1 : 2;
+    dst = (c == K - 1 && final_out) ? final_out : ((c & 1) ? s2 : s1);
+    const bool z = zero && c == 0;
+    if (ns == 2) {
+      if (z) launch_tile<T, KIND, 2, kSmooth, true>(P, L, nullptr, src, b, dst, nullptr, sc, s);
+      else launch_tile<T, KIND, 2, kSmooth, false>(P, L, nullptr, src, b, dst, nullptr, sc, s);
+    } else {
+      if (z) launch_tile<T, KIND, 1, kSmooth, true>(P, L, nullptr, src, b, dst, nullptr, sc, s);
+      else launch_tile<T, KIND, 1, kSmooth, false>(P, L, nullptr, src, b, dst, nullptr, sc, s);
+    }
+    src = dst;
+  }
+  return dst;
+}
+
+// Flexible-PCG dot products z.r and z.q_prev when they cannot be fused into
+// the last V-cycle kernel.
+template <typename T>
+__global__ __launch_bounds__(256) void pcg_dots_kernel(int ny, int nx, const T* __restrict__ z,
+                                                       const T* __restrict__ r,
+                                                       const T* __restrict__ q,
+                                                       double* partials, PcgScalars* sc) {
+  if (sc->done) return;
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+  double d0 = 0.0, d1 = 0.0;
+  if (i < nx && j < ny) {
+    const int64_t k = (int64_t)j * nx + i;
+    d0 = (double)z[k] * r[k];
+    if (!sc->first) d1 = (double)z[k] * q[k];
+  }
+  double v[2] = {d0, d1}, out[2];
+  if (block_reduce_last<2, 256>(v, partials, &sc->ticket[5], out)) {
+    if (threadIdx.x == 0) {
+      sc->beta = sc->first ? 0.0 : (-sc->alpha * out[1]) / sc->rz;
+      sc->rz = out[0];
+    }
+  }
+}
+
+// One V-cycle z = V(u0 or 0, rhs) at level 0.  Returns 1 in *fused when the
+// flexible-PCG dots against rhs and w were formed by the last kernel.
+template <typename T, int KIND>
+static int v_cycle_impl(Plan& P, const T* u0, const T* rhs, T* out, bool want_dots,
+                        const T* w, PcgScalars* sc, hipStream_t s, bool* fused) {
+  Hier<T>& H = hier<T>(P);
+  const int nl = P.nl, lc = P.lc;
+  if (fused) *fused = false;
+  if (lc == 0) {
+    const LevelDev<T>& L = H.lv[0];
+    const size_t bytes = sizeof(T) * (size_t)L.ny * L.nx;
+    if ((int64_t)L.ny * L.nx > kCoarseCells) {
+      // One large level (grid not divisible): the V-cycle is n_coarse sweeps.
+      if (P.n_coarse == 0) {
+        if (u0) FVX_HIP(hipMemcpyAsync(out, u0, bytes, hipMemcpyDeviceToDevice, s));
+        else FVX_HIP(hipMemsetAsync(out, 0, bytes, s));
+      } else {
+        smooth_chunks<T, KIND>(P, L, u0, u0 == nullptr, rhs, P.n_coarse, L.t, L.t2, out, sc, s);
+      }
+      FVX_LAUNCH_CHECK();
+      return kOk;
+    }
+    // Whole hierarchy in the single-workgroup kernel.
+    FVX_HIP(hipMemcpyAsync(L.b, rhs, bytes, hipMemcpyDeviceToDevice, s));
+    if (u0) FVX_HIP(hipMemcpyAsync(L.u, u0, bytes, hipMemcpyDeviceToDevice, s));
+    CoarseArgs<T> ca;
+    std::memset(&ca, 0, sizeof(ca));
+    for (int l = 0; l < nl; ++l) ca.lv[l] = H.lv[l];
+    ca.lc = 0;
+    ca.nl = nl;
+    ca.n_pre = P.n_pre;
+    ca.n_post = P.n_post;
+    ca.n_coarse = P.n_coarse;
+    ca.nonzero_init = u0 != nullptr;
+    ca.omega = (T)P.omega;
+    ca.done = sc ? &sc->done : nullptr;
+    hipLaunchKernelGGL((mg_coarse_kernel<T, KIND>), dim3(1), dim3(kCoarseThreads), 0, s, ca);
+    FVX_HIP(hipMemcpyAsync(out, L.u, bytes, hipMemcpyDeviceToDevice, s));
+    FVX_LAUNCH_CHECK();
+    return kOk;
+  }
+  // Down leg over the tiled levels 0 .. lc-1: result iterate in L.u.
+  for (int l = 0; l < lc; ++l) {
+    const LevelDev<T>& L = H.lv[l];
+    const LevelDev<T>& C = H.lv[l + 1];
+    const T* b = (l == 0) ? rhs : L.b;
+    bool zero = (l > 0) || (u0 == nullptr);
+    const T* src = (l == 0) ? u0 : nullptr;
+    const int npre = P.n_pre;
+    const int fz = npre >= 2 ? 2 : npre;  // sweeps fused with residual+restriction
+    if (npre - fz > 0) {
+      src = smooth_chunks<T, KIND>(P, L, src, zero, b, npre - fz, L.t, L.t2, nullptr, sc, s);
+      zero = false;
+    }
+    if (fz == 2) {
+      if (zero) launch_tile<T, KIND, 2, kDown, true>(P, L, &C, src, b, L.u, nullptr, sc, s);
+      else launch_tile<T, KIND, 2, kDown, false>(P, L, &C, src, b, L.u, nullptr, sc, s);
+    } else if (fz == 1) {
+      if (zero) launch_tile<T, KIND, 1, kDown, true>(P, L, &C, src, b, L.u, nullptr, sc, s);
+      else launch_tile<T, KIND, 1, kDown, false>(P, L, &C, src, b, L.u, nullptr, sc, s);
+    } else {
+      if (zero) launch_tile<T, KIND, 0, kDown, true>(P, L, &C, src, b, L.u, nullptr, sc, s);
+      else launch_tile<T, KIND, 0, kDown, false>(P, L, &C, src, b, L.u, nullptr, sc, s);
+    }
+  }
+  // Coarse levels lc .. nl-1 in one workgroup (zero guess at level lc).
+  if (lc < nl) {
+    CoarseArgs<T> ca;
+    std::memset(&ca, 0, sizeof(ca));
+    for (int l = 0; l < nl; ++l) ca.lv[l] = H.lv[l];
+    ca.lc = lc;
+    ca.nl = nl;
+    ca.n_pre = P.n_pre;
+    ca.n_post = P.n_post;
+    ca.n_coarse = P.n_coarse;
+    ca.omega = (T)P.omega;
+    ca.done = sc ? &sc->done : nullptr;
+    hipLaunchKernelGGL((mg_coarse_kernel<T, KIND>), dim3(1), dim3(kCoarseThreads), 0, s, ca);
+  }
+  // Up leg over the tiled levels lc-1 .. 0.  cur = array holding level l+1's
+  // corrected iterate; each up kernel reads L.u and writes a different array.
+  T* cur = (lc < nl) ? H.lv[lc].u : nullptr;
+  for (int l = lc - 1; l >= 0; --l) {
+    const LevelDev<T>& L = H.lv[l];
+    LevelDev<T> C = H.lv[l + 1];
+    C.u = cur;
+    const T* b = (l == 0) ? rhs : L.b;
+    const int npost = P.n_post;
+    const int fz = npost >= 2 ? 2 : npost;
+    const int tail = npost - fz;
+    const bool dk = want_dots && l == 0 && tail == 0;
+    T* dst = (l == 0 && tail == 0) ? out : L.t;
+    if (dk) {
+      if (fz == 2) launch_tile<T, KIND, 2, kUpDots, false>(P, L, &C, L.u, b, dst, w, sc, s);
+      else if (fz == 1) launch_tile<T, KIND, 1, kUpDots, false>(P, L, &C, L.u, b, dst, w, sc, s);
+      else launch_tile<T, KIND, 0, kUpDots, false>(P, L, &C, L.u, b, dst, w, sc, s);
+      if (fused) *fused = true;
+    } else {
+      if (fz == 2) launch_tile<T, KIND, 2, kUp, false>(P, L, &C, L.u, b, dst, nullptr, sc, s);
+      else if (fz == 1) launch_tile<T, KIND, 1, kUp, false>(P, L, &C, L.u, b, dst, nullptr, sc, s);
+      else launch_tile<T, KIND, 0, kUp, false>(P, L, &C, L.u, b, dst, nullptr, sc, s);
+    }
+    if (tail > 0) {
+      dst = smooth_chunks<T, KIND>(P, L, dst, false, b, tail, L.t2, L.t3, l == 0 ? out : nullptr,
+                                   sc, s);
+    }
+    cur = dst;
+  }
+  FVX_LAUNCH_CHECK();
+  return kOk;
+}
+
+template <typename T>
+static int v_cycle(Plan& P, const T* u0, const T* rhs, T* out, PcgScalars* sc, hipStream_t s) {
+  return P.kind == kDirichlet
+             ? v_cycle_impl<T, kDirichlet>(P, u0, rhs, out, false, nullptr, sc, s, nullptr)
+             : v_cycle_impl<T, kNoflux>(P, u0, rhs, out, false, nullptr, sc, s, nullptr);
+}
+
+static dim3 grid2(int ny, int nx) { return dim3(cdiv(nx, 64), cdiv(ny, 4)); }
+
+// Enqueue ``n`` flexible-PCG iterations (device-guarded) on hierarchy H.
+// Precondition: H.r holds the residual and sc->first / tol2 are set.
+template <typename T, int KIND>
+static int pcg_iters(Plan& P, Hier<T>& H, T* x, int n, PcgScalars* sc, hipStream_t s) {
+  const LevelDev<T>& L = H.lv[0];
+  for (int k = 0; k < n; ++k) {
+    bool fused = false;
+    int rc = v_cycle_impl<T, KIND>(P, nullptr, H.r, H.z, true, H.q, sc, s, &fused);
+    if (rc) return rc;
+    if (!fused)
+      hipLaunchKernelGGL(pcg_dots_kernel<T>, grid2(L.ny, L.nx), dim3(64, 4), 0, s, L.ny, L.nx,
+                         H.z, H.r, H.q, P.partials, sc);
+    hipLaunchKernelGGL((pcg_dir_kernel<T, KIND>), grid2(L.ny, L.nx), dim3(64, 4), 0, s, L, H.z,
+                       H.p0, H.p1, H.q, P.partials, sc);
+    std::swap(H.p0, H.p1);
+    hipLaunchKernelGGL(pcg_update_kernel<T>, grid2(L.ny, L.nx), dim3(64, 4), 0, s, L.ny, L.nx, x,
+                       H.p0, H.r, H.q, P.partials, sc);
+  }
+  FVX_LAUNCH_CHECK();
+  return kOk;
+}
+
+static int read_scalars(Plan& P, hipStream_t s) {
+  FVX_HIP(hipMemcpyAsync(P.hsc, P.sc, 2 * sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
+  FVX_HIP(hipStreamSynchronize(s));
+  return kOk;
+}
+
+}  // namespace fvx
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+using fvx::Plan;
+
+extern "C" {
+
+typedef struct fvx_cg_info {
+  int iterations;        // inner PCG iterations (summed over refinement sweeps)
+  int outer;             // refinement sweeps (mixed precision), else 0
+  int converged;         // 1 if the stopping rule was met
+  int pad;
+  double residual_norm;  // ||b - A x||_2 (FP64 true residual for mixed/FP64 solves)
+  double rhs_norm;       // ||b||_2
+} fvx_cg_info;
+
+int fvx_mg_plan_create(const double* mask, const double* coeff, int ny, int nx, double dx,
+                       double dy, double lambda_, int n_levels, int n_pre, int n_post,
+                       int n_coarse, double omega, int kind, int precision, void** plan_out) {
+  using namespace fvx;
+  FVX_CHECK_ARG(mask && plan_out, "null argument");
+  FVX_CHECK_ARG(ny >= 1 && nx >= 1, "empty grid");
+  FVX_CHECK_ARG(kind == kNoflux || kind == kDirichlet, "unknown operator kind %d", kind);
+  FVX_CHECK_ARG(precision == 32 || precision == 64 || precision == 3264,
+                "precision must be 32, 64 or 3264 (mixed)");
+  FVX_CHECK_ARG(n_pre >= 0 && n_post >= 0 && n_coarse >= 0, "negative sweep count");
+  FVX_CHECK_ARG(dx > 0 && dy > 0, "grid spacing must be positive");
+  int nl = n_levels > 0 ? n_levels : compute_n_levels(ny, nx);
+  FVX_CHECK_ARG(nl <= 16, "too many levels (%d)", nl);
+  const int f = 1 << (nl - 1);
+  if (ny % f != 0 || nx % f != 0) {
+    set_error("Grid shape (%d, %d) is not divisible by 2^(n_levels-1) = %d.  Choose a different "
+              "n_levels or pad the grid.", ny, nx, f);
+    return kInvalidArgument;
+  }
+  Plan* P = new Plan();
+  P->kind = kind;
+  P->prec = precision;
+  P->nl = nl;
+  P->ny = ny;
+  P->nx = nx;
+  P->n_pre = n_pre;
+  P->n_post = n_post;
+  P->n_coarse = n_coarse;
+  P->omega = omega;
+  P->lam = lambda_;
+  P->dx = dx;
+  P->dy = dy;
+  auto hl = build_host_levels(mask, coeff, ny, nx, dx, dy, lambda_, nl, kind);
+  // First level small enough for the single-workgroup kernel (all below too).
+  // (one-level hierarchies run entirely there; otherwise level 0 is tiled).
+  P->lc = nl;
+  for (int l = 0; l < nl; ++l)
+    if ((int64_t)hl[l].ny * hl[l].nx <= kCoarseCells) {
+      P->lc = l;
+      break;
+    }
+  // The bottom level is always the coarse kernel's (even when it is large).
+  if (P->lc > nl - 1) P->lc = nl - 1;
+  if (nl == 1) P->lc = 0;
+  else if (P->lc == 0) P->lc = 1;
+  int rc = 0;
+  if (precision == 64 || precision == 3264) {
+    rc = upload_hier<double>(*P, P->h64, hl);
+    P->has64 = true;
+  }
+  if (!rc && (precision == 32 || precision == 3264)) {
+    rc = upload_hier<float>(*P, P->h32, hl);
+    P->has32 = true;
+  }
+  if (!rc) {
+    const size_t maxblk = (size_t)cdiv(nx, 64) * cdiv(ny, 4) + 64;
+    hipError_t e = hipMalloc(&P->partials, maxblk * 4 * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&P->sc, 2 * sizeof(PcgScalars));
+    if (e == hipSuccess) e = hipMemset(P->sc, 0, 2 * sizeof(PcgScalars));
+    if (e == hipSuccess) e = hipHostMalloc(&P->hsc, 2 * sizeof(PcgScalars), hipHostMallocDefault);
+    if (e != hipSuccess) {
+      set_error("HIP allocation failed: %s", hipGetErrorString(e));
+      rc = kHipError;
+    }
+  }
+  if (rc) {
+    free_hier(P->h64);
+    free_hier(P->h32);
+    delete P;
+    return rc;
+  }
+  *plan_out = P;
+  return kOk;
+}
+
+int fvx_mg_plan_destroy(void* plan) {
+  using namespace fvx;
+  if (!plan) return kOk;
+  Plan* P = (Plan*)plan;
+  (void)hipDeviceSynchronize();
+  free_hier(P->h64);
+  free_hier(P->h32);
+  (void)hipFree(P->partials);
+  (void)hipFree(P->sc);
+  (void)hipHostFree(P->hsc);
+  delete P;
+  return kOk;
+}
+
+// Number of levels, level shapes (2 ints per level) and the coarse-kernel level.
+int fvx_mg_plan_info(void* plan, int* n_levels, int* shapes, int* coarse_level) {
+  using namespace fvx;
+  FVX_CHECK_ARG(plan, "null plan");
+  Plan* P = (Plan*)plan;
+  if (n_levels) *n_levels = P->nl;
+  if (coarse_level) *coarse_level = P->lc;
+  if (shapes) {
+    int ny = P->ny, nx = P->nx;
+    for (int l = 0; l < P->nl; ++l) {
+      shapes[2 * l] = ny;
+      shapes[2 * l + 1] = nx;
+      ny /= 2;
+      nx /= 2;
+    }
+  }
+  return kOk;
+}
+
+#define FVX_MG_T(SUF, T, HN)                                                                     \
+  int fvx_mg_apply_operator_##SUF(void* plan, int level, const T* u, T* out, void* stream) {     \
+    using namespace fvx;                                                                         \
+    FVX_CHECK_ARG(plan && u && out, "null argument");                                            \
+    Plan* P = (Plan*)plan;                                                                       \
+    FVX_CHECK_ARG(P->has##HN, "plan was not built for this precision");                          \
+    FVX_CHECK_ARG(level >= 0 && level < P->nl, "level %d out of range", level);                  \
+    const auto& L = hier<T>(*P).lv[level];                                                       \
+    if (P->kind == kDirichlet)                                                                   \
+      hipLaunchKernelGGL((apply_op_kernel<T, kDirichlet>), grid2(L.ny, L.nx), dim3(64, 4), 0,    \
+                         (hipStream_t)stream, L, u, out);                                        \
+    else                                                                                         \
+      hipLaunchKernelGGL((apply_op_kernel<T, kNoflux>), grid2(L.ny, L.nx), dim3(64, 4), 0,       \
+                         (hipStream_t)stream, L, u, out);                                        \
+    FVX_LAUNCH_CHECK();                                                                          \
+    return kOk;                                                                                  \
+  }                                                                                              \
+  int fvx_mg_jacobi_##SUF(void* plan, int level, const T* u, const T* rhs, T* out, int n_iters,  \
+                          void* stream) {                                                        \
+    using namespace fvx;                                                                         \
+    FVX_CHECK_ARG(plan && rhs && out, "null argument");                                          \
+    Plan* P = (Plan*)plan;                                                                       \
+    FVX_CHECK_ARG(P->has##HN, "plan was not built for this precision");                          \
+    FVX_CHECK_ARG(level >= 0 && level < P->nl, "level %d out of range", level);                  \
+    FVX_CHECK_ARG(n_iters >= 0, "negative n_iters");                                             \
+    FVX_CHECK_ARG(out != u && out != rhs, "out must not alias inputs");                          \
+    const auto& L = hier<T>(*P).lv[level];                                                       \
+    hipStream_t s = (hipStream_t)stream;                                                         \
+    if (n_iters == 0) {                                                                          \
+      hipLaunchKernelGGL(mask_copy_kernel<T>, grid2(L.ny, L.nx), dim3(64, 4), 0, s, L, u, out,   \
+                         (int64_t)L.nx);                                                         \
+      hipLaunchKernelGGL(mask_copy_kernel<T>, grid2(L.ny, L.nx), dim3(64, 4), 0, s, L, out, out, \
+                         (int64_t)L.nx);                                                         \
+      FVX_LAUNCH_CHECK();                                                                        \
+      return kOk;                                                                                \
+    }                                                                                            \
+    if (P->kind == kDirichlet)                                                                   \
+      smooth_chunks<T, kDirichlet>(*P, L, u, u == nullptr, rhs, n_iters, L.t, L.t2, out,         \
+                                   nullptr, s);                                                  \
+    else                                                                                         \
+      smooth_chunks<T, kNoflux>(*P, L, u, u == nullptr, rhs, n_iters, L.t, L.t2, out, nullptr,   \
+                                s);                                                              \
+    FVX_LAUNCH_CHECK();                                                                          \
+    return kOk;                                                                                  \
+  }                                                                                              \
+  int fvx_mg_restrict_##SUF(void* plan, int level, const T* v, T* out, void* stream) {           \
+    using namespace fvx;                                                                         \
+    FVX_CHECK_ARG(plan && v && out, "null argument");                                            \
+    Plan* P = (Plan*)plan;                                                                       \
+    FVX_CHECK_ARG(P->has##HN, "plan was not built for this precision");                          \
+    FVX_CHECK_ARG(level >= 0 && level + 1 < P->nl, "fine level %d has no coarse level", level);  \
+    const auto& L = hier<T>(*P).lv[level];                                                       \
+    const auto& C = hier<T>(*P).lv[level + 1];                                                   \
+    hipLaunchKernelGGL(restrict_kernel<T>, grid2(C.ny, C.nx), dim3(64, 4), 0,                    \
+                       (hipStream_t)stream, L, C, v, out);                                       \
+    FVX_LAUNCH_CHECK();                                                                          \
+    return kOk;                                                                                  \
+  }                                                                                              \
+  int fvx_mg_prolongate_##SUF(void* plan, int level, const T* vc, T* out, void* stream) {        \
+    using namespace fvx;                                                                         \
+    FVX_CHECK_ARG(plan && vc && out, "null argument");                                           \
+    Plan* P = (Plan*)plan;                                                                       \
+    FVX_CHECK_ARG(P->has##HN, "plan was not built for this precision");                          \
+    FVX_CHECK_ARG(level >= 0 && level + 1 < P->nl, "fine level %d has no coarse level", level);  \
+    const auto& L = hier<T>(*P).lv[level];                                                       \
+    auto C = hier<T>(*P).lv[level + 1];                                                          \
+    C.u = const_cast<T*>(vc);                                                                    \
+    if (P->kind == kDirichlet)                                                                   \
+      hipLaunchKernelGGL((prolong_kernel<T, kDirichlet>), grid2(L.ny, L.nx), dim3(64, 4), 0,     \
+                         (hipStream_t)stream, L, C, out);                                        \
+    else                                                                                         \
+      hipLaunchKernelGGL((prolong_kernel<T, kNoflux>), grid2(L.ny, L.nx), dim3(64, 4), 0,        \
+                         (hipStream_t)stream, L, C, out);                                        \
+    FVX_LAUNCH_CHECK();                                                                          \
+    return kOk;                                                                                  \
+  }                                                                                              \
+  int fvx_mg_v_cycle_##SUF(void* plan, const T* u, const T* rhs, T* out, void* stream) {         \
+    using namespace fvx;                                                                         \
+    FVX_CHECK_ARG(plan && rhs && out, "null argument");                                          \
+    Plan* P = (Plan*)plan;                                                                       \
+    FVX_CHECK_ARG(P->has##HN, "plan was not built for this precision");                          \
+    FVX_CHECK_ARG(out != rhs && (u == nullptr || out != u), "out must not alias inputs");        \
+    hipStream_t s = (hipStream_t)stream;                                                         \
+    return v_cycle<T>(*P, u, rhs, out, nullptr, s);                                                                             \
+  }                                                                                              \
+  int fvx_mg_pcg_solve_##SUF(void* plan, const T* rhs, int64_t ld_rhs, T* x, int64_t ld_x,       \
+                             double rtol, double atol, int max_iters, int use_x0,                \
+                             fvx_cg_info* info, void* stream) {                                  \
+    using namespace fvx;                                                                         \
+    FVX_CHECK_ARG(plan && rhs && x, "null argument");                                            \
+    Plan* P = (Plan*)plan;                                                                       \
+    FVX_CHECK_ARG(P->has##HN, "plan was not built for this precision");                          \
+    FVX_CHECK_ARG(max_iters >= 0, "negative max_iters");                                         \
+    hipStream_t s = (hipStream_t)stream;                                                         \
+    Hier<T>& H = hier<T>(*P);                                                                    \
+    const auto& L = H.lv[0];                                                                     \
+    PcgScalars* sc = &P->sc[0];                                                                  \
+    /* x lives in H.x (packed); copy in the warm start */                                        \
+    if (use_x0)                                                                                  \
+      FVX_HIP(hipMemcpy2DAsync(H.x, L.nx * sizeof(T), x, ld_x * sizeof(T), L.nx * sizeof(T),     \
+                               L.ny, hipMemcpyDeviceToDevice, s));                               \
+    else                                                                                         \
+      FVX_HIP(hipMemsetAsync(H.x, 0, sizeof(T) * (size_t)L.ny * L.nx, s));                       \
+    if (P->kind == kDirichlet)                                                                   \
+      hipLaunchKernelGGL((pcg_init_kernel<T, kDirichlet>), grid2(L.ny, L.nx), dim3(64, 4), 0, s, \
+                         L, rhs, ld_rhs, use_x0 ? H.x : nullptr, H.r, rtol * rtol, atol * atol,  \
+                         P->partials, sc);                                                       \
+    else                                                                                         \
+      hipLaunchKernelGGL((pcg_init_kernel<T, kNoflux>), grid2(L.ny, L.nx), dim3(64, 4), 0, s,    \
+                         L, rhs, ld_rhs, use_x0 ? H.x : nullptr, H.r, rtol * rtol, atol * atol,  \
+                         P->partials, sc);                                                       \
+    FVX_LAUNCH_CHECK();                                                                          \
+    int done_iters = 0;                                                                          \
+    int chunk = std::max(1, std::min(P->last_iters, max_iters));                                \
+    while (true) {                                                                               \
+      const int n = std::min(chunk, max_iters - done_iters);                                     \
+      if (n > 0) {                                                                               \
+        int rc = (P->kind == kDirichlet) ? pcg_iters<T, kDirichlet>(*P, H, H.x, n, sc, s)        \
+                                         : pcg_iters<T, kNoflux>(*P, H, H.x, n, sc, s);          \
+        if (rc) return rc;                                                                       \
+        done_iters += n;                                                                         \
+      }                                                                                          \
+      int rc = read_scalars(*P, s);                                                              \
+      if (rc) return rc;                                                                         \
+      if (P->hsc[0].done || done_iters >= max_iters) break;                                      \
+      chunk = 1;                                                                                 \
+    }                                                                                            \
+    const PcgScalars& h = P->hsc[0];                                                             \
+    P->last_iters = std::max(1, h.iter);                                                         \
+    FVX_HIP(hipMemcpy2DAsync(x, ld_x * sizeof(T), H.x, L.nx * sizeof(T), L.nx * sizeof(T), L.ny, \
+                             hipMemcpyDeviceToDevice, s));                                       \
+    if (info) {                                                                                  \
+      info->iterations = h.iter;                                                                 \
+      info->outer = 0;                                                                           \
+      info->converged = h.done;                                                                  \
+      info->residual_norm = std::sqrt(h.rr);                                                     \
+      info->rhs_norm = std::sqrt(h.bb);                                                          \
+    }                                                                                            \
+    return kOk;                                                                                  \
+  }
+
+FVX_MG_T(f64, double, 64)
+FVX_MG_T(f32, float, 32)
+#undef FVX_MG_T
+
+// FP32 multigrid-PCG inner solves with FP64 iterative refinement: the outer
+// residual b - A x, the correction update and all inner products are FP64;
+// the V-cycles, matvecs and vector updates of the inner solve are FP32.
+// Converges on the FP64 true residual ||b - A x|| <= max(rtol ||b||, atol).
+int fvx_mg_mixed_solve(void* plan, const double* rhs, int64_t ld_rhs, double* x, int64_t ld_x,
+                       double rtol, double atol, int max_outer, int max_inner, double inner_rtol,
+                       fvx_cg_info* info, void* stream) {
+  using namespace fvx;
+  FVX_CHECK_ARG(plan && rhs && x, "null argument");
+  Plan* P = (Plan*)plan;
+  FVX_CHECK_ARG(P->has64 && P->has32, "mixed solve needs a plan built with precision 3264");
+  FVX_CHECK_ARG(max_outer >= 1 && max_inner >= 1, "iteration caps must be >= 1");
+  hipStream_t s = (hipStream_t)stream;
+  Hier<float>& H = P->h32;
+  const auto& L64 = P->h64.lv[0];
+  const auto& L32 = H.lv[0];
+  PcgScalars* in = &P->sc[0];
+  PcgScalars* out = &P->sc[1];
+  FVX_HIP(hipMemsetAsync(out, 0, offsetof(PcgScalars, ticket), s));
+  int total_inner = 0, outer = 0;
+  int inner_pred = std::max(1, std::min(P->last_inner, max_inner));
+  int outer_pred = std::max(1, std::min(P->last_outer, max_outer));
+  bool converged = false;
+  while (outer < max_outer) {
+    // Enqueue outer_pred refinement sweeps, each with inner_pred PCG iterations.
+    const int no = std::min(outer_pred, max_outer - outer);
+    for (int o = 0; o < no; ++o) {
+      if (P->kind == kDirichlet)
+        hipLaunchKernelGGL((refine_residual_kernel<kDirichlet>), grid2(L64.ny, L64.nx),
+                           dim3(64, 4), 0, s, L64, rhs, ld_rhs, x, ld_x, H.r, H.x,
+                           rtol * rtol, atol * atol, P->partials, out);
+      else
+        hipLaunchKernelGGL((refine_residual_kernel<kNoflux>), grid2(L64.ny, L64.nx), dim3(64, 4),
+                           0, s, L64, rhs, ld_rhs, x, ld_x, H.r, H.x, rtol * rtol, atol * atol,
+                           P->partials, out);
+      hipLaunchKernelGGL(inner_reset_kernel, dim3(1), dim3(1), 0, s, in, out, inner_rtol);
+      int rc = (P->kind == kDirichlet) ? pcg_iters<float, kDirichlet>(*P, H, H.x, inner_pred, in, s)
+                                       : pcg_iters<float, kNoflux>(*P, H, H.x, inner_pred, in, s);
+      if (rc) return rc;
+      hipLaunchKernelGGL(refine_update_kernel, grid2(L32.ny, L32.nx), dim3(64, 4), 0, s, L32.ny,
+                         L32.nx, x, ld_x, H.x, out);
+      FVX_LAUNCH_CHECK();
+    }
+    // Final residual check of this batch.
+    if (P->kind == kDirichlet)
+      hipLaunchKernelGGL((refine_residual_kernel<kDirichlet>), grid2(L64.ny, L64.nx), dim3(64, 4),
+                         0, s, L64, rhs, ld_rhs, x, ld_x, H.r, H.x, rtol * rtol, atol * atol,
+                         P->partials, out);
+    else
+      hipLaunchKernelGGL((refine_residual_kernel<kNoflux>), grid2(L64.ny, L64.nx), dim3(64, 4), 0,
+                         s, L64, rhs, ld_rhs, x, ld_x, H.r, H.x, rtol * rtol, atol * atol,
+                         P->partials, out);
+    FVX_LAUNCH_CHECK();
+    int rc = read_scalars(*P, s);
+    if (rc) return rc;
+    const PcgScalars& ho = P->hsc[1];
+    const PcgScalars& hi = P->hsc[0];
+    outer = ho.outer;
+    total_inner += 0;
+    if (ho.out_done) {
+      converged = true;
+      break;
+    }
+    // Not converged: if the inner solve hit its cap, allow more inner steps.
+    if (!hi.done) inner_pred = std::min(max_inner, inner_pred + 2);
+    outer_pred = 1;
+  }
+  const PcgScalars& ho = P->hsc[1];
+  P->last_outer = std::max(1, ho.outer);
+  P->last_inner = inner_pred;
+  if (info) {
+    info->iterations = P->hsc[0].iter;  // inner iterations of the last sweep
+    info->outer = ho.outer;
+    info->converged = converged ? 1 : 0;
+    info->residual_norm = std::sqrt(ho.out_rr);
+    info->rhs_norm = std::sqrt(ho.bb);
+  }
+  return kOk;
+}
+
+}  // extern "C"

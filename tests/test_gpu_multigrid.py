@@ -1,0 +1,192 @@
+"""GPU parity: multigrid hierarchy, transfers, smoother, V-cycle and PCG.
+
+FP64 single operators match the oracle to 1e-12 relative (max-norm scaled by
+max|oracle|); composed V-cycles to 1e-10; FP32 to 1e-5 (operators) and 1e-4
+(V-cycles).  Converged solves are checked against direct sparse solves and
+by their FP64 true residual.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import finitevolx_amd as fx
+from finitevolx_amd import solvers as S
+from oracle import fvx
+from tests.gpu_util import assert_close_scaled, need_gpu, random_mask, to_np
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(ny, nx, kind, masked, varcoef, seed=0, lam=10.0):
+    rng = np.random.default_rng(seed)
+    m = random_mask(ny, nx, seed + 1, 0.2).astype(np.float64) if masked else np.ones((ny, nx))
+    if masked:  # keep the array edge partly wet so edge handling is exercised
+        m[0, ny // 3:] = 1.0
+    coeff = rng.uniform(0.5, 2.0, (ny, nx)) if varcoef else None
+    dx, dy = 1.0 / nx, 1.3 / ny
+    return m, coeff, dx, dy, lam
+
+
+CASES = [
+    (16, 16, "noflux", False, False), (32, 32, "noflux", True, True), (16, 32, "dirichlet", False, False),
+    (128, 96, "noflux", True, True), (256, 192, "dirichlet", True, False), (192, 256, "noflux", False, True),
+]
+
+
+@pytest.mark.parametrize("ny,nx,kind,masked,varcoef", CASES)
+def test_operator_transfers_jacobi(ny, nx, kind, masked, varcoef):
+    need_gpu()
+    m, coeff, dx, dy, lam = _setup(ny, nx, kind, masked, varcoef)
+    levels = fvx.build_multigrid(m, dx, dy, lam, coeff, kind=kind)
+    for prec, tol in (("float64", 1e-12), ("float32", 1e-5)):
+        mg = S.build_multigrid_solver(m, dx, dy, lam, coeff, kind=kind, precision=prec,
+                                      n_pre=2, n_post=2)
+        assert mg.n_levels == len(levels)
+        rng = np.random.default_rng(3)
+        for li, lev in enumerate(levels):
+            u = rng.standard_normal(lev.mask.shape)
+            assert_close_scaled(to_np(S.apply_operator(u, mg.levels[li])),
+                                fvx.apply_operator(u, lev), tol, f"A level {li} {prec}")
+            b = rng.standard_normal(lev.mask.shape)
+            for n in (1, 2, 3):
+                got = S.weighted_jacobi(u, b, mg.levels[li], 0.95, n)
+                assert_close_scaled(to_np(got), fvx.weighted_jacobi(u, b, lev, 0.95, n), 10 * tol,
+                                    f"jacobi{n} level {li} {prec}")
+            if li + 1 < len(levels):
+                cl = levels[li + 1]
+                assert_close_scaled(to_np(S.restrict(u, mg.levels[li])),
+                                    fvx.restrict(u, lev.mask, cl.mask), tol, f"restrict {li}")
+                vc = rng.standard_normal(cl.mask.shape)
+                assert_close_scaled(to_np(S.prolongate(vc, mg.levels[li])),
+                                    fvx.prolongate(vc, cl.mask, lev.mask), tol, f"prolong {li}")
+
+
+@pytest.mark.parametrize("ny,nx,kind,masked,varcoef", CASES)
+@pytest.mark.parametrize("npre,npost", [(6, 6), (2, 2), (1, 3)])
+def test_v_cycle_matches_oracle(ny, nx, kind, masked, varcoef, npre, npost):
+    need_gpu()
+    m, coeff, dx, dy, lam = _setup(ny, nx, kind, masked, varcoef, seed=4)
+    levels = fvx.build_multigrid(m, dx, dy, lam, coeff, kind=kind)
+    rng = np.random.default_rng(5)
+    b = rng.standard_normal((ny, nx)) * m
+    u0 = rng.standard_normal((ny, nx))
+    kw = dict(n_pre=npre, n_post=npost, n_coarse=30, omega=0.9)
+    mg = S.build_multigrid_solver(m, dx, dy, lam, coeff, kind=kind, **kw)
+    ref0 = fvx.v_cycle(levels, np.zeros_like(b), b, **kw)
+    assert_close_scaled(to_np(mg.v_cycle(None, b)), ref0, 1e-10, "V(0,b) f64")
+    ref1 = fvx.v_cycle(levels, u0, b, **kw)
+    assert_close_scaled(to_np(mg.v_cycle(u0, b)), ref1, 1e-10, "V(u0,b) f64")
+    mg32 = S.build_multigrid_solver(m, dx, dy, lam, coeff, kind=kind, precision="float32", **kw)
+    got32 = mg32.v_cycle(None, torch.tensor(b, dtype=torch.float32))
+    assert_close_scaled(to_np(got32), ref0, 1e-4, "V(0,b) f32")
+
+
+def test_solver_call_and_reference_thresholds():
+    """Mirrors reference tests/test_multigrid.py:222-301 thresholds."""
+    need_gpu()
+    ny = nx = 32
+    rng = np.random.default_rng(11)
+    rhs = rng.standard_normal((ny, nx))
+    mg = S.build_multigrid_solver(np.ones((ny, nx)), 1 / nx, 1 / ny, lambda_=10.0, n_cycles=20)
+    u = mg(rhs)
+    lev = fvx.build_multigrid(np.ones((ny, nx)), 1 / nx, 1 / ny, 10.0)
+    ref = fvx.mg_solve(lev, rhs, n_cycles=20)
+    assert_close_scaled(to_np(u), ref, 1e-9, "20 cycles")
+    res = np.linalg.norm(rhs - fvx.apply_operator(to_np(u), lev[0])) / np.linalg.norm(rhs)
+    assert res < 1e-2
+    with pytest.raises(ValueError, match="not divisible"):
+        S.build_multigrid_solver(np.ones((15, 15)), 1.0, 1.0, n_levels=2)
+
+
+def test_masked_outputs_zero_outside():
+    need_gpu()
+    ny = nx = 16
+    jj, ii = np.meshgrid(np.linspace(-.5, .5, ny), np.linspace(-.5, .5, nx), indexing="ij")
+    m = (jj**2 + ii**2 < 0.16).astype(np.float64)
+    mg = S.build_multigrid_solver(m, 1 / nx, 1 / ny, n_levels=1, n_cycles=10)
+    rhs = np.random.default_rng(7).standard_normal((ny, nx)) * m
+    u = to_np(mg(rhs))
+    assert np.all(u[m == 0] == 0.0)
+    Au = to_np(S.apply_operator(np.ones((ny, nx)), mg.levels[0]))
+    assert np.all(Au[m == 0] == 0.0)
+
+
+@pytest.mark.parametrize("ny,nx,kind,masked,varcoef", CASES)
+def test_pcg_converges_to_direct_solution(ny, nx, kind, masked, varcoef):
+    need_gpu()
+    m, coeff, dx, dy, lam = _setup(ny, nx, kind, masked, varcoef, seed=8)
+    rng = np.random.default_rng(9)
+    b = rng.standard_normal((ny, nx)) * m
+    lev = fvx.build_multigrid(m, dx, dy, lam, coeff, kind=kind)
+    A = fvx.operator_matrix(lambda u: fvx.apply_operator(u, lev[0]), (ny, nx))
+    wet = m.ravel() > 0
+    import scipy.sparse.linalg as spla
+    xd = np.zeros(ny * nx)
+    xd[wet] = spla.spsolve(A[wet][:, wet].tocsc(), b.ravel()[wet])
+    xd = xd.reshape(ny, nx)
+    mg = S.build_multigrid_solver(m, dx, dy, lam, coeff, kind=kind, n_pre=2, n_post=2)
+    x, info = fx.solve_cg(S.MultigridOperator(mg), b, preconditioner=S.make_multigrid_preconditioner(mg),
+                          rtol=1e-11, atol=0.0, max_steps=200)
+    assert info.converged, info
+    assert_close_scaled(to_np(x), xd, 1e-8, "pcg vs direct")
+    # the reported residual is the true residual
+    r = np.linalg.norm(b - fvx.apply_operator(to_np(x), lev[0]))
+    assert r <= 1.01e-11 * np.linalg.norm(b) * 1.5
+
+
+@pytest.mark.parametrize("kind", ["dirichlet", "noflux"])
+def test_mixed_precision_certifies_fp64_residual(kind):
+    need_gpu()
+    ny, nx = 256, 192
+    m, coeff, dx, dy, lam = _setup(ny, nx, kind, True, kind == "noflux", seed=12)
+    rng = np.random.default_rng(13)
+    b = rng.standard_normal((ny, nx)) * m
+    lev = fvx.build_multigrid(m, dx, dy, lam, coeff, kind=kind)
+    mix = S.build_multigrid_solver(m, dx, dy, lam, coeff, kind=kind, precision="mixed",
+                                   n_pre=2, n_post=2)
+    x, info = mix.pcg(b, rtol=1e-10, atol=0.0, max_steps=60)
+    assert info.converged
+    r = np.linalg.norm(b - fvx.apply_operator(to_np(x), lev[0]))
+    assert r <= 1e-10 * np.linalg.norm(b) * 1.0001, (r, info)
+    assert mix.last_info.outer >= 2  # FP32 inner solves alone cannot reach 1e-10
+    f64 = S.build_multigrid_solver(m, dx, dy, lam, coeff, kind=kind, n_pre=2, n_post=2)
+    x64, i64 = f64.pcg(b, rtol=1e-10, atol=0.0, max_steps=60)
+    assert i64.converged
+    assert_close_scaled(to_np(x), to_np(x64), 1e-8, "mixed vs fp64")
+
+
+def test_masked_laplacian_and_pv_inversion():
+    need_gpu()
+    ny, nx = 34, 50
+    rng = np.random.default_rng(2)
+    psi = rng.standard_normal((ny, nx))
+    m = random_mask(ny, nx, 4).astype(np.float64)
+    for dt, tol in ((torch.float64, 1e-12), (torch.float32, 1e-5)):
+        got = S.masked_laplacian(torch.tensor(psi, dtype=dt), torch.tensor(m, dtype=dt), 0.1, 0.2, 3.0)
+        assert_close_scaled(to_np(got), fvx.masked_laplacian(psi, m, 0.1, 0.2, 3.0), tol, "mlap")
+    q = rng.standard_normal((ny, nx)) * m
+    psi_cg = to_np(S.pv_inversion(q, 0.1, 0.2, 3.0, method="cg", mask=m))
+    # exact solve of the masked (Dirichlet) Helmholtz problem
+    ref = fvx.dirichlet_helmholtz_direct(q, m, 0.1, 0.2, 3.0)
+    assert_close_scaled(psi_cg, ref, 1e-8, "pv_inversion cg")
+    lam = np.array([1.0, 5.0])
+    q2 = np.stack([q, 2 * q])
+    out = to_np(S.pv_inversion(q2, 0.1, 0.2, lam, method="cg", mask=m))
+    assert_close_scaled(out[1], fvx.dirichlet_helmholtz_direct(2 * q, m, 0.1, 0.2, 5.0), 1e-8, "layer 1")
+    with pytest.raises(ValueError):
+        S.pv_inversion(q2, 0.1, 0.2, np.array([1.0, 2.0, 3.0]), method="cg", mask=m)
+
+
+def test_large_bottom_level_uses_coarse_kernel():
+    """130x130 gives levels 130^2 and 65^2 (> the coarse-kernel size)."""
+    need_gpu()
+    n = 130
+    m = np.ones((n, n))
+    lev = fvx.build_multigrid(m, 1 / n, 1 / n, 5.0)
+    assert len(lev) == 2
+    mg = S.build_multigrid_solver(m, 1 / n, 1 / n, 5.0, n_pre=2, n_post=2, n_coarse=10)
+    assert mg.coarse_level == 1
+    b = np.random.default_rng(0).standard_normal((n, n))
+    ref = fvx.v_cycle(lev, np.zeros_like(b), b, n_pre=2, n_post=2, n_coarse=10)
+    assert_close_scaled(to_np(mg.v_cycle(None, b)), ref, 1e-10, "V 130")
