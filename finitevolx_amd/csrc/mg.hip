@@ -26,33 +26,11 @@
 #include <vector>
 
 #include "fvx_common.hpp"
+#include "mg.hpp"
 
 namespace fvx {
 
 enum MgKind : int { kNoflux = 0, kDirichlet = 1 };
-
-// ---------------------------------------------------------------------------
-// Device-side scalars of one PCG solve.  Layout is part of the C ABI
-// (fvx_cg_info reads iterations / residual from it).
-// ---------------------------------------------------------------------------
-struct PcgScalars {
-  double rz;      // rho_k = r . z
-  double pq;      // p . A p
-  double rr;      // ||r||^2 (recursive residual)
-  double bb;      // ||b||^2
-  double tol2;    // stop when rr <= tol2
-  double alpha, beta;
-  double zq;      // z_{k+1} . q_k  (flexible beta)
-  double out_rr;  // outer (FP64) residual ||b - A x||^2 for refinement
-  double out_tol2;
-  int iter;       // inner iterations done
-  int done;       // 1 once rr <= tol2 (all kernels become no-ops)
-  int first;      // 1 before the first direction is formed
-  int out_done;   // outer refinement converged
-  int outer;      // outer iterations done
-  int pad[3];
-  unsigned int ticket[8];  // last-block counters, one per reduction site
-};
 
 template <typename T>
 struct LevelDev {
@@ -154,11 +132,43 @@ template <typename T, int KIND>
 __device__ __forceinline__ T glob_Au(const LevelDev<T>& L, const T* u, int j, int i) {
   const int64_t k = (int64_t)j * L.nx + i;
   const T m = mval<T>(L.mask, k);
-  const T c = u[k];
-  const T e = (i + 1 < L.nx) ? u[k + 1] : T(0);
-  const T w = (i > 0) ? u[k - 1] : T(0);
-  const T n = (j + 1 < L.ny) ? u[k + L.nx] : T(0);
-  const T s = (j > 0) ? u[k - L.nx] : T(0);
+  // The reference applies the stencil to u * mask (multigrid.py:383,
+  // iterative.py:183); with face coefficients that vanish across land only
+  // the Dirichlet operator needs the neighbours masked.
+  auto U = [&](int64_t kk) {
+    return KIND == kDirichlet ? u[kk] * mval<T>(L.mask, kk) : u[kk];
+  };
+  const T c = u[k] * m;
+  const T e = (i + 1 < L.nx) ? U(k + 1) : T(0);
+  const T w = (i > 0) ? U(k - 1) : T(0);
+  const T n = (j + 1 < L.ny) ? U(k + L.nx) : T(0);
+  const T s = (j > 0) ? U(k - L.nx) : T(0);
+  T lap;
+  if constexpr (KIND == kDirichlet) {
+    lap = (e + w - T(2) * c) * L.idx2 + (n + s - T(2) * c) * L.idy2;
+  } else {
+    const T cxe = L.cx[k], cxw = (i > 0) ? L.cx[k - 1] : T(0);
+    const T cyn = L.cy[k], cys = (j > 0) ? L.cy[k - L.nx] : T(0);
+    lap = (cxe * (e - c) - cxw * (c - w)) * L.idx2 + (cyn * (n - c) - cys * (c - s)) * L.idy2;
+  }
+  return (lap - L.lam * c) * m;
+}
+
+// Operator on a pitched array (row stride ld), zero outside the level.
+template <typename T, int KIND>
+__device__ __forceinline__ T glob_Au_pitched(const LevelDev<T>& L, const T* u, int64_t ld, int j,
+                                             int i) {
+  const int64_t k = (int64_t)j * L.nx + i;
+  const int64_t ku = (int64_t)j * ld + i;
+  const T m = mval<T>(L.mask, k);
+  auto U = [&](int64_t kk, int64_t km) {
+    return KIND == kDirichlet ? u[kk] * mval<T>(L.mask, km) : u[kk];
+  };
+  const T c = u[ku] * m;
+  const T e = (i + 1 < L.nx) ? U(ku + 1, k + 1) : T(0);
+  const T w = (i > 0) ? U(ku - 1, k - 1) : T(0);
+  const T n = (j + 1 < L.ny) ? U(ku + ld, k + L.nx) : T(0);
+  const T s = (j > 0) ? U(ku - ld, k - L.nx) : T(0);
   T lap;
   if constexpr (KIND == kDirichlet) {
     lap = (e + w - T(2) * c) * L.idx2 + (n + s - T(2) * c) * L.idy2;
@@ -254,7 +264,7 @@ __global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
     }
     T u = T(0);
     if (in) {
-      if constexpr (!ZERO) u = a.u_in[g];
+      if constexpr (!ZERO) u = a.u_in[g] * m;  // u_m = u * mask (multigrid.py:383)
       if constexpr (MODE == kUp || MODE == kUpDots) {
         // u = (u + P e_c) * mask  (multigrid.py:875-876)
         const T e = prolong_at<T, KIND>(a.C, gj, gi) * m;
@@ -498,16 +508,16 @@ __global__ __launch_bounds__(256) void prolong_kernel(LevelDev<T> L, LevelDev<T>
 template <typename T, int KIND>
 __global__ __launch_bounds__(256) void pcg_init_kernel(LevelDev<T> L, const T* __restrict__ b,
                                                        int64_t ld_b, const T* __restrict__ x,
-                                                       T* __restrict__ r, double rtol2,
-                                                       double atol2, double* partials,
-                                                       PcgScalars* sc) {
+                                                       int64_t ld_x, T* __restrict__ r,
+                                                       double rtol2, double atol2,
+                                                       double* partials, PcgScalars* sc) {
   const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
   double bb = 0.0, rr = 0.0;
   if (i < L.nx && j < L.ny) {
     const int64_t k = (int64_t)j * L.nx + i;
     const T m = mval<T>(L.mask, k);
     const T bv = b[(int64_t)j * ld_b + i] * m;
-    const T ax = x ? glob_Au<T, KIND>(L, x, j, i) : T(0);
+    const T ax = x ? glob_Au_pitched<T, KIND>(L, x, ld_x, j, i) : T(0);
     const T rv = (bv - ax) * m;
     r[k] = rv;
     bb = (double)bv * bv;
@@ -574,7 +584,7 @@ __global__ __launch_bounds__(256) void pcg_dir_kernel(LevelDev<T> L, const T* __
 // PCG: x += alpha p ; r -= alpha q ; ||r||^2 -> convergence flag.
 template <typename T>
 __global__ __launch_bounds__(256) void pcg_update_kernel(int ny, int nx, T* __restrict__ x,
-                                                         const T* __restrict__ p,
+                                                         int64_t ld_x, const T* __restrict__ p,
                                                          T* __restrict__ r,
                                                          const T* __restrict__ q,
                                                          double* partials, PcgScalars* sc) {
@@ -584,7 +594,7 @@ __global__ __launch_bounds__(256) void pcg_update_kernel(int ny, int nx, T* __re
   double rr = 0.0;
   if (i < nx && j < ny) {
     const int64_t k = (int64_t)j * nx + i;
-    x[k] += alpha * p[k];
+    x[(int64_t)j * ld_x + i] += alpha * p[k];
     const T rv = r[k] - alpha * q[k];
     r[k] = rv;
     rr = (double)rv * rv;
@@ -601,9 +611,9 @@ __global__ __launch_bounds__(256) void pcg_update_kernel(int ny, int nx, T* __re
 
 // Mixed precision: r32 = (b64 - A64 x64) m in FP64, ||r||^2 and ||b||^2 in
 // FP64; the inner FP32 solve starts from e = 0 so e is cleared here too.
-template <int KIND>
+template <int KIND, typename TB>
 __global__ __launch_bounds__(256) void refine_residual_kernel(
-    LevelDev<double> L, const double* __restrict__ b, int64_t ld_b, const double* __restrict__ x,
+    LevelDev<double> L, const TB* __restrict__ b, int64_t ld_b, const double* __restrict__ x,
     int64_t ld_x, float* __restrict__ r32, float* __restrict__ e32, double rtol2, double atol2,
     double* partials, PcgScalars* sc) {
   if (sc->out_done) return;
@@ -613,11 +623,14 @@ __global__ __launch_bounds__(256) void refine_residual_kernel(
     const int64_t k = (int64_t)j * L.nx + i;
     const int64_t kx = (int64_t)j * ld_x + i;
     const double m = mval<double>(L.mask, k);
-    const double c = x[kx];
-    const double e = (i + 1 < L.nx) ? x[kx + 1] : 0.0;
-    const double w = (i > 0) ? x[kx - 1] : 0.0;
-    const double n = (j + 1 < L.ny) ? x[kx + ld_x] : 0.0;
-    const double s = (j > 0) ? x[kx - ld_x] : 0.0;
+    auto X = [&](int64_t kk, int64_t km) {
+      return KIND == kDirichlet ? x[kk] * mval<double>(L.mask, km) : x[kk];
+    };
+    const double c = x[kx] * m;
+    const double e = (i + 1 < L.nx) ? X(kx + 1, k + 1) : 0.0;
+    const double w = (i > 0) ? X(kx - 1, k - 1) : 0.0;
+    const double n = (j + 1 < L.ny) ? X(kx + ld_x, k + L.nx) : 0.0;
+    const double s = (j > 0) ? X(kx - ld_x, k - L.nx) : 0.0;
     double lap;
     if constexpr (KIND == kDirichlet) {
       lap = (e + w - 2.0 * c) * L.idx2 + (n + s - 2.0 * c) * L.idy2;
@@ -626,7 +639,7 @@ __global__ __launch_bounds__(256) void refine_residual_kernel(
       const double cyn = L.cy[k], cys = (j > 0) ? L.cy[k - L.nx] : 0.0;
       lap = (cxe * (e - c) - cxw * (c - w)) * L.idx2 + (cyn * (n - c) - cys * (c - s)) * L.idy2;
     }
-    const double bv = b[(int64_t)j * ld_b + i] * m;
+    const double bv = (double)b[(int64_t)j * ld_b + i] * m;
     const double rv = (bv - (lap - L.lam * c) * m) * m;
     r32[k] = (float)rv;
     e32[k] = 0.0f;
@@ -668,6 +681,17 @@ __global__ void inner_reset_kernel(PcgScalars* inner, const PcgScalars* outer, d
   inner->iter = 0;
 }
 
+// Clear the flags and counters of a scalar block (graph-safe replacement for
+// a memset; tickets are left armed).
+__global__ void scalars_reset_kernel(PcgScalars* sc, int n) {
+  const int k = threadIdx.x;
+  if (k < n) {
+    PcgScalars& x = sc[k];
+    x.rz = x.pq = x.rr = x.bb = x.tol2 = x.alpha = x.beta = x.zq = x.out_rr = x.out_tol2 = 0.0;
+    x.iter = x.done = x.first = x.out_done = x.outer = 0;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Host-side plan
 // ---------------------------------------------------------------------------
@@ -689,7 +713,7 @@ struct Plan {
   Hier<double> h64;
   Hier<float> h32;
   double* partials = nullptr;
-  PcgScalars* sc = nullptr;   // [0] inner, [1] outer
+  PcgScalars* sc = nullptr;   // kSlots pairs: [2k] inner, [2k+1] outer
   PcgScalars* hsc = nullptr;  // pinned host mirror
   int last_iters = 4, last_outer = 2, last_inner = 4;
   std::vector<void*> allocs;
@@ -860,6 +884,8 @@ static void free_hier(Hier<T>& H) {
 
 // Cells at or below which a level is handled by the coarse single-workgroup kernel.
 constexpr int kCoarseCells = 64 * 64;
+// Independent solve slots (device scalar blocks) per plan.
+constexpr int kSlots = 8;
 
 template <typename T>
 static Hier<T>& hier(Plan& P);
@@ -1062,7 +1088,8 @@ static dim3 grid2(int ny, int nx) { return dim3(cdiv(nx, 64), cdiv(ny, 4)); }
 // Enqueue ``n`` flexible-PCG iterations (device-guarded) on hierarchy H.
 // Precondition: H.r holds the residual and sc->first / tol2 are set.
 template <typename T, int KIND>
-static int pcg_iters(Plan& P, Hier<T>& H, T* x, int n, PcgScalars* sc, hipStream_t s) {
+static int pcg_iters(Plan& P, Hier<T>& H, T* x, int64_t ld_x, int n, PcgScalars* sc,
+                     hipStream_t s) {
   const LevelDev<T>& L = H.lv[0];
   for (int k = 0; k < n; ++k) {
     bool fused = false;
@@ -1075,17 +1102,205 @@ static int pcg_iters(Plan& P, Hier<T>& H, T* x, int n, PcgScalars* sc, hipStream
                        H.p0, H.p1, H.q, P.partials, sc);
     std::swap(H.p0, H.p1);
     hipLaunchKernelGGL(pcg_update_kernel<T>, grid2(L.ny, L.nx), dim3(64, 4), 0, s, L.ny, L.nx, x,
-                       H.p0, H.r, H.q, P.partials, sc);
+                       ld_x, H.p0, H.r, H.q, P.partials, sc);
   }
   FVX_LAUNCH_CHECK();
   return kOk;
 }
 
-static int read_scalars(Plan& P, hipStream_t s) {
-  FVX_HIP(hipMemcpyAsync(P.hsc, P.sc, 2 * sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
-  FVX_HIP(hipStreamSynchronize(s));
+
+template <typename T>
+__global__ __launch_bounds__(256) void zero_pitched_kernel(int ny, int nx, T* __restrict__ x,
+                                                           int64_t ld) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+  if (i < nx && j < ny) x[(int64_t)j * ld + i] = T(0);
+}
+
+// Enqueue PCG init + ``iters`` guarded iterations on slot ``slot``; no sync.
+template <typename T>
+int pcg_enqueue(Plan& P, const T* rhs, int64_t ld_rhs, T* x, int64_t ld_x, double rtol,
+                double atol, bool use_x0, int iters, int slot, hipStream_t s) {
+  Hier<T>& H = hier<T>(P);
+  const auto& L = H.lv[0];
+  PcgScalars* sc = &P.sc[2 * slot];
+  if (!use_x0)
+    hipLaunchKernelGGL(zero_pitched_kernel<T>, grid2(L.ny, L.nx), dim3(64, 4), 0, s, L.ny, L.nx,
+                       x, ld_x);
+  if (P.kind == kDirichlet)
+    hipLaunchKernelGGL((pcg_init_kernel<T, kDirichlet>), grid2(L.ny, L.nx), dim3(64, 4), 0, s, L,
+                       rhs, ld_rhs, use_x0 ? x : nullptr, ld_x, H.r, rtol * rtol, atol * atol,
+                       P.partials, sc);
+  else
+    hipLaunchKernelGGL((pcg_init_kernel<T, kNoflux>), grid2(L.ny, L.nx), dim3(64, 4), 0, s, L,
+                       rhs, ld_rhs, use_x0 ? x : nullptr, ld_x, H.r, rtol * rtol, atol * atol,
+                       P.partials, sc);
+  FVX_LAUNCH_CHECK();
+  if (iters <= 0) return kOk;
+  return (P.kind == kDirichlet) ? pcg_iters<T, kDirichlet>(P, H, x, ld_x, iters, sc, s)
+                                : pcg_iters<T, kNoflux>(P, H, x, ld_x, iters, sc, s);
+}
+
+// Synchronous PCG: predicted chunk, then single iterations until converged.
+template <typename T>
+int pcg_solve(Plan& P, const T* rhs, int64_t ld_rhs, T* x, int64_t ld_x, double rtol,
+              double atol, int max_iters, bool use_x0, CgInfoC* info, int slot, hipStream_t s) {
+  Hier<T>& H = hier<T>(P);
+  PcgScalars* sc = &P.sc[2 * slot];
+  int chunk = std::max(1, std::min(P.last_iters, max_iters));
+  int rc = pcg_enqueue<T>(P, rhs, ld_rhs, x, ld_x, rtol, atol, use_x0, std::min(chunk, max_iters),
+                          slot, s);
+  if (rc) return rc;
+  int done_iters = std::min(chunk, max_iters);
+  while (true) {
+    FVX_HIP(hipMemcpyAsync(&P.hsc[2 * slot], sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
+    FVX_HIP(hipStreamSynchronize(s));
+    if (P.hsc[2 * slot].done || done_iters >= max_iters) break;
+    rc = (P.kind == kDirichlet) ? pcg_iters<T, kDirichlet>(P, H, x, ld_x, 1, sc, s)
+                                : pcg_iters<T, kNoflux>(P, H, x, ld_x, 1, sc, s);
+    if (rc) return rc;
+    done_iters += 1;
+  }
+  const PcgScalars& h = P.hsc[2 * slot];
+  P.last_iters = std::max(1, h.iter);
+  if (info) {
+    info->iterations = h.iter;
+    info->outer = 0;
+    info->converged = h.done;
+    info->residual_norm = std::sqrt(h.rr);
+    info->rhs_norm = std::sqrt(h.bb);
+  }
   return kOk;
 }
+
+// Enqueue the mixed-precision schedule: n_outer x [FP64 residual, FP32 PCG
+// (n_inner guarded iterations), FP64 update] and a closing FP64 residual.
+template <typename TB>
+int mixed_enqueue(Plan& P, const TB* rhs, int64_t ld_rhs, double* x, int64_t ld_x, double rtol,
+                  double atol, int n_outer, int n_inner, double inner_rtol, int slot,
+                  hipStream_t s) {
+  Hier<float>& H = P.h32;
+  const auto& L64 = P.h64.lv[0];
+  const auto& L32 = H.lv[0];
+  PcgScalars* in = &P.sc[2 * slot];
+  PcgScalars* out = &P.sc[2 * slot + 1];
+  hipLaunchKernelGGL(scalars_reset_kernel, dim3(1), dim3(64), 0, s, out, 1);
+  auto residual = [&]() {
+    if (P.kind == kDirichlet)
+      hipLaunchKernelGGL((refine_residual_kernel<kDirichlet, TB>), grid2(L64.ny, L64.nx),
+                         dim3(64, 4), 0, s, L64, rhs, ld_rhs, x, ld_x, H.r, H.x, rtol * rtol,
+                         atol * atol, P.partials, out);
+    else
+      hipLaunchKernelGGL((refine_residual_kernel<kNoflux, TB>), grid2(L64.ny, L64.nx),
+                         dim3(64, 4), 0, s, L64, rhs, ld_rhs, x, ld_x, H.r, H.x, rtol * rtol,
+                         atol * atol, P.partials, out);
+  };
+  for (int o = 0; o < n_outer; ++o) {
+    residual();
+    hipLaunchKernelGGL(inner_reset_kernel, dim3(1), dim3(1), 0, s, in, out, inner_rtol);
+    int rc = (P.kind == kDirichlet)
+                 ? pcg_iters<float, kDirichlet>(P, H, H.x, L32.nx, n_inner, in, s)
+                 : pcg_iters<float, kNoflux>(P, H, H.x, L32.nx, n_inner, in, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(refine_update_kernel, grid2(L32.ny, L32.nx), dim3(64, 4), 0, s, L32.ny,
+                       L32.nx, x, ld_x, H.x, out);
+  }
+  residual();
+  FVX_LAUNCH_CHECK();
+  return kOk;
+}
+
+// Synchronous mixed solve: predicted schedule, then one more sweep at a time.
+template <typename TB>
+int mixed_solve(Plan& P, const TB* rhs, int64_t ld_rhs, double* x, int64_t ld_x, double rtol,
+                double atol, int max_outer, int max_inner, double inner_rtol, CgInfoC* info,
+                int slot, hipStream_t s) {
+  int inner = std::max(1, std::min(P.last_inner, max_inner));
+  int outer_pred = std::max(1, std::min(P.last_outer, max_outer));
+  int outer_total = 0;
+  bool converged = false;
+  int rc = 0;
+  bool first = true;
+  while (true) {
+    const int no = first ? outer_pred : 1;
+    if (first) {
+      rc = mixed_enqueue<TB>(P, rhs, ld_rhs, x, ld_x, rtol, atol, no, inner, inner_rtol, slot, s);
+    } else {
+      // Continue: do not reset the outer scalars (keeps the outer count).
+      Hier<float>& H = P.h32;
+      const auto& L32 = H.lv[0];
+      PcgScalars* in = &P.sc[2 * slot];
+      PcgScalars* out = &P.sc[2 * slot + 1];
+      hipLaunchKernelGGL(inner_reset_kernel, dim3(1), dim3(1), 0, s, in, out, inner_rtol);
+      rc = (P.kind == kDirichlet) ? pcg_iters<float, kDirichlet>(P, H, H.x, L32.nx, inner, in, s)
+                                  : pcg_iters<float, kNoflux>(P, H, H.x, L32.nx, inner, in, s);
+      if (rc) return rc;
+      hipLaunchKernelGGL(refine_update_kernel, grid2(L32.ny, L32.nx), dim3(64, 4), 0, s, L32.ny,
+                         L32.nx, x, ld_x, H.x, out);
+      const auto& L64 = P.h64.lv[0];
+      if (P.kind == kDirichlet)
+        hipLaunchKernelGGL((refine_residual_kernel<kDirichlet, TB>), grid2(L64.ny, L64.nx),
+                           dim3(64, 4), 0, s, L64, rhs, ld_rhs, x, ld_x, H.r, H.x, rtol * rtol,
+                           atol * atol, P.partials, out);
+      else
+        hipLaunchKernelGGL((refine_residual_kernel<kNoflux, TB>), grid2(L64.ny, L64.nx),
+                           dim3(64, 4), 0, s, L64, rhs, ld_rhs, x, ld_x, H.r, H.x, rtol * rtol,
+                           atol * atol, P.partials, out);
+      FVX_LAUNCH_CHECK();
+    }
+    if (rc) return rc;
+    first = false;
+    FVX_HIP(hipMemcpyAsync(&P.hsc[2 * slot], &P.sc[2 * slot], 2 * sizeof(PcgScalars),
+                           hipMemcpyDeviceToHost, s));
+    FVX_HIP(hipStreamSynchronize(s));
+    const PcgScalars& ho = P.hsc[2 * slot + 1];
+    const PcgScalars& hi = P.hsc[2 * slot];
+    outer_total = ho.outer;
+    if (ho.out_done) {
+      converged = true;
+      break;
+    }
+    if (outer_total >= max_outer) break;
+    if (!hi.done) inner = std::min(max_inner, inner + 2);
+  }
+  const PcgScalars& ho = P.hsc[2 * slot + 1];
+  P.last_outer = std::max(1, ho.outer);
+  P.last_inner = inner;
+  if (info) {
+    info->iterations = P.hsc[2 * slot].iter;
+    info->outer = ho.outer;
+    info->converged = converged ? 1 : 0;
+    info->residual_norm = std::sqrt(ho.out_rr);
+    info->rhs_norm = std::sqrt(ho.bb);
+  }
+  return kOk;
+}
+
+const PcgScalars* fetch_slots(Plan& P, int first_slot, int n, hipStream_t s) {
+  if (hipMemcpyAsync(&P.hsc[2 * first_slot], &P.sc[2 * first_slot], 2 * n * sizeof(PcgScalars),
+                     hipMemcpyDeviceToHost, s) != hipSuccess)
+    return nullptr;
+  if (hipStreamSynchronize(s) != hipSuccess) return nullptr;
+  return &P.hsc[2 * first_slot];
+}
+int plan_levels(const Plan& P) { return P.nl; }
+int plan_kind(const Plan& P) { return P.kind; }
+
+template int pcg_enqueue<float>(Plan&, const float*, int64_t, float*, int64_t, double, double,
+                                bool, int, int, hipStream_t);
+template int pcg_enqueue<double>(Plan&, const double*, int64_t, double*, int64_t, double, double,
+                                 bool, int, int, hipStream_t);
+template int mixed_enqueue<float>(Plan&, const float*, int64_t, double*, int64_t, double, double,
+                                  int, int, double, int, hipStream_t);
+template int mixed_enqueue<double>(Plan&, const double*, int64_t, double*, int64_t, double,
+                                   double, int, int, double, int, hipStream_t);
+template int pcg_solve<double>(Plan&, const double*, int64_t, double*, int64_t, double, double,
+                               int, bool, CgInfoC*, int, hipStream_t);
+template int pcg_solve<float>(Plan&, const float*, int64_t, float*, int64_t, double, double, int,
+                              bool, CgInfoC*, int, hipStream_t);
+template int mixed_solve<float>(Plan&, const float*, int64_t, double*, int64_t, double, double,
+                                int, int, double, CgInfoC*, int, hipStream_t);
+template int mixed_solve<double>(Plan&, const double*, int64_t, double*, int64_t, double, double,
+                                 int, int, double, CgInfoC*, int, hipStream_t);
 
 }  // namespace fvx
 
@@ -1162,9 +1377,10 @@ int fvx_mg_plan_create(const double* mask, const double* coeff, int ny, int nx, 
   if (!rc) {
     const size_t maxblk = (size_t)cdiv(nx, 64) * cdiv(ny, 4) + 64;
     hipError_t e = hipMalloc(&P->partials, maxblk * 4 * sizeof(double));
-    if (e == hipSuccess) e = hipMalloc(&P->sc, 2 * sizeof(PcgScalars));
-    if (e == hipSuccess) e = hipMemset(P->sc, 0, 2 * sizeof(PcgScalars));
-    if (e == hipSuccess) e = hipHostMalloc(&P->hsc, 2 * sizeof(PcgScalars), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc(&P->sc, 2 * kSlots * sizeof(PcgScalars));
+    if (e == hipSuccess) e = hipMemset(P->sc, 0, 2 * kSlots * sizeof(PcgScalars));
+    if (e == hipSuccess)
+      e = hipHostMalloc(&P->hsc, 2 * kSlots * sizeof(PcgScalars), hipHostMallocDefault);
     if (e != hipSuccess) {
       set_error("HIP allocation failed: %s", hipGetErrorString(e));
       rc = kHipError;
@@ -1306,52 +1522,9 @@ int fvx_mg_plan_info(void* plan, int* n_levels, int* shapes, int* coarse_level) 
     Plan* P = (Plan*)plan;                                                                       \
     FVX_CHECK_ARG(P->has##HN, "plan was not built for this precision");                          \
     FVX_CHECK_ARG(max_iters >= 0, "negative max_iters");                                         \
-    hipStream_t s = (hipStream_t)stream;                                                         \
-    Hier<T>& H = hier<T>(*P);                                                                    \
-    const auto& L = H.lv[0];                                                                     \
-    PcgScalars* sc = &P->sc[0];                                                                  \
-    /* x lives in H.x (packed); copy in the warm start */                                        \
-    if (use_x0)                                                                                  \
-      FVX_HIP(hipMemcpy2DAsync(H.x, L.nx * sizeof(T), x, ld_x * sizeof(T), L.nx * sizeof(T),     \
-                               L.ny, hipMemcpyDeviceToDevice, s));                               \
-    else                                                                                         \
-      FVX_HIP(hipMemsetAsync(H.x, 0, sizeof(T) * (size_t)L.ny * L.nx, s));                       \
-    if (P->kind == kDirichlet)                                                                   \
-      hipLaunchKernelGGL((pcg_init_kernel<T, kDirichlet>), grid2(L.ny, L.nx), dim3(64, 4), 0, s, \
-                         L, rhs, ld_rhs, use_x0 ? H.x : nullptr, H.r, rtol * rtol, atol * atol,  \
-                         P->partials, sc);                                                       \
-    else                                                                                         \
-      hipLaunchKernelGGL((pcg_init_kernel<T, kNoflux>), grid2(L.ny, L.nx), dim3(64, 4), 0, s,    \
-                         L, rhs, ld_rhs, use_x0 ? H.x : nullptr, H.r, rtol * rtol, atol * atol,  \
-                         P->partials, sc);                                                       \
-    FVX_LAUNCH_CHECK();                                                                          \
-    int done_iters = 0;                                                                          \
-    int chunk = std::max(1, std::min(P->last_iters, max_iters));                                \
-    while (true) {                                                                               \
-      const int n = std::min(chunk, max_iters - done_iters);                                     \
-      if (n > 0) {                                                                               \
-        int rc = (P->kind == kDirichlet) ? pcg_iters<T, kDirichlet>(*P, H, H.x, n, sc, s)        \
-                                         : pcg_iters<T, kNoflux>(*P, H, H.x, n, sc, s);          \
-        if (rc) return rc;                                                                       \
-        done_iters += n;                                                                         \
-      }                                                                                          \
-      int rc = read_scalars(*P, s);                                                              \
-      if (rc) return rc;                                                                         \
-      if (P->hsc[0].done || done_iters >= max_iters) break;                                      \
-      chunk = 1;                                                                                 \
-    }                                                                                            \
-    const PcgScalars& h = P->hsc[0];                                                             \
-    P->last_iters = std::max(1, h.iter);                                                         \
-    FVX_HIP(hipMemcpy2DAsync(x, ld_x * sizeof(T), H.x, L.nx * sizeof(T), L.nx * sizeof(T), L.ny, \
-                             hipMemcpyDeviceToDevice, s));                                       \
-    if (info) {                                                                                  \
-      info->iterations = h.iter;                                                                 \
-      info->outer = 0;                                                                           \
-      info->converged = h.done;                                                                  \
-      info->residual_norm = std::sqrt(h.rr);                                                     \
-      info->rhs_norm = std::sqrt(h.bb);                                                          \
-    }                                                                                            \
-    return kOk;                                                                                  \
+    FVX_CHECK_ARG(ld_rhs >= P->nx && ld_x >= P->nx, "row pitch smaller than nx");               \
+    return pcg_solve<T>(*P, rhs, ld_rhs, x, ld_x, rtol, atol, max_iters, use_x0 != 0,            \
+                        (CgInfoC*)info, 0, (hipStream_t)stream);                                 \
   }
 
 FVX_MG_T(f64, double, 64)
@@ -1370,72 +1543,9 @@ int fvx_mg_mixed_solve(void* plan, const double* rhs, int64_t ld_rhs, double* x,
   Plan* P = (Plan*)plan;
   FVX_CHECK_ARG(P->has64 && P->has32, "mixed solve needs a plan built with precision 3264");
   FVX_CHECK_ARG(max_outer >= 1 && max_inner >= 1, "iteration caps must be >= 1");
-  hipStream_t s = (hipStream_t)stream;
-  Hier<float>& H = P->h32;
-  const auto& L64 = P->h64.lv[0];
-  const auto& L32 = H.lv[0];
-  PcgScalars* in = &P->sc[0];
-  PcgScalars* out = &P->sc[1];
-  FVX_HIP(hipMemsetAsync(out, 0, offsetof(PcgScalars, ticket), s));
-  int total_inner = 0, outer = 0;
-  int inner_pred = std::max(1, std::min(P->last_inner, max_inner));
-  int outer_pred = std::max(1, std::min(P->last_outer, max_outer));
-  bool converged = false;
-  while (outer < max_outer) {
-    // Enqueue outer_pred refinement sweeps, each with inner_pred PCG iterations.
-    const int no = std::min(outer_pred, max_outer - outer);
-    for (int o = 0; o < no; ++o) {
-      if (P->kind == kDirichlet)
-        hipLaunchKernelGGL((refine_residual_kernel<kDirichlet>), grid2(L64.ny, L64.nx),
-                           dim3(64, 4), 0, s, L64, rhs, ld_rhs, x, ld_x, H.r, H.x,
-                           rtol * rtol, atol * atol, P->partials, out);
-      else
-        hipLaunchKernelGGL((refine_residual_kernel<kNoflux>), grid2(L64.ny, L64.nx), dim3(64, 4),
-                           0, s, L64, rhs, ld_rhs, x, ld_x, H.r, H.x, rtol * rtol, atol * atol,
-                           P->partials, out);
-      hipLaunchKernelGGL(inner_reset_kernel, dim3(1), dim3(1), 0, s, in, out, inner_rtol);
-      int rc = (P->kind == kDirichlet) ? pcg_iters<float, kDirichlet>(*P, H, H.x, inner_pred, in, s)
-                                       : pcg_iters<float, kNoflux>(*P, H, H.x, inner_pred, in, s);
-      if (rc) return rc;
-      hipLaunchKernelGGL(refine_update_kernel, grid2(L32.ny, L32.nx), dim3(64, 4), 0, s, L32.ny,
-                         L32.nx, x, ld_x, H.x, out);
-      FVX_LAUNCH_CHECK();
-    }
-    // Final residual check of this batch.
-    if (P->kind == kDirichlet)
-      hipLaunchKernelGGL((refine_residual_kernel<kDirichlet>), grid2(L64.ny, L64.nx), dim3(64, 4),
-                         0, s, L64, rhs, ld_rhs, x, ld_x, H.r, H.x, rtol * rtol, atol * atol,
-                         P->partials, out);
-    else
-      hipLaunchKernelGGL((refine_residual_kernel<kNoflux>), grid2(L64.ny, L64.nx), dim3(64, 4), 0,
-                         s, L64, rhs, ld_rhs, x, ld_x, H.r, H.x, rtol * rtol, atol * atol,
-                         P->partials, out);
-    FVX_LAUNCH_CHECK();
-    int rc = read_scalars(*P, s);
-    if (rc) return rc;
-    const PcgScalars& ho = P->hsc[1];
-    const PcgScalars& hi = P->hsc[0];
-    outer = ho.outer;
-    total_inner += 0;
-    if (ho.out_done) {
-      converged = true;
-      break;
-    }
-    // Not converged: if the inner solve hit its cap, allow more inner steps.
-    if (!hi.done) inner_pred = std::min(max_inner, inner_pred + 2);
-    outer_pred = 1;
-  }
-  const PcgScalars& ho = P->hsc[1];
-  P->last_outer = std::max(1, ho.outer);
-  P->last_inner = inner_pred;
-  if (info) {
-    info->iterations = P->hsc[0].iter;  // inner iterations of the last sweep
-    info->outer = ho.outer;
-    info->converged = converged ? 1 : 0;
-    info->residual_norm = std::sqrt(ho.out_rr);
-    info->rhs_norm = std::sqrt(ho.bb);
-  }
-  return kOk;
+  FVX_CHECK_ARG(ld_rhs >= P->nx && ld_x >= P->nx, "row pitch smaller than nx");
+  return mixed_solve<double>(*P, rhs, ld_rhs, x, ld_x, rtol, atol, max_outer, max_inner,
+                             inner_rtol, (CgInfoC*)info, 0, (hipStream_t)stream);
 }
 
 }  // extern "C"

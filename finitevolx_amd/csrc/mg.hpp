@@ -1,0 +1,58 @@
+// Internal interface between the multigrid/PCG module and its in-library
+// callers (the QG step driver).  Not part of the C ABI.
+#pragma once
+
+#include "fvx_common.hpp"
+
+namespace fvx {
+
+// Device-side scalars of one PCG solve (one "slot" holds an inner and an
+// outer block).
+struct PcgScalars {
+  double rz;      // rho_k = r . z
+  double pq;      // p . A p
+  double rr;      // ||r||^2 (recursive residual)
+  double bb;      // ||b||^2
+  double tol2;    // stop when rr <= tol2
+  double alpha, beta;
+  double zq;      // z_{k+1} . q_k  (flexible beta)
+  double out_rr;  // outer (FP64) residual ||b - A x||^2 for refinement
+  double out_tol2;
+  int iter;       // inner iterations done
+  int done;       // 1 once rr <= tol2 (all kernels become no-ops)
+  int first;      // 1 before the first direction is formed
+  int out_done;   // outer refinement converged
+  int outer;      // outer iterations done
+  int pad[3];
+  unsigned int ticket[8];  // last-block counters, one per reduction site
+};
+
+struct CgInfoC {
+  int iterations, outer, converged, pad;
+  double residual_norm, rhs_norm;
+};
+
+struct Plan;
+
+template <typename T>
+int pcg_enqueue(Plan& P, const T* rhs, int64_t ld_rhs, T* x, int64_t ld_x, double rtol,
+                double atol, bool use_x0, int iters, int slot, hipStream_t s);
+template <typename T>
+int pcg_solve(Plan& P, const T* rhs, int64_t ld_rhs, T* x, int64_t ld_x, double rtol,
+              double atol, int max_iters, bool use_x0, CgInfoC* info, int slot, hipStream_t s);
+template <typename TB>
+int mixed_enqueue(Plan& P, const TB* rhs, int64_t ld_rhs, double* x, int64_t ld_x, double rtol,
+                  double atol, int n_outer, int n_inner, double inner_rtol, int slot,
+                  hipStream_t s);
+template <typename TB>
+int mixed_solve(Plan& P, const TB* rhs, int64_t ld_rhs, double* x, int64_t ld_x, double rtol,
+                double atol, int max_outer, int max_inner, double inner_rtol, CgInfoC* info,
+                int slot, hipStream_t s);
+
+// Copy ``n`` slots of device scalars to the plan's pinned mirror and wait;
+// returns the host pointer (2 blocks per slot).
+const PcgScalars* fetch_slots(Plan& P, int first_slot, int n, hipStream_t s);
+int plan_levels(const Plan& P);
+int plan_kind(const Plan& P);
+
+}  // namespace fvx

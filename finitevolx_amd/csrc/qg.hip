@@ -1,0 +1,378 @@
+// 1.5-layer quasi-geostrophic step (build-defined composition of reference
+// operators; nearest reference: scripts/qg_1p5_layer.py:420-497):
+//
+//   dq/dt = -J(psi, q + beta (y - y0)) + F(y) - nu4 D(D(q))
+//   (lap - 1/Ld^2) psi = q,  psi = 0 on land and beyond the basin walls
+//
+// with J = arakawa_jacobian (operators/jacobian.py:22-131), D the no-flux
+// Diffusion2D Laplacian (BiharmonicDiffusion2D, diffusion/diffusion.py:543-574),
+// F = -F0 sin(2 pi y / Ly) (qg_1p5_layer.py:363) and rk3_ssp_step stepping
+// (timestepping/explicit_rk.py:104-121).
+//
+// Per RK stage: one multigrid-PCG Helmholtz solve (FP64, or FP32 with FP64
+// refinement) warm-started from the previous psi, then ONE fused kernel that
+// evaluates the whole tendency and the Shu-Osher stage combination
+//   q_out = a q_n + b q_s + c dt rhs(q_s, psi)
+// from an LDS tile (q with a 2-cell halo, psi with 1): HBM traffic per cell is
+// read q_s, psi, q_n + write q_out.
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "fvx_common.hpp"
+#include "mg.hpp"
+
+extern "C" int fvx_mg_plan_create(const double*, const double*, int, int, double, double, double,
+                                  int, int, int, int, double, int, int, void**);
+extern "C" int fvx_mg_plan_destroy(void*);
+
+namespace fvx {
+
+constexpr int kQTx = 64, kQTy = 16, kQThreads = 256;
+
+struct QGStageParams {
+  double idx, idy, inv12, beta, dy, y0, F0, kF, nu4, a, b, cdt;
+};
+
+template <typename TS, typename TP, bool MASKED>
+__global__ __launch_bounds__(kQThreads) void qg_stage_kernel(
+    const TS* __restrict__ qs, const TP* __restrict__ psi, const TS* __restrict__ qn,
+    TS* __restrict__ out, const uint8_t* __restrict__ mask, int Ny, int Nx, QGStageParams p,
+    int tiles_x, int ntiles) {
+  using T = TS;
+  __shared__ T sq[kQTy + 4][kQTx + 4];
+  __shared__ T sp[kQTy + 2][kQTx + 2];
+  __shared__ T sl[kQTy + 2][kQTx + 2];
+  __shared__ T sby[kQTy + 2];
+  __shared__ T sF[kQTy];
+  const int t = xcd_swizzle(blockIdx.x, ntiles);
+  const int i0 = (t % tiles_x) * kQTx, j0 = (t / tiles_x) * kQTy;
+  const int tid = threadIdx.x;
+  const T idx = (T)p.idx, idy = (T)p.idy;
+
+  for (int k = tid; k < (kQTy + 4) * (kQTx + 4); k += kQThreads) {
+    const int yy = k / (kQTx + 4), xx = k % (kQTx + 4);
+    const int gj = j0 - 2 + yy, gi = i0 - 2 + xx;
+    sq[yy][xx] = (gj >= 0 && gj < Ny && gi >= 0 && gi < Nx) ? qs[(int64_t)gj * Nx + gi] : T(0);
+  }
+  for (int k = tid; k < (kQTy + 2) * (kQTx + 2); k += kQThreads) {
+    const int yy = k / (kQTx + 2), xx = k % (kQTx + 2);
+    const int gj = j0 - 1 + yy, gi = i0 - 1 + xx;
+    sp[yy][xx] =
+        (gj >= 0 && gj < Ny && gi >= 0 && gi < Nx) ? (T)psi[(int64_t)gj * Nx + gi] : T(0);
+  }
+  if (tid < kQTy + 2) {
+    // beta (y - y0) at cell centres y_j = (j - 1/2) dy, rows j0-1 .. j0+TY.
+    const int gj = j0 - 1 + tid;
+    sby[tid] = (T)(p.beta * ((gj - 0.5) * p.dy - p.y0));
+  } else if (tid >= 64 && tid < 64 + kQTy) {
+    const int gj = j0 + tid - 64;
+    sF[tid - 64] = (T)(-p.F0 * sin(p.kF * ((gj - 0.5) * p.dy)));
+  }
+  __syncthreads();
+
+  // First no-flux Laplacian on the tile + 1 halo (zero outside the interior).
+  for (int k = tid; k < (kQTy + 2) * (kQTx + 2); k += kQThreads) {
+    const int yy = k / (kQTx + 2), xx = k % (kQTx + 2);
+    const int gj = j0 - 1 + yy, gi = i0 - 1 + xx;
+    T v = T(0);
+    if (gj >= 1 && gj <= Ny - 2 && gi >= 1 && gi <= Nx - 2) {
+      const int y = yy + 1, x = xx + 1;  // sq coordinates
+      const T c = sq[y][x];
+      const T fe = (gi <= Nx - 3) ? (sq[y][x + 1] - c) * idx : T(0);
+      const T fw = (gi >= 2) ? (c - sq[y][x - 1]) * idx : T(0);
+      const T fn = (gj <= Ny - 3) ? (sq[y + 1][x] - c) * idy : T(0);
+      const T fs = (gj >= 2) ? (c - sq[y - 1][x]) * idy : T(0);
+      v = (fe - fw) * idx + (fn - fs) * idy;
+    }
+    sl[yy][xx] = v;
+  }
+  __syncthreads();
+
+  const T nu4 = (T)p.nu4, inv12 = (T)p.inv12;
+  const T a = (T)p.a, b = (T)p.b, cdt = (T)p.cdt;
+  for (int k = tid; k < kQTx * kQTy; k += kQThreads) {
+    const int ty = k / kQTx, tx = k % kQTx;
+    const int gj = j0 + ty, gi = i0 + tx;
+    if (gj >= Ny || gi >= Nx) continue;
+    const int64_t g = (int64_t)gj * Nx + gi;
+    T res = T(0);
+    if (gj >= 1 && gj <= Ny - 2 && gi >= 1 && gi <= Nx - 2) {
+      // Arakawa J(psi, q + beta (y - y0)).
+      const int y = ty + 1, x = tx + 1;  // sp / sl coordinates
+      const int qy = ty + 2, qx = tx + 2;
+      const T bN = sby[y + 1], bC = sby[y], bS = sby[y - 1];
+      const T fE = sp[y][x + 1], fW = sp[y][x - 1], fN = sp[y + 1][x], fS = sp[y - 1][x];
+      const T fNE = sp[y + 1][x + 1], fNW = sp[y + 1][x - 1], fSE = sp[y - 1][x + 1],
+              fSW = sp[y - 1][x - 1];
+      const T gE = sq[qy][qx + 1] + bC, gW = sq[qy][qx - 1] + bC;
+      const T gN = sq[qy + 1][qx] + bN, gS = sq[qy - 1][qx] + bS;
+      const T gNE = sq[qy + 1][qx + 1] + bN, gNW = sq[qy + 1][qx - 1] + bN;
+      const T gSE = sq[qy - 1][qx + 1] + bS, gSW = sq[qy - 1][qx - 1] + bS;
+      const T jpp = (fE - fW) * (gN - gS) - (fN - fS) * (gE - gW);
+      const T jpx = fE * (gNE - gSE) - fW * (gNW - gSW) - fN * (gNE - gNW) + fS * (gSE - gSW);
+      const T jxp = gN * (fNE - fNW) - gS * (fSE - fSW) - gE * (fNE - fSE) + gW * (fNW - fSW);
+      const T J = ((jpp + jpx) + jxp) * inv12;
+      // Second no-flux Laplacian -> biharmonic tendency -nu4 D(D q).
+      const T c = sl[y][x];
+      const T fe = (gi <= Nx - 3) ? (sl[y][x + 1] - c) * idx : T(0);
+      const T fw = (gi >= 2) ? (c - sl[y][x - 1]) * idx : T(0);
+      const T fn = (gj <= Ny - 3) ? (sl[y + 1][x] - c) * idy : T(0);
+      const T fs = (gj >= 2) ? (c - sl[y - 1][x]) * idy : T(0);
+      const T bih = -nu4 * ((fe - fw) * idx + (fn - fs) * idy);
+      T rhs = (-J + sF[ty]) + bih;
+      if constexpr (MASKED) rhs = rhs * (T)(mask[g] ? 1 : 0);
+      const T qsv = sq[qy][qx];
+      res = (a != T(0) ? a * qn[g] : T(0)) + b * qsv + cdt * rhs;
+    }
+    out[g] = res;
+  }
+}
+
+template <typename TS, typename TP>
+int qg_stage(const TS* qs, const TP* psi, const TS* qn, TS* out, const uint8_t* mask, int Ny,
+             int Nx, const QGStageParams& p, hipStream_t s) {
+  FVX_CHECK_ARG(qs && psi && qn && out, "null field pointer");
+  FVX_CHECK_ARG(Ny >= 3 && Nx >= 3, "array too small");
+  FVX_CHECK_ARG(out != qs && out != (const TS*)psi, "out must not alias q_s or psi");
+  const int tx = cdiv(Nx, kQTx), ty = cdiv(Ny, kQTy);
+  if (mask)
+    hipLaunchKernelGGL((qg_stage_kernel<TS, TP, true>), dim3(tx * ty), dim3(kQThreads), 0, s, qs,
+                       psi, qn, out, mask, Ny, Nx, p, tx, tx * ty);
+  else
+    hipLaunchKernelGGL((qg_stage_kernel<TS, TP, false>), dim3(tx * ty), dim3(kQThreads), 0, s, qs,
+                       psi, qn, out, mask, Ny, Nx, p, tx, tx * ty);
+  FVX_LAUNCH_CHECK();
+  return kOk;
+}
+
+// ---------------------------------------------------------------------------
+// QG plan and step driver
+// ---------------------------------------------------------------------------
+
+enum QGSolve : int { kSolveF64 = 64, kSolveMixed = 3264 };
+
+struct QGPlan {
+  int ny, nx, Ny, Nx;
+  double Lx, Ly, dx, dy, beta, lam, nu4, F0, dt, rtol, inner_rtol;
+  int state;  // 32 or 64
+  int solve;  // kSolveF64 or kSolveMixed
+  int max_iters, max_outer;
+  uint8_t* mask = nullptr;  // full-array wet mask (device), null = basin
+  Plan* mg = nullptr;
+  void* q1 = nullptr;
+  void* q2 = nullptr;
+};
+
+struct QGInfoC {
+  int solves, iterations, max_iterations, outer;
+  int converged, pad;
+  double max_rel_residual;
+};
+
+static QGStageParams stage_params(const QGPlan& Q, double a, double b, double c) {
+  QGStageParams p;
+  p.idx = 1.0 / Q.dx;
+  p.idy = 1.0 / Q.dy;
+  p.inv12 = 1.0 / (12.0 * Q.dx * Q.dy);
+  p.beta = Q.beta;
+  p.dy = Q.dy;
+  p.y0 = 0.5 * Q.Ly;
+  p.F0 = Q.F0;
+  p.kF = 2.0 * M_PI / Q.Ly;
+  p.nu4 = Q.nu4;
+  p.a = a;
+  p.b = b;
+  p.cdt = c * Q.dt;
+  return p;
+}
+
+template <typename TS>
+static int qg_step_impl(QGPlan& Q, TS* q, double* psi, int nsteps, QGInfoC* info,
+                        hipStream_t s) {
+  const int Nx = Q.Nx;
+  const int64_t off = (int64_t)Nx + 1;  // interior origin
+  TS* q1 = (TS*)Q.q1;
+  TS* q2 = (TS*)Q.q2;
+  const double coef[3][3] = {{0.0, 1.0, 1.0}, {0.75, 0.25, 0.25},
+                             {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
+  QGInfoC acc;
+  std::memset(&acc, 0, sizeof(acc));
+  acc.converged = 1;
+  for (int st = 0; st < nsteps; ++st) {
+    for (int k = 0; k < 3; ++k) {
+      const TS* qs = (k == 0) ? q : (k == 1 ? q1 : q2);
+      TS* out = (k == 0) ? q1 : (k == 1 ? q2 : q);
+      CgInfoC ci;
+      std::memset(&ci, 0, sizeof(ci));
+      int rc;
+      if (Q.solve == kSolveMixed) {
+        rc = mixed_solve<TS>(*Q.mg, qs + off, Nx, psi + off, Nx, Q.rtol, 0.0, Q.max_outer,
+                             Q.max_iters, Q.inner_rtol, &ci, 0, s);
+      } else {
+        if constexpr (std::is_same<TS, double>::value) {
+          rc = pcg_solve<double>(*Q.mg, qs + off, Nx, psi + off, Nx, Q.rtol, 0.0, Q.max_iters,
+                                 true, &ci, 0, s);
+        } else {
+          set_error("FP64 MG-PCG needs an FP64 state");
+          return kInvalidArgument;
+        }
+      }
+      if (rc) return rc;
+      acc.solves += 1;
+      acc.iterations += ci.iterations;
+      acc.max_iterations = std::max(acc.max_iterations, ci.iterations);
+      acc.outer += ci.outer;
+      if (!ci.converged) acc.converged = 0;
+      const double rel = ci.rhs_norm > 0 ? ci.residual_norm / ci.rhs_norm : ci.residual_norm;
+      acc.max_rel_residual = std::max(acc.max_rel_residual, rel);
+      rc = qg_stage<TS, double>(qs, psi, q, out, Q.mask, Q.Ny, Q.Nx,
+                                stage_params(Q, coef[k][0], coef[k][1], coef[k][2]), s);
+      if (rc) return rc;
+    }
+  }
+  if (info) *info = acc;
+  return kOk;
+}
+
+}  // namespace fvx
+
+extern "C" {
+
+typedef struct fvx_qg_info {
+  int solves;           // Helmholtz solves performed
+  int iterations;       // total inner PCG iterations
+  int max_iterations;   // largest per-solve inner iteration count
+  int outer;            // total refinement sweeps (mixed)
+  int converged;        // 1 if every solve met the tolerance
+  int pad;
+  double max_rel_residual;  // max over solves of ||b - A psi|| / ||b|| (FP64)
+} fvx_qg_info;
+
+int fvx_qg_plan_create(int ny, int nx, double Lx, double Ly, double beta, double Ld, double nu4,
+                       double F0, double dt, const double* mask_interior, int state_precision,
+                       int solve_mode, int n_pre, int n_post, int n_coarse, double omega,
+                       double rtol, double inner_rtol, int max_iters, int max_outer,
+                       void** plan_out) {
+  using namespace fvx;
+  FVX_CHECK_ARG(plan_out, "null plan_out");
+  FVX_CHECK_ARG(ny >= 8 && nx >= 8, "interior must be at least 8 x 8");
+  FVX_CHECK_ARG(state_precision == 32 || state_precision == 64, "state precision 32 or 64");
+  FVX_CHECK_ARG(solve_mode == kSolveF64 || solve_mode == kSolveMixed,
+                "solve mode must be 64 (FP64 MG-PCG) or 3264 (mixed)");
+  FVX_CHECK_ARG(!(solve_mode == kSolveF64 && state_precision == 32),
+                "the FP64 MG-PCG path needs an FP64 state");
+  FVX_CHECK_ARG(Lx > 0 && Ly > 0 && Ld > 0 && dt > 0, "physical parameters must be positive");
+  QGPlan* Q = new QGPlan();
+  Q->ny = ny;
+  Q->nx = nx;
+  Q->Ny = ny + 2;
+  Q->Nx = nx + 2;
+  Q->Lx = Lx;
+  Q->Ly = Ly;
+  Q->dx = Lx / nx;
+  Q->dy = Ly / ny;
+  Q->beta = beta;
+  Q->lam = 1.0 / (Ld * Ld);
+  Q->nu4 = nu4;
+  Q->F0 = F0;
+  Q->dt = dt;
+  Q->rtol = rtol;
+  Q->inner_rtol = inner_rtol;
+  Q->state = state_precision;
+  Q->solve = solve_mode;
+  Q->max_iters = max_iters;
+  Q->max_outer = max_outer;
+  std::vector<double> m((size_t)ny * nx, 1.0);
+  bool masked = false;
+  if (mask_interior) {
+    for (size_t k = 0; k < m.size(); ++k) {
+      m[k] = mask_interior[k] != 0.0 ? 1.0 : 0.0;
+      if (m[k] == 0.0) masked = true;
+    }
+  }
+  void* mg = nullptr;
+  int rc = fvx_mg_plan_create(m.data(), nullptr, ny, nx, Q->dx, Q->dy, Q->lam, 0, n_pre, n_post,
+                              n_coarse, omega, 1 /*dirichlet*/,
+                              solve_mode == kSolveMixed ? 3264 : 64, &mg);
+  if (rc) {
+    delete Q;
+    return rc;
+  }
+  Q->mg = (Plan*)mg;
+  const size_t N = (size_t)Q->Ny * Q->Nx;
+  const size_t es = state_precision == 64 ? 8 : 4;
+  hipError_t e = hipMalloc(&Q->q1, N * es);
+  if (e == hipSuccess) e = hipMalloc(&Q->q2, N * es);
+  if (e == hipSuccess) e = hipMemset(Q->q1, 0, N * es);
+  if (e == hipSuccess) e = hipMemset(Q->q2, 0, N * es);
+  if (e == hipSuccess && masked) {
+    std::vector<uint8_t> mf(N, 0);
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) mf[(size_t)(j + 1) * Q->Nx + (i + 1)] = m[(size_t)j * nx + i] != 0.0;
+    e = hipMalloc(&Q->mask, N);
+    if (e == hipSuccess) e = hipMemcpy(Q->mask, mf.data(), N, hipMemcpyHostToDevice);
+  }
+  if (e != hipSuccess) {
+    set_error("HIP allocation failed: %s", hipGetErrorString(e));
+    fvx_mg_plan_destroy(mg);
+    delete Q;
+    return kHipError;
+  }
+  *plan_out = Q;
+  return kOk;
+}
+
+int fvx_qg_plan_destroy(void* plan) {
+  using namespace fvx;
+  if (!plan) return kOk;
+  QGPlan* Q = (QGPlan*)plan;
+  (void)hipDeviceSynchronize();
+  fvx_mg_plan_destroy(Q->mg);
+  (void)hipFree(Q->q1);
+  (void)hipFree(Q->q2);
+  (void)hipFree(Q->mask);
+  delete Q;
+  return kOk;
+}
+
+// Advance q (full (ny+2) x (nx+2) array of the plan's state precision, ghost
+// ring zero) by nsteps rk3_ssp_step steps; psi (FP64, same shape) is the warm
+// start on entry and the streamfunction of the last stage on exit.
+int fvx_qg_step(void* plan, void* q, double* psi, int nsteps, fvx_qg_info* info, void* stream) {
+  using namespace fvx;
+  FVX_CHECK_ARG(plan && q && psi, "null argument");
+  FVX_CHECK_ARG(nsteps >= 0, "negative nsteps");
+  QGPlan* Q = (QGPlan*)plan;
+  hipStream_t s = (hipStream_t)stream;
+  if (Q->state == 64)
+    return qg_step_impl<double>(*Q, (double*)q, psi, nsteps, (QGInfoC*)info, s);
+  return qg_step_impl<float>(*Q, (float*)q, psi, nsteps, (QGInfoC*)info, s);
+}
+
+#define FVX_QG_STAGE(SUF, TS)                                                                     \
+  int fvx_qg_stage_##SUF(const TS* qs, const double* psi, const TS* qn, TS* out,                  \
+                         const uint8_t* mask, int Ny, int Nx, double dx, double dy, double beta,  \
+                         double Ly, double F0, double nu4, double a, double b, double cdt,        \
+                         void* stream) {                                                          \
+    using namespace fvx;                                                                          \
+    QGStageParams p;                                                                              \
+    p.idx = 1.0 / dx;                                                                             \
+    p.idy = 1.0 / dy;                                                                             \
+    p.inv12 = 1.0 / (12.0 * dx * dy);                                                             \
+    p.beta = beta;                                                                                \
+    p.dy = dy;                                                                                    \
+    p.y0 = 0.5 * Ly;                                                                              \
+    p.F0 = F0;                                                                                    \
+    p.kF = 2.0 * M_PI / Ly;                                                                       \
+    p.nu4 = nu4;                                                                                  \
+    p.a = a;                                                                                      \
+    p.b = b;                                                                                      \
+    p.cdt = cdt;                                                                                  \
+    return qg_stage<TS, double>(qs, psi, qn, out, mask, Ny, Nx, p, (hipStream_t)stream);          \
+  }
+FVX_QG_STAGE(f64, double)
+FVX_QG_STAGE(f32, float)
+#undef FVX_QG_STAGE
+}
