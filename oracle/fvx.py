@@ -822,18 +822,19 @@ def pcg(matvec, b, x0=None, precond=None, rtol=1e-6, atol=1e-6, max_steps=500,
     b = np.asarray(b, np.float64)
     x = np.zeros_like(b) if x0 is None else np.array(x0, np.float64)
     M = (lambda r: r) if precond is None else precond
-    # Work with -A (positive definite) and -M.
+    # A and M are both negative definite; CG on (A, M) gives the same
+    # iterates as CG on the positive pair (-A, -M).
     r = b - matvec(x)
     bn = np.linalg.norm(b)
     tol = max(rtol * bn, atol)
     rn = np.linalg.norm(r)
     if rn <= tol:
         return x, 0, rn, True
-    z = -M(r)
+    z = M(r)
     p = z.copy()
     rz = np.vdot(r, z)
     for k in range(1, max_steps + 1):
-        q = -matvec(p)
+        q = matvec(p)
         alpha = rz / np.vdot(p, q)
         x = x + alpha * p
         r_old = r
@@ -841,7 +842,7 @@ def pcg(matvec, b, x0=None, precond=None, rtol=1e-6, atol=1e-6, max_steps=500,
         rn = np.linalg.norm(r)
         if rn <= tol:
             return x, k, np.linalg.norm(b - matvec(x)), True
-        z = -M(r)
+        z = M(r)
         rz_new = np.vdot(r, z)
         beta = (np.vdot(z, r - r_old) if flexible else rz_new) / rz
         rz = rz_new
