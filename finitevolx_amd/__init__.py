@@ -2,8 +2,9 @@
 
 Same public names as the reference's ``finitevolx`` package for the
 quasi-geostrophic step: advection, Arakawa Jacobian, C-grid stencils,
-diffusion, multigrid-preconditioned Helmholtz solves and SSP-RK3 stepping.
-All compute runs in ``libfvx_hip.so``; there is no CPU fallback.
+diffusion, multigrid-preconditioned Helmholtz solves, vertical modes and
+SSP-RK3 stepping.  All compute runs in ``libfvx_hip.so``; there is no CPU
+fallback.
 """
 
 from finitevolx_amd.advection import Advection2D
@@ -19,11 +20,32 @@ from finitevolx_amd.operators import (
     divergence_2d,
     interior,
 )
+from finitevolx_amd.qg import QGModel
+from finitevolx_amd.solvers import (
+    CGInfo,
+    MultigridLevel,
+    MultigridSolver,
+    build_multigrid_solver,
+    make_multigrid_preconditioner,
+    masked_laplacian,
+    pv_inversion,
+    solve_cg,
+    streamfunction_from_vorticity,
+)
 from finitevolx_amd.timestepping import rk3_ssp_step
+from finitevolx_amd.vertical import (
+    build_coupling_matrix,
+    decompose_vertical_modes,
+    layer_to_mode,
+    mode_to_layer,
+)
 
 __all__ = [
-    "Advection2D", "ArakawaCGrid2D", "BiharmonicDiffusion2D", "CartesianGrid2D",
+    "Advection2D", "ArakawaCGrid2D", "BiharmonicDiffusion2D", "CGInfo", "CartesianGrid2D",
     "Difference2D", "Diffusion2D", "Divergence2D", "Interpolation2D", "Mask2D",
-    "StencilCapability2D", "arakawa_jacobian", "diffusion_2d", "divergence_2d",
-    "enforce_periodic", "interior", "pad_interior", "rk3_ssp_step",
+    "MultigridLevel", "MultigridSolver", "QGModel", "StencilCapability2D", "arakawa_jacobian",
+    "build_coupling_matrix", "build_multigrid_solver", "decompose_vertical_modes",
+    "diffusion_2d", "divergence_2d", "enforce_periodic", "interior", "layer_to_mode",
+    "make_multigrid_preconditioner", "masked_laplacian", "mode_to_layer", "pad_interior",
+    "pv_inversion", "rk3_ssp_step", "solve_cg", "streamfunction_from_vorticity",
 ]

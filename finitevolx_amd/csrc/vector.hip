@@ -33,9 +33,62 @@ int lincomb3(int64_t n, double a, const T* x, double b, const T* y, double c, co
   return kOk;
 }
 
+// out[k, p] = sum_l C[k, l] in[l, p] for nl <= 8 layers (layer_to_mode /
+// mode_to_layer, vertical/vertical_modes.py:193-263): one pass, each
+// thread owns one horizontal point and all its layers.
+struct Mix8 {
+  double c[64];
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void layer_mix_kernel(const T* __restrict__ in,
+                                                        T* __restrict__ out, Mix8 C, int nl,
+                                                        int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += stride) {
+    T v[8];
+#pragma unroll
+    for (int l = 0; l < 8; ++l) v[l] = (l < nl) ? in[(int64_t)l * n + p] : T(0);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      if (k >= nl) break;
+      double acc = 0.0;
+#pragma unroll
+      for (int l = 0; l < 8; ++l)
+        if (l < nl) acc += C.c[k * 8 + l] * (double)v[l];
+      out[(int64_t)k * n + p] = (T)acc;
+    }
+  }
+}
+
+template <typename T>
+int layer_mix(const T* in, T* out, const double* C, int nl, int64_t n, hipStream_t s) {
+  FVX_CHECK_ARG(in && out && C, "null pointer");
+  FVX_CHECK_ARG(nl >= 1 && nl <= 8, "1 <= nl <= 8 layers supported, got %d", nl);
+  FVX_CHECK_ARG(in != out, "in-place layer transform is not supported");
+  Mix8 m;
+  for (int k = 0; k < 64; ++k) m.c[k] = 0.0;
+  for (int k = 0; k < nl; ++k)
+    for (int l = 0; l < nl; ++l) m.c[k * 8 + l] = C[k * nl + l];
+  if (n == 0) return kOk;
+  const int blocks = (int)std::min<int64_t>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(layer_mix_kernel<T>, dim3(blocks), dim3(256), 0, s, in, out, m, nl, n);
+  FVX_LAUNCH_CHECK();
+  return kOk;
+}
+
 }  // namespace fvx
 
 extern "C" {
+// C: host row-major nl x nl matrix; in / out: device (nl, n).
+int fvx_layer_mix_f64(const double* in, double* out, const double* C, int nl, int64_t n,
+                      void* stream) {
+  return fvx::layer_mix<double>(in, out, C, nl, n, (hipStream_t)stream);
+}
+int fvx_layer_mix_f32(const float* in, float* out, const double* C, int nl, int64_t n,
+                      void* stream) {
+  return fvx::layer_mix<float>(in, out, C, nl, n, (hipStream_t)stream);
+}
 int fvx_lincomb3_f64(int64_t n, double a, const double* x, double b, const double* y, double c,
                      const double* z, double* out, void* stream) {
   return fvx::lincomb3<double>(n, a, x, b, y, c, z, out, (hipStream_t)stream);

@@ -1,0 +1,219 @@
+/* finitevolx MI355X (gfx950) hot-path library: the C ABI of libfvx_hip.so.
+ *
+ * Conventions
+ *  - Fields are row-major [ny][nx] device arrays (float or double; suffix
+ *    _f32 / _f64).  Operators that mirror the reference's ghost-ring contract
+ *    take the FULL array (one ghost ring included) and write a zero ring.
+ *  - Masks are device uint8 arrays of the same shape (0 = dry); NULL = all wet.
+ *  - Every function takes a hipStream_t (as void*) and only enqueues work,
+ *    except where noted (plan creation, and solves that report convergence,
+ *    which synchronise the stream once per solve).
+ *  - Return value: 0 = success, -1 = invalid argument, -2 = HIP error;
+ *    fvx_last_error() returns the thread's last message.  Non-convergence of
+ *    a solver is not an error (reported in the info structs).
+ *  - Reference citations are into finitevolX 0.0.40 (paths under finitevolx/_src/).
+ */
+#ifndef FVX_H
+#define FVX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+const char* fvx_last_error(void);
+int fvx_abi_version(void);
+int fvx_device_synchronize(void);
+
+/* ---- C-grid point stencils ------------------------------------------------
+ * op codes (FVX_OP_*) and the reference method each one replaces:            */
+enum {
+  FVX_OP_DIFF_X_T_TO_U = 0,  /* Difference2D.diff_x_T_to_U  operators/difference.py:194 */
+  FVX_OP_DIFF_Y_T_TO_V = 1,  /* Difference2D.diff_y_T_to_V  operators/difference.py:215 */
+  FVX_OP_DIFF_Y_U_TO_X = 2,  /* Difference2D.diff_y_U_to_X  operators/difference.py:236 */
+  FVX_OP_DIFF_X_V_TO_X = 3,  /* Difference2D.diff_x_V_to_X  operators/difference.py:258 */
+  FVX_OP_DIFF_Y_X_TO_U = 4,  /* Difference2D.diff_y_X_to_U  operators/difference.py:280 */
+  FVX_OP_DIFF_X_X_TO_V = 5,  /* Difference2D.diff_x_X_to_V  operators/difference.py:301 */
+  FVX_OP_DIFF_X_U_TO_T = 6,  /* Difference2D.diff_x_U_to_T  operators/difference.py:326 */
+  FVX_OP_DIFF_Y_V_TO_T = 7,  /* Difference2D.diff_y_V_to_T  operators/difference.py:347 */
+  FVX_OP_DIVERGENCE = 8,     /* Difference2D.divergence / Divergence2D.__call__ :372, divergence.py:113 */
+  FVX_OP_DIV_NOFLUX = 9,     /* Divergence2D.noflux         operators/divergence.py:138 */
+  FVX_OP_CURL = 10,          /* Difference2D.curl           operators/difference.py:401 */
+  FVX_OP_LAPLACIAN = 11,     /* Difference2D.laplacian      operators/difference.py:431 */
+  FVX_OP_GRAD_PERP_U = 12,   /* Difference2D.grad_perp (u)  operators/difference.py:456 */
+  FVX_OP_GRAD_PERP_V = 13,   /* Difference2D.grad_perp (v)  operators/difference.py:456 */
+  FVX_OP_T_TO_X = 14,        /* Interpolation2D.T_to_X      operators/interpolation.py:158 */
+  FVX_OP_V_TO_T = 15         /* Interpolation2D.V_to_T      operators/interpolation.py:206 */
+};
+/* out = op(a[, b]) * mask, zero ghost ring.  mask: the output stagger's mask. */
+int fvx_stencil2d_f32(int op, const float* a, const float* b, float* out, int ny, int nx,
+                      double dx, double dy, const uint8_t* mask, void* stream);
+int fvx_stencil2d_f64(int op, const double* a, const double* b, double* out, int ny, int nx,
+                      double dx, double dy, const uint8_t* mask, void* stream);
+
+/* ---- Advection2D.__call__ (advection/advection.py:329-420) -----------------
+ * method: FVX_ADV_*; limiter used when method == FVX_ADV_TVD.
+ * tier_x / tier_y: NULL for the unmasked path, else per-cell largest usable
+ * stencil in {0,2,4,6} (Mask2D.get_adaptive_masks, mask/cartesian.py:495-553)
+ * for the mask-dispatchable methods (weno3, weno5, wenoz5, TVD;
+ * advection/flux.py:100-253).  mask_h (nullable) multiplies the tendency.   */
+enum {
+  FVX_ADV_NAIVE = 0, FVX_ADV_UPWIND1 = 1, FVX_ADV_UPWIND2 = 2, FVX_ADV_UPWIND3 = 3,
+  FVX_ADV_WENO3 = 4, FVX_ADV_WENOZ3 = 5, FVX_ADV_WENO5 = 6, FVX_ADV_WENOZ5 = 7, FVX_ADV_TVD = 8
+};
+enum { FVX_LIM_MINMOD = 0, FVX_LIM_VAN_LEER = 1, FVX_LIM_SUPERBEE = 2, FVX_LIM_MC = 3 };
+int fvx_advection2d_f32(int method, int limiter, const float* h, const float* u, const float* v,
+                        float* out, const uint8_t* tier_x, const uint8_t* tier_y,
+                        const uint8_t* mask_h, int ny, int nx, double dx, double dy,
+                        void* stream);
+int fvx_advection2d_f64(int method, int limiter, const double* h, const double* u,
+                        const double* v, double* out, const uint8_t* tier_x,
+                        const uint8_t* tier_y, const uint8_t* mask_h, int ny, int nx, double dx,
+                        double dy, void* stream);
+
+/* ---- Diffusion2D (diffusion/diffusion.py:120-329) --------------------------
+ * kappa: device field or NULL (then kappa0 is used everywhere).             */
+int fvx_diffusion2d_f32(const float* h, const float* kappa, double kappa0, float* out, int ny,
+                        int nx, double dx, double dy, const uint8_t* mask_h,
+                        const uint8_t* mask_u, const uint8_t* mask_v, void* stream);
+int fvx_diffusion2d_f64(const double* h, const double* kappa, double kappa0, double* out,
+                        int ny, int nx, double dx, double dy, const uint8_t* mask_h,
+                        const uint8_t* mask_u, const uint8_t* mask_v, void* stream);
+/* Diffusion2D.fluxes (diffusion/diffusion.py:290-329) */
+int fvx_diffusion2d_fluxes_f32(const float* h, const float* kappa, double kappa0, float* fx,
+                               float* fy, int ny, int nx, double dx, double dy,
+                               const uint8_t* mask_u, const uint8_t* mask_v, void* stream);
+int fvx_diffusion2d_fluxes_f64(const double* h, const double* kappa, double kappa0, double* fx,
+                               double* fy, int ny, int nx, double dx, double dy,
+                               const uint8_t* mask_u, const uint8_t* mask_v, void* stream);
+/* BiharmonicDiffusion2D.__call__ (diffusion/diffusion.py:543-574), fused. */
+int fvx_biharmonic2d_f32(const float* h, float* out, int ny, int nx, double dx, double dy,
+                         double kappa, const uint8_t* mask_h, void* stream);
+int fvx_biharmonic2d_f64(const double* h, double* out, int ny, int nx, double dx, double dy,
+                         double kappa, const uint8_t* mask_h, void* stream);
+
+/* ---- arakawa_jacobian (operators/jacobian.py:22-131) -----------------------
+ * f, g: (batch, ny, nx); out: (batch, ny-2, nx-2).                          */
+int fvx_arakawa_jacobian_f32(const float* f, const float* g, float* out, int batch, int ny,
+                             int nx, double dx, double dy, void* stream);
+int fvx_arakawa_jacobian_f64(const double* f, const double* g, double* out, int batch, int ny,
+                             int nx, double dx, double dy, void* stream);
+
+/* ---- masked_laplacian (solvers/iterative.py:144-191); mask: same dtype ---- */
+int fvx_masked_laplacian_f32(const float* psi, const float* mask, float* out, int ny, int nx,
+                             double dx, double dy, double lambda_, void* stream);
+int fvx_masked_laplacian_f64(const double* psi, const double* mask, double* out, int ny,
+                             int nx, double dx, double dy, double lambda_, void* stream);
+
+/* ---- enforce_periodic (boundary/boundary.py:52-78), ghost width ng, in place */
+int fvx_enforce_periodic_f32(float* field, int ny, int nx, int ng, void* stream);
+int fvx_enforce_periodic_f64(double* field, int ny, int nx, int ng, void* stream);
+
+/* ---- rk3_ssp_step stage combination (timestepping/explicit_rk.py:104-121) --
+ * out = a*x + b*y + c*z over n elements (y, z nullable).                    */
+int fvx_lincomb3_f32(int64_t n, double a, const float* x, double b, const float* y, double c,
+                     const float* z, float* out, void* stream);
+int fvx_lincomb3_f64(int64_t n, double a, const double* x, double b, const double* y, double c,
+                     const double* z, double* out, void* stream);
+
+/* ---- layer_to_mode / mode_to_layer (vertical/vertical_modes.py:193-263) ---
+ * out[k, :] = sum_l C[k, l] in[l, :]; C is a HOST row-major nl x nl matrix
+ * (Cl2m or Cm2l from decompose_vertical_modes), nl <= 8, n points per layer. */
+int fvx_layer_mix_f32(const float* in, float* out, const double* C, int nl, int64_t n,
+                      void* stream);
+int fvx_layer_mix_f64(const double* in, double* out, const double* C, int nl, int64_t n,
+                      void* stream);
+
+/* ---- Multigrid (solvers/multigrid.py) --------------------------------------
+ * fvx_mg_plan_create replaces build_multigrid_solver (multigrid.py:905-1062):
+ * mask / coeff are HOST float64 arrays (ny, nx) (coeff nullable = 1);
+ * n_levels <= 0 = automatic (multigrid.py:50-76).  kind: 0 = the reference's
+ * no-flux variable-coefficient operator (_apply_operator, :327-405);
+ * 1 = masked_laplacian's Dirichlet operator with zero padding beyond the
+ * array.  precision: 32, 64, or 3264 (FP32 hierarchy + FP64 fine operator for
+ * mixed-precision solves).  Non-divisible grids fail with -1.             */
+int fvx_mg_plan_create(const double* mask, const double* coeff, int ny, int nx, double dx,
+                       double dy, double lambda_, int n_levels, int n_pre, int n_post,
+                       int n_coarse, double omega, int kind, int precision, void** plan_out);
+int fvx_mg_plan_destroy(void* plan);
+/* Level count, level shapes (2 ints per level) and the first level solved by
+ * the single-workgroup coarse kernel. */
+int fvx_mg_plan_info(void* plan, int* n_levels, int* shapes, int* coarse_level);
+/* _apply_operator at one level (multigrid.py:327-405) */
+int fvx_mg_apply_operator_f32(void* plan, int level, const float* u, float* out, void* stream);
+int fvx_mg_apply_operator_f64(void* plan, int level, const double* u, double* out, void* stream);
+/* _weighted_jacobi (multigrid.py:408-461) with the plan's omega; u NULL = 0 */
+int fvx_mg_jacobi_f32(void* plan, int level, const float* u, const float* rhs, float* out,
+                      int n_iters, void* stream);
+int fvx_mg_jacobi_f64(void* plan, int level, const double* u, const double* rhs, double* out,
+                      int n_iters, void* stream);
+/* _restrict (multigrid.py:464-507) from `level` to level+1 */
+int fvx_mg_restrict_f32(void* plan, int level, const float* v, float* out, void* stream);
+int fvx_mg_restrict_f64(void* plan, int level, const double* v, double* out, void* stream);
+/* _prolongate (multigrid.py:510-603) from level+1 onto `level` */
+int fvx_mg_prolongate_f32(void* plan, int level, const float* vc, float* out, void* stream);
+int fvx_mg_prolongate_f64(void* plan, int level, const double* vc, double* out, void* stream);
+/* MultigridSolver.v_cycle at level 0 (multigrid.py:808-883); u NULL = 0.
+ * make_multigrid_preconditioner (preconditioners.py:207-244) is V(0, r). */
+int fvx_mg_v_cycle_f32(void* plan, const float* u, const float* rhs, float* out, void* stream);
+int fvx_mg_v_cycle_f64(void* plan, const double* u, const double* rhs, double* out,
+                       void* stream);
+
+typedef struct fvx_cg_info {
+  int iterations;       /* inner PCG iterations (last refinement sweep for mixed) */
+  int outer;            /* refinement sweeps (mixed precision), else 0 */
+  int converged;        /* 1 if ||b - A x|| <= max(rtol ||b||, atol) */
+  int pad;
+  double residual_norm; /* ||b - A x||_2 (FP64 true residual for mixed solves) */
+  double rhs_norm;      /* ||b||_2 */
+} fvx_cg_info;
+
+/* solve_cg(A = level-0 operator, preconditioner = V-cycle)
+ * (solvers/iterative.py:48-136): flexible (Polak-Ribiere) PCG with device-side
+ * reductions and convergence flags; rhs / x are pitched (row strides ld_*),
+ * x is the warm start when use_x0 != 0.  Synchronises once per solve.      */
+int fvx_mg_pcg_solve_f32(void* plan, const float* rhs, int64_t ld_rhs, float* x, int64_t ld_x,
+                         double rtol, double atol, int max_iters, int use_x0,
+                         fvx_cg_info* info, void* stream);
+int fvx_mg_pcg_solve_f64(void* plan, const double* rhs, int64_t ld_rhs, double* x,
+                         int64_t ld_x, double rtol, double atol, int max_iters, int use_x0,
+                         fvx_cg_info* info, void* stream);
+/* Same solve with FP32 multigrid-PCG inner solves and FP64 iterative
+ * refinement (no reference counterpart: the precision scheme of this build).
+ * Needs a plan created with precision 3264.                                */
+int fvx_mg_mixed_solve(void* plan, const double* rhs, int64_t ld_rhs, double* x, int64_t ld_x,
+                       double rtol, double atol, int max_outer, int max_inner, double inner_rtol,
+                       fvx_cg_info* info, void* stream);
+
+/* ---- QG 1.5-layer step (composition; nearest reference
+ * scripts/qg_1p5_layer.py:420-497; operators as cited above) -------------- */
+typedef struct fvx_qg_info {
+  int solves, iterations, max_iterations, outer, converged, pad;
+  double max_rel_residual;
+} fvx_qg_info;
+/* state_precision 32|64; solve_mode 64 (FP64 MG-PCG) | 3264 (mixed);
+ * mask_interior: host float64 (ny, nx) or NULL (rectangular basin).       */
+int fvx_qg_plan_create(int ny, int nx, double Lx, double Ly, double beta, double Ld, double nu4,
+                       double F0, double dt, const double* mask_interior, int state_precision,
+                       int solve_mode, int n_pre, int n_post, int n_coarse, double omega,
+                       double rtol, double inner_rtol, int max_iters, int max_outer,
+                       void** plan_out);
+int fvx_qg_plan_destroy(void* plan);
+/* nsteps rk3_ssp_step steps of q (full (ny+2, nx+2), state precision) in
+ * place; psi (FP64 full array) is the warm start and the last solution.    */
+int fvx_qg_step(void* plan, void* q, double* psi, int nsteps, fvx_qg_info* info, void* stream);
+/* One fused tendency + stage combination out = a q_n + b q_s + cdt rhs(q_s, psi). */
+int fvx_qg_stage_f32(const float* qs, const double* psi, const float* qn, float* out,
+                     const uint8_t* mask, int Ny, int Nx, double dx, double dy, double beta,
+                     double Ly, double F0, double nu4, double a, double b, double cdt,
+                     void* stream);
+int fvx_qg_stage_f64(const double* qs, const double* psi, const double* qn, double* out,
+                     const uint8_t* mask, int Ny, int Nx, double dx, double dy, double beta,
+                     double Ly, double F0, double nu4, double a, double b, double cdt,
+                     void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FVX_H */
