@@ -29,7 +29,14 @@ from finitevolx_amd.solvers import (
     make_multigrid_preconditioner,
     masked_laplacian,
     pv_inversion,
+    dst1_eigenvalues,
     solve_cg,
+    solve_helmholtz_dct,
+    solve_helmholtz_dst,
+    solve_helmholtz_fft,
+    solve_poisson_dct,
+    solve_poisson_dst,
+    solve_poisson_fft,
     streamfunction_from_vorticity,
 )
 from finitevolx_amd.timestepping import rk3_ssp_step
@@ -48,4 +55,6 @@ __all__ = [
     "diffusion_2d", "divergence_2d", "enforce_periodic", "interior", "layer_to_mode",
     "make_multigrid_preconditioner", "masked_laplacian", "mode_to_layer", "pad_interior",
     "pv_inversion", "rk3_ssp_step", "solve_cg", "streamfunction_from_vorticity",
+    "dst1_eigenvalues", "solve_helmholtz_dct", "solve_helmholtz_dst", "solve_helmholtz_fft",
+    "solve_poisson_dct", "solve_poisson_dst", "solve_poisson_fft",
 ]

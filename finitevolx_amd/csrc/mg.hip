@@ -57,12 +57,18 @@ __device__ __forceinline__ T mval(const uint8_t* m, int64_t k) {
 // Reductions finished by the last workgroup (fixed summation order ->
 // deterministic).  Partials: nblk x NV doubles.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ int lin_tid() {
+  return threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+}
+
 template <int NV, int kThreads>
 __device__ bool block_reduce_last(const double (&v)[NV], double* partials,
                                   unsigned int* ticket, double (&out)[NV]) {
   __shared__ double red[kThreads / 64][NV];
   __shared__ bool is_last;
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  // Linear thread id: the callers use 1-D (256) and 2-D (64 x 4) blocks.
+  const int tid = lin_tid();
+  const int lane = tid & 63, wid = tid >> 6;
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
     const double s = wave_sum(v[k]);
@@ -71,7 +77,7 @@ __device__ bool block_reduce_last(const double (&v)[NV], double* partials,
   __syncthreads();
   const int nblk = gridDim.x * gridDim.y;
   const int bid = blockIdx.y * gridDim.x + blockIdx.x;
-  if (threadIdx.x == 0) {
+  if (tid == 0) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
       double s = 0.0;
@@ -90,14 +96,14 @@ __device__ bool block_reduce_last(const double (&v)[NV], double* partials,
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
     double s = 0.0;
-    for (int b = threadIdx.x; b < nblk; b += kThreads) {
+    for (int b = tid; b < nblk; b += kThreads) {
       s += __builtin_nontemporal_load(&partials[(int64_t)b * NV + k]);
     }
     s = wave_sum(s);
     if (lane == 0) red[wid][k] = s;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (tid == 0) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
       double s = 0.0;
@@ -345,7 +351,7 @@ __global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
   if constexpr (MODE == kUpDots) {
     double v[2] = {d0, d1}, out[2];
     if (block_reduce_last<2, kTileThreads>(v, a.partials, &a.sc->ticket[0], out)) {
-      if (threadIdx.x == 0) {
+      if (lin_tid() == 0) {
         PcgScalars* sc = a.sc;
         // Flexible (Polak-Ribiere) beta: z_{k+1}.(r_{k+1} - r_k) / rho_k with
         // r_{k+1} - r_k = -alpha q_k.
@@ -525,7 +531,7 @@ __global__ __launch_bounds__(256) void pcg_init_kernel(LevelDev<T> L, const T* _
   }
   double v[2] = {bb, rr}, out[2];
   if (block_reduce_last<2, 256>(v, partials, &sc->ticket[1], out)) {
-    if (threadIdx.x == 0) {
+    if (lin_tid() == 0) {
       sc->bb = out[0];
       sc->rr = out[1];
       sc->tol2 = fmax(rtol2 * out[0], atol2);
@@ -573,7 +579,7 @@ __global__ __launch_bounds__(256) void pcg_dir_kernel(LevelDev<T> L, const T* __
   }
   double v[1] = {pq}, out[1];
   if (block_reduce_last<1, 256>(v, partials, &sc->ticket[2], out)) {
-    if (threadIdx.x == 0) {
+    if (lin_tid() == 0) {
       sc->pq = out[0];
       sc->alpha = sc->rz / out[0];
       sc->first = 0;
@@ -601,7 +607,7 @@ __global__ __launch_bounds__(256) void pcg_update_kernel(int ny, int nx, T* __re
   }
   double v[1] = {rr}, out[1];
   if (block_reduce_last<1, 256>(v, partials, &sc->ticket[3], out)) {
-    if (threadIdx.x == 0) {
+    if (lin_tid() == 0) {
       sc->rr = out[0];
       sc->iter += 1;
       if (out[0] <= sc->tol2) sc->done = 1;
@@ -648,7 +654,7 @@ __global__ __launch_bounds__(256) void refine_residual_kernel(
   }
   double v[2] = {bb, rr}, out[2];
   if (block_reduce_last<2, 256>(v, partials, &sc->ticket[4], out)) {
-    if (threadIdx.x == 0) {
+    if (lin_tid() == 0) {
       sc->bb = out[0];
       sc->out_rr = out[1];
       sc->out_tol2 = fmax(rtol2 * out[0], atol2);
@@ -957,7 +963,7 @@ __global__ __launch_bounds__(256) void pcg_dots_kernel(int ny, int nx, const T* 
   }
   double v[2] = {d0, d1}, out[2];
   if (block_reduce_last<2, 256>(v, partials, &sc->ticket[5], out)) {
-    if (threadIdx.x == 0) {
+    if (lin_tid() == 0) {
       sc->beta = sc->first ? 0.0 : (-sc->alpha * out[1]) / sc->rz;
       sc->rz = out[0];
     }

@@ -408,6 +408,97 @@ def _generic_pcg(matvec, rhs, x0, precond, rtol, atol, max_steps):
 
 
 # ---------------------------------------------------------------------------
+# Spectral rectangular solvers (solvers/spectral.py:14-128 re-exports of
+# spectraldiffx v0.0.10): each inverts one 5-point operator exactly.
+# ---------------------------------------------------------------------------
+
+_lib.extra_signatures({
+    "fvx_spectral_plan_create": [_i, _i, _i, _i, C.POINTER(_vp)],
+    "fvx_spectral_plan_destroy": [_vp],
+    "fvx_spectral_helmholtz_f64": [_vp, _vp, _i64, _vp, _i64, _d, _d, _d, _vp],
+    "fvx_spectral_helmholtz_f32": [_vp, _vp, _i64, _vp, _i64, _d, _d, _d, _vp],
+})
+
+_BC = {"dst": 0, "dct": 1, "fft": 2}
+
+
+class _SpecPlan:
+    def __init__(self, ny, nx, bc, bits):
+        h = _vp()
+        _lib.call("fvx_spectral_plan_create", ny, nx, _BC[bc], bits, C.byref(h))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            try:
+                _lib.call("fvx_spectral_plan_destroy", self.h)
+            except Exception:
+                pass
+            self.h = None
+
+
+_SPEC_PLANS: dict = {}
+
+
+def _spectral(rhs, dx, dy, lambda_, bc):
+    b = _lib.as_field(rhs)
+    if b.dim() < 2:
+        raise ValueError("spectral solves need at least 2-D input")
+    ny, nx = b.shape[-2:]
+    bits = 64 if b.dtype == torch.float64 else 32
+    key = (ny, nx, bc, bits)
+    plan = _SPEC_PLANS.get(key)
+    if plan is None:
+        if len(_SPEC_PLANS) > 8:
+            _SPEC_PLANS.clear()
+        plan = _SPEC_PLANS[key] = _SpecPlan(ny, nx, bc, bits)
+    flat = b.reshape(-1, ny, nx)
+    out = torch.empty_like(flat)
+    for k in range(flat.shape[0]):
+        _lib.call(f"fvx_spectral_helmholtz_{_lib.suffix(b)}", plan.h, flat[k].data_ptr(), nx,
+                  out[k].data_ptr(), nx, float(dx), float(dy), float(lambda_), _lib.stream())
+    return out.reshape(b.shape)
+
+
+def solve_helmholtz_dst(rhs, dx, dy, lambda_=0.0):
+    """Dirichlet ``(lap - lambda) psi = rhs`` on an interior-only array (DST-I)."""
+    return _spectral(rhs, dx, dy, lambda_, "dst")
+
+
+def solve_helmholtz_dct(rhs, dx, dy, lambda_=0.0):
+    """Neumann (no-flux, cell-centred) Helmholtz solve (DCT-II); the mean
+    mode is set to zero when ``lambda_ == 0``."""
+    return _spectral(rhs, dx, dy, lambda_, "dct")
+
+
+def solve_helmholtz_fft(rhs, dx, dy, lambda_=0.0):
+    """Periodic Helmholtz solve (2-D real FFT); zero mean mode when singular."""
+    return _spectral(rhs, dx, dy, lambda_, "fft")
+
+
+def solve_poisson_dst(rhs, dx, dy):
+    return solve_helmholtz_dst(rhs, dx, dy, 0.0)
+
+
+def solve_poisson_dct(rhs, dx, dy):
+    return solve_helmholtz_dct(rhs, dx, dy, 0.0)
+
+
+def solve_poisson_fft(rhs, dx, dy):
+    return solve_helmholtz_fft(rhs, dx, dy, 0.0)
+
+
+def dst1_eigenvalues(n: int, d: float) -> np.ndarray:
+    """Eigenvalues of the 1-D Dirichlet second difference (DST-I basis)."""
+    k = np.arange(1, n + 1)
+    return -4.0 / d**2 * np.sin(np.pi * k / (2.0 * (n + 1))) ** 2
+
+
+_HELMHOLTZ_DISPATCH = {"dst": solve_helmholtz_dst, "dct": solve_helmholtz_dct,
+                       "fft": solve_helmholtz_fft}
+
+
+# ---------------------------------------------------------------------------
 # Wrappers (elliptic.py) for method="cg"
 # ---------------------------------------------------------------------------
 
@@ -467,13 +558,22 @@ def _solve_cg_method(rhs, dx, dy, lam, mask, preconditioner, rtol=1e-10, atol=0.
     return out * _lib.as_field(m, b.dtype)
 
 
+def _spectral_dispatch(rhs, dx, dy, lam, bc):
+    fn = _HELMHOLTZ_DISPATCH.get(bc)
+    if fn is None:
+        raise ValueError(f"bc must be 'fft', 'dst', or 'dct'; got {bc!r}")
+    return fn(rhs, dx, dy, lam)
+
+
 def streamfunction_from_vorticity(zeta, dx, dy, bc="dst", lambda_=0.0, method="spectral",
                                   mask=None, capacitance_solver=None, preconditioner=None):
-    """``(lap - lambda) psi = zeta`` (``elliptic.py:276-339``); ``method='cg'`` only."""
+    """``(lap - lambda) psi = zeta`` (``elliptic.py:276-339``)."""
     if method == "cg":
         return _solve_cg_method(zeta, dx, dy, lambda_, mask, preconditioner)
-    if method in ("spectral", "capacitance"):
-        raise NotImplementedError(f"method={method!r} is outside this build's hot path")
+    if method == "spectral":
+        return _spectral_dispatch(zeta, dx, dy, lambda_, bc)
+    if method == "capacitance":
+        raise NotImplementedError("method='capacitance' is outside this build's hot path")
     raise ValueError(f"method must be 'spectral', 'cg', or 'capacitance'; got {method!r}")
 
 
@@ -488,14 +588,17 @@ def pv_inversion(pv, dx, dy, lambda_, bc="dst", method="spectral", mask=None,
         raise NotImplementedError("method='capacitance' is outside this build's hot path")
     if method not in ("cg", "spectral"):
         raise ValueError(f"method must be 'spectral', 'cg', or 'capacitance'; got {method!r}")
-    if method == "spectral":
-        raise NotImplementedError("method='spectral' is outside this build's hot path")
     pv = _lib.as_field(pv)
     lam = np.asarray(lambda_, dtype=np.float64)
+
+    def one(f, lk):
+        if method == "spectral":
+            return _spectral_dispatch(f, dx, dy, lk, bc)
+        return _solve_cg_method(f, dx, dy, lk, mask, preconditioner)
+
     if lam.ndim == 0:
         flat = pv.reshape(-1, *pv.shape[-2:])
-        out = torch.stack([_solve_cg_method(f, dx, dy, float(lam), mask, preconditioner)
-                           for f in flat])
+        out = torch.stack([one(f, float(lam)) for f in flat])
         return out.reshape(pv.shape)
     if pv.dim() < 3:
         raise ValueError(f"pv must have at least 3 dims when lambda_ is an array, got shape "
@@ -504,7 +607,6 @@ def pv_inversion(pv, dx, dy, lambda_, bc="dst", method="spectral", mask=None,
     if pv.shape[-3] != nl:
         raise ValueError(f"pv.shape[-3]={pv.shape[-3]} does not match lambda_ length {nl}")
     flat = pv.reshape(-1, nl, *pv.shape[-2:])
-    out = torch.stack([torch.stack([_solve_cg_method(f[k], dx, dy, float(lam[k]), mask,
-                                                     preconditioner) for k in range(nl)])
+    out = torch.stack([torch.stack([one(f[k], float(lam[k])) for k in range(nl)])
                        for f in flat])
     return out.reshape(pv.shape)

@@ -186,6 +186,21 @@ int fvx_mg_mixed_solve(void* plan, const double* rhs, int64_t ld_rhs, double* x,
                        double rtol, double atol, int max_outer, int max_inner, double inner_rtol,
                        fvx_cg_info* info, void* stream);
 
+/* ---- Spectral rectangular Helmholtz solvers: solve_helmholtz_{dst,dct,fft}
+ * (solvers/spectral.py:14-128, re-exported from spectraldiffx v0.0.10;
+ * used by scripts/qg_1p5_layer.py:443).  bc 0 = DST-I (Dirichlet just
+ * outside the array), 1 = DCT-II (no-flux cell-centred Neumann), 2 = FFT
+ * (periodic); each is the exact inverse of the matching 5-point operator.
+ * The plan owns hipFFT plans and work buffers for one (ny, nx, bc, dtype). */
+int fvx_spectral_plan_create(int ny, int nx, int bc, int precision, void** plan_out);
+int fvx_spectral_plan_destroy(void* plan);
+int fvx_spectral_helmholtz_f32(void* plan, const float* rhs, int64_t ld_rhs, float* out,
+                               int64_t ld_out, double dx, double dy, double lambda_,
+                               void* stream);
+int fvx_spectral_helmholtz_f64(void* plan, const double* rhs, int64_t ld_rhs, double* out,
+                               int64_t ld_out, double dx, double dy, double lambda_,
+                               void* stream);
+
 /* ---- QG 1.5-layer step (composition; nearest reference
  * scripts/qg_1p5_layer.py:420-497; operators as cited above) -------------- */
 typedef struct fvx_qg_info {

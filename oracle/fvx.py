@@ -863,6 +863,46 @@ def operator_matrix(apply, shape):
     return sp.hstack(cols).tocsr()
 
 
+def level_matrix(lev):
+    """Sparse (ny*nx)^2 matrix of ``apply_operator`` at one level, assembled
+    directly from the stencil coefficients (vectorised)."""
+    import scipy.sparse as sp
+
+    m = lev.mask
+    ny, nx = m.shape
+    n = ny * nx
+    idx = np.arange(n).reshape(ny, nx)
+    ix2, iy2 = 1.0 / lev.dx**2, 1.0 / lev.dy**2
+    rows, cols, vals = [], [], []
+
+    def add(r, c, v):
+        rows.append(r.ravel())
+        cols.append(c.ravel())
+        vals.append(v.ravel())
+
+    if lev.kind == "noflux":
+        cx, cy = lev.cx, lev.cy
+        cxw = np.zeros_like(cx)
+        cxw[:, 1:] = cx[:, :-1]
+        cys = np.zeros_like(cy)
+        cys[1:, :] = cy[:-1, :]
+        diag = (-(cx + cxw) * ix2 - (cy + cys) * iy2 - lev.lambda_) * m * m
+        add(idx, idx, diag)
+        add(idx[:, :-1], idx[:, 1:], (cx[:, :-1] * ix2) * m[:, :-1] * m[:, 1:])
+        add(idx[:, 1:], idx[:, :-1], (cxw[:, 1:] * ix2) * m[:, 1:] * m[:, :-1])
+        add(idx[:-1, :], idx[1:, :], (cy[:-1, :] * iy2) * m[:-1, :] * m[1:, :])
+        add(idx[1:, :], idx[:-1, :], (cys[1:, :] * iy2) * m[1:, :] * m[:-1, :])
+    else:
+        diag = (-2.0 * ix2 - 2.0 * iy2 - lev.lambda_) * m * m
+        add(idx, idx, diag)
+        add(idx[:, :-1], idx[:, 1:], ix2 * m[:, :-1] * m[:, 1:])
+        add(idx[:, 1:], idx[:, :-1], ix2 * m[:, 1:] * m[:, :-1])
+        add(idx[:-1, :], idx[1:, :], iy2 * m[:-1, :] * m[1:, :])
+        add(idx[1:, :], idx[:-1, :], iy2 * m[1:, :] * m[:-1, :])
+    return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                         shape=(n, n))
+
+
 def dirichlet_helmholtz_matrix(mask, dx, dy, lambda_):
     """Sparse (wet x wet) matrix of the zero-padded masked Dirichlet operator."""
     import scipy.sparse as sp

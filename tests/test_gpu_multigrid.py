@@ -119,7 +119,7 @@ def test_pcg_converges_to_direct_solution(ny, nx, kind, masked, varcoef):
     rng = np.random.default_rng(9)
     b = rng.standard_normal((ny, nx)) * m
     lev = fvx.build_multigrid(m, dx, dy, lam, coeff, kind=kind)
-    A = fvx.operator_matrix(lambda u: fvx.apply_operator(u, lev[0]), (ny, nx))
+    A = fvx.level_matrix(lev[0])
     wet = m.ravel() > 0
     import scipy.sparse.linalg as spla
     xd = np.zeros(ny * nx)

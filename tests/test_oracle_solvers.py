@@ -121,3 +121,14 @@ def test_flexible_beta_needed_for_nonsymmetric_vcycle():
     _, it_f, _, ok_f = fvx.pcg(A, b, precond=M, rtol=1e-12, atol=0, max_steps=200, flexible=True)
     _, it_p, _, ok_p = fvx.pcg(A, b, precond=M, rtol=1e-12, atol=0, max_steps=200, flexible=False)
     assert ok_f and (not ok_p or it_p > it_f)
+
+
+@pytest.mark.parametrize("kind", ["noflux", "dirichlet"])
+def test_level_matrix_matches_brute_force(kind):
+    ny, nx = 9, 12
+    m = _mask(ny, nx, 11)
+    coeff = np.random.default_rng(12).uniform(0.5, 2, (ny, nx)) if kind == "noflux" else None
+    lev = fvx.build_multigrid(m, 0.3, 0.7, 1.5, coeff, n_levels=1, kind=kind)[0]
+    A1 = fvx.operator_matrix(lambda u: fvx.apply_operator(u, lev), (ny, nx)).toarray()
+    A2 = fvx.level_matrix(lev).toarray()
+    np.testing.assert_allclose(A2, A1, atol=1e-12 * np.abs(A1).max())
