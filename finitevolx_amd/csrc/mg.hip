@@ -687,6 +687,35 @@ __global__ void inner_reset_kernel(PcgScalars* inner, const PcgScalars* outer, d
   inner->iter = 0;
 }
 
+// Certification: ||b - A x||^2 and ||b||^2 in FP64 into ``res`` (out_rr, bb,
+// out_tol2, out_done).  Reads b and x pitched; x is not modified.
+template <typename T, int KIND>
+__global__ __launch_bounds__(256) void true_residual_kernel(LevelDev<T> L, const T* __restrict__ b,
+                                                            int64_t ld_b, const T* __restrict__ x,
+                                                            int64_t ld_x, double rtol2,
+                                                            double atol2, double* partials,
+                                                            PcgScalars* res) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+  double bb = 0.0, rr = 0.0;
+  if (i < L.nx && j < L.ny) {
+    const int64_t k = (int64_t)j * L.nx + i;
+    const double m = (double)mval<T>(L.mask, k);
+    const double bv = (double)b[(int64_t)j * ld_b + i] * m;
+    const double rv = (bv - (double)glob_Au_pitched<T, KIND>(L, x, ld_x, j, i)) * m;
+    bb = bv * bv;
+    rr = rv * rv;
+  }
+  double v[2] = {bb, rr}, out[2];
+  if (block_reduce_last<2, 256>(v, partials, &res->ticket[6], out)) {
+    if (lin_tid() == 0) {
+      res->bb = out[0];
+      res->out_rr = out[1];
+      res->out_tol2 = fmax(rtol2 * out[0], atol2);
+      res->out_done = out[1] <= res->out_tol2 ? 1 : 0;
+    }
+  }
+}
+
 // Clear the flags and counters of a scalar block (graph-safe replacement for
 // a memset; tickets are left armed).
 __global__ void scalars_reset_kernel(PcgScalars* sc, int n) {
@@ -1146,6 +1175,23 @@ int pcg_enqueue(Plan& P, const T* rhs, int64_t ld_rhs, T* x, int64_t ld_x, doubl
                                 : pcg_iters<T, kNoflux>(P, H, x, ld_x, iters, sc, s);
 }
 
+// FP64-accumulated true residual of x into the slot's outer block.
+template <typename T>
+int certify_enqueue(Plan& P, const T* rhs, int64_t ld_rhs, const T* x, int64_t ld_x, double rtol,
+                    double atol, int slot, hipStream_t s) {
+  Hier<T>& H = hier<T>(P);
+  const auto& L = H.lv[0];
+  PcgScalars* res = &P.sc[2 * slot + 1];
+  if (P.kind == kDirichlet)
+    hipLaunchKernelGGL((true_residual_kernel<T, kDirichlet>), grid2(L.ny, L.nx), dim3(64, 4), 0, s,
+                       L, rhs, ld_rhs, x, ld_x, rtol * rtol, atol * atol, P.partials, res);
+  else
+    hipLaunchKernelGGL((true_residual_kernel<T, kNoflux>), grid2(L.ny, L.nx), dim3(64, 4), 0, s,
+                       L, rhs, ld_rhs, x, ld_x, rtol * rtol, atol * atol, P.partials, res);
+  FVX_LAUNCH_CHECK();
+  return kOk;
+}
+
 // Synchronous PCG: predicted chunk, then single iterations until converged.
 template <typename T>
 int pcg_solve(Plan& P, const T* rhs, int64_t ld_rhs, T* x, int64_t ld_x, double rtol,
@@ -1166,14 +1212,20 @@ int pcg_solve(Plan& P, const T* rhs, int64_t ld_rhs, T* x, int64_t ld_x, double 
     if (rc) return rc;
     done_iters += 1;
   }
+  P.last_iters = std::max(1, P.hsc[2 * slot].iter);
+  rc = certify_enqueue<T>(P, rhs, ld_rhs, x, ld_x, rtol, atol, slot, s);
+  if (rc) return rc;
+  FVX_HIP(hipMemcpyAsync(&P.hsc[2 * slot], &P.sc[2 * slot], 2 * sizeof(PcgScalars),
+                         hipMemcpyDeviceToHost, s));
+  FVX_HIP(hipStreamSynchronize(s));
   const PcgScalars& h = P.hsc[2 * slot];
-  P.last_iters = std::max(1, h.iter);
+  const PcgScalars& c = P.hsc[2 * slot + 1];
   if (info) {
     info->iterations = h.iter;
     info->outer = 0;
-    info->converged = h.done;
-    info->residual_norm = std::sqrt(h.rr);
-    info->rhs_norm = std::sqrt(h.bb);
+    info->converged = h.done && c.out_done;
+    info->residual_norm = std::sqrt(c.out_rr);
+    info->rhs_norm = std::sqrt(c.bb);
   }
   return kOk;
 }
@@ -1299,6 +1351,10 @@ template int mixed_enqueue<float>(Plan&, const float*, int64_t, double*, int64_t
                                   int, int, double, int, hipStream_t);
 template int mixed_enqueue<double>(Plan&, const double*, int64_t, double*, int64_t, double,
                                    double, int, int, double, int, hipStream_t);
+template int certify_enqueue<double>(Plan&, const double*, int64_t, const double*, int64_t,
+                                     double, double, int, hipStream_t);
+template int certify_enqueue<float>(Plan&, const float*, int64_t, const float*, int64_t, double,
+                                    double, int, hipStream_t);
 template int pcg_solve<double>(Plan&, const double*, int64_t, double*, int64_t, double, double,
                                int, bool, CgInfoC*, int, hipStream_t);
 template int pcg_solve<float>(Plan&, const float*, int64_t, float*, int64_t, double, double, int,

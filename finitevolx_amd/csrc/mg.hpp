@@ -38,6 +38,9 @@ template <typename T>
 int pcg_enqueue(Plan& P, const T* rhs, int64_t ld_rhs, T* x, int64_t ld_x, double rtol,
                 double atol, bool use_x0, int iters, int slot, hipStream_t s);
 template <typename T>
+int certify_enqueue(Plan& P, const T* rhs, int64_t ld_rhs, const T* x, int64_t ld_x, double rtol,
+                    double atol, int slot, hipStream_t s);
+template <typename T>
 int pcg_solve(Plan& P, const T* rhs, int64_t ld_rhs, T* x, int64_t ld_x, double rtol,
               double atol, int max_iters, bool use_x0, CgInfoC* info, int slot, hipStream_t s);
 template <typename TB>

@@ -163,6 +163,17 @@ struct QGPlan {
   Plan* mg = nullptr;
   void* q1 = nullptr;
   void* q2 = nullptr;
+  // Graph replay: ping-pong state buffers, each graph = one full step.
+  void* qa = nullptr;
+  void* qb = nullptr;
+  double* psi = nullptr;
+  hipStream_t cap = nullptr;
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  int b_iters = 0, b_outer = 0, b_inner = 0;  // budgets the graphs hold
+  int obs_iters = 0, obs_outer = 0;           // calibration observations
+  bool calibrated = false;
+  int recaptures = 0;
 };
 
 struct QGInfoC {
@@ -212,8 +223,10 @@ static int qg_step_impl(QGPlan& Q, TS* q, double* psi, int nsteps, QGInfoC* info
                              Q.max_iters, Q.inner_rtol, &ci, 0, s);
       } else {
         if constexpr (std::is_same<TS, double>::value) {
-          rc = pcg_solve<double>(*Q.mg, qs + off, Nx, psi + off, Nx, Q.rtol, 0.0, Q.max_iters,
-                                 true, &ci, 0, s);
+          // Recursive residual to rtol/2 so the certified FP64 true residual
+          // meets rtol despite recurrence drift.
+          rc = pcg_solve<double>(*Q.mg, qs + off, Nx, psi + off, Nx, 0.5 * Q.rtol, 0.0,
+                                 Q.max_iters, true, &ci, 0, s);
         } else {
           set_error("FP64 MG-PCG needs an FP64 state");
           return kInvalidArgument;
@@ -232,6 +245,179 @@ static int qg_step_impl(QGPlan& Q, TS* q, double* psi, int nsteps, QGInfoC* info
       if (rc) return rc;
     }
   }
+  if (info) *info = acc;
+  return kOk;
+}
+
+// Enqueue one whole step q_in -> q_out (all kernels, no host work): the
+// sequence a graph captures.  Solves use the plan's budgets and slots 0..2.
+template <typename TS>
+static int qg_step_enqueue(QGPlan& Q, const TS* q_in, TS* q_out, hipStream_t s) {
+  const int Nx = Q.Nx;
+  const int64_t off = (int64_t)Nx + 1;
+  TS* q1 = (TS*)Q.q1;
+  TS* q2 = (TS*)Q.q2;
+  double* psi = Q.psi;
+  const double coef[3][3] = {{0.0, 1.0, 1.0}, {0.75, 0.25, 0.25},
+                             {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
+  for (int k = 0; k < 3; ++k) {
+    const TS* qs = (k == 0) ? q_in : (k == 1 ? q1 : q2);
+    TS* out = (k == 0) ? q1 : (k == 1 ? q2 : q_out);
+    int rc;
+    if (Q.solve == kSolveMixed) {
+      rc = mixed_enqueue<TS>(*Q.mg, qs + off, Nx, psi + off, Nx, Q.rtol, 0.0, Q.b_outer,
+                             Q.b_inner, Q.inner_rtol, k, s);
+    } else {
+      if constexpr (std::is_same<TS, double>::value) {
+        rc = pcg_enqueue<double>(*Q.mg, qs + off, Nx, psi + off, Nx, 0.5 * Q.rtol, 0.0, true,
+                                 Q.b_iters, k, s);
+        if (!rc) rc = certify_enqueue<double>(*Q.mg, qs + off, Nx, psi + off, Nx, Q.rtol, 0.0, k, s);
+      } else {
+        set_error("FP64 MG-PCG needs an FP64 state");
+        return kInvalidArgument;
+      }
+    }
+    if (rc) return rc;
+    rc = qg_stage<TS, double>(qs, psi, q_in, out, Q.mask, Q.Ny, Q.Nx,
+                              stage_params(Q, coef[k][0], coef[k][1], coef[k][2]), s);
+    if (rc) return rc;
+  }
+  return kOk;
+}
+
+static void drop_graphs(QGPlan& Q) {
+  for (int g = 0; g < 2; ++g) {
+    if (Q.exec[g]) (void)hipGraphExecDestroy(Q.exec[g]);
+    if (Q.graph[g]) (void)hipGraphDestroy(Q.graph[g]);
+    Q.exec[g] = nullptr;
+    Q.graph[g] = nullptr;
+  }
+}
+
+template <typename TS>
+static int capture_graphs(QGPlan& Q) {
+  drop_graphs(Q);
+  for (int g = 0; g < 2; ++g) {
+    const TS* in = (const TS*)(g == 0 ? Q.qa : Q.qb);
+    TS* out = (TS*)(g == 0 ? Q.qb : Q.qa);
+    FVX_HIP(hipStreamBeginCapture(Q.cap, hipStreamCaptureModeThreadLocal));
+    int rc = qg_step_enqueue<TS>(Q, in, out, Q.cap);
+    hipGraph_t gr = nullptr;
+    hipError_t e = hipStreamEndCapture(Q.cap, &gr);
+    if (rc) {
+      if (gr) (void)hipGraphDestroy(gr);
+      return rc;
+    }
+    if (e != hipSuccess) {
+      set_error("graph capture failed: %s", hipGetErrorString(e));
+      return kHipError;
+    }
+    Q.graph[g] = gr;
+    FVX_HIP(hipGraphInstantiate(&Q.exec[g], gr, nullptr, nullptr, 0));
+  }
+  return kOk;
+}
+
+// Convergence of the three solves of the last replayed step.
+static bool step_converged(QGPlan& Q, QGInfoC& acc, hipStream_t s) {
+  const PcgScalars* h = fetch_slots(*Q.mg, 0, 3, s);
+  if (!h) return false;
+  bool ok = true;
+  for (int k = 0; k < 3; ++k) {
+    const PcgScalars& in = h[2 * k];
+    const PcgScalars& out = h[2 * k + 1];
+    const bool conv = out.out_done != 0;  // FP64 true residual (both modes)
+    ok = ok && conv;
+    acc.solves += 1;
+    acc.iterations += in.iter;
+    acc.max_iterations = std::max(acc.max_iterations, in.iter);
+    acc.outer += out.outer;
+    const double rel = out.bb > 0 ? std::sqrt(out.out_rr / out.bb) : std::sqrt(out.out_rr);
+    acc.max_rel_residual = std::max(acc.max_rel_residual, rel);
+  }
+  return ok;
+}
+
+template <typename TS>
+static int qg_run_graph(QGPlan& Q, TS* q, double* psi, int nsteps, QGInfoC* info,
+                        hipStream_t s) {
+  const size_t N = (size_t)Q.Ny * Q.Nx;
+  QGInfoC acc;
+  std::memset(&acc, 0, sizeof(acc));
+  acc.converged = 1;
+  if (nsteps == 0) {
+    if (info) *info = acc;
+    return kOk;
+  }
+  FVX_HIP(hipMemcpyAsync(Q.psi, psi, N * sizeof(double), hipMemcpyDeviceToDevice, s));
+  int done_steps = 0;
+  TS* cur = q;
+  if (!Q.calibrated) {
+    // One synchronous step to observe iteration counts (advances q in place).
+    QGInfoC ci;
+    int rc = qg_step_impl<TS>(Q, q, Q.psi, 1, &ci, s);
+    if (rc) return rc;
+    acc.solves += ci.solves;
+    acc.iterations += ci.iterations;
+    acc.max_iterations = std::max(acc.max_iterations, ci.max_iterations);
+    acc.outer += ci.outer;
+    acc.max_rel_residual = std::max(acc.max_rel_residual, ci.max_rel_residual);
+    if (!ci.converged) acc.converged = 0;
+    Q.obs_iters = ci.max_iterations;
+    Q.obs_outer = std::max(1, (ci.outer + ci.solves - 1) / std::max(1, ci.solves));
+    done_steps = 1;
+  }
+  if (done_steps < nsteps) {
+    // Budgets: observed + 1 inner iteration of margin.
+    int want_iters = std::max(2, Q.obs_iters + 1);
+    int want_outer = std::max(1, Q.obs_outer);
+    int want_inner = std::max(2, Q.obs_iters + 1);
+    if (!Q.calibrated || Q.exec[0] == nullptr) {
+      Q.b_iters = want_iters;
+      Q.b_outer = want_outer;
+      Q.b_inner = want_inner;
+      int rc = capture_graphs<TS>(Q);
+      if (rc) return rc;
+      Q.calibrated = true;
+    }
+    FVX_HIP(hipMemcpyAsync(Q.qa, q, N * sizeof(TS), hipMemcpyDeviceToDevice, s));
+    int g = 0;  // exec[0]: qa -> qb
+    while (done_steps < nsteps) {
+      FVX_HIP(hipGraphLaunch(Q.exec[g], s));
+      QGInfoC st;
+      std::memset(&st, 0, sizeof(st));
+      if (step_converged(Q, st, s)) {
+        acc.solves += st.solves;
+        acc.iterations += st.iterations;
+        acc.max_iterations = std::max(acc.max_iterations, st.max_iterations);
+        acc.outer += st.outer;
+        acc.max_rel_residual = std::max(acc.max_rel_residual, st.max_rel_residual);
+        ++done_steps;
+        g ^= 1;
+        continue;
+      }
+      // A solve missed its tolerance within the budget: widen and redo the
+      // step from its (untouched) input buffer.
+      if (Q.recaptures > 50 || Q.b_iters >= Q.max_iters || Q.b_outer >= Q.max_outer) {
+        acc.converged = 0;
+        acc.solves += st.solves;
+        acc.max_rel_residual = std::max(acc.max_rel_residual, st.max_rel_residual);
+        ++done_steps;
+        g ^= 1;
+        continue;
+      }
+      Q.recaptures += 1;
+      Q.b_iters = std::min(Q.max_iters, Q.b_iters + 2);
+      Q.b_inner = std::min(Q.max_iters, Q.b_inner + 1);
+      Q.b_outer = std::min(Q.max_outer, Q.b_outer + 1);
+      int rc = capture_graphs<TS>(Q);
+      if (rc) return rc;
+    }
+    cur = (TS*)(g == 0 ? Q.qa : Q.qb);
+    FVX_HIP(hipMemcpyAsync(q, cur, N * sizeof(TS), hipMemcpyDeviceToDevice, s));
+  }
+  FVX_HIP(hipMemcpyAsync(psi, Q.psi, N * sizeof(double), hipMemcpyDeviceToDevice, s));
+  FVX_HIP(hipStreamSynchronize(s));
   if (info) *info = acc;
   return kOk;
 }
@@ -305,6 +491,12 @@ int fvx_qg_plan_create(int ny, int nx, double Lx, double Ly, double beta, double
   const size_t es = state_precision == 64 ? 8 : 4;
   hipError_t e = hipMalloc(&Q->q1, N * es);
   if (e == hipSuccess) e = hipMalloc(&Q->q2, N * es);
+  if (e == hipSuccess) e = hipMalloc(&Q->qa, N * es);
+  if (e == hipSuccess) e = hipMalloc(&Q->qb, N * es);
+  if (e == hipSuccess) e = hipMalloc(&Q->psi, N * sizeof(double));
+  if (e == hipSuccess) e = hipMemset(Q->qa, 0, N * es);
+  if (e == hipSuccess) e = hipMemset(Q->qb, 0, N * es);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&Q->cap, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMemset(Q->q1, 0, N * es);
   if (e == hipSuccess) e = hipMemset(Q->q2, 0, N * es);
   if (e == hipSuccess && masked) {
@@ -329,9 +521,14 @@ int fvx_qg_plan_destroy(void* plan) {
   if (!plan) return kOk;
   QGPlan* Q = (QGPlan*)plan;
   (void)hipDeviceSynchronize();
+  drop_graphs(*Q);
+  if (Q->cap) (void)hipStreamDestroy(Q->cap);
   fvx_mg_plan_destroy(Q->mg);
   (void)hipFree(Q->q1);
   (void)hipFree(Q->q2);
+  (void)hipFree(Q->qa);
+  (void)hipFree(Q->qb);
+  (void)hipFree(Q->psi);
   (void)hipFree(Q->mask);
   delete Q;
   return kOk;
@@ -349,6 +546,24 @@ int fvx_qg_step(void* plan, void* q, double* psi, int nsteps, fvx_qg_info* info,
   if (Q->state == 64)
     return qg_step_impl<double>(*Q, (double*)q, psi, nsteps, (QGInfoC*)info, s);
   return qg_step_impl<float>(*Q, (float*)q, psi, nsteps, (QGInfoC*)info, s);
+}
+
+// Same as fvx_qg_step, replaying one captured HIP graph per step: each graph
+// holds the three solves (fixed iteration budgets, device-side convergence
+// guards, FP64 certification) and the three stage kernels.  The host checks
+// every solve's FP64 residual flag after each step and redoes a step with a
+// wider budget when one is missed.  The first call calibrates the budgets
+// with one synchronous step.
+int fvx_qg_step_graph(void* plan, void* q, double* psi, int nsteps, fvx_qg_info* info,
+                      void* stream) {
+  using namespace fvx;
+  FVX_CHECK_ARG(plan && q && psi, "null argument");
+  FVX_CHECK_ARG(nsteps >= 0, "negative nsteps");
+  QGPlan* Q = (QGPlan*)plan;
+  hipStream_t s = (hipStream_t)stream;
+  if (Q->state == 64)
+    return qg_run_graph<double>(*Q, (double*)q, psi, nsteps, (QGInfoC*)info, s);
+  return qg_run_graph<float>(*Q, (float*)q, psi, nsteps, (QGInfoC*)info, s);
 }
 
 #define FVX_QG_STAGE(SUF, TS)                                                                     \

@@ -35,6 +35,7 @@ _lib.extra_signatures({
                            _d, _i, _i, C.POINTER(_vp)],
     "fvx_qg_plan_destroy": [_vp],
     "fvx_qg_step": [_vp, _vp, _vp, _i, C.POINTER(_QGInfoC), _vp],
+    "fvx_qg_step_graph": [_vp, _vp, _vp, _i, C.POINTER(_QGInfoC), _vp],
     "fvx_qg_stage_f64": [_vp, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _d, _d, _d, _d, _d, _vp],
     "fvx_qg_stage_f32": [_vp, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _d, _d, _d, _d, _d, _vp],
 })
@@ -110,10 +111,14 @@ class QGModel:
         q[1:-1, 1:-1] = qi
         return q
 
-    def step(self, q: torch.Tensor, psi: torch.Tensor | None = None, nsteps: int = 1):
+    def step(self, q: torch.Tensor, psi: torch.Tensor | None = None, nsteps: int = 1,
+             graph: bool = True):
         """Advance ``q`` in place by ``nsteps`` RK3 steps; returns ``(q, psi, info)``.
 
         ``psi`` (FP64, full shape) is used as the warm start and updated.
+        ``graph=True`` replays one captured HIP graph per step (solver budgets
+        calibrated on the first call, every solve certified by its FP64
+        residual flag); ``graph=False`` drives each solve from the host.
         """
         if q.shape != self.shape or q.dtype != self.dtype or not q.is_cuda or not q.is_contiguous():
             raise ValueError(f"q must be a contiguous {self.dtype} CUDA tensor of shape {self.shape}")
@@ -122,8 +127,8 @@ class QGModel:
         if psi.shape != self.shape or psi.dtype != torch.float64 or not psi.is_contiguous():
             raise ValueError("psi must be a contiguous float64 tensor of the full shape")
         ci = _QGInfoC()
-        _lib.call("fvx_qg_step", self._h, q.data_ptr(), psi.data_ptr(), int(nsteps), C.byref(ci),
-                  _lib.stream())
+        _lib.call("fvx_qg_step_graph" if graph else "fvx_qg_step", self._h, q.data_ptr(),
+                  psi.data_ptr(), int(nsteps), C.byref(ci), _lib.stream())
         info = QGInfo(solves=ci.solves, iterations=ci.iterations,
                       max_iterations=ci.max_iterations, outer=ci.outer,
                       converged=bool(ci.converged), max_rel_residual=ci.max_rel_residual)

@@ -218,6 +218,12 @@ int fvx_qg_plan_destroy(void* plan);
 /* nsteps rk3_ssp_step steps of q (full (ny+2, nx+2), state precision) in
  * place; psi (FP64 full array) is the warm start and the last solution.    */
 int fvx_qg_step(void* plan, void* q, double* psi, int nsteps, fvx_qg_info* info, void* stream);
+/* fvx_qg_step replaying one captured HIP graph per step (three solves with
+ * fixed iteration budgets and device-side convergence guards, FP64 residual
+ * certification, three stage kernels); a step whose solve misses rtol is
+ * redone with a wider budget.  Budgets are calibrated by the first call. */
+int fvx_qg_step_graph(void* plan, void* q, double* psi, int nsteps, fvx_qg_info* info,
+                      void* stream);
 /* One fused tendency + stage combination out = a q_n + b q_s + cdt rhs(q_s, psi). */
 int fvx_qg_stage_f32(const float* qs, const double* psi, const float* qn, float* out,
                      const uint8_t* mask, int Ny, int Nx, double dx, double dy, double beta,

@@ -90,3 +90,23 @@ def test_mixed_paths_track_fp64(precision):
     assert im["max_rel_residual"] <= 1e-8 and i64["max_rel_residual"] <= 1e-8
     err = np.max(np.abs(to_np(qm) - to_np(q64))) / np.max(np.abs(to_np(q64)))
     assert err <= 1e-6, err
+
+
+@pytest.mark.parametrize("precision", ["float64", "mixed"])
+def test_graph_replay_matches_host_driven_steps(precision):
+    need_gpu()
+    ny, nx = 256, 192
+    p = _params(nx, ny)
+    q0 = inputs.qg_initial_pv(ny, nx, seed=4)
+    tol = 1e-12 if precision == "float64" else 1e-9
+    m1 = QGModel(nx, ny, p.Lx, p.Ly, precision=precision, rtol=tol)
+    m2 = QGModel(nx, ny, p.Lx, p.Ly, precision=precision, rtol=tol)
+    qa, _, ia = m1.step(m1.new_state(q0), nsteps=6, graph=False)
+    qb, _, ib = m2.step(m2.new_state(q0), nsteps=6, graph=True)
+    assert ia["converged"] and ib["converged"] and ib["solves"] == 18
+    assert ib["max_rel_residual"] <= tol
+    err = np.max(np.abs(to_np(qa) - to_np(qb))) / np.max(np.abs(to_np(qa)))
+    assert err <= (1e-10 if precision == "float64" else 1e-6), err
+    # a second call reuses the captured graphs
+    qc, _, ic = m2.step(qb, nsteps=2, graph=True)
+    assert ic["converged"] and ic["solves"] == 6
