@@ -538,6 +538,7 @@ __global__ __launch_bounds__(256) void pcg_init_kernel(LevelDev<T> L, const T* _
       sc->done = out[1] <= sc->tol2 ? 1 : 0;
       sc->first = 1;
       sc->iter = 0;
+      sc->total = 0;
       sc->alpha = 0.0;
       sc->beta = 0.0;
     }
@@ -610,6 +611,7 @@ __global__ __launch_bounds__(256) void pcg_update_kernel(int ny, int nx, T* __re
     if (lin_tid() == 0) {
       sc->rr = out[0];
       sc->iter += 1;
+      sc->total += 1;
       if (out[0] <= sc->tol2) sc->done = 1;
     }
   }
@@ -723,7 +725,7 @@ __global__ void scalars_reset_kernel(PcgScalars* sc, int n) {
   if (k < n) {
     PcgScalars& x = sc[k];
     x.rz = x.pq = x.rr = x.bb = x.tol2 = x.alpha = x.beta = x.zq = x.out_rr = x.out_tol2 = 0.0;
-    x.iter = x.done = x.first = x.out_done = x.outer = 0;
+    x.iter = x.done = x.first = x.out_done = x.outer = x.total = 0;
   }
 }
 
@@ -1124,11 +1126,17 @@ static dim3 grid2(int ny, int nx) { return dim3(cdiv(nx, 64), cdiv(ny, 4)); }
 // Precondition: H.r holds the residual and sc->first / tol2 are set.
 template <typename T, int KIND>
 static int pcg_iters(Plan& P, Hier<T>& H, T* x, int64_t ld_x, int n, PcgScalars* sc,
-                     hipStream_t s) {
+                     hipStream_t s, PrecondFn pf = nullptr, void* ctx = nullptr) {
   const LevelDev<T>& L = H.lv[0];
   for (int k = 0; k < n; ++k) {
     bool fused = false;
-    int rc = v_cycle_impl<T, KIND>(P, nullptr, H.r, H.z, true, H.q, sc, s, &fused);
+    int rc;
+    if constexpr (std::is_same<T, float>::value) {
+      if (pf) rc = pf(ctx, H.r, H.z, L.ny, L.nx, &sc->done, s);
+      else rc = v_cycle_impl<T, KIND>(P, nullptr, H.r, H.z, true, H.q, sc, s, &fused);
+    } else {
+      rc = v_cycle_impl<T, KIND>(P, nullptr, H.r, H.z, true, H.q, sc, s, &fused);
+    }
     if (rc) return rc;
     if (!fused)
       hipLaunchKernelGGL(pcg_dots_kernel<T>, grid2(L.ny, L.nx), dim3(64, 4), 0, s, L.ny, L.nx,
@@ -1235,13 +1243,13 @@ int pcg_solve(Plan& P, const T* rhs, int64_t ld_rhs, T* x, int64_t ld_x, double 
 template <typename TB>
 int mixed_enqueue(Plan& P, const TB* rhs, int64_t ld_rhs, double* x, int64_t ld_x, double rtol,
                   double atol, int n_outer, int n_inner, double inner_rtol, int slot,
-                  hipStream_t s) {
+                  hipStream_t s, PrecondFn pf, void* ctx) {
   Hier<float>& H = P.h32;
   const auto& L64 = P.h64.lv[0];
   const auto& L32 = H.lv[0];
   PcgScalars* in = &P.sc[2 * slot];
   PcgScalars* out = &P.sc[2 * slot + 1];
-  hipLaunchKernelGGL(scalars_reset_kernel, dim3(1), dim3(64), 0, s, out, 1);
+  hipLaunchKernelGGL(scalars_reset_kernel, dim3(1), dim3(64), 0, s, in, 2);
   auto residual = [&]() {
     if (P.kind == kDirichlet)
       hipLaunchKernelGGL((refine_residual_kernel<kDirichlet, TB>), grid2(L64.ny, L64.nx),
@@ -1256,8 +1264,8 @@ int mixed_enqueue(Plan& P, const TB* rhs, int64_t ld_rhs, double* x, int64_t ld_
     residual();
     hipLaunchKernelGGL(inner_reset_kernel, dim3(1), dim3(1), 0, s, in, out, inner_rtol);
     int rc = (P.kind == kDirichlet)
-                 ? pcg_iters<float, kDirichlet>(P, H, H.x, L32.nx, n_inner, in, s)
-                 : pcg_iters<float, kNoflux>(P, H, H.x, L32.nx, n_inner, in, s);
+                 ? pcg_iters<float, kDirichlet>(P, H, H.x, L32.nx, n_inner, in, s, pf, ctx)
+                 : pcg_iters<float, kNoflux>(P, H, H.x, L32.nx, n_inner, in, s, pf, ctx);
     if (rc) return rc;
     hipLaunchKernelGGL(refine_update_kernel, grid2(L32.ny, L32.nx), dim3(64, 4), 0, s, L32.ny,
                        L32.nx, x, ld_x, H.x, out);
@@ -1271,7 +1279,8 @@ int mixed_enqueue(Plan& P, const TB* rhs, int64_t ld_rhs, double* x, int64_t ld_
 template <typename TB>
 int mixed_solve(Plan& P, const TB* rhs, int64_t ld_rhs, double* x, int64_t ld_x, double rtol,
                 double atol, int max_outer, int max_inner, double inner_rtol, CgInfoC* info,
-                int slot, hipStream_t s) {
+                int slot, hipStream_t s, PrecondFn pf, void* ctx) {
+  int fallbacks = 0;
   int inner = std::max(1, std::min(P.last_inner, max_inner));
   int outer_pred = std::max(1, std::min(P.last_outer, max_outer));
   int outer_total = 0;
@@ -1281,7 +1290,8 @@ int mixed_solve(Plan& P, const TB* rhs, int64_t ld_rhs, double* x, int64_t ld_x,
   while (true) {
     const int no = first ? outer_pred : 1;
     if (first) {
-      rc = mixed_enqueue<TB>(P, rhs, ld_rhs, x, ld_x, rtol, atol, no, inner, inner_rtol, slot, s);
+      rc = mixed_enqueue<TB>(P, rhs, ld_rhs, x, ld_x, rtol, atol, no, inner, inner_rtol, slot, s,
+                             pf, ctx);
     } else {
       // Continue: do not reset the outer scalars (keeps the outer count).
       Hier<float>& H = P.h32;
@@ -1289,8 +1299,9 @@ int mixed_solve(Plan& P, const TB* rhs, int64_t ld_rhs, double* x, int64_t ld_x,
       PcgScalars* in = &P.sc[2 * slot];
       PcgScalars* out = &P.sc[2 * slot + 1];
       hipLaunchKernelGGL(inner_reset_kernel, dim3(1), dim3(1), 0, s, in, out, inner_rtol);
-      rc = (P.kind == kDirichlet) ? pcg_iters<float, kDirichlet>(P, H, H.x, L32.nx, inner, in, s)
-                                  : pcg_iters<float, kNoflux>(P, H, H.x, L32.nx, inner, in, s);
+      rc = (P.kind == kDirichlet)
+               ? pcg_iters<float, kDirichlet>(P, H, H.x, L32.nx, inner, in, s, pf, ctx)
+               : pcg_iters<float, kNoflux>(P, H, H.x, L32.nx, inner, in, s, pf, ctx);
       if (rc) return rc;
       hipLaunchKernelGGL(refine_update_kernel, grid2(L32.ny, L32.nx), dim3(64, 4), 0, s, L32.ny,
                          L32.nx, x, ld_x, H.x, out);
@@ -1318,19 +1329,35 @@ int mixed_solve(Plan& P, const TB* rhs, int64_t ld_rhs, double* x, int64_t ld_x,
       break;
     }
     if (outer_total >= max_outer) break;
-    if (!hi.done) inner = std::min(max_inner, inner + 2);
+    if (!hi.done) {
+      if (pf && inner >= max_inner) {
+        // The learned inner solve hit its cap: fall back to the V-cycle.
+        pf = nullptr;
+        ctx = nullptr;
+        ++fallbacks;
+      }
+      inner = std::min(max_inner, inner + 2);
+    }
   }
   const PcgScalars& ho = P.hsc[2 * slot + 1];
   P.last_outer = std::max(1, ho.outer);
   P.last_inner = inner;
   if (info) {
-    info->iterations = P.hsc[2 * slot].iter;
+    info->iterations = P.hsc[2 * slot].total;
     info->outer = ho.outer;
     info->converged = converged ? 1 : 0;
     info->residual_norm = std::sqrt(ho.out_rr);
     info->rhs_norm = std::sqrt(ho.bb);
+    info->pad = fallbacks;
   }
   return kOk;
+}
+
+const uint8_t* plan_mask0(const Plan& P) {
+  return P.has32 ? P.h32.lv[0].mask : P.h64.lv[0].mask;
+}
+double plan_inv_diag0(const Plan& P) {
+  return 1.0 / (2.0 / (P.dx * P.dx) + 2.0 / (P.dy * P.dy) + P.lam);
 }
 
 const PcgScalars* fetch_slots(Plan& P, int first_slot, int n, hipStream_t s) {
@@ -1341,6 +1368,8 @@ const PcgScalars* fetch_slots(Plan& P, int first_slot, int n, hipStream_t s) {
   return &P.hsc[2 * first_slot];
 }
 int plan_levels(const Plan& P) { return P.nl; }
+bool plan_has_mixed(const Plan& P) { return P.has32 && P.has64; }
+int plan_nx(const Plan& P) { return P.has64 ? P.h64.lv[0].nx : P.h32.lv[0].nx; }
 int plan_kind(const Plan& P) { return P.kind; }
 
 template int pcg_enqueue<float>(Plan&, const float*, int64_t, float*, int64_t, double, double,
@@ -1348,9 +1377,9 @@ template int pcg_enqueue<float>(Plan&, const float*, int64_t, float*, int64_t, d
 template int pcg_enqueue<double>(Plan&, const double*, int64_t, double*, int64_t, double, double,
                                  bool, int, int, hipStream_t);
 template int mixed_enqueue<float>(Plan&, const float*, int64_t, double*, int64_t, double, double,
-                                  int, int, double, int, hipStream_t);
+                                  int, int, double, int, hipStream_t, PrecondFn, void*);
 template int mixed_enqueue<double>(Plan&, const double*, int64_t, double*, int64_t, double,
-                                   double, int, int, double, int, hipStream_t);
+                                   double, int, int, double, int, hipStream_t, PrecondFn, void*);
 template int certify_enqueue<double>(Plan&, const double*, int64_t, const double*, int64_t,
                                      double, double, int, hipStream_t);
 template int certify_enqueue<float>(Plan&, const float*, int64_t, const float*, int64_t, double,
@@ -1360,9 +1389,9 @@ template int pcg_solve<double>(Plan&, const double*, int64_t, double*, int64_t, 
 template int pcg_solve<float>(Plan&, const float*, int64_t, float*, int64_t, double, double, int,
                               bool, CgInfoC*, int, hipStream_t);
 template int mixed_solve<float>(Plan&, const float*, int64_t, double*, int64_t, double, double,
-                                int, int, double, CgInfoC*, int, hipStream_t);
+                                int, int, double, CgInfoC*, int, hipStream_t, PrecondFn, void*);
 template int mixed_solve<double>(Plan&, const double*, int64_t, double*, int64_t, double, double,
-                                 int, int, double, CgInfoC*, int, hipStream_t);
+                                 int, int, double, CgInfoC*, int, hipStream_t, PrecondFn, void*);
 
 }  // namespace fvx
 

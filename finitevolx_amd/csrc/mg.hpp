@@ -23,7 +23,8 @@ struct PcgScalars {
   int first;      // 1 before the first direction is formed
   int out_done;   // outer refinement converged
   int outer;      // outer iterations done
-  int pad[3];
+  int total;      // inner iterations summed over refinement steps
+  int pad[2];
   unsigned int ticket[8];  // last-block counters, one per reduction site
 };
 
@@ -33,6 +34,12 @@ struct CgInfoC {
 };
 
 struct Plan;
+
+// A preconditioner replacing the V-cycle inside flexible PCG: z = M(r) on the
+// finest level (FP32 arrays of nx columns); kernels should exit early when
+// *guard is nonzero (the solve has converged).
+using PrecondFn = int (*)(void* ctx, const float* r, float* z, int ny, int nx, const int* guard,
+                          hipStream_t s);
 
 template <typename T>
 int pcg_enqueue(Plan& P, const T* rhs, int64_t ld_rhs, T* x, int64_t ld_x, double rtol,
@@ -46,16 +53,24 @@ int pcg_solve(Plan& P, const T* rhs, int64_t ld_rhs, T* x, int64_t ld_x, double 
 template <typename TB>
 int mixed_enqueue(Plan& P, const TB* rhs, int64_t ld_rhs, double* x, int64_t ld_x, double rtol,
                   double atol, int n_outer, int n_inner, double inner_rtol, int slot,
-                  hipStream_t s);
+                  hipStream_t s, PrecondFn pf = nullptr, void* ctx = nullptr);
+// Synchronous mixed solve.  With ``pf`` the inner FP32 solves use it; an
+// inner solve that hits ``max_inner`` without reaching its tolerance makes the
+// remaining sweeps fall back to the multigrid V-cycle (counted in
+// info->pad).
 template <typename TB>
 int mixed_solve(Plan& P, const TB* rhs, int64_t ld_rhs, double* x, int64_t ld_x, double rtol,
                 double atol, int max_outer, int max_inner, double inner_rtol, CgInfoC* info,
-                int slot, hipStream_t s);
+                int slot, hipStream_t s, PrecondFn pf = nullptr, void* ctx = nullptr);
+const uint8_t* plan_mask0(const Plan& P);
+double plan_inv_diag0(const Plan& P);
 
 // Copy ``n`` slots of device scalars to the plan's pinned mirror and wait;
 // returns the host pointer (2 blocks per slot).
 const PcgScalars* fetch_slots(Plan& P, int first_slot, int n, hipStream_t s);
 int plan_levels(const Plan& P);
+bool plan_has_mixed(const Plan& P);
+int plan_nx(const Plan& P);
 int plan_kind(const Plan& P);
 
 }  // namespace fvx

@@ -23,6 +23,12 @@
 #include "fvx_common.hpp"
 #include "mg.hpp"
 
+namespace fvx {
+template <typename T>
+int unet_apply(void* net, const T* r, int64_t ld_r, T* z, int64_t ld_z, int H, int W,
+               double out_scale, const uint8_t* mask, hipStream_t s, const int* guard);
+}
+
 extern "C" int fvx_mg_plan_create(const double*, const double*, int, int, double, double, double,
                                   int, int, int, int, double, int, int, void**);
 extern "C" int fvx_mg_plan_destroy(void*);
@@ -174,7 +180,19 @@ struct QGPlan {
   int obs_iters = 0, obs_outer = 0;           // calibration observations
   bool calibrated = false;
   int recaptures = 0;
+  // Learned preconditioner for the inner FP32 solves (mixed mode only).
+  void* net = nullptr;
+  double net_scale = 1.0;
+  int fallbacks = 0;
 };
+
+// PrecondFn adapter: z = net(r) on the interior grid.
+static int net_precond(void* ctx, const float* r, float* z, int ny, int nx, const int* guard,
+                       hipStream_t s) {
+  QGPlan* Q = (QGPlan*)ctx;
+  return unet_apply<float>(Q->net, r, nx, z, nx, ny, nx, Q->net_scale, plan_mask0(*Q->mg), s,
+                           guard);
+}
 
 struct QGInfoC {
   int solves, iterations, max_iterations, outer;
@@ -220,7 +238,9 @@ static int qg_step_impl(QGPlan& Q, TS* q, double* psi, int nsteps, QGInfoC* info
       int rc;
       if (Q.solve == kSolveMixed) {
         rc = mixed_solve<TS>(*Q.mg, qs + off, Nx, psi + off, Nx, Q.rtol, 0.0, Q.max_outer,
-                             Q.max_iters, Q.inner_rtol, &ci, 0, s);
+                             Q.max_iters, Q.inner_rtol, &ci, 0, s,
+                             Q.net ? &net_precond : nullptr, Q.net ? &Q : nullptr);
+        Q.fallbacks += ci.pad;
       } else {
         if constexpr (std::is_same<TS, double>::value) {
           // Recursive residual to rtol/2 so the certified FP64 true residual
@@ -266,7 +286,8 @@ static int qg_step_enqueue(QGPlan& Q, const TS* q_in, TS* q_out, hipStream_t s) 
     int rc;
     if (Q.solve == kSolveMixed) {
       rc = mixed_enqueue<TS>(*Q.mg, qs + off, Nx, psi + off, Nx, Q.rtol, 0.0, Q.b_outer,
-                             Q.b_inner, Q.inner_rtol, k, s);
+                             Q.b_inner, Q.inner_rtol, k, s, Q.net ? &net_precond : nullptr,
+                             Q.net ? &Q : nullptr);
     } else {
       if constexpr (std::is_same<TS, double>::value) {
         rc = pcg_enqueue<double>(*Q.mg, qs + off, Nx, psi + off, Nx, 0.5 * Q.rtol, 0.0, true,
@@ -329,8 +350,8 @@ static bool step_converged(QGPlan& Q, QGInfoC& acc, hipStream_t s) {
     const bool conv = out.out_done != 0;  // FP64 true residual (both modes)
     ok = ok && conv;
     acc.solves += 1;
-    acc.iterations += in.iter;
-    acc.max_iterations = std::max(acc.max_iterations, in.iter);
+    acc.iterations += in.total;
+    acc.max_iterations = std::max(acc.max_iterations, in.total);
     acc.outer += out.outer;
     const double rel = out.bb > 0 ? std::sqrt(out.out_rr / out.bb) : std::sqrt(out.out_rr);
     acc.max_rel_residual = std::max(acc.max_rel_residual, rel);
@@ -398,6 +419,15 @@ static int qg_run_graph(QGPlan& Q, TS* q, double* psi, int nsteps, QGInfoC* info
       }
       // A solve missed its tolerance within the budget: widen and redo the
       // step from its (untouched) input buffer.
+      if (Q.net && (Q.b_inner >= Q.max_iters || Q.b_outer >= Q.max_outer)) {
+        // The learned preconditioner cannot meet the tolerance within the
+        // caps: fall back to multigrid-PCG for the rest of the run.
+        Q.net = nullptr;
+        Q.fallbacks += 1;
+        int rc = capture_graphs<TS>(Q);
+        if (rc) return rc;
+        continue;
+      }
       if (Q.recaptures > 50 || Q.b_iters >= Q.max_iters || Q.b_outer >= Q.max_outer) {
         acc.converged = 0;
         acc.solves += st.solves;
@@ -547,6 +577,26 @@ int fvx_qg_step(void* plan, void* q, double* psi, int nsteps, fvx_qg_info* info,
     return qg_step_impl<double>(*Q, (double*)q, psi, nsteps, (QGInfoC*)info, s);
   return qg_step_impl<float>(*Q, (float*)q, psi, nsteps, (QGInfoC*)info, s);
 }
+
+// Use a learned U-Net (fvx_unet_create) as the preconditioner of the inner
+// FP32 solves of a mixed-precision plan (net = NULL restores multigrid);
+// z = out_scale * rms(r) * net(r / rms(r)).  out_scale <= 0 selects the
+// inverse Jacobi diagonal of the fine operator.  Invalidates captured graphs.
+int fvx_qg_set_preconditioner(void* plan, void* net, double out_scale) {
+  using namespace fvx;
+  FVX_CHECK_ARG(plan, "null plan");
+  QGPlan* Q = (QGPlan*)plan;
+  FVX_CHECK_ARG(net == nullptr || Q->solve == kSolveMixed,
+                "a learned preconditioner needs the mixed-precision solve");
+  Q->net = net;
+  Q->net_scale = out_scale > 0 ? out_scale : plan_inv_diag0(*Q->mg);
+  drop_graphs(*Q);
+  Q->calibrated = false;
+  Q->fallbacks = 0;
+  return kOk;
+}
+
+int fvx_qg_fallbacks(void* plan) { return plan ? ((fvx::QGPlan*)plan)->fallbacks : -1; }
 
 // Same as fvx_qg_step, replaying one captured HIP graph per step: each graph
 // holds the three solves (fixed iteration budgets, device-side convergence

@@ -36,6 +36,8 @@ _lib.extra_signatures({
     "fvx_qg_plan_destroy": [_vp],
     "fvx_qg_step": [_vp, _vp, _vp, _i, C.POINTER(_QGInfoC), _vp],
     "fvx_qg_step_graph": [_vp, _vp, _vp, _i, C.POINTER(_QGInfoC), _vp],
+    "fvx_qg_set_preconditioner": [_vp, _vp, _d],
+    "fvx_qg_fallbacks": [_vp],
     "fvx_qg_stage_f64": [_vp, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _d, _d, _d, _d, _d, _vp],
     "fvx_qg_stage_f32": [_vp, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _d, _d, _d, _d, _d, _vp],
 })
@@ -101,6 +103,20 @@ class QGModel:
     @property
     def shape(self):
         return (self.ny + 2, self.nx + 2)
+
+    def set_preconditioner(self, net=None, out_scale: float = 0.0):
+        """Use a trained ``learned.UNet`` for the inner FP32 solves (mixed
+        precision only); ``None`` restores the multigrid V-cycle."""
+        if net is not None and (net.H < self.ny or net.W < self.nx):
+            raise ValueError("the network plan is smaller than the grid")
+        self._net = net  # keep the plan alive while the model uses it
+        _lib.call("fvx_qg_set_preconditioner", self._h, None if net is None else net._h,
+                  float(out_scale))
+
+    @property
+    def fallbacks(self) -> int:
+        """Times the learned preconditioner gave way to multigrid-PCG."""
+        return _lib.call("fvx_qg_fallbacks", self._h)
 
     def new_state(self, q_interior):
         """Full ``(ny+2, nx+2)`` state with a zero ghost ring (and zero land)."""

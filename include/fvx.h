@@ -185,6 +185,16 @@ int fvx_mg_pcg_solve_f64(void* plan, const double* rhs, int64_t ld_rhs, double* 
 int fvx_mg_mixed_solve(void* plan, const double* rhs, int64_t ld_rhs, double* x, int64_t ld_x,
                        double rtol, double atol, int max_outer, int max_inner, double inner_rtol,
                        fvx_cg_info* info, void* stream);
+/* The same mixed solve with the inner preconditioner chosen by ``precond``:
+ * 0 multigrid V-cycle (make_multigrid_preconditioner,
+ * solvers/preconditioners.py:86-134), 1 learned U-Net ``net`` (from
+ * fvx_unet_create; no reference counterpart), 2 diagonal (Jacobi).
+ * out_scale <= 0 selects 1 / diag(A).  info->pad counts inner solves that hit
+ * max_inner and were replaced by the V-cycle.                             */
+int fvx_mg_precond_solve(void* plan, int precond, void* net, double out_scale,
+                         const double* rhs, int64_t ld_rhs, double* x, int64_t ld_x, double rtol,
+                         double atol, int max_outer, int max_inner, double inner_rtol,
+                         fvx_cg_info* info, void* stream);
 
 /* ---- Spectral rectangular Helmholtz solvers: solve_helmholtz_{dst,dct,fft}
  * (solvers/spectral.py:14-128, re-exported from spectraldiffx v0.0.10;
@@ -218,6 +228,12 @@ int fvx_qg_plan_destroy(void* plan);
 /* nsteps rk3_ssp_step steps of q (full (ny+2, nx+2), state precision) in
  * place; psi (FP64 full array) is the warm start and the last solution.    */
 int fvx_qg_step(void* plan, void* q, double* psi, int nsteps, fvx_qg_info* info, void* stream);
+/* Learned U-Net preconditioner for the inner FP32 solves of a mixed plan
+ * (NULL restores multigrid); out_scale <= 0 = inverse Jacobi diagonal.
+ * Multigrid-PCG takes over when a learned inner solve hits its cap;
+ * fvx_qg_fallbacks counts those events. */
+int fvx_qg_set_preconditioner(void* plan, void* net, double out_scale);
+int fvx_qg_fallbacks(void* plan);
 /* fvx_qg_step replaying one captured HIP graph per step (three solves with
  * fixed iteration budgets and device-side convergence guards, FP64 residual
  * certification, three stage kernels); a step whose solve misses rtol is
@@ -233,6 +249,31 @@ int fvx_qg_stage_f64(const double* qs, const double* psi, const double* qn, doub
                      const uint8_t* mask, int Ny, int Nx, double dx, double dy, double beta,
                      double Ly, double F0, double nu4, double a, double b, double cdt,
                      void* stream);
+
+/* ---- Learned U-Net preconditioner (no reference counterpart) -------------
+ * Bias-free U-Net: nlev levels of two 3x3 convs + ReLU (channels[l], each a
+ * multiple of 16 <= 64), 2x2 average pooling, nearest upsampling, additive
+ * skips, 1-channel input/output; BF16 weights/activations, FP32 accumulation
+ * on matrix cores; FP32 master weights trained with Adam.  The CPU oracle is
+ * oracle/unet.py.  H, W: largest field the plan runs (divisible by 2^(nlev-1)). */
+int fvx_unet_create(int nlev, const int* channels, int H, int W, void** out);
+int fvx_unet_destroy(void* net);
+int64_t fvx_unet_num_weights(void* net);
+/* direction 0: load fp32 weights from device memory (resets Adam); 1: read. */
+int fvx_unet_weights(void* net, float* w, int direction, void* stream);
+int fvx_unet_forward_raw(void* net, const float* x, float* y, int H, int W, void* stream);
+int fvx_unet_backward_raw(void* net, const float* x, const float* gy, int H, int W,
+                          void* stream);
+int fvx_unet_grads(void* net, float* g, int zero_after, void* stream);
+/* Adam step on a batch of B (x, e) pairs, loss = mean (net(x) - e)^2. */
+int fvx_unet_train_step(void* net, const float* xb, const float* eb, int B, int H, int W,
+                        double lr, double* loss_out, void* stream);
+/* z = out_scale * rms(r) * net(r / rms(r)) * mask  (the PCG preconditioner). */
+int fvx_unet_apply_f32(void* net, const float* r, int64_t ld_r, float* z, int64_t ld_z, int H,
+                       int W, double out_scale, const uint8_t* mask, void* stream);
+/* One 3x3 conv on matrix cores (bf16 channels-last in/out, fp32 weights). */
+int fvx_conv3x3_bf16(const uint16_t* x, int Cin, const float* w, int Cout, uint16_t* out, int H,
+                     int W, int ups, int relu, uint16_t* scratch, void* stream);
 
 #ifdef __cplusplus
 }

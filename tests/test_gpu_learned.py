@@ -1,0 +1,108 @@
+"""GPU: the learned U-Net preconditioner inside mixed-precision flexible PCG.
+
+The network is trained on (residual, error) pairs that the multigrid solver
+produces for the QG Helmholtz operator (lap - 1/Ld^2, Ld = 16 cells, zero
+streamfunction on a basin coastline).  Checks:
+
+* the learned-PCG solve certifies its FP64 true residual at rtol 1e-8 and
+  matches the multigrid-PCG solution to 1e-7 (max-norm relative);
+* gain: it needs fewer inner iterations than diagonal (Jacobi) PCG and than
+  the same network before training, with no fallback to the V-cycle;
+* QGModel with the learned preconditioner tracks the FP64 model.
+
+The network has no reference counterpart; its kernels are checked against
+oracle/unet.py in test_gpu_unet.py.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from finitevolx_amd import learned, solvers as S
+from finitevolx_amd.inputs import basin_mask, spectral_field
+from oracle import fvx
+from tests.gpu_util import assert_close_scaled, need_gpu, to_np
+
+pytestmark = pytest.mark.gpu
+
+N = 128
+LD_CELLS = 16.0
+
+
+def _operator():
+    m = basin_mask(N, N, 21).astype(np.float64)
+    dx = dy = 1.0 / N
+    lam = 1.0 / (LD_CELLS * dx) ** 2
+    return m, dx, dy, lam
+
+
+@pytest.fixture(scope="module")
+def trained():
+    need_gpu()
+    m, dx, dy, lam = _operator()
+    mg64 = S.build_multigrid_solver(m, dx, dy, lam, kind="dirichlet", n_pre=2, n_post=2,
+                                    n_coarse=20, omega=0.8)
+    rhs = [spectral_field(N, N, 3.0, 100 + s) * m for s in range(8)]
+    rs, es = learned.generate_pairs(mg64, rhs, vcycles=(1, 2, 3))
+    out_scale = 1.0 / (2 / dx**2 + 2 / dy**2 + lam)
+    net = learned.UNet(H=N, W=N, seed=0)
+    untrained = learned.UNet(H=N, W=N, seed=0)
+    torch.manual_seed(0)
+    hist = learned.train(net, rs, es, out_scale, steps=400, crop=64, batch=8, lr=2e-3,
+                         lr_final=2e-4, seed=0)
+    return net, untrained, hist, out_scale
+
+
+def test_training_lowers_loss(trained):
+    _, _, hist, _ = trained
+    first, last = np.mean(hist[:20]), np.mean(hist[-20:])
+    assert np.isfinite(last)
+    assert last < 0.5 * first, (first, last)
+
+
+def test_learned_pcg_certifies_and_beats_diagonal(trained):
+    net, untrained, _, out_scale = trained
+    m, dx, dy, lam = _operator()
+    b = spectral_field(N, N, 3.0, 7) * m  # held out of training
+    mix = S.build_multigrid_solver(m, dx, dy, lam, kind="dirichlet", precision="mixed",
+                                   n_pre=2, n_post=2, n_coarse=20, omega=0.8)
+    lev = fvx.build_multigrid(m, dx, dy, lam, kind="dirichlet")
+    bn = np.linalg.norm(b)
+    results = {}
+    for name, kw in (("multigrid", {}), ("learned", {"net": net}), ("untrained", {"net": untrained}),
+                     ("diagonal", {})):
+        pre = "learned" if name == "untrained" else name
+        x, info = mix.precond_solve(b, precond=pre, out_scale=out_scale, rtol=1e-8, atol=0.0,
+                                    max_outer=30, max_inner=400, **kw)
+        assert info.converged, (name, info)
+        r = np.linalg.norm(b - fvx.apply_operator(to_np(x), lev[0]))
+        assert r <= 1e-8 * bn * 1.0001, (name, r / bn)
+        results[name] = (to_np(x), info.iterations, mix.last_info.pad)
+        print(f"[learned] {name}: {info.iterations} inner iterations, "
+              f"{mix.last_info.outer} outer, fallbacks {mix.last_info.pad}", flush=True)
+    assert_close_scaled(results["learned"][0], results["multigrid"][0], 1e-7, "learned vs mg")
+    assert results["learned"][2] == 0
+    assert results["diagonal"][2] == 0
+    assert results["learned"][1] < results["diagonal"][1]
+    assert results["learned"][1] < results["untrained"][1]
+
+
+def test_qg_model_with_learned_preconditioner(trained):
+    from finitevolx_amd.qg import QGModel
+
+    net, _, _, out_scale = trained
+    m, dx, dy, lam = _operator()
+    Lx = 1.0
+    kw = dict(beta=2.0, Ld=LD_CELLS * dx, nu4=1e-9, F0=0.5, dt=1e-3, mask=m, rtol=1e-8,
+              max_iters=400)
+    q0 = spectral_field(N, N, 3.0, 11) * m * 10.0
+    ref = QGModel(N, N, Lx, Lx, precision="float64", **kw)
+    qr, pr, _ = ref.step(ref.new_state(q0), nsteps=3, graph=False)
+    mod = QGModel(N, N, Lx, Lx, precision="mixed", **kw)
+    mod.set_preconditioner(net, out_scale)
+    q, p, info = mod.step(mod.new_state(q0), nsteps=3, graph=False)
+    assert info["converged"] and info["max_rel_residual"] <= 1e-8 * 1.0001, info
+    print(f"[learned] QG 3 steps: {info}, fallbacks {mod.fallbacks}", flush=True)
+    assert_close_scaled(to_np(p).astype(np.float64), to_np(pr), 1e-5, "psi learned vs fp64")
+    assert_close_scaled(to_np(q).astype(np.float64), to_np(qr), 1e-5, "q learned vs fp64")
+    mod.set_preconditioner(None)
