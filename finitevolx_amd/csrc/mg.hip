@@ -54,19 +54,24 @@ __device__ __forceinline__ T mval(const uint8_t* m, int64_t k) {
 }
 
 // ---------------------------------------------------------------------------
-// Reductions finished by the last workgroup (fixed summation order ->
-// deterministic).  Partials: nblk x NV doubles.
+// Reductions finished by the last workgroup, in two levels so that no
+// counter takes more than ~nblk/64 atomics (same-address device atomics
+// serialise; one counter per 16k-block grid cost ~0.4 ms).  Block b belongs
+// to group b % G (G = min(nblk, kRedGroups)); the last block of each group
+// sums that group's partials in block order, the last group finisher sums the
+// G group totals in group order -> deterministic.
+// partials: [kRedGroups][NV] group totals, then [nblk][NV] block partials.
+// tk: kRedGroups + 1 counters (tk[0] top level), left at zero on exit.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int lin_tid() {
   return threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
 }
 
 template <int NV, int kThreads>
-__device__ bool block_reduce_last(const double (&v)[NV], double* partials,
-                                  unsigned int* ticket, double (&out)[NV]) {
+__device__ bool block_reduce_last(const double (&v)[NV], double* partials, unsigned int* tk,
+                                  double (&out)[NV]) {
   __shared__ double red[kThreads / 64][NV];
-  __shared__ bool is_last;
-  // Linear thread id: the callers use 1-D (256) and 2-D (64 x 4) blocks.
+  __shared__ int stage;
   const int tid = lin_tid();
   const int lane = tid & 63, wid = tid >> 6;
 #pragma unroll
@@ -77,29 +82,57 @@ __device__ bool block_reduce_last(const double (&v)[NV], double* partials,
   __syncthreads();
   const int nblk = gridDim.x * gridDim.y;
   const int bid = blockIdx.y * gridDim.x + blockIdx.x;
+  const int G = nblk < kRedGroups ? nblk : kRedGroups;
+  const int g = bid % G;
+  const int gsize = nblk / G + (g < nblk % G ? 1 : 0);
+  double* gpart = partials;
+  double* bpart = partials + kRedGroups * NV;
   if (tid == 0) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
       double s = 0.0;
       for (int w = 0; w < kThreads / 64; ++w) s += red[w][k];
-      partials[(int64_t)bid * NV + k] = s;
+      bpart[(int64_t)bid * NV + k] = s;
     }
     __threadfence();
-    const unsigned int prev = atomicAdd(ticket, 1u);
-    is_last = (prev == (unsigned int)nblk - 1);
+    const unsigned int prev = atomicAdd(&tk[1 + g], 1u);
+    stage = (prev == (unsigned int)gsize - 1) ? 1 : 0;
   }
   __syncthreads();
-  if (!is_last) return false;
+  if (!stage) return false;
   __threadfence();
-  // Last workgroup: sum the partials in block order (each lane a strided
-  // subset, then a fixed-shape tree).
+  // Last block of group g: its blocks are g, g + G, g + 2G, ...
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
     double s = 0.0;
-    for (int b = tid; b < nblk; b += kThreads) {
-      s += __builtin_nontemporal_load(&partials[(int64_t)b * NV + k]);
-    }
+    for (int m = tid; m < gsize; m += kThreads)
+      s += __builtin_nontemporal_load(&bpart[(int64_t)(g + m * G) * NV + k]);
     s = wave_sum(s);
+    __syncthreads();
+    if (lane == 0) red[wid][k] = s;
+  }
+  __syncthreads();
+  if (tid == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double s = 0.0;
+      for (int w = 0; w < kThreads / 64; ++w) s += red[w][k];
+      gpart[g * NV + k] = s;
+    }
+    tk[1 + g] = 0u;  // every block of the group has counted
+    __threadfence();
+    const unsigned int prev = atomicAdd(&tk[0], 1u);
+    stage = (prev == (unsigned int)G - 1) ? 2 : 0;
+  }
+  __syncthreads();
+  if (stage != 2) return false;
+  __threadfence();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double s = 0.0;
+    for (int m = tid; m < G; m += kThreads) s += __builtin_nontemporal_load(&gpart[m * NV + k]);
+    s = wave_sum(s);
+    __syncthreads();
     if (lane == 0) red[wid][k] = s;
   }
   __syncthreads();
@@ -110,7 +143,7 @@ __device__ bool block_reduce_last(const double (&v)[NV], double* partials,
       for (int w = 0; w < kThreads / 64; ++w) s += red[w][k];
       out[k] = s;
     }
-    *ticket = 0u;  // re-arm for the next launch
+    tk[0] = 0u;  // re-arm for the next launch
   }
   return true;  // out[] valid in thread 0
 }
@@ -350,7 +383,7 @@ __global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
   }
   if constexpr (MODE == kUpDots) {
     double v[2] = {d0, d1}, out[2];
-    if (block_reduce_last<2, kTileThreads>(v, a.partials, &a.sc->ticket[0], out)) {
+    if (block_reduce_last<2, kTileThreads>(v, a.partials, a.sc->ticket[0], out)) {
       if (lin_tid() == 0) {
         PcgScalars* sc = a.sc;
         // Flexible (Polak-Ribiere) beta: z_{k+1}.(r_{k+1} - r_k) / rho_k with
@@ -473,6 +506,24 @@ __global__ __launch_bounds__(256) void apply_op_kernel(LevelDev<T> L, const T* _
   out[(int64_t)j * L.nx + i] = glob_Au<T, KIND>(L, u, j, i);
 }
 
+// res = (b - A u) * mask on one level.
+template <typename T, int KIND>
+__global__ __launch_bounds__(256) void residual_kernel(LevelDev<T> L, const T* __restrict__ u,
+                                                       const T* __restrict__ b,
+                                                       T* __restrict__ res) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+  if (i >= L.nx || j >= L.ny) return;
+  const int64_t k = (int64_t)j * L.nx + i;
+  res[k] = (b[k] - glob_Au<T, KIND>(L, u, j, i)) * mval<T>(L.mask, k);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void add_into_kernel(int n, const T* __restrict__ a,
+                                                       T* __restrict__ b) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k < n) b[k] += a[k];
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void mask_copy_kernel(LevelDev<T> L, const T* __restrict__ in,
                                                         T* __restrict__ out, int64_t ld_in) {
@@ -530,7 +581,7 @@ __global__ __launch_bounds__(256) void pcg_init_kernel(LevelDev<T> L, const T* _
     rr = (double)rv * rv;
   }
   double v[2] = {bb, rr}, out[2];
-  if (block_reduce_last<2, 256>(v, partials, &sc->ticket[1], out)) {
+  if (block_reduce_last<2, 256>(v, partials, sc->ticket[1], out)) {
     if (lin_tid() == 0) {
       sc->bb = out[0];
       sc->rr = out[1];
@@ -579,7 +630,7 @@ __global__ __launch_bounds__(256) void pcg_dir_kernel(LevelDev<T> L, const T* __
     pq = (double)c * qv;
   }
   double v[1] = {pq}, out[1];
-  if (block_reduce_last<1, 256>(v, partials, &sc->ticket[2], out)) {
+  if (block_reduce_last<1, 256>(v, partials, sc->ticket[2], out)) {
     if (lin_tid() == 0) {
       sc->pq = out[0];
       sc->alpha = sc->rz / out[0];
@@ -607,7 +658,7 @@ __global__ __launch_bounds__(256) void pcg_update_kernel(int ny, int nx, T* __re
     rr = (double)rv * rv;
   }
   double v[1] = {rr}, out[1];
-  if (block_reduce_last<1, 256>(v, partials, &sc->ticket[3], out)) {
+  if (block_reduce_last<1, 256>(v, partials, sc->ticket[3], out)) {
     if (lin_tid() == 0) {
       sc->rr = out[0];
       sc->iter += 1;
@@ -655,7 +706,7 @@ __global__ __launch_bounds__(256) void refine_residual_kernel(
     rr = rv * rv;
   }
   double v[2] = {bb, rr}, out[2];
-  if (block_reduce_last<2, 256>(v, partials, &sc->ticket[4], out)) {
+  if (block_reduce_last<2, 256>(v, partials, sc->ticket[4], out)) {
     if (lin_tid() == 0) {
       sc->bb = out[0];
       sc->out_rr = out[1];
@@ -708,7 +759,7 @@ __global__ __launch_bounds__(256) void true_residual_kernel(LevelDev<T> L, const
     rr = rv * rv;
   }
   double v[2] = {bb, rr}, out[2];
-  if (block_reduce_last<2, 256>(v, partials, &res->ticket[6], out)) {
+  if (block_reduce_last<2, 256>(v, partials, res->ticket[6], out)) {
     if (lin_tid() == 0) {
       res->bb = out[0];
       res->out_rr = out[1];
@@ -753,6 +804,7 @@ struct Plan {
   PcgScalars* sc = nullptr;   // kSlots pairs: [2k] inner, [2k+1] outer
   PcgScalars* hsc = nullptr;  // pinned host mirror
   int last_iters = 4, last_outer = 2, last_inner = 4;
+  float* work0[3] = {nullptr, nullptr, nullptr};  // fine-level FP32 work (lazy)
   std::vector<void*> allocs;
 };
 
@@ -993,7 +1045,7 @@ __global__ __launch_bounds__(256) void pcg_dots_kernel(int ny, int nx, const T* 
     if (!sc->first) d1 = (double)z[k] * q[k];
   }
   double v[2] = {d0, d1}, out[2];
-  if (block_reduce_last<2, 256>(v, partials, &sc->ticket[5], out)) {
+  if (block_reduce_last<2, 256>(v, partials, sc->ticket[5], out)) {
     if (lin_tid() == 0) {
       sc->beta = sc->first ? 0.0 : (-sc->alpha * out[1]) / sc->rz;
       sc->rz = out[0];
@@ -1353,6 +1405,93 @@ int mixed_solve(Plan& P, const TB* rhs, int64_t ld_rhs, double* x, int64_t ld_x,
   return kOk;
 }
 
+// Fine-level (FP32) helpers for preconditioners built around the smoother.
+float* fine_work(Plan& P, int k) {
+  if (k < 0 || k >= 3) return nullptr;
+  if (!P.work0[k]) {
+    const auto& L = P.h32.lv[0];
+    void* q = nullptr;
+    if (hipMalloc(&q, (size_t)L.ny * L.nx * sizeof(float)) != hipSuccess) return nullptr;
+    P.allocs.push_back(q);
+    P.work0[k] = (float*)q;
+  }
+  return P.work0[k];
+}
+
+int fine_presmooth(Plan& P, const float* r, float* z, float* res, int sweeps, hipStream_t s) {
+  const auto& L = P.h32.lv[0];
+  if (sweeps <= 0) {
+    hipLaunchKernelGGL(mask_copy_kernel<float>, grid2(L.ny, L.nx), dim3(64, 4), 0, s, L, nullptr,
+                       z, (int64_t)L.nx);
+    hipLaunchKernelGGL(mask_copy_kernel<float>, grid2(L.ny, L.nx), dim3(64, 4), 0, s, L, r, res,
+                       (int64_t)L.nx);
+  } else if (P.kind == kDirichlet) {
+    smooth_chunks<float, kDirichlet>(P, L, nullptr, true, r, sweeps, L.t, L.t2, z, nullptr, s);
+    hipLaunchKernelGGL((residual_kernel<float, kDirichlet>), grid2(L.ny, L.nx), dim3(64, 4), 0, s,
+                       L, z, r, res);
+  } else {
+    smooth_chunks<float, kNoflux>(P, L, nullptr, true, r, sweeps, L.t, L.t2, z, nullptr, s);
+    hipLaunchKernelGGL((residual_kernel<float, kNoflux>), grid2(L.ny, L.nx), dim3(64, 4), 0, s, L,
+                       z, r, res);
+  }
+  FVX_LAUNCH_CHECK();
+  return kOk;
+}
+
+int fine_postsmooth(Plan& P, float* z_in, const float* add, const float* r, float* z_out,
+                    int sweeps, hipStream_t s) {
+  const auto& L = P.h32.lv[0];
+  const int n = L.ny * L.nx;
+  if (add)
+    hipLaunchKernelGGL(add_into_kernel<float>, dim3(cdiv(n, 256)), dim3(256), 0, s, n, add, z_in);
+  if (sweeps <= 0) {
+    hipLaunchKernelGGL(mask_copy_kernel<float>, grid2(L.ny, L.nx), dim3(64, 4), 0, s, L, z_in,
+                       z_out, (int64_t)L.nx);
+  } else if (P.kind == kDirichlet) {
+    smooth_chunks<float, kDirichlet>(P, L, z_in, false, r, sweeps, L.t, L.t2, z_out, nullptr, s);
+  } else {
+    smooth_chunks<float, kNoflux>(P, L, z_in, false, r, sweeps, L.t, L.t2, z_out, nullptr, s);
+  }
+  FVX_LAUNCH_CHECK();
+  return kOk;
+}
+
+// Profiling hook support: a scalar block that never converges, so the
+// guarded PCG kernels do their full work when launched back to back.
+__global__ void bench_scalars_kernel(PcgScalars* sc) {
+  sc->done = 0;
+  sc->first = 0;
+  sc->beta = 0.5;
+  sc->alpha = 1e-6;
+  sc->rz = 1.0;
+  sc->tol2 = -1.0;
+}
+
+template <typename T, int KIND>
+static int kernel_bench_impl(Plan& P, int which, int reps, hipStream_t s) {
+  Hier<T>& H = hier<T>(P);
+  const LevelDev<T>& L = H.lv[0];
+  PcgScalars* sc = &P.sc[2 * (kSlots - 1)];
+  if (reps == 0) {  // prepare: zero the vectors, arm the scalars
+    for (T* v : {H.r, H.z, H.p0, H.p1, H.q, H.x})
+      hipLaunchKernelGGL(zero_pitched_kernel<T>, grid2(L.ny, L.nx), dim3(64, 4), 0, s, L.ny, L.nx,
+                         v, (int64_t)L.nx);
+    hipLaunchKernelGGL(bench_scalars_kernel, dim3(1), dim3(1), 0, s, sc);
+    FVX_LAUNCH_CHECK();
+    return kOk;
+  }
+  for (int k = 0; k < reps; ++k) {
+    if (which == 0)
+      hipLaunchKernelGGL((pcg_dir_kernel<T, KIND>), grid2(L.ny, L.nx), dim3(64, 4), 0, s, L, H.z,
+                         H.p0, H.p1, H.q, P.partials, sc);
+    else
+      hipLaunchKernelGGL(pcg_update_kernel<T>, grid2(L.ny, L.nx), dim3(64, 4), 0, s, L.ny, L.nx,
+                         H.x, (int64_t)L.nx, H.p0, H.r, H.q, P.partials, sc);
+  }
+  FVX_LAUNCH_CHECK();
+  return kOk;
+}
+
 const uint8_t* plan_mask0(const Plan& P) {
   return P.has32 ? P.h32.lv[0].mask : P.h64.lv[0].mask;
 }
@@ -1370,6 +1509,7 @@ const PcgScalars* fetch_slots(Plan& P, int first_slot, int n, hipStream_t s) {
 int plan_levels(const Plan& P) { return P.nl; }
 bool plan_has_mixed(const Plan& P) { return P.has32 && P.has64; }
 int plan_nx(const Plan& P) { return P.has64 ? P.h64.lv[0].nx : P.h32.lv[0].nx; }
+int plan_ny(const Plan& P) { return P.has64 ? P.h64.lv[0].ny : P.h32.lv[0].ny; }
 int plan_kind(const Plan& P) { return P.kind; }
 
 template int pcg_enqueue<float>(Plan&, const float*, int64_t, float*, int64_t, double, double,
@@ -1467,7 +1607,7 @@ int fvx_mg_plan_create(const double* mask, const double* coeff, int ny, int nx, 
   }
   if (!rc) {
     const size_t maxblk = (size_t)cdiv(nx, 64) * cdiv(ny, 4) + 64;
-    hipError_t e = hipMalloc(&P->partials, maxblk * 4 * sizeof(double));
+    hipError_t e = hipMalloc(&P->partials, (maxblk + kRedGroups) * 4 * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&P->sc, 2 * kSlots * sizeof(PcgScalars));
     if (e == hipSuccess) e = hipMemset(P->sc, 0, 2 * kSlots * sizeof(PcgScalars));
     if (e == hipSuccess)
@@ -1637,6 +1777,28 @@ int fvx_mg_mixed_solve(void* plan, const double* rhs, int64_t ld_rhs, double* x,
   FVX_CHECK_ARG(ld_rhs >= P->nx && ld_x >= P->nx, "row pitch smaller than nx");
   return mixed_solve<double>(*P, rhs, ld_rhs, x, ld_x, rtol, atol, max_outer, max_inner,
                              inner_rtol, (CgInfoC*)info, 0, (hipStream_t)stream);
+}
+
+// Profiling hook (no reference counterpart): launch ``reps`` back-to-back
+// instances of one fine-level PCG kernel on the plan's own vectors with a
+// scalar block that never converges.  which: 0 pcg_dir (p = z + beta p,
+// q = A p, p.q), 1 pcg_update (x += alpha p, r -= alpha q, r.r); bits 32 or
+// 64 select the hierarchy.  reps = 0 prepares (zeroes the vectors, arms the
+// scalars) and must precede timed calls.
+int fvx_mg_kernel_bench(void* plan, int which, int bits, int reps, void* stream) {
+  using namespace fvx;
+  FVX_CHECK_ARG(plan, "null plan");
+  Plan* P = (Plan*)plan;
+  FVX_CHECK_ARG(which == 0 || which == 1, "which must be 0 (pcg_dir) or 1 (pcg_update)");
+  FVX_CHECK_ARG((bits == 32 && P->has32) || (bits == 64 && P->has64),
+                "plan has no hierarchy of that precision");
+  FVX_CHECK_ARG(reps >= 0, "negative reps");
+  hipStream_t s = (hipStream_t)stream;
+  if (bits == 32)
+    return P->kind == kDirichlet ? kernel_bench_impl<float, kDirichlet>(*P, which, reps, s)
+                                 : kernel_bench_impl<float, kNoflux>(*P, which, reps, s);
+  return P->kind == kDirichlet ? kernel_bench_impl<double, kDirichlet>(*P, which, reps, s)
+                               : kernel_bench_impl<double, kNoflux>(*P, which, reps, s);
 }
 
 }  // extern "C"

@@ -6,6 +6,9 @@
 
 namespace fvx {
 
+// Counters per reduction site: one top level + one per block group.
+constexpr int kRedGroups = 64;
+
 // Device-side scalars of one PCG solve (one "slot" holds an inner and an
 // outer block).
 struct PcgScalars {
@@ -25,7 +28,7 @@ struct PcgScalars {
   int outer;      // outer iterations done
   int total;      // inner iterations summed over refinement steps
   int pad[2];
-  unsigned int ticket[8];  // last-block counters, one per reduction site
+  unsigned int ticket[8][kRedGroups + 1];  // last-block counters per reduction site
 };
 
 struct CgInfoC {
@@ -70,7 +73,20 @@ double plan_inv_diag0(const Plan& P);
 const PcgScalars* fetch_slots(Plan& P, int first_slot, int n, hipStream_t s);
 int plan_levels(const Plan& P);
 bool plan_has_mixed(const Plan& P);
+// Fine-level FP32 helpers (level 0 of the FP32 hierarchy, contiguous fields):
+// lazily allocated work arrays k = 0..2; ``sweeps`` damped-Jacobi sweeps on
+// A z = r from zero with res = (r - A z) mask; and sweeps from z_in (+ add)
+// into z_out.  z_in is overwritten when ``add`` is given.
+float* fine_work(Plan& P, int k);
+int fine_presmooth(Plan& P, const float* r, float* z, float* res, int sweeps, hipStream_t s);
+int fine_postsmooth(Plan& P, float* z_in, const float* add, const float* r, float* z_out,
+                    int sweeps, hipStream_t s);
 int plan_nx(const Plan& P);
+int plan_ny(const Plan& P);
+// Learned preconditioner application (precond.hip); sweeps as in
+// fvx_mg_precond_solve.
+int learned_precond_apply(Plan& P, void* net, double scale, int sweeps, const float* r, float* z,
+                          const int* guard, hipStream_t s);
 int plan_kind(const Plan& P);
 
 }  // namespace fvx

@@ -26,16 +26,40 @@ __global__ __launch_bounds__(256) void diag_precond_kernel(int ny, int nx, float
 }
 
 struct PrecondCtx {
-  const Plan* plan;
+  Plan* plan;
   void* net;
   double scale;
+  int sweeps;
 };
+
+// Learned preconditioner.  With sweeps = 0: z = N(r).  With sweeps > 0 the
+// network is the smooth-error correction between damped-Jacobi sweeps (a
+// two-grid cycle whose coarse solve is learned):
+//   z0 = S^nu(0; r),  z1 = z0 + N(r - A z0),  z = S^nu(z1; r).
+int learned_precond_apply(Plan& P, void* net, double scale, int sweeps, const float* r, float* z,
+                          const int* guard, hipStream_t s) {
+  const int nx = plan_nx(P), ny = plan_ny(P);
+  if (sweeps <= 0) return unet_apply<float>(net, r, nx, z, nx, ny, nx, scale, plan_mask0(P), s,
+                                            guard);
+  float* z0 = fine_work(P, 0);
+  float* res = fine_work(P, 1);
+  float* zn = fine_work(P, 2);
+  if (!z0 || !res || !zn) {
+    set_error("allocation of the preconditioner work arrays failed");
+    return kHipError;
+  }
+  int rc = fine_presmooth(P, r, z0, res, sweeps, s);
+  if (!rc) rc = unet_apply<float>(net, res, nx, zn, nx, ny, nx, scale, plan_mask0(P), s, guard);
+  if (!rc) rc = fine_postsmooth(P, zn, z0, r, z, sweeps, s);
+  return rc;
+}
 
 static int learned_fn(void* ctx, const float* r, float* z, int ny, int nx, const int* guard,
                       hipStream_t s) {
+  (void)ny;
+  (void)nx;
   PrecondCtx* c = (PrecondCtx*)ctx;
-  return unet_apply<float>(c->net, r, nx, z, nx, ny, nx, c->scale, plan_mask0(*c->plan), s,
-                           guard);
+  return learned_precond_apply(*c->plan, c->net, c->scale, c->sweeps, r, z, guard, s);
 }
 
 static int diag_fn(void* ctx, const float* r, float* z, int ny, int nx, const int* guard,
@@ -54,10 +78,12 @@ extern "C" {
 // Mixed-precision PCG (FP32 inner flexible PCG, FP64 refinement) with the
 // inner preconditioner chosen by ``precond``: 0 multigrid V-cycle, 1 learned
 // U-Net ``net`` (z = out_scale rms(r) net(r / rms(r)) on wet cells;
-// out_scale <= 0 selects 1 / diag(A)), 2 diagonal.  info->pad returns the
+// out_scale <= 0 selects 1 / diag(A); with sweeps > 0 the network corrects
+// the smooth error between ``sweeps`` damped-Jacobi pre- and post-sweeps),
+// 2 diagonal.  info->pad returns the
 // number of learned/diagonal inner solves that hit max_inner and were
 // replaced by the V-cycle.
-int fvx_mg_precond_solve(void* plan, int precond, void* net, double out_scale,
+int fvx_mg_precond_solve(void* plan, int precond, void* net, double out_scale, int sweeps,
                          const double* rhs, int64_t ld_rhs, double* x, int64_t ld_x, double rtol,
                          double atol, int max_outer, int max_inner, double inner_rtol,
                          void* info, void* stream) {
@@ -70,7 +96,8 @@ int fvx_mg_precond_solve(void* plan, int precond, void* net, double out_scale,
   FVX_CHECK_ARG(plan_has_mixed(*P), "mixed solve needs a plan built with precision 3264");
   FVX_CHECK_ARG(max_outer >= 1 && max_inner >= 1, "iteration caps must be >= 1");
   FVX_CHECK_ARG(ld_rhs >= plan_nx(*P) && ld_x >= plan_nx(*P), "row pitch smaller than nx");
-  PrecondCtx ctx{P, net, out_scale > 0 ? out_scale : plan_inv_diag0(*P)};
+  FVX_CHECK_ARG(sweeps >= 0 && sweeps <= 8, "sweeps must be in [0, 8]");
+  PrecondCtx ctx{P, net, out_scale > 0 ? out_scale : plan_inv_diag0(*P), sweeps};
   PrecondFn pf = precond == 1 ? &learned_fn : precond == 2 ? &diag_fn : nullptr;
   return mixed_solve<double>(*P, rhs, ld_rhs, x, ld_x, rtol, atol, max_outer, max_inner,
                              inner_rtol, (CgInfoC*)info, 0, (hipStream_t)stream, pf,

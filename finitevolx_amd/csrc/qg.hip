@@ -183,17 +183,18 @@ struct QGPlan {
   // Learned preconditioner for the inner FP32 solves (mixed mode only).
   void* net = nullptr;
   double net_scale = 1.0;
+  int net_sweeps = 0;
   int fallbacks = 0;
 };
 
 // PrecondFn adapter: z = net(r) on the interior grid.
 static int net_precond(void* ctx, const float* r, float* z, int ny, int nx, const int* guard,
                        hipStream_t s) {
+  (void)ny;
+  (void)nx;
   QGPlan* Q = (QGPlan*)ctx;
-  return unet_apply<float>(Q->net, r, nx, z, nx, ny, nx, Q->net_scale, plan_mask0(*Q->mg), s,
-                           guard);
+  return learned_precond_apply(*Q->mg, Q->net, Q->net_scale, Q->net_sweeps, r, z, guard, s);
 }
-
 struct QGInfoC {
   int solves, iterations, max_iterations, outer;
   int converged, pad;
@@ -582,13 +583,19 @@ int fvx_qg_step(void* plan, void* q, double* psi, int nsteps, fvx_qg_info* info,
 // FP32 solves of a mixed-precision plan (net = NULL restores multigrid);
 // z = out_scale * rms(r) * net(r / rms(r)).  out_scale <= 0 selects the
 // inverse Jacobi diagonal of the fine operator.  Invalidates captured graphs.
-int fvx_qg_set_preconditioner(void* plan, void* net, double out_scale) {
+int fvx_qg_set_preconditioner(void* plan, void* net, double out_scale, int sweeps) {
   using namespace fvx;
   FVX_CHECK_ARG(plan, "null plan");
   QGPlan* Q = (QGPlan*)plan;
   FVX_CHECK_ARG(net == nullptr || Q->solve == kSolveMixed,
                 "a learned preconditioner needs the mixed-precision solve");
+  FVX_CHECK_ARG(sweeps >= 0 && sweeps <= 8, "sweeps must be in [0, 8]");
+  if (net && sweeps > 0) {  // allocate the work arrays outside any graph capture
+    for (int k = 0; k < 3; ++k)
+      FVX_CHECK_ARG(fine_work(*Q->mg, k) != nullptr, "work array allocation failed");
+  }
   Q->net = net;
+  Q->net_sweeps = sweeps;
   Q->net_scale = out_scale > 0 ? out_scale : plan_inv_diag0(*Q->mg);
   drop_graphs(*Q);
   Q->calibrated = false;

@@ -173,31 +173,48 @@ def conv3x3_bf16(x: torch.Tensor, w: np.ndarray, ups: bool = False, relu: bool =
 # ---------------------------------------------------------------------------
 
 
-def generate_pairs(mg, rhs_fields, vcycles=(1, 2, 3), rtol=1e-11):
-    """(residual, error) pairs generated by the multigrid solver.
+def generate_pairs(mg, rhs_fields, vcycles=(1, 2, 3), sweeps: int = 2, rtol=1e-11):
+    """(residual, error) training pairs generated by the multigrid solver.
 
     For each right-hand side b: the converged solution x* (multigrid-PCG to
-    ``rtol``), then the iterates x_k of k plain V-cycles from zero; each pair is
-    (r_k = b - A x_k, e_k = x* - x_k), plus (b, x*) for a cold start.  ``mg``
-    is an FP64 ``MultigridSolver``; returns lists of FP32 device tensors.
+    ``rtol``) and the iterates x_k of k plain V-cycles from zero (k in
+    ``vcycles``, plus x_0 = 0), giving residual/error pairs (r_k = b - A x_k,
+    e_k = x* - x_k).  With ``sweeps`` > 0 each pair is then taken through the
+    pre-smoothing the preconditioner applies before the network:
+    z = S^sweeps(0; r_k), pair (r_k - A z, e_k - z).  ``mg`` is an FP64
+    ``MultigridSolver`` built with the same operator and omega as the solve.
+    Returns FP32 device tensors (list of residuals, list of errors) and the
+    output scale (median rms(error) / rms(residual)) to train and apply with.
     """
-    from finitevolx_amd.solvers import apply_operator
+    from finitevolx_amd.solvers import apply_operator, weighted_jacobi
 
-    rs, es = [], []
+    lev0 = mg.levels[0]
+    rs, es, ratios = [], [], []
     for b in rhs_fields:
         b = _lib.as_field(b, torch.float64)
         xs, info = mg.pcg(b, rtol=rtol, atol=0.0, max_steps=400)
         if not info.converged:
             raise RuntimeError(f"pair generation solve did not converge: {info}")
+        iterates = [torch.zeros_like(b)]
         u = None
         for k in range(1, max(vcycles) + 1):
             u = mg.v_cycle(u, b)
             if k in vcycles:
-                rs.append((b - apply_operator(u, mg.levels[0])).float())
-                es.append((xs - u).float())
-        rs.append(b.float())
-        es.append(xs.float())
-    return rs, es
+                iterates.append(u)
+        for x in iterates:
+            r = b - apply_operator(x, lev0)
+            e = xs - x
+            if sweeps > 0:
+                z = weighted_jacobi(None, r, lev0, mg.omega, sweeps)
+                e = e - z
+                r = r - apply_operator(z, lev0)
+            rn = float(torch.sqrt(torch.mean(r * r)))
+            if rn == 0.0:
+                continue
+            ratios.append(float(torch.sqrt(torch.mean(e * e))) / rn)
+            rs.append(r.float())
+            es.append(e.float())
+    return rs, es, float(np.median(ratios))
 
 
 def crop_batch(rs, es, crop: int, B: int, out_scale: float, gen: torch.Generator):
@@ -232,3 +249,33 @@ def train(net: UNet, rs, es, out_scale: float, steps: int = 600, crop: int = 128
         if log_every and it % log_every == 0:
             print(f"[train] step {it} loss {loss:.4e}", flush=True)
     return hist
+
+
+def fit_preconditioner(mask, dx: float, dy: float, lambda_: float, rhs_fields, *,
+                       kind: str = "dirichlet", n_pre: int = 2, n_post: int = 2,
+                       n_coarse: int = 20, omega: float = 0.8, sweeps: int = 2,
+                       channels=(16, 32, 48, 64), steps: int = 600, crop: int = 128,
+                       batch: int = 8, lr: float = 2e-3, lr_final: float = 2e-4, seed: int = 0,
+                       vcycles=(1, 2, 3)):
+    """Train a U-Net preconditioner for one Helmholtz operator on the GPU.
+
+    Builds the FP64 multigrid solver of the operator, generates (residual,
+    error) pairs from it on ``rhs_fields`` and trains the network with this
+    build's kernels.  Returns ``(net, out_scale, loss_history)``; pass the
+    network, ``out_scale`` and the same ``sweeps`` to
+    ``MultigridSolver.precond_solve`` or ``QGModel.set_preconditioner``.
+    """
+    from finitevolx_amd.solvers import build_multigrid_solver
+
+    m = np.asarray(mask, dtype=np.float64)
+    mg = build_multigrid_solver(m, dx, dy, lambda_, kind=kind, n_pre=n_pre, n_post=n_post,
+                                n_coarse=n_coarse, omega=omega)
+    rs, es, out_scale = generate_pairs(mg, [np.asarray(f) * m for f in rhs_fields],
+                                       vcycles=vcycles, sweeps=sweeps)
+    del mg
+    H, W = m.shape
+    net = UNet(channels, H, W, seed=seed)
+    crop = min(crop, H, W)
+    hist = train(net, rs, es, out_scale, steps=steps, crop=crop, batch=batch, lr=lr,
+                 lr_final=lr_final, seed=seed)
+    return net, out_scale, hist

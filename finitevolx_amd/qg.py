@@ -36,7 +36,7 @@ _lib.extra_signatures({
     "fvx_qg_plan_destroy": [_vp],
     "fvx_qg_step": [_vp, _vp, _vp, _i, C.POINTER(_QGInfoC), _vp],
     "fvx_qg_step_graph": [_vp, _vp, _vp, _i, C.POINTER(_QGInfoC), _vp],
-    "fvx_qg_set_preconditioner": [_vp, _vp, _d],
+    "fvx_qg_set_preconditioner": [_vp, _vp, _d, _i],
     "fvx_qg_fallbacks": [_vp],
     "fvx_qg_stage_f64": [_vp, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _d, _d, _d, _d, _d, _vp],
     "fvx_qg_stage_f32": [_vp, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _d, _d, _d, _d, _d, _vp],
@@ -104,14 +104,15 @@ class QGModel:
     def shape(self):
         return (self.ny + 2, self.nx + 2)
 
-    def set_preconditioner(self, net=None, out_scale: float = 0.0):
+    def set_preconditioner(self, net=None, out_scale: float = 0.0, sweeps: int = 2):
         """Use a trained ``learned.UNet`` for the inner FP32 solves (mixed
-        precision only); ``None`` restores the multigrid V-cycle."""
+        precision only), between ``sweeps`` damped-Jacobi pre- and
+        post-sweeps; ``None`` restores the multigrid V-cycle."""
         if net is not None and (net.H < self.ny or net.W < self.nx):
             raise ValueError("the network plan is smaller than the grid")
         self._net = net  # keep the plan alive while the model uses it
         _lib.call("fvx_qg_set_preconditioner", self._h, None if net is None else net._h,
-                  float(out_scale))
+                  float(out_scale), int(sweeps))
 
     @property
     def fallbacks(self) -> int:

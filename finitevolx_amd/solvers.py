@@ -47,7 +47,7 @@ _sigs = {
     "fvx_mg_plan_info": [_vp, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i)],
     "fvx_mg_mixed_solve": [_vp, _vp, _i64, _vp, _i64, _d, _d, _i, _i, _d,
                            C.POINTER(_CgInfoC), _vp],
-    "fvx_mg_precond_solve": [_vp, _i, _vp, _d, _vp, _i64, _vp, _i64, _d, _d, _i, _i, _d,
+    "fvx_mg_precond_solve": [_vp, _i, _vp, _d, _i, _vp, _i64, _vp, _i64, _d, _d, _i, _i, _d,
                              C.POINTER(_CgInfoC), _vp],
 }
 PRECONDS = {"multigrid": 0, "learned": 1, "diagonal": 2}
@@ -241,13 +241,15 @@ class MultigridSolver:
                       converged=bool(ci.converged))
 
     def precond_solve(self, rhs, x0=None, precond: str = "multigrid", net=None,
-                      out_scale: float = 0.0, rtol=1e-8, atol=0.0, max_outer=20, max_inner=200,
-                      inner_rtol=1e-4):
+                      out_scale: float = 0.0, sweeps: int = 2, rtol=1e-8, atol=0.0,
+                      max_outer=20, max_inner=200, inner_rtol=1e-4):
         """Mixed-precision PCG with the inner preconditioner chosen by name:
         ``"multigrid"`` (this hierarchy's V-cycle), ``"learned"`` (a trained
-        ``learned.UNet``), or ``"diagonal"`` (Jacobi).  Needs ``precision=
-        "mixed"``.  Returns ``(x, info)``; ``self.last_info.pad`` counts inner
-        solves that hit ``max_inner`` and were redone with the V-cycle."""
+        ``learned.UNet``; with ``sweeps`` > 0 it corrects the smooth error
+        between that many damped-Jacobi pre- and post-sweeps), or
+        ``"diagonal"`` (Jacobi).  Needs ``precision="mixed"``.  Returns
+        ``(x, info)``; ``self.last_info.pad`` counts inner solves that hit
+        ``max_inner`` and were redone with the V-cycle."""
         if self.precision != "mixed":
             raise ValueError("precond_solve needs a solver built with precision='mixed'")
         if precond not in PRECONDS:
@@ -260,7 +262,7 @@ class MultigridSolver:
         x = torch.zeros_like(rhs_t) if x0 is None else _lib.as_field(x0, torch.float64).clone()
         ci = _CgInfoC()
         _lib.call("fvx_mg_precond_solve", self._h, PRECONDS[precond],
-                  None if net is None else net._h, float(out_scale), rhs_t.data_ptr(),
+                  None if net is None else net._h, float(out_scale), int(sweeps), rhs_t.data_ptr(),
                   rhs_t.stride(0), x.data_ptr(), x.stride(0), float(rtol), float(atol),
                   int(max_outer), int(max_inner), float(inner_rtol), C.byref(ci), _lib.stream())
         self.last_info = ci

@@ -189,12 +189,22 @@ int fvx_mg_mixed_solve(void* plan, const double* rhs, int64_t ld_rhs, double* x,
  * 0 multigrid V-cycle (make_multigrid_preconditioner,
  * solvers/preconditioners.py:86-134), 1 learned U-Net ``net`` (from
  * fvx_unet_create; no reference counterpart), 2 diagonal (Jacobi).
- * out_scale <= 0 selects 1 / diag(A).  info->pad counts inner solves that hit
- * max_inner and were replaced by the V-cycle.                             */
-int fvx_mg_precond_solve(void* plan, int precond, void* net, double out_scale,
+ * out_scale <= 0 selects 1 / diag(A).  For the learned preconditioner,
+ * sweeps > 0 makes it a two-grid cycle with a learned coarse correction:
+ * z0 = S^sweeps(0; r), z1 = z0 + net(r - A z0), z = S^sweeps(z1; r), S the
+ * damped-Jacobi smoother of the plan.  info->pad counts inner solves that
+ * hit max_inner and were replaced by the V-cycle.                         */
+int fvx_mg_precond_solve(void* plan, int precond, void* net, double out_scale, int sweeps,
                          const double* rhs, int64_t ld_rhs, double* x, int64_t ld_x, double rtol,
                          double atol, int max_outer, int max_inner, double inner_rtol,
                          fvx_cg_info* info, void* stream);
+
+/* Profiling hook (no reference counterpart; for benchmarks): reps
+ * back-to-back launches of one fine-level PCG kernel on the plan's own
+ * vectors, never converging.  which 0 = direction/matvec (p = z + beta p,
+ * q = A p, p.q), 1 = update (x += alpha p, r -= alpha q, r.r); bits 32/64
+ * picks the hierarchy; reps = 0 prepares and must come first.             */
+int fvx_mg_kernel_bench(void* plan, int which, int bits, int reps, void* stream);
 
 /* ---- Spectral rectangular Helmholtz solvers: solve_helmholtz_{dst,dct,fft}
  * (solvers/spectral.py:14-128, re-exported from spectraldiffx v0.0.10;
@@ -229,10 +239,12 @@ int fvx_qg_plan_destroy(void* plan);
  * place; psi (FP64 full array) is the warm start and the last solution.    */
 int fvx_qg_step(void* plan, void* q, double* psi, int nsteps, fvx_qg_info* info, void* stream);
 /* Learned U-Net preconditioner for the inner FP32 solves of a mixed plan
- * (NULL restores multigrid); out_scale <= 0 = inverse Jacobi diagonal.
+ * (NULL restores multigrid; no reference counterpart); out_scale <= 0 =
+ * inverse Jacobi diagonal; sweeps > 0 wraps the network between that many
+ * damped-Jacobi pre- and post-sweeps (as in fvx_mg_precond_solve).
  * Multigrid-PCG takes over when a learned inner solve hits its cap;
  * fvx_qg_fallbacks counts those events. */
-int fvx_qg_set_preconditioner(void* plan, void* net, double out_scale);
+int fvx_qg_set_preconditioner(void* plan, void* net, double out_scale, int sweeps);
 int fvx_qg_fallbacks(void* plan);
 /* fvx_qg_step replaying one captured HIP graph per step (three solves with
  * fixed iteration budgets and device-side convergence guards, FP64 residual

@@ -11,6 +11,12 @@ Tensors are channels-last ``(H, W, C)``; conv weights ``(3, 3, Cout, Cin)``
 
 Backward is hand-written and pinned by finite differences in
 ``tests/test_oracle_unet.py``.
+
+``bf16=True`` rounds to bfloat16 (round-to-nearest-even) at the points where
+the device network stores activations and activation gradients (every
+hidden conv output after its ReLU, pooled inputs, and each masked gradient
+tensor), so that the GPU kernels can be compared against the same storage
+precision; with ``bf16=False`` the network is exact FP64.
 """
 
 from __future__ import annotations
@@ -64,6 +70,17 @@ def relu(x):
     return np.maximum(x, 0.0)
 
 
+def bf16_round(x):
+    """Round to the nearest bfloat16 (ties to even) via float32."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def _identity(x):
+    return x
+
+
 def layer_names(nlev):
     names = []
     for l in range(nlev):
@@ -95,28 +112,29 @@ def init_weights(channels, seed=0):
             for k, s in weight_shapes(channels).items()}
 
 
-def forward(Wt, r, channels):
+def forward(Wt, r, channels, bf16=False):
     """Returns the output (H, W) and the cache needed by ``backward``."""
+    q = bf16_round if bf16 else _identity
     L = len(channels)
     cache = {}
     x = r[:, :, None]
     e = []
     for l in range(L):
         if l > 0:
-            x = pool2(e[-1])
+            x = q(pool2(e[-1]))
         a1 = conv3x3(x, Wt[f"e{l}a"])
-        h1 = relu(a1)
+        h1 = q(relu(a1))
         a2 = conv3x3(h1, Wt[f"e{l}b"])
-        h2 = relu(a2)
+        h2 = q(relu(a2))
         cache[f"e{l}"] = (x, h1, h2)
         e.append(h2)
     d = e[-1]
     for l in range(L - 2, -1, -1):
         u = up2(d)
         a1 = conv3x3(u, Wt[f"u{l}"]) + e[l]
-        h1 = relu(a1)
+        h1 = q(relu(a1))
         a2 = conv3x3(h1, Wt[f"d{l}"])
-        h2 = relu(a2)
+        h2 = q(relu(a2))
         cache[f"d{l}"] = (u, h1, h2)
         d = h2
     y = conv3x3(d, Wt["out"])
@@ -124,8 +142,9 @@ def forward(Wt, r, channels):
     return y[:, :, 0], cache
 
 
-def backward(Wt, cache, gy, channels):
+def backward(Wt, cache, gy, channels, bf16=False):
     """Weight gradients of sum(gy * y)."""
+    q = bf16_round if bf16 else _identity
     L = len(channels)
     g = {}
     gd, g["out"] = conv3x3_bwd(cache["out"], Wt["out"], gy[:, :, None])
@@ -134,23 +153,23 @@ def backward(Wt, cache, gy, channels):
     grad_d = gd
     for l in range(0, L - 1):
         u, h1, h2 = cache[f"d{l}"]
-        ga2 = grad_d * (h2 > 0)
+        ga2 = q(grad_d * (h2 > 0))
         gh1, g[f"d{l}"] = conv3x3_bwd(h1, Wt[f"d{l}"], ga2)
-        ga1 = gh1 * (h1 > 0)
+        ga1 = q(gh1 * (h1 > 0))
         ge[l] = ga1 if ge[l] is None else ge[l] + ga1
         gu, g[f"u{l}"] = conv3x3_bwd(u, Wt[f"u{l}"], ga1)
-        grad_d = up2_bwd(gu)
+        grad_d = up2_bwd(q(gu))
     # encoder, from the coarsest level back to the finest
     gcarry = grad_d  # gradient into e_{L-1}
     for l in range(L - 1, -1, -1):
         x, h1, h2 = cache[f"e{l}"]
         gh2 = gcarry if l == L - 1 else ge[l] + gcarry
-        ga2 = gh2 * (h2 > 0)
+        ga2 = q(gh2 * (h2 > 0))
         gh1, g[f"e{l}b"] = conv3x3_bwd(h1, Wt[f"e{l}b"], ga2)
-        ga1 = gh1 * (h1 > 0)
+        ga1 = q(gh1 * (h1 > 0))
         gx, g[f"e{l}a"] = conv3x3_bwd(x, Wt[f"e{l}a"], ga1)
         if l > 0:
-            gcarry = pool2_bwd(gx)
+            gcarry = pool2_bwd(q(gx))
     return g
 
 

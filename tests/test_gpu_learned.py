@@ -27,6 +27,7 @@ pytestmark = pytest.mark.gpu
 
 N = 128
 LD_CELLS = 16.0
+SWEEPS = 2
 
 
 def _operator():
@@ -43,8 +44,7 @@ def trained():
     mg64 = S.build_multigrid_solver(m, dx, dy, lam, kind="dirichlet", n_pre=2, n_post=2,
                                     n_coarse=20, omega=0.8)
     rhs = [spectral_field(N, N, 3.0, 100 + s) * m for s in range(8)]
-    rs, es = learned.generate_pairs(mg64, rhs, vcycles=(1, 2, 3))
-    out_scale = 1.0 / (2 / dx**2 + 2 / dy**2 + lam)
+    rs, es, out_scale = learned.generate_pairs(mg64, rhs, vcycles=(1, 2, 3), sweeps=SWEEPS)
     net = learned.UNet(H=N, W=N, seed=0)
     untrained = learned.UNet(H=N, W=N, seed=0)
     torch.manual_seed(0)
@@ -72,8 +72,9 @@ def test_learned_pcg_certifies_and_beats_diagonal(trained):
     for name, kw in (("multigrid", {}), ("learned", {"net": net}), ("untrained", {"net": untrained}),
                      ("diagonal", {})):
         pre = "learned" if name == "untrained" else name
-        x, info = mix.precond_solve(b, precond=pre, out_scale=out_scale, rtol=1e-8, atol=0.0,
-                                    max_outer=30, max_inner=400, **kw)
+        scale = out_scale if pre == "learned" else 0.0
+        x, info = mix.precond_solve(b, precond=pre, out_scale=scale, sweeps=SWEEPS, rtol=1e-8,
+                                    atol=0.0, max_outer=30, max_inner=400, **kw)
         assert info.converged, (name, info)
         r = np.linalg.norm(b - fvx.apply_operator(to_np(x), lev[0]))
         assert r <= 1e-8 * bn * 1.0001, (name, r / bn)
@@ -99,7 +100,7 @@ def test_qg_model_with_learned_preconditioner(trained):
     ref = QGModel(N, N, Lx, Lx, precision="float64", **kw)
     qr, pr, _ = ref.step(ref.new_state(q0), nsteps=3, graph=False)
     mod = QGModel(N, N, Lx, Lx, precision="mixed", **kw)
-    mod.set_preconditioner(net, out_scale)
+    mod.set_preconditioner(net, out_scale, SWEEPS)
     q, p, info = mod.step(mod.new_state(q0), nsteps=3, graph=False)
     assert info["converged"] and info["max_rel_residual"] <= 1e-8 * 1.0001, info
     print(f"[learned] QG 3 steps: {info}, fallbacks {mod.fallbacks}", flush=True)

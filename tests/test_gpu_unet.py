@@ -1,8 +1,14 @@
 """Learned preconditioner kernels against the U-Net oracle (oracle/unet.py).
 
 Single conv: bf16-exact inputs and weights, FP32 accumulation -> within one
-bf16 ulp of the oracle (rounded to bf16).  Whole network: bf16 activations
-between layers -> 3e-2 relative (max-norm); weight gradients 5e-2.
+bf16 ulp of the oracle (rounded to bf16).  Whole network: against the oracle
+run at the same storage precision (``bf16=True``: activations and activation
+gradients rounded to bfloat16 where the device stores them; matrix-core
+layers with bf16 weights, first/last layers FP32): output within 5e-3 and
+weight gradients within 2e-2 (max-norm relative), leaving room only for FP32
+accumulation order flipping a stored value by one bf16 ulp.  Against the
+exact FP64 network the same bf16 storage alone moves deep-layer gradients by
+up to ~20 %, so the exact comparison is not a useful gate.
 """
 
 import numpy as np
@@ -43,19 +49,21 @@ def test_forward_and_gradients_match_oracle(channels):
     need_gpu()
     H = W = 64
     net = Lm.UNet(channels, H, W, seed=1)
-    Wt = {k: Lm.to_bf16(v).astype(np.float64) for k, v in net.get_weights().items()}
+    # matrix-core layers run on bf16 weights; the 1-channel edge layers on FP32
+    Wt = {k: (v if k in ("e0a", "out") else Lm.to_bf16(v).astype(np.float64))
+          for k, v in net.get_weights().items()}
     rng = np.random.default_rng(2)
     x = Lm.to_bf16(rng.standard_normal((H, W))).astype(np.float64)
-    ref, cache = U.forward(Wt, x, channels)
+    ref, cache = U.forward(Wt, x, channels, bf16=True)
     got = net.forward(torch.tensor(x, dtype=torch.float32)).cpu().numpy()
     err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
-    assert err < 3e-2, err
+    assert err < 5e-3, err
     gy = rng.standard_normal((H, W))
-    gref = U.backward(Wt, cache, gy, channels)
+    gref = U.backward(Wt, cache, gy, channels, bf16=True)
     g = net.grads(torch.tensor(x, dtype=torch.float32), torch.tensor(gy, dtype=torch.float32))
     for k in gref:
         e = np.max(np.abs(g[k] - gref[k])) / max(np.max(np.abs(gref[k])), 1e-30)
-        assert e < 5e-2, (k, e)
+        assert e < 2e-2, (k, e)
 
 
 def test_training_reduces_loss():

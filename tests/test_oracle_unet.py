@@ -67,3 +67,21 @@ def test_adam_first_step_is_signed_lr():
     v = {"a": np.zeros(3)}
     U.adam_step(Wt, g, m, v, 1, lr=0.1)
     np.testing.assert_allclose(Wt["a"], [1.0 - 0.1, -2.0 + 0.1, 3.0], atol=1e-7)
+
+
+def test_bf16_rounding_and_storage_mode():
+    # ties to even at 1 + 2^-8 (halfway between 1 and 1 + 2^-7), up above it
+    vals = np.array([1.0, 1 + 2.0**-8, 1 + 3 * 2.0**-8, 1 + 2.0**-8 + 2.0**-12, -2.5, 0.0])
+    np.testing.assert_array_equal(U.bf16_round(vals),
+                                  [1.0, 1.0, 1 + 2.0**-6, 1 + 2.0**-7, -2.5, 0.0])
+    ch = (16, 32)
+    Wt = U.init_weights(ch, seed=5)
+    r = np.random.default_rng(6).standard_normal((16, 16))
+    y, _ = U.forward(Wt, r, ch)
+    yb, cache = U.forward(Wt, r, ch, bf16=True)
+    assert 0 < np.max(np.abs(yb - y)) <= 3e-2 * np.max(np.abs(y))
+    # every stored activation is representable in bf16
+    for k, v in cache.items():
+        for t in (v if isinstance(v, tuple) else (v,)):
+            if k != "e0" or t is not v[0]:
+                np.testing.assert_array_equal(U.bf16_round(t), t)
