@@ -54,24 +54,21 @@ __device__ __forceinline__ T mval(const uint8_t* m, int64_t k) {
 }
 
 // ---------------------------------------------------------------------------
-// Reductions finished by the last workgroup, in two levels so that no
-// counter takes more than ~nblk/64 atomics (same-address device atomics
-// serialise; one counter per 16k-block grid cost ~0.4 ms).  Block b belongs
-// to group b % G (G = min(nblk, kRedGroups)); the last block of each group
-// sums that group's partials in block order, the last group finisher sums the
-// G group totals in group order -> deterministic.
-// partials: [kRedGroups][NV] group totals, then [nblk][NV] block partials.
-// tk: kRedGroups + 1 counters (tk[0] top level), left at zero on exit.
+// Reductions: every block reduces its values in registers and LDS and writes
+// one partial per value with plain stores; a one-workgroup finishing kernel
+// launched right after sums the partials in block order (deterministic) and
+// updates the solver scalars on the device.  The kernel boundary orders the
+// partial stores before their reads, so no per-block device-scope fence is
+// needed (on this GPU such a fence writes back the XCD's L2: with one per
+// block a 16k-block reduction cost ~0.4 ms).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int lin_tid() {
   return threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
 }
 
 template <int NV, int kThreads>
-__device__ bool block_reduce_last(const double (&v)[NV], double* partials, unsigned int* tk,
-                                  double (&out)[NV]) {
+__device__ __forceinline__ void block_partials(const double (&v)[NV], double* partials) {
   __shared__ double red[kThreads / 64][NV];
-  __shared__ int stage;
   const int tid = lin_tid();
   const int lane = tid & 63, wid = tid >> 6;
 #pragma unroll
@@ -80,72 +77,92 @@ __device__ bool block_reduce_last(const double (&v)[NV], double* partials, unsig
     if (lane == 0) red[wid][k] = s;
   }
   __syncthreads();
-  const int nblk = gridDim.x * gridDim.y;
-  const int bid = blockIdx.y * gridDim.x + blockIdx.x;
-  const int G = nblk < kRedGroups ? nblk : kRedGroups;
-  const int g = bid % G;
-  const int gsize = nblk / G + (g < nblk % G ? 1 : 0);
-  double* gpart = partials;
-  double* bpart = partials + kRedGroups * NV;
   if (tid == 0) {
+    const int64_t bid = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
       double s = 0.0;
       for (int w = 0; w < kThreads / 64; ++w) s += red[w][k];
-      bpart[(int64_t)bid * NV + k] = s;
+      partials[bid * NV + k] = s;
     }
-    __threadfence();
-    const unsigned int prev = atomicAdd(&tk[1 + g], 1u);
-    stage = (prev == (unsigned int)gsize - 1) ? 1 : 0;
   }
-  __syncthreads();
-  if (!stage) return false;
-  __threadfence();
-  // Last block of group g: its blocks are g, g + G, g + 2G, ...
+}
+
+// Reduction sites (which scalars the finishing kernel updates).
+enum RedSite : int {
+  kRedUpDots = 0,  // z.r, z.q_prev -> flexible beta, rho
+  kRedInit = 1,    // ||b||^2, ||r0||^2 -> tolerance, flags
+  kRedDir = 2,     // p.Ap -> alpha
+  kRedUpdate = 3,  // ||r||^2 -> convergence
+  kRedRefine = 4,  // FP64 refinement residual
+  kRedDots = 5,    // as kRedUpDots, from the separate dots kernel
+  kRedTrue = 6,    // FP64 certification residual
+};
+constexpr int kFinishThreads = 1024;
+
+template <int NV, int SITE>
+__global__ __launch_bounds__(kFinishThreads) void reduce_finish_kernel(
+    const double* __restrict__ partials, int nblk, PcgScalars* sc, double rtol2, double atol2) {
+  // The same guards as the producing kernels.
+  if constexpr (SITE == kRedRefine) {
+    if (sc->out_done) return;
+  } else if constexpr (SITE != kRedInit && SITE != kRedTrue) {
+    if (sc->done) return;
+  }
+  __shared__ double red[kFinishThreads / 64][NV];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  double acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+#pragma unroll 8
+  for (int b = tid; b < nblk; b += kFinishThreads) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] += partials[(int64_t)b * NV + k];
+  }
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
-    double s = 0.0;
-    for (int m = tid; m < gsize; m += kThreads)
-      s += __builtin_nontemporal_load(&bpart[(int64_t)(g + m * G) * NV + k]);
-    s = wave_sum(s);
-    __syncthreads();
+    const double s = wave_sum(acc[k]);
     if (lane == 0) red[wid][k] = s;
   }
   __syncthreads();
-  if (tid == 0) {
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-      double s = 0.0;
-      for (int w = 0; w < kThreads / 64; ++w) s += red[w][k];
-      gpart[g * NV + k] = s;
-    }
-    tk[1 + g] = 0u;  // every block of the group has counted
-    __threadfence();
-    const unsigned int prev = atomicAdd(&tk[0], 1u);
-    stage = (prev == (unsigned int)G - 1) ? 2 : 0;
-  }
-  __syncthreads();
-  if (stage != 2) return false;
-  __threadfence();
+  if (tid != 0) return;
+  double out[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
     double s = 0.0;
-    for (int m = tid; m < G; m += kThreads) s += __builtin_nontemporal_load(&gpart[m * NV + k]);
-    s = wave_sum(s);
-    __syncthreads();
-    if (lane == 0) red[wid][k] = s;
+    for (int w = 0; w < kFinishThreads / 64; ++w) s += red[w][k];
+    out[k] = s;
   }
-  __syncthreads();
-  if (tid == 0) {
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-      double s = 0.0;
-      for (int w = 0; w < kThreads / 64; ++w) s += red[w][k];
-      out[k] = s;
-    }
-    tk[0] = 0u;  // re-arm for the next launch
+  if constexpr (SITE == kRedUpDots || SITE == kRedDots) {
+    // Flexible (Polak-Ribiere) beta: z_{k+1}.(r_{k+1} - r_k) / rho_k with
+    // r_{k+1} - r_k = -alpha q_k.
+    sc->beta = sc->first ? 0.0 : (-sc->alpha * out[1]) / sc->rz;
+    sc->rz = out[0];
+  } else if constexpr (SITE == kRedInit) {
+    sc->bb = out[0];
+    sc->rr = out[1];
+    sc->tol2 = fmax(rtol2 * out[0], atol2);
+    sc->done = out[1] <= sc->tol2 ? 1 : 0;
+    sc->first = 1;
+    sc->iter = 0;
+    sc->total = 0;
+    sc->alpha = 0.0;
+    sc->beta = 0.0;
+  } else if constexpr (SITE == kRedDir) {
+    sc->pq = out[0];
+    sc->alpha = sc->rz / out[0];
+    sc->first = 0;
+  } else if constexpr (SITE == kRedUpdate) {
+    sc->rr = out[0];
+    sc->iter += 1;
+    sc->total += 1;
+    if (out[0] <= sc->tol2) sc->done = 1;
+  } else {  // kRedRefine, kRedTrue
+    sc->bb = out[0];
+    sc->out_rr = out[1];
+    sc->out_tol2 = fmax(rtol2 * out[0], atol2);
+    sc->out_done = out[1] <= sc->out_tol2 ? 1 : 0;
   }
-  return true;  // out[] valid in thread 0
 }
 
 // ---------------------------------------------------------------------------
@@ -382,16 +399,8 @@ __global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
     }
   }
   if constexpr (MODE == kUpDots) {
-    double v[2] = {d0, d1}, out[2];
-    if (block_reduce_last<2, kTileThreads>(v, a.partials, a.sc->ticket[0], out)) {
-      if (lin_tid() == 0) {
-        PcgScalars* sc = a.sc;
-        // Flexible (Polak-Ribiere) beta: z_{k+1}.(r_{k+1} - r_k) / rho_k with
-        // r_{k+1} - r_k = -alpha q_k.
-        sc->beta = sc->first ? 0.0 : (-sc->alpha * out[1]) / sc->rz;
-        sc->rz = out[0];
-      }
-    }
+    double v[2] = {d0, d1};
+    block_partials<2, kTileThreads>(v, a.partials);
   }
 }
 
@@ -517,12 +526,6 @@ __global__ __launch_bounds__(256) void residual_kernel(LevelDev<T> L, const T* _
   res[k] = (b[k] - glob_Au<T, KIND>(L, u, j, i)) * mval<T>(L.mask, k);
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void add_into_kernel(int n, const T* __restrict__ a,
-                                                       T* __restrict__ b) {
-  const int k = blockIdx.x * 256 + threadIdx.x;
-  if (k < n) b[k] += a[k];
-}
 
 template <typename T>
 __global__ __launch_bounds__(256) void mask_copy_kernel(LevelDev<T> L, const T* __restrict__ in,
@@ -580,20 +583,8 @@ __global__ __launch_bounds__(256) void pcg_init_kernel(LevelDev<T> L, const T* _
     bb = (double)bv * bv;
     rr = (double)rv * rv;
   }
-  double v[2] = {bb, rr}, out[2];
-  if (block_reduce_last<2, 256>(v, partials, sc->ticket[1], out)) {
-    if (lin_tid() == 0) {
-      sc->bb = out[0];
-      sc->rr = out[1];
-      sc->tol2 = fmax(rtol2 * out[0], atol2);
-      sc->done = out[1] <= sc->tol2 ? 1 : 0;
-      sc->first = 1;
-      sc->iter = 0;
-      sc->total = 0;
-      sc->alpha = 0.0;
-      sc->beta = 0.0;
-    }
-  }
+  double v[2] = {bb, rr};
+  block_partials<2, 256>(v, partials);
 }
 
 // PCG: p_out = z + beta p_in ; q = A p_out ; partial p.q -> alpha = rho / pq.
@@ -629,14 +620,8 @@ __global__ __launch_bounds__(256) void pcg_dir_kernel(LevelDev<T> L, const T* __
     q[k] = qv;
     pq = (double)c * qv;
   }
-  double v[1] = {pq}, out[1];
-  if (block_reduce_last<1, 256>(v, partials, sc->ticket[2], out)) {
-    if (lin_tid() == 0) {
-      sc->pq = out[0];
-      sc->alpha = sc->rz / out[0];
-      sc->first = 0;
-    }
-  }
+  double v[1] = {pq};
+  block_partials<1, 256>(v, partials);
 }
 
 // PCG: x += alpha p ; r -= alpha q ; ||r||^2 -> convergence flag.
@@ -657,15 +642,8 @@ __global__ __launch_bounds__(256) void pcg_update_kernel(int ny, int nx, T* __re
     r[k] = rv;
     rr = (double)rv * rv;
   }
-  double v[1] = {rr}, out[1];
-  if (block_reduce_last<1, 256>(v, partials, sc->ticket[3], out)) {
-    if (lin_tid() == 0) {
-      sc->rr = out[0];
-      sc->iter += 1;
-      sc->total += 1;
-      if (out[0] <= sc->tol2) sc->done = 1;
-    }
-  }
+  double v[1] = {rr};
+  block_partials<1, 256>(v, partials);
 }
 
 // Mixed precision: r32 = (b64 - A64 x64) m in FP64, ||r||^2 and ||b||^2 in
@@ -705,15 +683,8 @@ __global__ __launch_bounds__(256) void refine_residual_kernel(
     bb = bv * bv;
     rr = rv * rv;
   }
-  double v[2] = {bb, rr}, out[2];
-  if (block_reduce_last<2, 256>(v, partials, sc->ticket[4], out)) {
-    if (lin_tid() == 0) {
-      sc->bb = out[0];
-      sc->out_rr = out[1];
-      sc->out_tol2 = fmax(rtol2 * out[0], atol2);
-      sc->out_done = out[1] <= sc->out_tol2 ? 1 : 0;
-    }
-  }
+  double v[2] = {bb, rr};
+  block_partials<2, 256>(v, partials);
 }
 
 __global__ __launch_bounds__(256) void refine_update_kernel(int ny, int nx, double* __restrict__ x,
@@ -758,19 +729,12 @@ __global__ __launch_bounds__(256) void true_residual_kernel(LevelDev<T> L, const
     bb = bv * bv;
     rr = rv * rv;
   }
-  double v[2] = {bb, rr}, out[2];
-  if (block_reduce_last<2, 256>(v, partials, res->ticket[6], out)) {
-    if (lin_tid() == 0) {
-      res->bb = out[0];
-      res->out_rr = out[1];
-      res->out_tol2 = fmax(rtol2 * out[0], atol2);
-      res->out_done = out[1] <= res->out_tol2 ? 1 : 0;
-    }
-  }
+  double v[2] = {bb, rr};
+  block_partials<2, 256>(v, partials);
 }
 
 // Clear the flags and counters of a scalar block (graph-safe replacement for
-// a memset; tickets are left armed).
+// a memset).
 __global__ void scalars_reset_kernel(PcgScalars* sc, int n) {
   const int k = threadIdx.x;
   if (k < n) {
@@ -804,7 +768,8 @@ struct Plan {
   PcgScalars* sc = nullptr;   // kSlots pairs: [2k] inner, [2k+1] outer
   PcgScalars* hsc = nullptr;  // pinned host mirror
   int last_iters = 4, last_outer = 2, last_inner = 4;
-  float* work0[3] = {nullptr, nullptr, nullptr};  // fine-level FP32 work (lazy)
+  float* work0[4] = {nullptr, nullptr, nullptr, nullptr};  // fine-level FP32 work (lazy)
+  double* dscal = nullptr;  // device scalars of the learned correction (lazy)
   std::vector<void*> allocs;
 };
 
@@ -1001,6 +966,18 @@ static void launch_tile(const Plan& P, const LevelDev<T>& L, const LevelDev<T>* 
   a.ntiles = a.tiles_x * cdiv(L.ny, kTy);
   hipLaunchKernelGGL((mg_tile_kernel<T, KIND, NS, MODE, ZERO>), dim3(a.ntiles), dim3(kTileThreads),
                      0, s, a);
+  if constexpr (MODE == kUpDots)
+    hipLaunchKernelGGL((reduce_finish_kernel<2, kRedUpDots>), dim3(1), dim3(kFinishThreads), 0, s,
+                       P.partials, a.ntiles, sc, 0.0, 0.0);
+}
+
+// Finishing kernel of a reduction over the blocks of grid2(ny, nx).
+template <int NV, int SITE>
+static void finish2(const Plan& P, int ny, int nx, PcgScalars* sc, hipStream_t s,
+                    double rtol2 = 0.0, double atol2 = 0.0) {
+  const int nblk = cdiv(nx, 64) * cdiv(ny, 4);
+  hipLaunchKernelGGL((reduce_finish_kernel<NV, SITE>), dim3(1), dim3(kFinishThreads), 0, s,
+                     P.partials, nblk, sc, rtol2, atol2);
 }
 
 // nu Jacobi sweeps on level L, up to 2 per kernel.  Ping-pongs between the
@@ -1044,13 +1021,8 @@ __global__ __launch_bounds__(256) void pcg_dots_kernel(int ny, int nx, const T* 
     d0 = (double)z[k] * r[k];
     if (!sc->first) d1 = (double)z[k] * q[k];
   }
-  double v[2] = {d0, d1}, out[2];
-  if (block_reduce_last<2, 256>(v, partials, sc->ticket[5], out)) {
-    if (lin_tid() == 0) {
-      sc->beta = sc->first ? 0.0 : (-sc->alpha * out[1]) / sc->rz;
-      sc->rz = out[0];
-    }
-  }
+  double v[2] = {d0, d1};
+  block_partials<2, 256>(v, partials);
 }
 
 // One V-cycle z = V(u0 or 0, rhs) at level 0.  Returns 1 in *fused when the
@@ -1190,14 +1162,18 @@ static int pcg_iters(Plan& P, Hier<T>& H, T* x, int64_t ld_x, int n, PcgScalars*
       rc = v_cycle_impl<T, KIND>(P, nullptr, H.r, H.z, true, H.q, sc, s, &fused);
     }
     if (rc) return rc;
-    if (!fused)
+    if (!fused) {
       hipLaunchKernelGGL(pcg_dots_kernel<T>, grid2(L.ny, L.nx), dim3(64, 4), 0, s, L.ny, L.nx,
                          H.z, H.r, H.q, P.partials, sc);
+      finish2<2, kRedDots>(P, L.ny, L.nx, sc, s);
+    }
     hipLaunchKernelGGL((pcg_dir_kernel<T, KIND>), grid2(L.ny, L.nx), dim3(64, 4), 0, s, L, H.z,
                        H.p0, H.p1, H.q, P.partials, sc);
+    finish2<1, kRedDir>(P, L.ny, L.nx, sc, s);
     std::swap(H.p0, H.p1);
     hipLaunchKernelGGL(pcg_update_kernel<T>, grid2(L.ny, L.nx), dim3(64, 4), 0, s, L.ny, L.nx, x,
                        ld_x, H.p0, H.r, H.q, P.partials, sc);
+    finish2<1, kRedUpdate>(P, L.ny, L.nx, sc, s);
   }
   FVX_LAUNCH_CHECK();
   return kOk;
@@ -1229,6 +1205,7 @@ int pcg_enqueue(Plan& P, const T* rhs, int64_t ld_rhs, T* x, int64_t ld_x, doubl
     hipLaunchKernelGGL((pcg_init_kernel<T, kNoflux>), grid2(L.ny, L.nx), dim3(64, 4), 0, s, L,
                        rhs, ld_rhs, use_x0 ? x : nullptr, ld_x, H.r, rtol * rtol, atol * atol,
                        P.partials, sc);
+  finish2<2, kRedInit>(P, L.ny, L.nx, sc, s, rtol * rtol, atol * atol);
   FVX_LAUNCH_CHECK();
   if (iters <= 0) return kOk;
   return (P.kind == kDirichlet) ? pcg_iters<T, kDirichlet>(P, H, x, ld_x, iters, sc, s)
@@ -1248,6 +1225,7 @@ int certify_enqueue(Plan& P, const T* rhs, int64_t ld_rhs, const T* x, int64_t l
   else
     hipLaunchKernelGGL((true_residual_kernel<T, kNoflux>), grid2(L.ny, L.nx), dim3(64, 4), 0, s,
                        L, rhs, ld_rhs, x, ld_x, rtol * rtol, atol * atol, P.partials, res);
+  finish2<2, kRedTrue>(P, L.ny, L.nx, res, s, rtol * rtol, atol * atol);
   FVX_LAUNCH_CHECK();
   return kOk;
 }
@@ -1311,6 +1289,7 @@ int mixed_enqueue(Plan& P, const TB* rhs, int64_t ld_rhs, double* x, int64_t ld_
       hipLaunchKernelGGL((refine_residual_kernel<kNoflux, TB>), grid2(L64.ny, L64.nx),
                          dim3(64, 4), 0, s, L64, rhs, ld_rhs, x, ld_x, H.r, H.x, rtol * rtol,
                          atol * atol, P.partials, out);
+    finish2<2, kRedRefine>(P, L64.ny, L64.nx, out, s, rtol * rtol, atol * atol);
   };
   for (int o = 0; o < n_outer; ++o) {
     residual();
@@ -1366,6 +1345,7 @@ int mixed_solve(Plan& P, const TB* rhs, int64_t ld_rhs, double* x, int64_t ld_x,
         hipLaunchKernelGGL((refine_residual_kernel<kNoflux, TB>), grid2(L64.ny, L64.nx),
                            dim3(64, 4), 0, s, L64, rhs, ld_rhs, x, ld_x, H.r, H.x, rtol * rtol,
                            atol * atol, P.partials, out);
+      finish2<2, kRedRefine>(P, L64.ny, L64.nx, out, s, rtol * rtol, atol * atol);
       FVX_LAUNCH_CHECK();
     }
     if (rc) return rc;
@@ -1405,9 +1385,67 @@ int mixed_solve(Plan& P, const TB* rhs, int64_t ld_rhs, double* x, int64_t ld_x,
   return kOk;
 }
 
-// Fine-level (FP32) helpers for preconditioners built around the smoother.
+// Learned two-grid preconditioner on the FP32 hierarchy (see mg.hpp).
+template <int KIND>
+__global__ __launch_bounds__(256) void energy_dots_kernel(LevelDev<float> L,
+                                                          const float* __restrict__ c,
+                                                          const float* __restrict__ res,
+                                                          double* partials,
+                                                          const int* __restrict__ guard) {
+  if (guard && *guard) return;
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+  double d0 = 0.0, d1 = 0.0;
+  if (i < L.nx && j < L.ny) {
+    const int64_t k = (int64_t)j * L.nx + i;
+    const double cv = (double)c[k];
+    d0 = (double)res[k] * cv;
+    d1 = cv * (double)glob_Au<float, KIND>(L, c, j, i);
+  }
+  double v[2] = {d0, d1};
+  block_partials<2, 256>(v, partials);
+}
+
+// alpha = <res, c> / <c, A c>: the multiple of c that minimises the energy
+// norm of the error (A definite of either sign).
+__global__ __launch_bounds__(kFinishThreads) void energy_finish_kernel(
+    const double* __restrict__ partials, int nblk, double* alpha, const int* __restrict__ guard) {
+  if (guard && *guard) return;
+  __shared__ double red[kFinishThreads / 64][2];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  double a0 = 0.0, a1 = 0.0;
+#pragma unroll 8
+  for (int b = tid; b < nblk; b += kFinishThreads) {
+    a0 += partials[2 * (int64_t)b];
+    a1 += partials[2 * (int64_t)b + 1];
+  }
+  a0 = wave_sum(a0);
+  a1 = wave_sum(a1);
+  if (lane == 0) {
+    red[wid][0] = a0;
+    red[wid][1] = a1;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double s0 = 0.0, s1 = 0.0;
+    for (int w = 0; w < kFinishThreads / 64; ++w) {
+      s0 += red[w][0];
+      s1 += red[w][1];
+    }
+    *alpha = s1 != 0.0 ? s0 / s1 : 0.0;
+  }
+}
+
+__global__ __launch_bounds__(256) void add_scaled_kernel(int n, const double* __restrict__ alpha,
+                                                         const float* __restrict__ c,
+                                                         float* __restrict__ z,
+                                                         const int* __restrict__ guard) {
+  if (guard && *guard) return;
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k < n) z[k] += (float)(*alpha) * c[k];
+}
+
 float* fine_work(Plan& P, int k) {
-  if (k < 0 || k >= 3) return nullptr;
+  if (k < 0 || k >= 4) return nullptr;
   if (!P.work0[k]) {
     const auto& L = P.h32.lv[0];
     void* q = nullptr;
@@ -1415,45 +1453,81 @@ float* fine_work(Plan& P, int k) {
     P.allocs.push_back(q);
     P.work0[k] = (float*)q;
   }
+  if (!P.dscal) {
+    void* q = nullptr;
+    if (hipMalloc(&q, 8 * sizeof(double)) != hipSuccess) return nullptr;
+    P.allocs.push_back(q);
+    P.dscal = (double*)q;
+  }
   return P.work0[k];
 }
 
-int fine_presmooth(Plan& P, const float* r, float* z, float* res, int sweeps, hipStream_t s) {
+int learned_coarse_dims(const Plan& P, int* nyc, int* nxc) {
+  if (!P.has32 || P.nl < 2) return kInvalidArgument;
+  *nyc = P.h32.lv[1].ny;
+  *nxc = P.h32.lv[1].nx;
+  return kOk;
+}
+
+template <int KIND>
+static int learned_two_grid_impl(Plan& P, CoarseNetFn fn, void* ctx, int sweeps, const float* r,
+                                 float* z, const int* guard, hipStream_t s) {
   const auto& L = P.h32.lv[0];
-  if (sweeps <= 0) {
+  LevelDev<float> C = P.h32.lv[1];
+  const int n = L.ny * L.nx;
+  float* z0 = fine_work(P, 0);
+  float* res = fine_work(P, 1);
+  float* c = fine_work(P, 2);
+  float* cw = fine_work(P, 3);
+  if (!z0 || !res || !c || !cw) {
+    set_error("allocation of the preconditioner work arrays failed");
+    return kHipError;
+  }
+  float* rc = cw;
+  float* zc = cw + (size_t)C.ny * C.nx;  // level 1 holds a quarter of level 0
+  // z0 = S^sweeps(0; r), res = (r - A z0) m
+  if (sweeps > 0) {
+    smooth_chunks<float, KIND>(P, L, nullptr, true, r, sweeps, L.t, L.t2, z0, nullptr, s);
+    hipLaunchKernelGGL((residual_kernel<float, KIND>), grid2(L.ny, L.nx), dim3(64, 4), 0, s, L,
+                       z0, r, res);
+  } else {
     hipLaunchKernelGGL(mask_copy_kernel<float>, grid2(L.ny, L.nx), dim3(64, 4), 0, s, L, nullptr,
-                       z, (int64_t)L.nx);
+                       z0, (int64_t)L.nx);
     hipLaunchKernelGGL(mask_copy_kernel<float>, grid2(L.ny, L.nx), dim3(64, 4), 0, s, L, r, res,
                        (int64_t)L.nx);
-  } else if (P.kind == kDirichlet) {
-    smooth_chunks<float, kDirichlet>(P, L, nullptr, true, r, sweeps, L.t, L.t2, z, nullptr, s);
-    hipLaunchKernelGGL((residual_kernel<float, kDirichlet>), grid2(L.ny, L.nx), dim3(64, 4), 0, s,
-                       L, z, r, res);
-  } else {
-    smooth_chunks<float, kNoflux>(P, L, nullptr, true, r, sweeps, L.t, L.t2, z, nullptr, s);
-    hipLaunchKernelGGL((residual_kernel<float, kNoflux>), grid2(L.ny, L.nx), dim3(64, 4), 0, s, L,
-                       z, r, res);
   }
+  // coarse correction c = P net(R res), scaled by the energy-minimising alpha
+  hipLaunchKernelGGL(restrict_kernel<float>, grid2(C.ny, C.nx), dim3(64, 4), 0, s, L, C, res, rc);
+  FVX_LAUNCH_CHECK();
+  int rc_ = fn(ctx, rc, zc, C.ny, C.nx, C.mask, guard, s);
+  if (rc_) return rc_;
+  C.u = zc;
+  hipLaunchKernelGGL((prolong_kernel<float, KIND>), grid2(L.ny, L.nx), dim3(64, 4), 0, s, L, C, c);
+  hipLaunchKernelGGL(energy_dots_kernel<KIND>, grid2(L.ny, L.nx), dim3(64, 4), 0, s, L, c, res,
+                     P.partials, guard);
+  hipLaunchKernelGGL(energy_finish_kernel, dim3(1), dim3(kFinishThreads), 0, s, P.partials,
+                     cdiv(L.nx, 64) * cdiv(L.ny, 4), P.dscal, guard);
+  hipLaunchKernelGGL(add_scaled_kernel, dim3(cdiv(n, 256)), dim3(256), 0, s, n, P.dscal, c, z0,
+                     guard);
+  // post-smoothing from z0 into z
+  if (sweeps > 0)
+    smooth_chunks<float, KIND>(P, L, z0, false, r, sweeps, L.t, L.t2, z, nullptr, s);
+  else
+    hipLaunchKernelGGL(mask_copy_kernel<float>, grid2(L.ny, L.nx), dim3(64, 4), 0, s, L, z0, z,
+                       (int64_t)L.nx);
   FVX_LAUNCH_CHECK();
   return kOk;
 }
 
-int fine_postsmooth(Plan& P, float* z_in, const float* add, const float* r, float* z_out,
-                    int sweeps, hipStream_t s) {
-  const auto& L = P.h32.lv[0];
-  const int n = L.ny * L.nx;
-  if (add)
-    hipLaunchKernelGGL(add_into_kernel<float>, dim3(cdiv(n, 256)), dim3(256), 0, s, n, add, z_in);
-  if (sweeps <= 0) {
-    hipLaunchKernelGGL(mask_copy_kernel<float>, grid2(L.ny, L.nx), dim3(64, 4), 0, s, L, z_in,
-                       z_out, (int64_t)L.nx);
-  } else if (P.kind == kDirichlet) {
-    smooth_chunks<float, kDirichlet>(P, L, z_in, false, r, sweeps, L.t, L.t2, z_out, nullptr, s);
-  } else {
-    smooth_chunks<float, kNoflux>(P, L, z_in, false, r, sweeps, L.t, L.t2, z_out, nullptr, s);
+int learned_two_grid(Plan& P, CoarseNetFn fn, void* ctx, int sweeps, const float* r, float* z,
+                     const int* guard, hipStream_t s) {
+  if (!P.has32 || P.nl < 2) {
+    set_error("the learned coarse correction needs an FP32 hierarchy with at least two levels");
+    return kInvalidArgument;
   }
-  FVX_LAUNCH_CHECK();
-  return kOk;
+  return P.kind == kDirichlet
+             ? learned_two_grid_impl<kDirichlet>(P, fn, ctx, sweeps, r, z, guard, s)
+             : learned_two_grid_impl<kNoflux>(P, fn, ctx, sweeps, r, z, guard, s);
 }
 
 // Profiling hook support: a scalar block that never converges, so the
@@ -1607,7 +1681,7 @@ int fvx_mg_plan_create(const double* mask, const double* coeff, int ny, int nx, 
   }
   if (!rc) {
     const size_t maxblk = (size_t)cdiv(nx, 64) * cdiv(ny, 4) + 64;
-    hipError_t e = hipMalloc(&P->partials, (maxblk + kRedGroups) * 4 * sizeof(double));
+    hipError_t e = hipMalloc(&P->partials, maxblk * 4 * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&P->sc, 2 * kSlots * sizeof(PcgScalars));
     if (e == hipSuccess) e = hipMemset(P->sc, 0, 2 * kSlots * sizeof(PcgScalars));
     if (e == hipSuccess)

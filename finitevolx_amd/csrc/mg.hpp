@@ -6,9 +6,6 @@
 
 namespace fvx {
 
-// Counters per reduction site: one top level + one per block group.
-constexpr int kRedGroups = 64;
-
 // Device-side scalars of one PCG solve (one "slot" holds an inner and an
 // outer block).
 struct PcgScalars {
@@ -28,7 +25,6 @@ struct PcgScalars {
   int outer;      // outer iterations done
   int total;      // inner iterations summed over refinement steps
   int pad[2];
-  unsigned int ticket[8][kRedGroups + 1];  // last-block counters per reduction site
 };
 
 struct CgInfoC {
@@ -73,14 +69,20 @@ double plan_inv_diag0(const Plan& P);
 const PcgScalars* fetch_slots(Plan& P, int first_slot, int n, hipStream_t s);
 int plan_levels(const Plan& P);
 bool plan_has_mixed(const Plan& P);
-// Fine-level FP32 helpers (level 0 of the FP32 hierarchy, contiguous fields):
-// lazily allocated work arrays k = 0..2; ``sweeps`` damped-Jacobi sweeps on
-// A z = r from zero with res = (r - A z) mask; and sweeps from z_in (+ add)
-// into z_out.  z_in is overwritten when ``add`` is given.
+// Learned two-grid preconditioner on the FP32 hierarchy, z = M(r):
+//   z0 = S^sweeps(0; r), res = r - A z0          (damped-Jacobi smoother S)
+//   c  = Prolong(net(Restrict(res)))              (network on level 1)
+//   z1 = z0 + alpha c, alpha = <res, c> / <c, A c> (energy-minimising)
+//   z  = S^sweeps(z1; r)
+// ``fn`` applies the network to a level-1 field (contiguous, nyc x nxc).
+typedef int (*CoarseNetFn)(void* ctx, const float* rc, float* zc, int nyc, int nxc,
+                           const uint8_t* maskc, const int* guard, hipStream_t s);
+int learned_two_grid(Plan& P, CoarseNetFn fn, void* ctx, int sweeps, const float* r, float* z,
+                     const int* guard, hipStream_t s);
+// Level-1 dimensions of the FP32 hierarchy (the network's grid).
+int learned_coarse_dims(const Plan& P, int* nyc, int* nxc);
+// Lazily allocated fine-level FP32 work arrays k = 0..3.
 float* fine_work(Plan& P, int k);
-int fine_presmooth(Plan& P, const float* r, float* z, float* res, int sweeps, hipStream_t s);
-int fine_postsmooth(Plan& P, float* z_in, const float* add, const float* r, float* z_out,
-                    int sweeps, hipStream_t s);
 int plan_nx(const Plan& P);
 int plan_ny(const Plan& P);
 // Learned preconditioner application (precond.hip); sweeps as in

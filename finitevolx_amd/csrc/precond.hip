@@ -1,7 +1,8 @@
 // Preconditioner choice for the mixed-precision PCG Helmholtz solve: the
 // multigrid V-cycle (the reference's make_multigrid_preconditioner,
-// solvers/preconditioners.py:86-134), the learned U-Net, or the diagonal
-// (Jacobi, z = r / diag(A) on wet cells).  The diagonal option is the
+// solvers/preconditioners.py:86-134), the learned U-Net (as the coarse solve
+// of a two-grid cycle), or the diagonal (Jacobi, z = r / diag(A) on wet
+// cells).  The diagonal option is the
 // baseline a learned preconditioner has to beat; the V-cycle is the one it
 // has to match.
 
@@ -32,26 +33,24 @@ struct PrecondCtx {
   int sweeps;
 };
 
-// Learned preconditioner.  With sweeps = 0: z = N(r).  With sweeps > 0 the
-// network is the smooth-error correction between damped-Jacobi sweeps (a
-// two-grid cycle whose coarse solve is learned):
-//   z0 = S^nu(0; r),  z1 = z0 + N(r - A z0),  z = S^nu(z1; r).
+struct NetCtx {
+  void* net;
+  double scale;
+};
+
+static int coarse_net(void* ctx, const float* rc, float* zc, int nyc, int nxc,
+                      const uint8_t* maskc, const int* guard, hipStream_t s) {
+  NetCtx* c = (NetCtx*)ctx;
+  return unet_apply<float>(c->net, rc, nxc, zc, nxc, nyc, nxc, c->scale, maskc, s, guard);
+}
+
+// Learned preconditioner: two-grid cycle whose coarse solve is the network
+// (damped-Jacobi pre/post sweeps on the fine level, network on level 1,
+// energy-minimising scaling of the coarse correction).
 int learned_precond_apply(Plan& P, void* net, double scale, int sweeps, const float* r, float* z,
                           const int* guard, hipStream_t s) {
-  const int nx = plan_nx(P), ny = plan_ny(P);
-  if (sweeps <= 0) return unet_apply<float>(net, r, nx, z, nx, ny, nx, scale, plan_mask0(P), s,
-                                            guard);
-  float* z0 = fine_work(P, 0);
-  float* res = fine_work(P, 1);
-  float* zn = fine_work(P, 2);
-  if (!z0 || !res || !zn) {
-    set_error("allocation of the preconditioner work arrays failed");
-    return kHipError;
-  }
-  int rc = fine_presmooth(P, r, z0, res, sweeps, s);
-  if (!rc) rc = unet_apply<float>(net, res, nx, zn, nx, ny, nx, scale, plan_mask0(P), s, guard);
-  if (!rc) rc = fine_postsmooth(P, zn, z0, r, z, sweeps, s);
-  return rc;
+  NetCtx ctx{net, scale};
+  return learned_two_grid(P, &coarse_net, &ctx, sweeps, r, z, guard, s);
 }
 
 static int learned_fn(void* ctx, const float* r, float* z, int ny, int nx, const int* guard,
@@ -77,10 +76,9 @@ extern "C" {
 
 // Mixed-precision PCG (FP32 inner flexible PCG, FP64 refinement) with the
 // inner preconditioner chosen by ``precond``: 0 multigrid V-cycle, 1 learned
-// U-Net ``net`` (z = out_scale rms(r) net(r / rms(r)) on wet cells;
-// out_scale <= 0 selects 1 / diag(A); with sweeps > 0 the network corrects
-// the smooth error between ``sweeps`` damped-Jacobi pre- and post-sweeps),
-// 2 diagonal.  info->pad returns the
+// two-grid cycle (``sweeps`` damped-Jacobi pre/post sweeps; coarse solve
+// out_scale rms(rc) net(rc / rms(rc)) on level 1, scaled to minimise the
+// energy norm), 2 diagonal (out_scale <= 0 selects 1 / diag(A)).  info->pad returns the
 // number of learned/diagonal inner solves that hit max_inner and were
 // replaced by the V-cycle.
 int fvx_mg_precond_solve(void* plan, int precond, void* net, double out_scale, int sweeps,
@@ -92,6 +90,9 @@ int fvx_mg_precond_solve(void* plan, int precond, void* net, double out_scale, i
   FVX_CHECK_ARG(precond >= 0 && precond <= 2, "precond must be 0 (multigrid), 1 (learned) or 2 "
                                               "(diagonal)");
   FVX_CHECK_ARG(precond != 1 || net, "learned preconditioner needs a network");
+  int nyc = 0, nxc = 0;
+  FVX_CHECK_ARG(precond != 1 || learned_coarse_dims(*(Plan*)plan, &nyc, &nxc) == 0,
+                "the learned preconditioner needs a mixed plan with at least two levels");
   Plan* P = (Plan*)plan;
   FVX_CHECK_ARG(plan_has_mixed(*P), "mixed solve needs a plan built with precision 3264");
   FVX_CHECK_ARG(max_outer >= 1 && max_inner >= 1, "iteration caps must be >= 1");

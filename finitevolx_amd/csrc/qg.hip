@@ -590,8 +590,11 @@ int fvx_qg_set_preconditioner(void* plan, void* net, double out_scale, int sweep
   FVX_CHECK_ARG(net == nullptr || Q->solve == kSolveMixed,
                 "a learned preconditioner needs the mixed-precision solve");
   FVX_CHECK_ARG(sweeps >= 0 && sweeps <= 8, "sweeps must be in [0, 8]");
-  if (net && sweeps > 0) {  // allocate the work arrays outside any graph capture
-    for (int k = 0; k < 3; ++k)
+  if (net) {  // allocate the work arrays outside any graph capture
+    int nyc = 0, nxc = 0;
+    FVX_CHECK_ARG(learned_coarse_dims(*Q->mg, &nyc, &nxc) == 0,
+                  "the learned preconditioner needs at least two multigrid levels");
+    for (int k = 0; k < 4; ++k)
       FVX_CHECK_ARG(fine_work(*Q->mg, k) != nullptr, "work array allocation failed");
   }
   Q->net = net;

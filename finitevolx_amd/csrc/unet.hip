@@ -384,17 +384,15 @@ __global__ void poolbwd_add_mask_kernel(const uint16_t* __restrict__ skip,
 // ---------------------------------------------------------------------------
 // Input normalisation, loss, Adam.
 // ---------------------------------------------------------------------------
-// s = rms of the masked residual (FP64 accumulate, last-block finish):
+// s = rms of the residual (FP64 accumulate): per-block partials, then a
+// one-workgroup finish (kernel boundary in between, no device fences):
 // scal[0] = 1/s (input scale), scal[1] = s * out_scale (output scale).
 template <typename T>
 __global__ __launch_bounds__(256) void rms_kernel(const T* __restrict__ r, int64_t ld, int H, int W,
                                                    double* __restrict__ part,
-                                                   unsigned int* __restrict__ ticket,
-                                                   float* __restrict__ scal, double out_scale,
                                                    const int* __restrict__ guard) {
   if (guard && *guard) return;
   __shared__ double red[256];
-  __shared__ bool last;
   const int tid = threadIdx.x;
   double acc = 0.0;
   const int64_t N = (int64_t)H * W;
@@ -408,21 +406,28 @@ __global__ __launch_bounds__(256) void rms_kernel(const T* __restrict__ r, int64
     if (tid < s) red[tid] += red[tid + s];
     __syncthreads();
   }
-  if (tid == 0) {
-    part[blockIdx.x] = red[0];
-    __threadfence();
-    last = atomicAdd(ticket, 1u) == gridDim.x - 1;
-  }
+  if (tid == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void rms_finish_kernel(const double* __restrict__ part, int nb,
+                                                          int64_t N, float* __restrict__ scal,
+                                                          double out_scale,
+                                                          const int* __restrict__ guard) {
+  if (guard && *guard) return;
+  __shared__ double red[256];
+  const int tid = threadIdx.x;
+  double acc = 0.0;
+  for (int b = tid; b < nb; b += 256) acc += part[b];
+  red[tid] = acc;
   __syncthreads();
-  if (!last) return;
-  __threadfence();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (tid < s) red[tid] += red[tid + s];
+    __syncthreads();
+  }
   if (tid == 0) {
-    double s = 0.0;
-    for (int b = 0; b < (int)gridDim.x; ++b) s += __builtin_nontemporal_load(&part[b]);
-    const double rms = sqrt(s / (double)N);
+    const double rms = sqrt(red[0] / (double)N);
     scal[0] = rms > 0.0 ? (float)(1.0 / rms) : 0.f;
     scal[1] = (float)(rms * out_scale);
-    *ticket = 0u;
   }
 }
 
@@ -496,7 +501,6 @@ struct UNet {
   float* part = nullptr;
   size_t part_elems = 0;
   double* rms_part = nullptr;
-  unsigned int* ticket = nullptr;
   float* scal = nullptr;
   double* loss = nullptr;
   const int* guard = nullptr;  // PCG convergence flag: kernels exit when set
@@ -769,7 +773,6 @@ int fvx_unet_create(int nlev, const int* channels, int H, int W, void** out) {
   N->part_elems = (size_t)64 * 9 * 64 * 64;
   rc |= dev_alloc(*N, &N->part, N->part_elems);
   rc |= dev_alloc(*N, &N->rms_part, 4096);
-  rc |= dev_alloc(*N, &N->ticket, 4);
   rc |= dev_alloc(*N, &N->scal, 4);
   rc |= dev_alloc(*N, &N->loss, 1);
   if (!rc) {
@@ -781,7 +784,6 @@ int fvx_unet_create(int nlev, const int* channels, int H, int W, void** out) {
     hipError_t e = hipMemset(N->g, 0, off * sizeof(float));
     if (e == hipSuccess) e = hipMemset(N->m, 0, off * sizeof(float));
     if (e == hipSuccess) e = hipMemset(N->v, 0, off * sizeof(float));
-    if (e == hipSuccess) e = hipMemset(N->ticket, 0, 4 * sizeof(unsigned int));
     if (e != hipSuccess) rc = kHipError;
   }
   if (rc) {
@@ -917,7 +919,9 @@ int unet_apply(void* net, const T* r, int64_t ld_r, T* z, int64_t ld_z, int H, i
                 "field size not divisible by 2^(levels-1)");
   const int nb = 256;
   hipLaunchKernelGGL(rms_kernel<T>, dim3(nb), dim3(256), 0, s, r, ld_r, H, W, N->rms_part,
-                     N->ticket, N->scal, out_scale, N->guard);
+                     N->guard);
+  hipLaunchKernelGGL(rms_finish_kernel, dim3(1), dim3(256), 0, s, N->rms_part, nb,
+                     (int64_t)H * W, N->scal, out_scale, N->guard);
   hipLaunchKernelGGL(scale_in_kernel<T>, g2(H, W), dim3(64, 4), 0, s, r, ld_r, H, W, N->scal,
                      N->rin, N->guard);
   int rc = unet_forward(*N, H, W, s, false);

@@ -174,20 +174,23 @@ def conv3x3_bf16(x: torch.Tensor, w: np.ndarray, ups: bool = False, relu: bool =
 
 
 def generate_pairs(mg, rhs_fields, vcycles=(1, 2, 3), sweeps: int = 2, rtol=1e-11):
-    """(residual, error) training pairs generated by the multigrid solver.
+    """Coarse-level (residual, error) training pairs generated by the
+    multigrid solver, for the network's place in the learned two-grid cycle.
 
     For each right-hand side b: the converged solution x* (multigrid-PCG to
     ``rtol``) and the iterates x_k of k plain V-cycles from zero (k in
-    ``vcycles``, plus x_0 = 0), giving residual/error pairs (r_k = b - A x_k,
-    e_k = x* - x_k).  With ``sweeps`` > 0 each pair is then taken through the
-    pre-smoothing the preconditioner applies before the network:
-    z = S^sweeps(0; r_k), pair (r_k - A z, e_k - z).  ``mg`` is an FP64
-    ``MultigridSolver`` built with the same operator and omega as the solve.
-    Returns FP32 device tensors (list of residuals, list of errors) and the
+    ``vcycles``, plus x_0 = 0) give residual/error pairs (r_k = b - A x_k,
+    e_k = x* - x_k).  Each pair is taken through the cycle's pre-smoothing,
+    z = S^sweeps(0; r_k), res = r_k - A z, e = e_k - z, and restricted to
+    level 1 with the solver's own restriction: (R res, R e).  ``mg`` is an
+    FP64 ``MultigridSolver`` with the same operator and omega as the solve.
+    Returns FP32 device tensors (coarse residuals, coarse errors) and the
     output scale (median rms(error) / rms(residual)) to train and apply with.
     """
-    from finitevolx_amd.solvers import apply_operator, weighted_jacobi
+    from finitevolx_amd.solvers import apply_operator, restrict, weighted_jacobi
 
+    if mg.n_levels < 2:
+        raise ValueError("the learned two-grid cycle needs at least two multigrid levels")
     lev0 = mg.levels[0]
     rs, es, ratios = [], [], []
     for b in rhs_fields:
@@ -208,12 +211,14 @@ def generate_pairs(mg, rhs_fields, vcycles=(1, 2, 3), sweeps: int = 2, rtol=1e-1
                 z = weighted_jacobi(None, r, lev0, mg.omega, sweeps)
                 e = e - z
                 r = r - apply_operator(z, lev0)
-            rn = float(torch.sqrt(torch.mean(r * r)))
+            rc = restrict(r, lev0)
+            ec = restrict(e, lev0)
+            rn = float(torch.sqrt(torch.mean(rc * rc)))
             if rn == 0.0:
                 continue
-            ratios.append(float(torch.sqrt(torch.mean(e * e))) / rn)
-            rs.append(r.float())
-            es.append(e.float())
+            ratios.append(float(torch.sqrt(torch.mean(ec * ec))) / rn)
+            rs.append(rc.float())
+            es.append(ec.float())
     return rs, es, float(np.median(ratios))
 
 
@@ -259,11 +264,12 @@ def fit_preconditioner(mask, dx: float, dy: float, lambda_: float, rhs_fields, *
                        vcycles=(1, 2, 3)):
     """Train a U-Net preconditioner for one Helmholtz operator on the GPU.
 
-    Builds the FP64 multigrid solver of the operator, generates (residual,
-    error) pairs from it on ``rhs_fields`` and trains the network with this
-    build's kernels.  Returns ``(net, out_scale, loss_history)``; pass the
-    network, ``out_scale`` and the same ``sweeps`` to
-    ``MultigridSolver.precond_solve`` or ``QGModel.set_preconditioner``.
+    Builds the FP64 multigrid solver of the operator, generates coarse
+    (residual, error) pairs from it on ``rhs_fields`` and trains the network
+    (sized for multigrid level 1) with this build's kernels.  Returns
+    ``(net, out_scale, loss_history)``; pass the network, ``out_scale`` and
+    the same ``sweeps`` to ``MultigridSolver.precond_solve`` or
+    ``QGModel.set_preconditioner``.
     """
     from finitevolx_amd.solvers import build_multigrid_solver
 
@@ -273,7 +279,7 @@ def fit_preconditioner(mask, dx: float, dy: float, lambda_: float, rhs_fields, *
     rs, es, out_scale = generate_pairs(mg, [np.asarray(f) * m for f in rhs_fields],
                                        vcycles=vcycles, sweeps=sweeps)
     del mg
-    H, W = m.shape
+    H, W = rs[0].shape
     net = UNet(channels, H, W, seed=seed)
     crop = min(crop, H, W)
     hist = train(net, rs, es, out_scale, steps=steps, crop=crop, batch=batch, lr=lr,

@@ -105,11 +105,12 @@ class QGModel:
         return (self.ny + 2, self.nx + 2)
 
     def set_preconditioner(self, net=None, out_scale: float = 0.0, sweeps: int = 2):
-        """Use a trained ``learned.UNet`` for the inner FP32 solves (mixed
-        precision only), between ``sweeps`` damped-Jacobi pre- and
-        post-sweeps; ``None`` restores the multigrid V-cycle."""
-        if net is not None and (net.H < self.ny or net.W < self.nx):
-            raise ValueError("the network plan is smaller than the grid")
+        """Precondition the inner FP32 solves (mixed precision only) with the
+        learned two-grid cycle: ``sweeps`` damped-Jacobi pre/post sweeps and a
+        trained ``learned.UNet`` (sized for the half-resolution grid) as the
+        coarse solve; ``None`` restores the multigrid V-cycle."""
+        if net is not None and (net.H < self.ny // 2 or net.W < self.nx // 2):
+            raise ValueError("the network plan is smaller than the half-resolution grid")
         self._net = net  # keep the plan alive while the model uses it
         _lib.call("fvx_qg_set_preconditioner", self._h, None if net is None else net._h,
                   float(out_scale), int(sweeps))

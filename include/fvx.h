@@ -189,11 +189,14 @@ int fvx_mg_mixed_solve(void* plan, const double* rhs, int64_t ld_rhs, double* x,
  * 0 multigrid V-cycle (make_multigrid_preconditioner,
  * solvers/preconditioners.py:86-134), 1 learned U-Net ``net`` (from
  * fvx_unet_create; no reference counterpart), 2 diagonal (Jacobi).
- * out_scale <= 0 selects 1 / diag(A).  For the learned preconditioner,
- * sweeps > 0 makes it a two-grid cycle with a learned coarse correction:
- * z0 = S^sweeps(0; r), z1 = z0 + net(r - A z0), z = S^sweeps(z1; r), S the
- * damped-Jacobi smoother of the plan.  info->pad counts inner solves that
- * hit max_inner and were replaced by the V-cycle.                         */
+ * out_scale <= 0 selects 1 / diag(A).  The learned preconditioner is a
+ * two-grid cycle whose coarse solve is the network on multigrid level 1:
+ * z0 = S^sweeps(0; r), res = r - A z0, c = Prolong(net(Restrict(res))),
+ * z1 = z0 + alpha c with alpha = <res, c> / <c, A c>, z = S^sweeps(z1; r),
+ * S the damped-Jacobi smoother of the plan; the network is applied as
+ * out_scale rms(rc) net(rc / rms(rc)) and its plan must cover level 1.
+ * info->pad counts inner solves that hit max_inner and were replaced by
+ * the V-cycle.                                                            */
 int fvx_mg_precond_solve(void* plan, int precond, void* net, double out_scale, int sweeps,
                          const double* rhs, int64_t ld_rhs, double* x, int64_t ld_x, double rtol,
                          double atol, int max_outer, int max_inner, double inner_rtol,
@@ -240,8 +243,8 @@ int fvx_qg_plan_destroy(void* plan);
 int fvx_qg_step(void* plan, void* q, double* psi, int nsteps, fvx_qg_info* info, void* stream);
 /* Learned U-Net preconditioner for the inner FP32 solves of a mixed plan
  * (NULL restores multigrid; no reference counterpart); out_scale <= 0 =
- * inverse Jacobi diagonal; sweeps > 0 wraps the network between that many
- * damped-Jacobi pre- and post-sweeps (as in fvx_mg_precond_solve).
+ * inverse Jacobi diagonal; the network is the coarse solve of the two-grid
+ * cycle described at fvx_mg_precond_solve, with ``sweeps`` pre/post sweeps.
  * Multigrid-PCG takes over when a learned inner solve hits its cap;
  * fvx_qg_fallbacks counts those events. */
 int fvx_qg_set_preconditioner(void* plan, void* net, double out_scale, int sweeps);

@@ -1,13 +1,15 @@
 """GPU: the learned U-Net preconditioner inside mixed-precision flexible PCG.
 
-The network is trained on (residual, error) pairs that the multigrid solver
-produces for the QG Helmholtz operator (lap - 1/Ld^2, Ld = 16 cells, zero
-streamfunction on a basin coastline).  Checks:
+The network is the coarse solve of a two-grid cycle (damped-Jacobi sweeps on
+the fine grid, network on the half-resolution grid, energy-minimising scaling
+of its correction), trained on coarse (residual, error) pairs that the
+multigrid solver produces for the QG Helmholtz operator (lap - 1/Ld^2,
+Ld = 16 cells, zero streamfunction on a basin coastline).  Checks:
 
 * the learned-PCG solve certifies its FP64 true residual at rtol 1e-8 and
   matches the multigrid-PCG solution to 1e-7 (max-norm relative);
 * gain: it needs fewer inner iterations than diagonal (Jacobi) PCG and than
-  the same network before training, with no fallback to the V-cycle;
+  the same cycle with the untrained network, with no fallback to the V-cycle;
 * QGModel with the learned preconditioner tracks the FP64 model.
 
 The network has no reference counterpart; its kernels are checked against
@@ -16,7 +18,6 @@ oracle/unet.py in test_gpu_unet.py.
 
 import numpy as np
 import pytest
-import torch
 
 from finitevolx_amd import learned, solvers as S
 from finitevolx_amd.inputs import basin_mask, spectral_field
@@ -45,10 +46,9 @@ def trained():
                                     n_coarse=20, omega=0.8)
     rhs = [spectral_field(N, N, 3.0, 100 + s) * m for s in range(8)]
     rs, es, out_scale = learned.generate_pairs(mg64, rhs, vcycles=(1, 2, 3), sweeps=SWEEPS)
-    net = learned.UNet(H=N, W=N, seed=0)
-    untrained = learned.UNet(H=N, W=N, seed=0)
-    torch.manual_seed(0)
-    hist = learned.train(net, rs, es, out_scale, steps=400, crop=64, batch=8, lr=2e-3,
+    net = learned.UNet(H=N // 2, W=N // 2, seed=0)
+    untrained = learned.UNet(H=N // 2, W=N // 2, seed=0)
+    hist = learned.train(net, rs, es, out_scale, steps=800, crop=32, batch=8, lr=2e-3,
                          lr_final=2e-4, seed=0)
     return net, untrained, hist, out_scale
 
