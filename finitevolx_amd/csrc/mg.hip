@@ -446,14 +446,49 @@ __device__ void cw_sweeps(const LevelDev<T>& L, int n, bool zero, T om) {
   }
 }
 
-template <typename T, int KIND>
+// Level l of the coarse kernel with its iterate, scratch and right-hand side
+// in LDS (u, t, b at cell offset off), coefficients and masks in global memory.
+template <typename T, bool LDS>
+__device__ __forceinline__ LevelDev<T> lds_level(const CoarseArgs<T>& a, int l, T* sm, int off) {
+  LevelDev<T> L = a.lv[l];
+  if constexpr (!LDS) return L;
+  const int n = L.ny * L.nx;
+  L.u = sm + off;
+  L.t = sm + off + n;
+  L.b = sm + off + 2 * n;
+  return L;
+}
+
+// LDS = false keeps every level in global memory (bottom levels too large
+// for LDS: grids that stop halving early).
+template <typename T, int KIND, bool LDS>
 __global__ __launch_bounds__(kCoarseThreads) void mg_coarse_kernel(CoarseArgs<T> a) {
   if (a.done && *a.done) return;
+  extern __shared__ __align__(16) unsigned char coarse_smem[];
+  T* sm = reinterpret_cast<T*>(coarse_smem);
+  int off[16];
+  {
+    int o = 0;
+    for (int l = a.lc; l < a.nl; ++l) {
+      off[l] = o;
+      o += 3 * a.lv[l].ny * a.lv[l].nx;
+    }
+  }
   const T om = a.omega;
+  // Level lc's right-hand side (and guess) from global memory.
+  if constexpr (LDS) {
+    const LevelDev<T>& G = a.lv[a.lc];
+    const LevelDev<T> L = lds_level<T, LDS>(a, a.lc, sm, off[a.lc]);
+    for (int k = threadIdx.x; k < G.ny * G.nx; k += kCoarseThreads) {
+      L.b[k] = G.b[k];
+      if (a.nonzero_init) L.u[k] = G.u[k];
+    }
+    __syncthreads();
+  }
   // Down leg.
   for (int l = a.lc; l < a.nl - 1; ++l) {
-    const LevelDev<T>& L = a.lv[l];
-    const LevelDev<T>& C = a.lv[l + 1];
+    const LevelDev<T> L = lds_level<T, LDS>(a, l, sm, off[l]);
+    const LevelDev<T> C = lds_level<T, LDS>(a, l + 1, sm, off[l + 1]);
     const bool zero = !(a.nonzero_init && l == a.lc);
     if (a.n_pre > 0) {
       cw_sweeps<T, KIND>(L, a.n_pre, zero, om);
@@ -478,10 +513,10 @@ __global__ __launch_bounds__(kCoarseThreads) void mg_coarse_kernel(CoarseArgs<T>
     }
     __syncthreads();
   }
-  // Bottom solve: n_coarse sweeps from zero (from lv[lc].u when the whole
+  // Bottom solve: n_coarse sweeps from zero (from the guess when the whole
   // hierarchy is one level and a guess was given).
   {
-    const LevelDev<T>& B = a.lv[a.nl - 1];
+    const LevelDev<T> B = lds_level<T, LDS>(a, a.nl - 1, sm, off[a.nl - 1]);
     const bool zero = !(a.nonzero_init && a.lc == a.nl - 1);
     if (a.n_coarse > 0) {
       cw_sweeps<T, KIND>(B, a.n_coarse, zero, om);
@@ -492,15 +527,58 @@ __global__ __launch_bounds__(kCoarseThreads) void mg_coarse_kernel(CoarseArgs<T>
   }
   // Up leg.
   for (int l = a.nl - 2; l >= a.lc; --l) {
-    const LevelDev<T>& L = a.lv[l];
+    const LevelDev<T> L = lds_level<T, LDS>(a, l, sm, off[l]);
+    const LevelDev<T> C = lds_level<T, LDS>(a, l + 1, sm, off[l + 1]);
     for (int k = threadIdx.x; k < L.ny * L.nx; k += kCoarseThreads) {
       const int j = k / L.nx, i = k % L.nx;
       const T m = mval<T>(L.mask, k);
-      L.u[k] = (L.u[k] + prolong_at<T, KIND>(a.lv[l + 1], j, i) * m) * m;
+      L.u[k] = (L.u[k] + prolong_at<T, KIND>(C, j, i) * m) * m;
     }
     __syncthreads();
     cw_sweeps<T, KIND>(L, a.n_post, false, om);
   }
+  // Level lc's corrected iterate back to global memory.
+  if constexpr (LDS) {
+    const LevelDev<T>& G = a.lv[a.lc];
+    const LevelDev<T> L = lds_level<T, LDS>(a, a.lc, sm, off[a.lc]);
+    for (int k = threadIdx.x; k < G.ny * G.nx; k += kCoarseThreads) G.u[k] = L.u[k];
+  }
+}
+
+// Dynamic LDS of the coarse kernel for levels lc .. nl-1 (u, t, b each).
+template <typename T>
+size_t coarse_lds_bytes(const CoarseArgs<T>& a) {
+  size_t n = 0;
+  for (int l = a.lc; l < a.nl; ++l) n += 3 * (size_t)a.lv[l].ny * a.lv[l].nx;
+  return n * sizeof(T);
+}
+constexpr size_t kCoarseLdsMax = 160 * 1024;
+
+template <typename T, int KIND>
+static void launch_coarse(const CoarseArgs<T>& ca, hipStream_t s) {
+  const size_t lds = coarse_lds_bytes(ca);
+  if (lds <= kCoarseLdsMax)
+    hipLaunchKernelGGL((mg_coarse_kernel<T, KIND, true>), dim3(1), dim3(kCoarseThreads), lds, s,
+                       ca);
+  else
+    hipLaunchKernelGGL((mg_coarse_kernel<T, KIND, false>), dim3(1), dim3(kCoarseThreads), 0, s,
+                       ca);
+}
+
+// Allow the LDS-resident coarse kernels the full 160 KB (once per process).
+static void coarse_lds_attributes() {
+  static bool done = false;
+  if (done) return;
+  done = true;
+  const int big = (int)kCoarseLdsMax;
+  (void)hipFuncSetAttribute((const void*)mg_coarse_kernel<float, kDirichlet, true>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, big);
+  (void)hipFuncSetAttribute((const void*)mg_coarse_kernel<float, kNoflux, true>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, big);
+  (void)hipFuncSetAttribute((const void*)mg_coarse_kernel<double, kDirichlet, true>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, big);
+  (void)hipFuncSetAttribute((const void*)mg_coarse_kernel<double, kNoflux, true>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, big);
 }
 
 // ---------------------------------------------------------------------------
@@ -588,43 +666,74 @@ __global__ __launch_bounds__(256) void pcg_init_kernel(LevelDev<T> L, const T* _
 }
 
 // PCG: p_out = z + beta p_in ; q = A p_out ; partial p.q -> alpha = rho / pq.
+// Each wave owns a 64-column strip of kDirRows rows and slides a three-row
+// register window down it (every row of z, p read once per strip); east/west
+// neighbours come from lane shuffles, the strip's edge columns from memory.
+constexpr int kDirRows = 8;
+static inline dim3 grid_dir(int ny, int nx) { return dim3(cdiv(nx, 64), cdiv(ny, 4 * kDirRows)); }
+
 template <typename T, int KIND>
 __global__ __launch_bounds__(256) void pcg_dir_kernel(LevelDev<T> L, const T* __restrict__ z,
                                                       const T* __restrict__ p_in,
                                                       T* __restrict__ p_out, T* __restrict__ q,
                                                       double* partials, PcgScalars* sc) {
   if (sc->done) return;
-  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+  const int lane = threadIdx.x;
+  const int i = blockIdx.x * 64 + lane;
+  const int j0 = (blockIdx.y * 4 + threadIdx.y) * kDirRows;  // uniform within a wave
   const bool first = sc->first;
   const T beta = first ? T(0) : (T)sc->beta;
+  const int nx = L.nx, ny = L.ny;
+  auto P = [&](int j, int ii) -> T {
+    if (j < 0 || j >= ny || ii < 0 || ii >= nx) return T(0);
+    const int64_t k = (int64_t)j * nx + ii;
+    return first ? z[k] : z[k] + beta * p_in[k];
+  };
+  auto CY = [&](int j) -> T {  // south-face coefficient of row j + 1 is cy[j]
+    return (KIND == kNoflux && j >= 0 && j < ny && i < nx) ? L.cy[(int64_t)j * nx + i] : T(0);
+  };
   double pq = 0.0;
-  if (i < L.nx && j < L.ny) {
-    const int64_t k = (int64_t)j * L.nx + i;
-    auto P = [&](int64_t kk) { return first ? z[kk] : z[kk] + beta * p_in[kk]; };
-    const T m = mval<T>(L.mask, k);
-    const T c = P(k);
-    const T e = (i + 1 < L.nx) ? P(k + 1) : T(0);
-    const T w = (i > 0) ? P(k - 1) : T(0);
-    const T n = (j + 1 < L.ny) ? P(k + L.nx) : T(0);
-    const T s = (j > 0) ? P(k - L.nx) : T(0);
+  T ps = P(j0 - 1, i), pc = P(j0, i);
+  T cys = CY(j0 - 1);
+  for (int r = 0; r < kDirRows; ++r) {
+    const int j = j0 + r;
+    if (j >= ny) break;
+    const T pn = P(j + 1, i);
+    T w = __shfl_up(pc, 1, 64);
+    T e = __shfl_down(pc, 1, 64);
+    if (lane == 0) w = P(j, i - 1);
+    if (lane == 63) e = P(j, i + 1);
     T lap;
+    T cyn = T(0);
     if constexpr (KIND == kDirichlet) {
-      lap = (e + w - T(2) * c) * L.idx2 + (n + s - T(2) * c) * L.idy2;
+      lap = (e + w - T(2) * pc) * L.idx2 + (pn + ps - T(2) * pc) * L.idy2;
     } else {
-      const T cxe = L.cx[k], cxw = (i > 0) ? L.cx[k - 1] : T(0);
-      const T cyn = L.cy[k], cys = (j > 0) ? L.cy[k - L.nx] : T(0);
-      lap = (cxe * (e - c) - cxw * (c - w)) * L.idx2 + (cyn * (n - c) - cys * (c - s)) * L.idy2;
+      const int64_t k = (int64_t)j * nx + i;
+      const T cxe = (i < nx) ? L.cx[k] : T(0);
+      T cxw = __shfl_up(cxe, 1, 64);
+      if (lane == 0) cxw = (i > 0 && i - 1 < nx) ? L.cx[k - 1] : T(0);
+      if (i == 0) cxw = T(0);
+      cyn = CY(j);
+      lap = (cxe * (e - pc) - cxw * (pc - w)) * L.idx2 + (cyn * (pn - pc) - cys * (pc - ps)) * L.idy2;
     }
-    const T qv = (lap - L.lam * c) * m;
-    p_out[k] = c;
-    q[k] = qv;
-    pq = (double)c * qv;
+    if (i < nx) {
+      const int64_t k = (int64_t)j * nx + i;
+      const T qv = (lap - L.lam * pc) * mval<T>(L.mask, k);
+      p_out[k] = pc;
+      q[k] = qv;
+      pq += (double)pc * qv;
+    }
+    ps = pc;
+    pc = pn;
+    cys = cyn;
   }
   double v[1] = {pq};
   block_partials<1, 256>(v, partials);
 }
 
-// PCG: x += alpha p ; r -= alpha q ; ||r||^2 -> convergence flag.
+// PCG: x += alpha p ; r -= alpha q ; ||r||^2 -> convergence flag.  Each wave
+// updates a 64-column strip of kDirRows rows (grid_dir), so the finishing
+// reduction has as few partials as the direction kernel's.
 template <typename T>
 __global__ __launch_bounds__(256) void pcg_update_kernel(int ny, int nx, T* __restrict__ x,
                                                          int64_t ld_x, const T* __restrict__ p,
@@ -632,15 +741,22 @@ __global__ __launch_bounds__(256) void pcg_update_kernel(int ny, int nx, T* __re
                                                          const T* __restrict__ q,
                                                          double* partials, PcgScalars* sc) {
   if (sc->done) return;
-  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int j0 = (blockIdx.y * 4 + threadIdx.y) * kDirRows;
   const T alpha = (T)sc->alpha;
   double rr = 0.0;
-  if (i < nx && j < ny) {
-    const int64_t k = (int64_t)j * nx + i;
-    x[(int64_t)j * ld_x + i] += alpha * p[k];
-    const T rv = r[k] - alpha * q[k];
-    r[k] = rv;
-    rr = (double)rv * rv;
+  if (i < nx) {
+#pragma unroll
+    for (int t = 0; t < kDirRows; ++t) {
+      const int j = j0 + t;
+      if (j < ny) {
+        const int64_t k = (int64_t)j * nx + i;
+        x[(int64_t)j * ld_x + i] += alpha * p[k];
+        const T rv = r[k] - alpha * q[k];
+        r[k] = rv;
+        rr += (double)rv * rv;
+      }
+    }
   }
   double v[1] = {rr};
   block_partials<1, 256>(v, partials);
@@ -1061,7 +1177,7 @@ static int v_cycle_impl(Plan& P, const T* u0, const T* rhs, T* out, bool want_do
     ca.nonzero_init = u0 != nullptr;
     ca.omega = (T)P.omega;
     ca.done = sc ? &sc->done : nullptr;
-    hipLaunchKernelGGL((mg_coarse_kernel<T, KIND>), dim3(1), dim3(kCoarseThreads), 0, s, ca);
+    launch_coarse<T, KIND>(ca, s);
     FVX_HIP(hipMemcpyAsync(out, L.u, bytes, hipMemcpyDeviceToDevice, s));
     FVX_LAUNCH_CHECK();
     return kOk;
@@ -1102,7 +1218,7 @@ static int v_cycle_impl(Plan& P, const T* u0, const T* rhs, T* out, bool want_do
     ca.n_coarse = P.n_coarse;
     ca.omega = (T)P.omega;
     ca.done = sc ? &sc->done : nullptr;
-    hipLaunchKernelGGL((mg_coarse_kernel<T, KIND>), dim3(1), dim3(kCoarseThreads), 0, s, ca);
+    launch_coarse<T, KIND>(ca, s);
   }
   // Up leg over the tiled levels lc-1 .. 0.  cur = array holding level l+1's
   // corrected iterate; each up kernel reads L.u and writes a different array.
@@ -1167,13 +1283,17 @@ static int pcg_iters(Plan& P, Hier<T>& H, T* x, int64_t ld_x, int n, PcgScalars*
                          H.z, H.r, H.q, P.partials, sc);
       finish2<2, kRedDots>(P, L.ny, L.nx, sc, s);
     }
-    hipLaunchKernelGGL((pcg_dir_kernel<T, KIND>), grid2(L.ny, L.nx), dim3(64, 4), 0, s, L, H.z,
-                       H.p0, H.p1, H.q, P.partials, sc);
-    finish2<1, kRedDir>(P, L.ny, L.nx, sc, s);
+    hipLaunchKernelGGL((pcg_dir_kernel<T, KIND>), grid_dir(L.ny, L.nx), dim3(64, 4), 0, s, L,
+                       H.z, H.p0, H.p1, H.q, P.partials, sc);
+    hipLaunchKernelGGL((reduce_finish_kernel<1, kRedDir>), dim3(1), dim3(kFinishThreads), 0, s,
+                       P.partials, (int)(grid_dir(L.ny, L.nx).x * grid_dir(L.ny, L.nx).y), sc,
+                       0.0, 0.0);
     std::swap(H.p0, H.p1);
-    hipLaunchKernelGGL(pcg_update_kernel<T>, grid2(L.ny, L.nx), dim3(64, 4), 0, s, L.ny, L.nx, x,
-                       ld_x, H.p0, H.r, H.q, P.partials, sc);
-    finish2<1, kRedUpdate>(P, L.ny, L.nx, sc, s);
+    hipLaunchKernelGGL(pcg_update_kernel<T>, grid_dir(L.ny, L.nx), dim3(64, 4), 0, s, L.ny, L.nx,
+                       x, ld_x, H.p0, H.r, H.q, P.partials, sc);
+    hipLaunchKernelGGL((reduce_finish_kernel<1, kRedUpdate>), dim3(1), dim3(kFinishThreads), 0, s,
+                       P.partials, (int)(grid_dir(L.ny, L.nx).x * grid_dir(L.ny, L.nx).y), sc,
+                       0.0, 0.0);
   }
   FVX_LAUNCH_CHECK();
   return kOk;
@@ -1556,11 +1676,11 @@ static int kernel_bench_impl(Plan& P, int which, int reps, hipStream_t s) {
   }
   for (int k = 0; k < reps; ++k) {
     if (which == 0)
-      hipLaunchKernelGGL((pcg_dir_kernel<T, KIND>), grid2(L.ny, L.nx), dim3(64, 4), 0, s, L, H.z,
-                         H.p0, H.p1, H.q, P.partials, sc);
+      hipLaunchKernelGGL((pcg_dir_kernel<T, KIND>), grid_dir(L.ny, L.nx), dim3(64, 4), 0, s, L,
+                         H.z, H.p0, H.p1, H.q, P.partials, sc);
     else
-      hipLaunchKernelGGL(pcg_update_kernel<T>, grid2(L.ny, L.nx), dim3(64, 4), 0, s, L.ny, L.nx,
-                         H.x, (int64_t)L.nx, H.p0, H.r, H.q, P.partials, sc);
+      hipLaunchKernelGGL(pcg_update_kernel<T>, grid_dir(L.ny, L.nx), dim3(64, 4), 0, s, L.ny,
+                         L.nx, H.x, (int64_t)L.nx, H.p0, H.r, H.q, P.partials, sc);
   }
   FVX_LAUNCH_CHECK();
   return kOk;
@@ -1660,6 +1780,7 @@ int fvx_mg_plan_create(const double* mask, const double* coeff, int ny, int nx, 
   auto hl = build_host_levels(mask, coeff, ny, nx, dx, dy, lambda_, nl, kind);
   // First level small enough for the single-workgroup kernel (all below too).
   // (one-level hierarchies run entirely there; otherwise level 0 is tiled).
+  coarse_lds_attributes();
   P->lc = nl;
   for (int l = 0; l < nl; ++l)
     if ((int64_t)hl[l].ny * hl[l].nx <= kCoarseCells) {
