@@ -305,7 +305,12 @@ __global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
   const int i0 = (t % a.tiles_x) * kTx, j0 = (t / a.tiles_x) * kTy;
   const int tid = threadIdx.x;
 
-  for (int k = tid; k < N; k += kTileThreads) {
+  // Fully unrolled (the trip count is a compile-time constant) so the
+  // global loads of all iterations are in flight together.
+#pragma unroll
+  for (int it = 0; it < (N + kTileThreads - 1) / kTileThreads; ++it) {
+    const int k = tid + it * kTileThreads;
+    if (k >= N) break;
     const int y = k / W, x = k % W;
     const int gj = j0 - H + y, gi = i0 - H + x;
     const bool in = (gj >= 0 && gj < L.ny && gi >= 0 && gi < L.nx);
@@ -384,6 +389,7 @@ __global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
 
   // Write the tile interior.
   double d0 = 0.0, d1 = 0.0;
+#pragma unroll
   for (int k = tid; k < kTx * kTy; k += kTileThreads) {
     const int y = H + k / kTx, x = H + k % kTx;
     const int gj = j0 + k / kTx, gi = i0 + k % kTx;
