@@ -103,12 +103,11 @@ constexpr int kFinishThreads = 1024;
 template <int NV, int SITE>
 __global__ __launch_bounds__(kFinishThreads) void reduce_finish_kernel(
     const double* __restrict__ partials, int nblk, PcgScalars* sc, double rtol2, double atol2) {
-  // The same guards as the producing kernels.
-  if constexpr (SITE == kRedRefine) {
-    if (sc->out_done) return;
-  } else if constexpr (SITE != kRedInit && SITE != kRedTrue) {
-    if (sc->done) return;
-  }
+  // The same guards as the producing kernels, read together with the
+  // partials so that the two memory round trips overlap.
+  int skip = 0;
+  if constexpr (SITE == kRedRefine) skip = sc->out_done;
+  else if constexpr (SITE != kRedInit && SITE != kRedTrue) skip = sc->done;
   __shared__ double red[kFinishThreads / 64][NV];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   double acc[NV];
@@ -119,6 +118,7 @@ __global__ __launch_bounds__(kFinishThreads) void reduce_finish_kernel(
 #pragma unroll
     for (int k = 0; k < NV; ++k) acc[k] += partials[(int64_t)b * NV + k];
   }
+  if (skip) return;
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
     const double s = wave_sum(acc[k]);
@@ -290,7 +290,9 @@ __device__ __forceinline__ T prolong_at(const LevelDev<T>& C, int j, int i) {
 
 template <typename T, int KIND, int NS, int MODE, bool ZERO>
 __global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
-  if (a.sc && a.sc->done) return;
+  // The convergence guard is checked after the halo loads are issued, so its
+  // round trip overlaps theirs.
+  const bool skip = a.sc && a.sc->done;
   constexpr int H = NS + (MODE == kDown ? 1 : 0);
   constexpr int W = kTx + 2 * H, Hh = kTy + 2 * H, N = W * Hh;
   __shared__ T sU[2][N];
@@ -334,6 +336,7 @@ __global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
     }
     sU[0][k] = u;
   }
+  if (skip) return;
   __syncthreads();
 
   const T om = a.omega;
@@ -469,7 +472,8 @@ __device__ __forceinline__ LevelDev<T> lds_level(const CoarseArgs<T>& a, int l, 
 // for LDS: grids that stop halving early).
 template <typename T, int KIND, bool LDS>
 __global__ __launch_bounds__(kCoarseThreads) void mg_coarse_kernel(CoarseArgs<T> a) {
-  if (a.done && *a.done) return;
+  const bool skip = a.done && *a.done;
+  if (!LDS && skip) return;
   extern __shared__ __align__(16) unsigned char coarse_smem[];
   T* sm = reinterpret_cast<T*>(coarse_smem);
   int off[16];
@@ -489,6 +493,7 @@ __global__ __launch_bounds__(kCoarseThreads) void mg_coarse_kernel(CoarseArgs<T>
       L.b[k] = G.b[k];
       if (a.nonzero_init) L.u[k] = G.u[k];
     }
+    if (skip) return;
     __syncthreads();
   }
   // Down leg.
@@ -683,7 +688,7 @@ __global__ __launch_bounds__(256) void pcg_dir_kernel(LevelDev<T> L, const T* __
                                                       const T* __restrict__ p_in,
                                                       T* __restrict__ p_out, T* __restrict__ q,
                                                       double* partials, PcgScalars* sc) {
-  if (sc->done) return;
+  const int done = sc->done;  // checked once the first rows are in flight
   const int lane = threadIdx.x;
   const int i = blockIdx.x * 64 + lane;
   const int j0 = (blockIdx.y * 4 + threadIdx.y) * kDirRows;  // uniform within a wave
@@ -701,6 +706,7 @@ __global__ __launch_bounds__(256) void pcg_dir_kernel(LevelDev<T> L, const T* __
   double pq = 0.0;
   T ps = P(j0 - 1, i), pc = P(j0, i);
   T cys = CY(j0 - 1);
+  if (done) return;
   for (int r = 0; r < kDirRows; ++r) {
     const int j = j0 + r;
     if (j >= ny) break;
@@ -746,22 +752,33 @@ __global__ __launch_bounds__(256) void pcg_update_kernel(int ny, int nx, T* __re
                                                          T* __restrict__ r,
                                                          const T* __restrict__ q,
                                                          double* partials, PcgScalars* sc) {
-  if (sc->done) return;
+  const int done = sc->done;
   const int i = blockIdx.x * 64 + threadIdx.x;
   const int j0 = (blockIdx.y * 4 + threadIdx.y) * kDirRows;
   const T alpha = (T)sc->alpha;
   double rr = 0.0;
-  if (i < nx) {
+  // All loads first (their round trip overlaps the scalars'), then the guard.
+  T xv[kDirRows], pv[kDirRows], rv0[kDirRows], qv[kDirRows];
 #pragma unroll
-    for (int t = 0; t < kDirRows; ++t) {
-      const int j = j0 + t;
-      if (j < ny) {
-        const int64_t k = (int64_t)j * nx + i;
-        x[(int64_t)j * ld_x + i] += alpha * p[k];
-        const T rv = r[k] - alpha * q[k];
-        r[k] = rv;
-        rr += (double)rv * rv;
-      }
+  for (int t = 0; t < kDirRows; ++t) {
+    const int j = j0 + t;
+    const bool in = i < nx && j < ny;
+    const int64_t k = (int64_t)j * nx + i;
+    xv[t] = in ? x[(int64_t)j * ld_x + i] : T(0);
+    pv[t] = in ? p[k] : T(0);
+    rv0[t] = in ? r[k] : T(0);
+    qv[t] = in ? q[k] : T(0);
+  }
+  if (done) return;
+#pragma unroll
+  for (int t = 0; t < kDirRows; ++t) {
+    const int j = j0 + t;
+    if (i < nx && j < ny) {
+      const int64_t k = (int64_t)j * nx + i;
+      x[(int64_t)j * ld_x + i] = xv[t] + alpha * pv[t];
+      const T rv = rv0[t] - alpha * qv[t];
+      r[k] = rv;
+      rr += (double)rv * rv;
     }
   }
   double v[1] = {rr};
@@ -775,7 +792,7 @@ __global__ __launch_bounds__(256) void refine_residual_kernel(
     LevelDev<double> L, const TB* __restrict__ b, int64_t ld_b, const double* __restrict__ x,
     int64_t ld_x, float* __restrict__ r32, float* __restrict__ e32, double rtol2, double atol2,
     double* partials, PcgScalars* sc) {
-  if (sc->out_done) return;
+  const int done = sc->out_done;
   const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
   double bb = 0.0, rr = 0.0;
   if (i < L.nx && j < L.ny) {
@@ -800,11 +817,14 @@ __global__ __launch_bounds__(256) void refine_residual_kernel(
     }
     const double bv = (double)b[(int64_t)j * ld_b + i] * m;
     const double rv = (bv - (lap - L.lam * c) * m) * m;
-    r32[k] = (float)rv;
-    e32[k] = 0.0f;
+    if (!done) {
+      r32[k] = (float)rv;
+      e32[k] = 0.0f;
+    }
     bb = bv * bv;
     rr = rv * rv;
   }
+  if (done) return;
   double v[2] = {bb, rr};
   block_partials<2, 256>(v, partials);
 }
@@ -1135,14 +1155,16 @@ __global__ __launch_bounds__(256) void pcg_dots_kernel(int ny, int nx, const T* 
                                                        const T* __restrict__ r,
                                                        const T* __restrict__ q,
                                                        double* partials, PcgScalars* sc) {
-  if (sc->done) return;
+  const int done = sc->done;
+  const int first = sc->first;
   const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
   double d0 = 0.0, d1 = 0.0;
   if (i < nx && j < ny) {
     const int64_t k = (int64_t)j * nx + i;
     d0 = (double)z[k] * r[k];
-    if (!sc->first) d1 = (double)z[k] * q[k];
+    if (!first) d1 = (double)z[k] * q[k];
   }
+  if (done) return;
   double v[2] = {d0, d1};
   block_partials<2, 256>(v, partials);
 }
