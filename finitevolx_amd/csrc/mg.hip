@@ -307,16 +307,27 @@ __global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
   const int i0 = (t % a.tiles_x) * kTx, j0 = (t / a.tiles_x) * kTy;
   const int tid = threadIdx.x;
 
+  constexpr bool kUpMode = (MODE == kUp || MODE == kUpDots);
+  constexpr int IT = (N + kTileThreads - 1) / kTileThreads;
+  // Level l+1's iterate (times its mask) and mask around this tile for the
+  // prolongation, staged in LDS: rows cj0 .. cj0 + CH - 1, columns
+  // ci0 .. ci0 + CW - 1 of the coarse level (zero outside it).
+  constexpr int CW = kTx / 2 + 4, CH = kTy / 2 + 4;
+  __shared__ T sCu[kUpMode ? CW * CH : 1];
+  __shared__ T sCm[kUpMode ? CW * CH : 1];
+  const int cj0 = ((j0 - H) >> 1) - 1, ci0 = ((i0 - H) >> 1) - 1;
+  T rU[kUpMode ? IT : 1], rM[kUpMode ? IT : 1];
+
   // Fully unrolled (the trip count is a compile-time constant) so the
-  // global loads of all iterations are in flight together.
+  // global loads of all iterations are in flight together; 32-bit offsets.
 #pragma unroll
-  for (int it = 0; it < (N + kTileThreads - 1) / kTileThreads; ++it) {
+  for (int it = 0; it < IT; ++it) {
     const int k = tid + it * kTileThreads;
     if (k >= N) break;
     const int y = k / W, x = k % W;
     const int gj = j0 - H + y, gi = i0 - H + x;
     const bool in = (gj >= 0 && gj < L.ny && gi >= 0 && gi < L.nx);
-    const int64_t g = (int64_t)gj * L.nx + gi;
+    const int g = in ? gj * L.nx + gi : 0;
     const T m = in ? mval<T>(L.mask, g) : T(0);
     sM[k] = m;
     sB[k] = in ? a.b[g] : T(0);
@@ -326,15 +337,48 @@ __global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
       sI[k] = in ? L.inv[g] : T(0);
     }
     T u = T(0);
-    if (in) {
-      if constexpr (!ZERO) u = a.u_in[g] * m;  // u_m = u * mask (multigrid.py:383)
-      if constexpr (MODE == kUp || MODE == kUpDots) {
-        // u = (u + P e_c) * mask  (multigrid.py:875-876)
-        const T e = prolong_at<T, KIND>(a.C, gj, gi) * m;
-        u = (u + e) * m;
-      }
+    if constexpr (!ZERO) u = in ? a.u_in[g] * m : T(0);  // u_m = u * mask (multigrid.py:383)
+    if constexpr (kUpMode) {
+      rU[it] = u;
+      rM[it] = m;
+    } else {
+      sU[0][k] = u;
     }
-    sU[0][k] = u;
+  }
+  if constexpr (kUpMode) {
+    const LevelDev<T>& C = a.C;
+#pragma unroll
+    for (int it = 0; it < (CW * CH + kTileThreads - 1) / kTileThreads; ++it) {
+      const int k = tid + it * kTileThreads;
+      if (k >= CW * CH) break;
+      const int J = cj0 + k / CW, I = ci0 + k % CW;
+      const bool in = (J >= 0 && J < C.ny && I >= 0 && I < C.nx);
+      const int g = in ? J * C.nx + I : 0;
+      const T m = in ? mval<T>(C.mask, g) : T(0);
+      sCm[k] = m;
+      sCu[k] = in ? C.u[g] * m : T(0);
+    }
+    __syncthreads();
+    // u = (u + P e_c) * mask  (multigrid.py:875-876), _prolongate
+    // (multigrid.py:510-603) from the staged coarse tile.
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int k = tid + it * kTileThreads;
+      if (k >= N) break;
+      const int y = k / W, x = k % W;
+      const int gj = j0 - H + y, gi = i0 - H + x;
+      const int cy = (gj >> 1) - cj0, cx = (gi >> 1) - ci0;
+      const int sj = (gj & 1) ? CW : -CW, si = (gi & 1) ? 1 : -1;
+      const int q = cy * CW + cx;
+      const T w = T(9) * sCu[q] + T(3) * sCu[q + si] + T(3) * sCu[q + sj] + sCu[q + sj + si];
+      const T d = T(9) * sCm[q] + T(3) * sCm[q + si] + T(3) * sCm[q + sj] + sCm[q + sj + si];
+      const T dd = d > T(1) ? d : T(1);
+      T e;
+      if constexpr (std::is_same<T, float>::value) e = w * __builtin_amdgcn_rcpf(dd);
+      else e = w / dd;
+      const T m = rM[it];
+      sU[0][k] = (rU[it] + e * m) * m;
+    }
   }
   if (skip) return;
   __syncthreads();
@@ -695,41 +739,47 @@ __global__ __launch_bounds__(256) void pcg_dir_kernel(LevelDev<T> L, const T* __
   const bool first = sc->first;
   const T beta = first ? T(0) : (T)sc->beta;
   const int nx = L.nx, ny = L.ny;
-  auto P = [&](int j, int ii) -> T {
-    if (j < 0 || j >= ny || ii < 0 || ii >= nx) return T(0);
-    const int64_t k = (int64_t)j * nx + ii;
+  const bool col = i < nx;
+  // p at (row j, this lane's column); rows outside the level are zero
+  // (j is uniform across the wave, col per lane).  32-bit offsets.
+  auto Prow = [&](int j) -> T {
+    if (j < 0 || j >= ny || !col) return T(0);
+    const int k = j * nx + i;
     return first ? z[k] : z[k] + beta * p_in[k];
   };
-  auto CY = [&](int j) -> T {  // south-face coefficient of row j + 1 is cy[j]
-    return (KIND == kNoflux && j >= 0 && j < ny && i < nx) ? L.cy[(int64_t)j * nx + i] : T(0);
+  auto Pat = [&](int j, int ii) -> T {  // edge lanes' outer neighbours
+    if (ii < 0 || ii >= nx) return T(0);
+    const int k = j * nx + ii;
+    return first ? z[k] : z[k] + beta * p_in[k];
+  };
+  auto CY = [&](int j) -> T {  // north-face coefficient of row j
+    return (KIND == kNoflux && j >= 0 && j < ny && col) ? L.cy[j * nx + i] : T(0);
   };
   double pq = 0.0;
-  T ps = P(j0 - 1, i), pc = P(j0, i);
+  T ps = Prow(j0 - 1), pc = Prow(j0);
   T cys = CY(j0 - 1);
   if (done) return;
   for (int r = 0; r < kDirRows; ++r) {
     const int j = j0 + r;
     if (j >= ny) break;
-    const T pn = P(j + 1, i);
+    const T pn = Prow(j + 1);
     T w = __shfl_up(pc, 1, 64);
     T e = __shfl_down(pc, 1, 64);
-    if (lane == 0) w = P(j, i - 1);
-    if (lane == 63) e = P(j, i + 1);
+    if (lane == 0) w = Pat(j, i - 1);
+    if (lane == 63) e = Pat(j, i + 1);
     T lap;
     T cyn = T(0);
+    const int k = j * nx + i;
     if constexpr (KIND == kDirichlet) {
       lap = (e + w - T(2) * pc) * L.idx2 + (pn + ps - T(2) * pc) * L.idy2;
     } else {
-      const int64_t k = (int64_t)j * nx + i;
-      const T cxe = (i < nx) ? L.cx[k] : T(0);
+      const T cxe = col ? L.cx[k] : T(0);
       T cxw = __shfl_up(cxe, 1, 64);
-      if (lane == 0) cxw = (i > 0 && i - 1 < nx) ? L.cx[k - 1] : T(0);
-      if (i == 0) cxw = T(0);
+      if (lane == 0) cxw = (i > 0) ? L.cx[k - 1] : T(0);
       cyn = CY(j);
       lap = (cxe * (e - pc) - cxw * (pc - w)) * L.idx2 + (cyn * (pn - pc) - cys * (pc - ps)) * L.idy2;
     }
-    if (i < nx) {
-      const int64_t k = (int64_t)j * nx + i;
+    if (col) {
       const T qv = (lap - L.lam * pc) * mval<T>(L.mask, k);
       p_out[k] = pc;
       q[k] = qv;
@@ -1092,7 +1142,8 @@ Hier<float>& hier<float>(Plan& P) { return P.h32; }
 
 template <typename T, int KIND, int NS, int MODE, bool ZERO>
 static void launch_tile(const Plan& P, const LevelDev<T>& L, const LevelDev<T>* C, const T* u_in,
-                        const T* b, T* u_out, const T* w, PcgScalars* sc, hipStream_t s) {
+                        const T* b, T* u_out, const T* w, PcgScalars* sc, hipStream_t s,
+                        bool finish = true) {
   TileArgs<T> a;
   std::memset(&a, 0, sizeof(a));
   a.L = L;
@@ -1109,8 +1160,19 @@ static void launch_tile(const Plan& P, const LevelDev<T>& L, const LevelDev<T>* 
   hipLaunchKernelGGL((mg_tile_kernel<T, KIND, NS, MODE, ZERO>), dim3(a.ntiles), dim3(kTileThreads),
                      0, s, a);
   if constexpr (MODE == kUpDots)
-    hipLaunchKernelGGL((reduce_finish_kernel<2, kRedUpDots>), dim3(1), dim3(kFinishThreads), 0, s,
-                       P.partials, a.ntiles, sc, 0.0, 0.0);
+    if (finish)
+      hipLaunchKernelGGL((reduce_finish_kernel<2, kRedUpDots>), dim3(1), dim3(kFinishThreads), 0,
+                         s, P.partials, a.ntiles, sc, 0.0, 0.0);
+}
+
+// Level-0 up kernel of a preconditioning V-cycle with the flexible-PCG dots
+// (two post-sweeps), as the V-cycle launches it, without its finishing
+// kernel (profiling hook).
+template <typename T, int KIND>
+static void launch_vcycle_up_dots(const Plan& P, const LevelDev<T>& L, const LevelDev<T>& C,
+                                  const T* u, const T* b, T* out, const T* w, PcgScalars* sc,
+                                  hipStream_t s) {
+  launch_tile<T, KIND, 2, kUpDots, false>(P, L, &C, u, b, out, w, sc, s, false);
 }
 
 // Finishing kernel of a reduction over the blocks of grid2(ny, nx).
@@ -1698,17 +1760,32 @@ static int kernel_bench_impl(Plan& P, int which, int reps, hipStream_t s) {
     for (T* v : {H.r, H.z, H.p0, H.p1, H.q, H.x})
       hipLaunchKernelGGL(zero_pitched_kernel<T>, grid2(L.ny, L.nx), dim3(64, 4), 0, s, L.ny, L.nx,
                          v, (int64_t)L.nx);
+    if (P.nl >= 2) {
+      const LevelDev<T>& C = H.lv[1];
+      hipLaunchKernelGGL(zero_pitched_kernel<T>, grid2(C.ny, C.nx), dim3(64, 4), 0, s, C.ny, C.nx,
+                         C.u, (int64_t)C.nx);
+    }
     hipLaunchKernelGGL(bench_scalars_kernel, dim3(1), dim3(1), 0, s, sc);
     FVX_LAUNCH_CHECK();
     return kOk;
   }
+  if (which == 2 && P.nl < 2) {
+    set_error("the prolongation kernel needs two levels");
+    return kInvalidArgument;
+  }
   for (int k = 0; k < reps; ++k) {
-    if (which == 0)
+    if (which == 0) {
       hipLaunchKernelGGL((pcg_dir_kernel<T, KIND>), grid_dir(L.ny, L.nx), dim3(64, 4), 0, s, L,
                          H.z, H.p0, H.p1, H.q, P.partials, sc);
-    else
+    } else if (which == 1) {
       hipLaunchKernelGGL(pcg_update_kernel<T>, grid_dir(L.ny, L.nx), dim3(64, 4), 0, s, L.ny,
                          L.nx, H.x, (int64_t)L.nx, H.p0, H.r, H.q, P.partials, sc);
+    } else {
+      // The last kernel of a preconditioning V-cycle on level 0: prolongation
+      // of level 1's iterate, correction, two sweeps, flexible-PCG dots.
+      LevelDev<T> C = H.lv[1];
+      launch_vcycle_up_dots<T, KIND>(P, L, C, H.p0, H.r, H.z, H.q, sc, s);
+    }
   }
   FVX_LAUNCH_CHECK();
   return kOk;
@@ -2006,13 +2083,16 @@ int fvx_mg_mixed_solve(void* plan, const double* rhs, int64_t ld_rhs, double* x,
 // instances of one fine-level PCG kernel on the plan's own vectors with a
 // scalar block that never converges.  which: 0 pcg_dir (p = z + beta p,
 // q = A p, p.q), 1 pcg_update (x += alpha p, r -= alpha q, r.r); bits 32 or
-// 64 select the hierarchy.  reps = 0 prepares (zeroes the vectors, arms the
+// 64 select the hierarchy; which 2 is the level-0 up kernel of a
+// preconditioning V-cycle (prolongation of level 1, correction, two sweeps,
+// flexible-PCG dots).  reps = 0 prepares (zeroes the vectors, arms the
 // scalars) and must precede timed calls.
 int fvx_mg_kernel_bench(void* plan, int which, int bits, int reps, void* stream) {
   using namespace fvx;
   FVX_CHECK_ARG(plan, "null plan");
   Plan* P = (Plan*)plan;
-  FVX_CHECK_ARG(which == 0 || which == 1, "which must be 0 (pcg_dir) or 1 (pcg_update)");
+  FVX_CHECK_ARG(which >= 0 && which <= 2,
+                "which must be 0 (pcg_dir), 1 (pcg_update) or 2 (V-cycle up + dots)");
   FVX_CHECK_ARG((bits == 32 && P->has32) || (bits == 64 && P->has64),
                 "plan has no hierarchy of that precision");
   FVX_CHECK_ARG(reps >= 0, "negative reps");
