@@ -600,6 +600,183 @@ __global__ __launch_bounds__(kCoarseThreads) void mg_coarse_kernel(CoarseArgs<T>
   }
 }
 
+// LDS-resident coarse V-cycle.  Levels lc .. nl-1 keep their iterate u,
+// scratch t and right-hand side b in dynamic LDS, addressed by cell offsets
+// (ds_read/ds_write); the 1024 threads tile each level as 32 x 32 blocks of
+// cells, so no integer division is needed.  Every stored iterate is already
+// multiplied by its mask, which is the reference's u * mask before the
+// stencil (multigrid.py:383).
+template <typename T, int KIND>
+struct CoarseLds {
+  T* sm;
+  __device__ __forceinline__ T m_at(const LevelDev<T>& L, int k) const {
+    return mval<T>(L.mask, k);
+  }
+  __device__ __forceinline__ T inv_at(const LevelDev<T>& L, int k, T m) const {
+    if constexpr (KIND == kDirichlet) return L.inv_d * m;
+    return L.inv[k];
+  }
+  // (A u)(j, i) for u at LDS offset ou (zero outside the level).
+  __device__ __forceinline__ T Au(const LevelDev<T>& L, int ou, int j, int i, int k, T m) const {
+    const T c = sm[ou + k];
+    const T e = (i + 1 < L.nx) ? sm[ou + k + 1] : T(0);
+    const T w = (i > 0) ? sm[ou + k - 1] : T(0);
+    const T n = (j + 1 < L.ny) ? sm[ou + k + L.nx] : T(0);
+    const T s = (j > 0) ? sm[ou + k - L.nx] : T(0);
+    T lap;
+    if constexpr (KIND == kDirichlet) {
+      lap = (e + w - T(2) * c) * L.idx2 + (n + s - T(2) * c) * L.idy2;
+    } else {
+      const T cxe = L.cx[k], cxw = (i > 0) ? L.cx[k - 1] : T(0);
+      const T cyn = L.cy[k], cys = (j > 0) ? L.cy[k - L.nx] : T(0);
+      lap = (cxe * (e - c) - cxw * (c - w)) * L.idx2 + (cyn * (n - c) - cys * (c - s)) * L.idy2;
+    }
+    return (lap - L.lam * c) * m;
+  }
+  // n damped-Jacobi sweeps of level L (u at ou, scratch at ot, b at ob); the
+  // result ends in ou.
+  __device__ void sweeps(const LevelDev<T>& L, int ou, int ot, int ob, int n, bool zero, T om,
+                         int tx, int ty) const {
+    int src = ou, dst = ot;
+    for (int sw = 0; sw < n; ++sw) {
+      for (int j = ty; j < L.ny; j += 32)
+        for (int i = tx; i < L.nx; i += 32) {
+          const int k = j * L.nx + i;
+          const T m = m_at(L, k);
+          const T inv = inv_at(L, k, m);
+          T un;
+          if (zero && sw == 0) un = (om * inv * sm[ob + k]) * m;
+          else un = (sm[src + k] + om * inv * (sm[ob + k] - Au(L, src, j, i, k, m))) * m;
+          sm[dst + k] = un;
+        }
+      __syncthreads();
+      const int tmp = src;
+      src = dst;
+      dst = tmp;
+    }
+    if (src != ou) {  // odd number of sweeps: copy back
+      for (int j = ty; j < L.ny; j += 32)
+        for (int i = tx; i < L.nx; i += 32) sm[ou + j * L.nx + i] = sm[src + j * L.nx + i];
+      __syncthreads();
+    }
+  }
+};
+
+template <typename T, int KIND>
+__global__ __launch_bounds__(kCoarseThreads) void mg_coarse_lds_kernel(CoarseArgs<T> a) {
+  const bool skip = a.done && *a.done;
+  extern __shared__ __align__(16) unsigned char coarse_smem[];
+  CoarseLds<T, KIND> S{reinterpret_cast<T*>(coarse_smem)};
+  T* sm = S.sm;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  int ou[16], ot[16], ob[16];
+  {
+    int o = 0;
+    for (int l = a.lc; l < a.nl; ++l) {
+      const int n = a.lv[l].ny * a.lv[l].nx;
+      ou[l] = o;
+      ot[l] = o + n;
+      ob[l] = o + 2 * n;
+      o += 3 * n;
+    }
+  }
+  const T om = a.omega;
+  // Level lc's right-hand side (and masked guess) from global memory.
+  {
+    const LevelDev<T>& G = a.lv[a.lc];
+    for (int j = ty; j < G.ny; j += 32)
+      for (int i = tx; i < G.nx; i += 32) {
+        const int k = j * G.nx + i;
+        sm[ob[a.lc] + k] = G.b[k];
+        if (a.nonzero_init) sm[ou[a.lc] + k] = G.u[k] * S.m_at(G, k);
+      }
+    if (skip) return;
+    __syncthreads();
+  }
+  // Down leg: pre-smoothing, residual, mask-weighted 2x2 restriction.
+  for (int l = a.lc; l < a.nl - 1; ++l) {
+    const LevelDev<T>& L = a.lv[l];
+    const LevelDev<T>& C = a.lv[l + 1];
+    const bool zero = !(a.nonzero_init && l == a.lc);
+    if (a.n_pre > 0) {
+      S.sweeps(L, ou[l], ot[l], ob[l], a.n_pre, zero, om, tx, ty);
+    } else if (zero) {
+      for (int j = ty; j < L.ny; j += 32)
+        for (int i = tx; i < L.nx; i += 32) sm[ou[l] + j * L.nx + i] = T(0);
+      __syncthreads();
+    }
+    // residual r = (b - A u) m into the scratch array
+    for (int j = ty; j < L.ny; j += 32)
+      for (int i = tx; i < L.nx; i += 32) {
+        const int k = j * L.nx + i;
+        const T m = S.m_at(L, k);
+        sm[ot[l] + k] = (sm[ob[l] + k] - S.Au(L, ou[l], j, i, k, m)) * m;
+      }
+    __syncthreads();
+    for (int J = ty; J < C.ny; J += 32)
+      for (int I = tx; I < C.nx; I += 32) {
+        const int f = (2 * J) * L.nx + 2 * I;
+        const T m00 = S.m_at(L, f), m01 = S.m_at(L, f + 1), m10 = S.m_at(L, f + L.nx),
+                m11 = S.m_at(L, f + L.nx + 1);
+        const T s4 = sm[ot[l] + f] * m00 + sm[ot[l] + f + 1] * m01 + sm[ot[l] + f + L.nx] * m10 +
+                     sm[ot[l] + f + L.nx + 1] * m11;
+        const T cnt = m00 + m01 + m10 + m11;
+        const int kc = J * C.nx + I;
+        sm[ob[l + 1] + kc] = (s4 / (cnt > T(1) ? cnt : T(1))) * S.m_at(C, kc);
+      }
+    __syncthreads();
+  }
+  // Bottom solve.
+  {
+    const int lb = a.nl - 1;
+    const LevelDev<T>& B = a.lv[lb];
+    const bool zero = !(a.nonzero_init && a.lc == lb);
+    if (a.n_coarse > 0) {
+      S.sweeps(B, ou[lb], ot[lb], ob[lb], a.n_coarse, zero, om, tx, ty);
+    } else if (zero) {
+      for (int j = ty; j < B.ny; j += 32)
+        for (int i = tx; i < B.nx; i += 32) sm[ou[lb] + j * B.nx + i] = T(0);
+      __syncthreads();
+    }
+  }
+  // Up leg: u += P(u_c) (9/3/3/1 weights, mask renormalised), post-smoothing.
+  for (int l = a.nl - 2; l >= a.lc; --l) {
+    const LevelDev<T>& L = a.lv[l];
+    const LevelDev<T>& C = a.lv[l + 1];
+    for (int j = ty; j < L.ny; j += 32)
+      for (int i = tx; i < L.nx; i += 32) {
+        const int k = j * L.nx + i;
+        const T m = S.m_at(L, k);
+        const int J = j >> 1, I = i >> 1;
+        const int sj = (j & 1) ? 1 : -1, si = (i & 1) ? 1 : -1;
+        auto vm = [&](int jj, int ii, T& mm) -> T {
+          if (jj < 0 || jj >= C.ny || ii < 0 || ii >= C.nx) {
+            mm = T(0);
+            return T(0);
+          }
+          const int kc = jj * C.nx + ii;
+          mm = S.m_at(C, kc);
+          return sm[ou[l + 1] + kc];  // stored iterates are already masked
+        };
+        T mc, mx, my, md;
+        const T vc = vm(J, I, mc), vx = vm(J, I + si, mx), vy = vm(J + sj, I, my),
+                vd = vm(J + sj, I + si, md);
+        const T w = T(9) * vc + T(3) * vx + T(3) * vy + vd;
+        const T d = T(9) * mc + T(3) * mx + T(3) * my + md;
+        const T e = w / (d > T(1) ? d : T(1));
+        sm[ou[l] + k] = (sm[ou[l] + k] + e * m) * m;
+      }
+    __syncthreads();
+    S.sweeps(L, ou[l], ot[l], ob[l], a.n_post, false, om, tx, ty);
+  }
+  // Level lc's corrected iterate back to global memory.
+  {
+    const LevelDev<T>& G = a.lv[a.lc];
+    for (int j = ty; j < G.ny; j += 32)
+      for (int i = tx; i < G.nx; i += 32) G.u[j * G.nx + i] = sm[ou[a.lc] + j * G.nx + i];
+  }
+}
+
 // Dynamic LDS of the coarse kernel for levels lc .. nl-1 (u, t, b each).
 template <typename T>
 size_t coarse_lds_bytes(const CoarseArgs<T>& a) {
@@ -613,7 +790,7 @@ template <typename T, int KIND>
 static void launch_coarse(const CoarseArgs<T>& ca, hipStream_t s) {
   const size_t lds = coarse_lds_bytes(ca);
   if (lds <= kCoarseLdsMax)
-    hipLaunchKernelGGL((mg_coarse_kernel<T, KIND, true>), dim3(1), dim3(kCoarseThreads), lds, s,
+    hipLaunchKernelGGL((mg_coarse_lds_kernel<T, KIND>), dim3(1), dim3(kCoarseThreads), lds, s,
                        ca);
   else
     hipLaunchKernelGGL((mg_coarse_kernel<T, KIND, false>), dim3(1), dim3(kCoarseThreads), 0, s,
@@ -626,13 +803,13 @@ static void coarse_lds_attributes() {
   if (done) return;
   done = true;
   const int big = (int)kCoarseLdsMax;
-  (void)hipFuncSetAttribute((const void*)mg_coarse_kernel<float, kDirichlet, true>,
+  (void)hipFuncSetAttribute((const void*)mg_coarse_lds_kernel<float, kDirichlet>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, big);
-  (void)hipFuncSetAttribute((const void*)mg_coarse_kernel<float, kNoflux, true>,
+  (void)hipFuncSetAttribute((const void*)mg_coarse_lds_kernel<float, kNoflux>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, big);
-  (void)hipFuncSetAttribute((const void*)mg_coarse_kernel<double, kDirichlet, true>,
+  (void)hipFuncSetAttribute((const void*)mg_coarse_lds_kernel<double, kDirichlet>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, big);
-  (void)hipFuncSetAttribute((const void*)mg_coarse_kernel<double, kNoflux, true>,
+  (void)hipFuncSetAttribute((const void*)mg_coarse_lds_kernel<double, kNoflux>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, big);
 }
 
