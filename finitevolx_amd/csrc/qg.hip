@@ -197,7 +197,7 @@ static int net_precond(void* ctx, const float* r, float* z, int ny, int nx, cons
 }
 struct QGInfoC {
   int solves, iterations, max_iterations, outer;
-  int converged, pad;
+  int converged, retries;
   double max_rel_residual;
 };
 
@@ -425,6 +425,7 @@ static int qg_run_graph(QGPlan& Q, TS* q, double* psi, int nsteps, QGInfoC* info
         // caps: fall back to multigrid-PCG for the rest of the run.
         Q.net = nullptr;
         Q.fallbacks += 1;
+        acc.retries += 1;
         int rc = capture_graphs<TS>(Q);
         if (rc) return rc;
         continue;
@@ -438,6 +439,7 @@ static int qg_run_graph(QGPlan& Q, TS* q, double* psi, int nsteps, QGInfoC* info
         continue;
       }
       Q.recaptures += 1;
+      acc.retries += 1;
       Q.b_iters = std::min(Q.max_iters, Q.b_iters + 2);
       Q.b_inner = std::min(Q.max_iters, Q.b_inner + 1);
       Q.b_outer = std::min(Q.max_outer, Q.b_outer + 1);
@@ -463,7 +465,7 @@ typedef struct fvx_qg_info {
   int max_iterations;   // largest per-solve inner iteration count
   int outer;            // total refinement sweeps (mixed)
   int converged;        // 1 if every solve met the tolerance
-  int pad;
+  int retries;          // graph steps redone with a wider iteration budget
   double max_rel_residual;  // max over solves of ||b - A psi|| / ||b|| (FP64)
 } fvx_qg_info;
 
@@ -607,6 +609,20 @@ int fvx_qg_set_preconditioner(void* plan, void* net, double out_scale, int sweep
 }
 
 int fvx_qg_fallbacks(void* plan) { return plan ? ((fvx::QGPlan*)plan)->fallbacks : -1; }
+
+// Iteration budgets of the captured step graphs: out[0] FP64 PCG iterations,
+// out[1] inner FP32 iterations per refinement sweep, out[2] refinement
+// sweeps, out[3] recaptures so far.
+int fvx_qg_budgets(void* plan, int* out) {
+  using namespace fvx;
+  FVX_CHECK_ARG(plan && out, "null argument");
+  QGPlan* Q = (QGPlan*)plan;
+  out[0] = Q->b_iters;
+  out[1] = Q->b_inner;
+  out[2] = Q->b_outer;
+  out[3] = Q->recaptures;
+  return kOk;
+}
 
 // Same as fvx_qg_step, replaying one captured HIP graph per step: each graph
 // holds the three solves (fixed iteration budgets, device-side convergence

@@ -26,7 +26,7 @@ _vp, _i, _d = _lib._vp, _lib._i, _lib._d
 
 class _QGInfoC(C.Structure):
     _fields_ = [("solves", C.c_int), ("iterations", C.c_int), ("max_iterations", C.c_int),
-                ("outer", C.c_int), ("converged", C.c_int), ("pad", C.c_int),
+                ("outer", C.c_int), ("converged", C.c_int), ("retries", C.c_int),
                 ("max_rel_residual", C.c_double)]
 
 
@@ -38,6 +38,7 @@ _lib.extra_signatures({
     "fvx_qg_step_graph": [_vp, _vp, _vp, _i, C.POINTER(_QGInfoC), _vp],
     "fvx_qg_set_preconditioner": [_vp, _vp, _d, _i],
     "fvx_qg_fallbacks": [_vp],
+    "fvx_qg_budgets": [_vp, C.POINTER(C.c_int)],
     "fvx_qg_stage_f64": [_vp, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _d, _d, _d, _d, _d, _vp],
     "fvx_qg_stage_f32": [_vp, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _d, _d, _d, _d, _d, _vp],
 })
@@ -116,6 +117,13 @@ class QGModel:
                   float(out_scale), int(sweeps))
 
     @property
+    def budgets(self) -> dict:
+        """Iteration budgets of the captured step graphs and recaptures so far."""
+        out = (C.c_int * 4)()
+        _lib.call("fvx_qg_budgets", self._h, out)
+        return dict(iters=out[0], inner=out[1], outer=out[2], recaptures=out[3])
+
+    @property
     def fallbacks(self) -> int:
         """Times the learned preconditioner gave way to multigrid-PCG."""
         return _lib.call("fvx_qg_fallbacks", self._h)
@@ -149,7 +157,8 @@ class QGModel:
                   psi.data_ptr(), int(nsteps), C.byref(ci), _lib.stream())
         info = QGInfo(solves=ci.solves, iterations=ci.iterations,
                       max_iterations=ci.max_iterations, outer=ci.outer,
-                      converged=bool(ci.converged), max_rel_residual=ci.max_rel_residual)
+                      converged=bool(ci.converged), retries=ci.retries,
+                      max_rel_residual=ci.max_rel_residual)
         return q, psi, info
 
     def stage(self, q_s, psi, q_n, a: float, b: float, c: float):

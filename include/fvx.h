@@ -229,7 +229,8 @@ int fvx_spectral_helmholtz_f64(void* plan, const double* rhs, int64_t ld_rhs, do
 /* ---- QG 1.5-layer step (composition; nearest reference
  * scripts/qg_1p5_layer.py:420-497; operators as cited above) -------------- */
 typedef struct fvx_qg_info {
-  int solves, iterations, max_iterations, outer, converged, pad;
+  int solves, iterations, max_iterations, outer, converged;
+  int retries; /* graph steps redone with a wider iteration budget */
   double max_rel_residual;
 } fvx_qg_info;
 /* state_precision 32|64; solve_mode 64 (FP64 MG-PCG) | 3264 (mixed);
@@ -251,6 +252,10 @@ int fvx_qg_step(void* plan, void* q, double* psi, int nsteps, fvx_qg_info* info,
  * fvx_qg_fallbacks counts those events. */
 int fvx_qg_set_preconditioner(void* plan, void* net, double out_scale, int sweeps);
 int fvx_qg_fallbacks(void* plan);
+/* Budgets of the captured step graphs: out[0] FP64 PCG iterations, out[1]
+ * inner FP32 iterations per refinement sweep, out[2] refinement sweeps,
+ * out[3] recaptures so far (no reference counterpart).                    */
+int fvx_qg_budgets(void* plan, int* out);
 /* fvx_qg_step replaying one captured HIP graph per step (three solves with
  * fixed iteration budgets and device-side convergence guards, FP64 residual
  * certification, three stage kernels); a step whose solve misses rtol is
