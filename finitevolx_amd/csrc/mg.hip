@@ -1073,6 +1073,7 @@ __global__ void inner_reset_kernel(PcgScalars* inner, const PcgScalars* outer, d
     inner->done = 1;
     return;
   }
+  inner->max_inner = max(inner->max_inner, inner->iter);
   inner->rr = outer->out_rr;
   inner->tol2 = inner_rtol * inner_rtol * outer->out_rr;
   inner->done = 0;
@@ -1109,7 +1110,7 @@ __global__ void scalars_reset_kernel(PcgScalars* sc, int n) {
   if (k < n) {
     PcgScalars& x = sc[k];
     x.rz = x.pq = x.rr = x.bb = x.tol2 = x.alpha = x.beta = x.zq = x.out_rr = x.out_tol2 = 0.0;
-    x.iter = x.done = x.first = x.out_done = x.outer = x.total = 0;
+    x.iter = x.done = x.first = x.out_done = x.outer = x.total = x.max_inner = 0;
   }
 }
 
@@ -1986,6 +1987,10 @@ int plan_levels(const Plan& P) { return P.nl; }
 bool plan_has_mixed(const Plan& P) { return P.has32 && P.has64; }
 int plan_nx(const Plan& P) { return P.has64 ? P.h64.lv[0].nx : P.h32.lv[0].nx; }
 int plan_ny(const Plan& P) { return P.has64 ? P.h64.lv[0].ny : P.h32.lv[0].ny; }
+int plan_max_inner(const Plan& P, int slot) {
+  const PcgScalars& h = P.hsc[2 * slot];
+  return std::max(h.max_inner, h.iter);
+}
 int plan_kind(const Plan& P) { return P.kind; }
 
 template int pcg_enqueue<float>(Plan&, const float*, int64_t, float*, int64_t, double, double,

@@ -24,7 +24,8 @@ struct PcgScalars {
   int out_done;   // outer refinement converged
   int outer;      // outer iterations done
   int total;      // inner iterations summed over refinement steps
-  int pad[2];
+  int max_inner;  // largest inner iteration count of one refinement step
+  int pad;
 };
 
 struct CgInfoC {
@@ -58,6 +59,9 @@ int mixed_enqueue(Plan& P, const TB* rhs, int64_t ld_rhs, double* x, int64_t ld_
 // remaining sweeps fall back to the multigrid V-cycle (counted in
 // info->pad).
 template <typename TB>
+// info->pad returns the number of learned inner solves replaced by the
+// V-cycle; plan_max_inner then gives the largest iteration count of one inner
+// solve.
 int mixed_solve(Plan& P, const TB* rhs, int64_t ld_rhs, double* x, int64_t ld_x, double rtol,
                 double atol, int max_outer, int max_inner, double inner_rtol, CgInfoC* info,
                 int slot, hipStream_t s, PrecondFn pf = nullptr, void* ctx = nullptr);
@@ -85,6 +89,9 @@ int learned_coarse_dims(const Plan& P, int* nyc, int* nxc);
 float* fine_work(Plan& P, int k);
 int plan_nx(const Plan& P);
 int plan_ny(const Plan& P);
+// Largest inner iteration count of one refinement step of the last
+// synchronous mixed solve on ``slot``.
+int plan_max_inner(const Plan& P, int slot);
 // Learned preconditioner application (precond.hip); sweeps as in
 // fvx_mg_precond_solve.
 int learned_precond_apply(Plan& P, void* net, double scale, int sweeps, const float* r, float* z,

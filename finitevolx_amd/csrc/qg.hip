@@ -178,6 +178,7 @@ struct QGPlan {
   hipGraphExec_t exec[2] = {nullptr, nullptr};
   int b_iters = 0, b_outer = 0, b_inner = 0;  // budgets the graphs hold
   int obs_iters = 0, obs_outer = 0;           // calibration observations
+  int obs_inner = 0;                          // largest inner solve (mixed)
   bool calibrated = false;
   int recaptures = 0;
   // Learned preconditioner for the inner FP32 solves (mixed mode only).
@@ -242,6 +243,7 @@ static int qg_step_impl(QGPlan& Q, TS* q, double* psi, int nsteps, QGInfoC* info
                              Q.max_iters, Q.inner_rtol, &ci, 0, s,
                              Q.net ? &net_precond : nullptr, Q.net ? &Q : nullptr);
         Q.fallbacks += ci.pad;
+        if (!rc) Q.obs_inner = std::max(Q.obs_inner, plan_max_inner(*Q.mg, 0));
       } else {
         if constexpr (std::is_same<TS, double>::value) {
           // Recursive residual to rtol/2 so the certified FP64 true residual
@@ -376,6 +378,7 @@ static int qg_run_graph(QGPlan& Q, TS* q, double* psi, int nsteps, QGInfoC* info
   TS* cur = q;
   if (!Q.calibrated) {
     // One synchronous step to observe iteration counts (advances q in place).
+    Q.obs_inner = 0;
     QGInfoC ci;
     int rc = qg_step_impl<TS>(Q, q, Q.psi, 1, &ci, s);
     if (rc) return rc;
@@ -393,7 +396,8 @@ static int qg_run_graph(QGPlan& Q, TS* q, double* psi, int nsteps, QGInfoC* info
     // Budgets: observed + 1 inner iteration of margin.
     int want_iters = std::max(2, Q.obs_iters + 1);
     int want_outer = std::max(1, Q.obs_outer);
-    int want_inner = std::max(2, Q.obs_iters + 1);
+    // Inner budget per refinement sweep: the largest inner solve seen.
+    int want_inner = std::max(2, (Q.solve == kSolveMixed ? Q.obs_inner : Q.obs_iters) + 1);
     if (!Q.calibrated || Q.exec[0] == nullptr) {
       Q.b_iters = want_iters;
       Q.b_outer = want_outer;
