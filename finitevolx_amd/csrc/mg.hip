@@ -1069,13 +1069,19 @@ __global__ __launch_bounds__(256) void refine_update_kernel(int ny, int nx, doub
 // Reset an inner solve on an already-formed residual r (mixed precision):
 // tol from ||r0||, flags cleared.  Single thread.
 __global__ void inner_reset_kernel(PcgScalars* inner, const PcgScalars* outer, double inner_rtol) {
+  // Refinement step outer->outer - 1 has just finished: keep its count.
+  if (outer->outer >= 1 && outer->outer <= 4) inner->it_outer[outer->outer - 1] = inner->iter;
   if (outer->out_done) {
     inner->done = 1;
     return;
   }
   inner->max_inner = max(inner->max_inner, inner->iter);
   inner->rr = outer->out_rr;
-  inner->tol2 = inner_rtol * inner_rtol * outer->out_rr;
+  // Reduce the residual by inner_rtol, but no further than half the outer
+  // target: later refinement steps then stop as soon as the FP64 residual
+  // can certify (FP32 rounding of the correction is ~eps32 kappa ||r||,
+  // far below that target once ||r|| is small).
+  inner->tol2 = fmax(inner_rtol * inner_rtol * outer->out_rr, 0.25 * outer->out_tol2);
   inner->done = 0;
   inner->first = 1;
   inner->iter = 0;
@@ -1111,6 +1117,7 @@ __global__ void scalars_reset_kernel(PcgScalars* sc, int n) {
     PcgScalars& x = sc[k];
     x.rz = x.pq = x.rr = x.bb = x.tol2 = x.alpha = x.beta = x.zq = x.out_rr = x.out_tol2 = 0.0;
     x.iter = x.done = x.first = x.out_done = x.outer = x.total = x.max_inner = 0;
+    x.it_outer[0] = x.it_outer[1] = x.it_outer[2] = x.it_outer[3] = 0;
   }
 }
 
@@ -1657,10 +1664,10 @@ int pcg_solve(Plan& P, const T* rhs, int64_t ld_rhs, T* x, int64_t ld_x, double 
 }
 
 // Enqueue the mixed-precision schedule: n_outer x [FP64 residual, FP32 PCG
-// (n_inner guarded iterations), FP64 update] and a closing FP64 residual.
+// (n_inner[o] guarded iterations), FP64 update] and a closing FP64 residual.
 template <typename TB>
 int mixed_enqueue(Plan& P, const TB* rhs, int64_t ld_rhs, double* x, int64_t ld_x, double rtol,
-                  double atol, int n_outer, int n_inner, double inner_rtol, int slot,
+                  double atol, int n_outer, const int* n_inner, double inner_rtol, int slot,
                   hipStream_t s, PrecondFn pf, void* ctx) {
   Hier<float>& H = P.h32;
   const auto& L64 = P.h64.lv[0];
@@ -1683,8 +1690,8 @@ int mixed_enqueue(Plan& P, const TB* rhs, int64_t ld_rhs, double* x, int64_t ld_
     residual();
     hipLaunchKernelGGL(inner_reset_kernel, dim3(1), dim3(1), 0, s, in, out, inner_rtol);
     int rc = (P.kind == kDirichlet)
-                 ? pcg_iters<float, kDirichlet>(P, H, H.x, L32.nx, n_inner, in, s, pf, ctx)
-                 : pcg_iters<float, kNoflux>(P, H, H.x, L32.nx, n_inner, in, s, pf, ctx);
+                 ? pcg_iters<float, kDirichlet>(P, H, H.x, L32.nx, n_inner[o], in, s, pf, ctx)
+                 : pcg_iters<float, kNoflux>(P, H, H.x, L32.nx, n_inner[o], in, s, pf, ctx);
     if (rc) return rc;
     hipLaunchKernelGGL(refine_update_kernel, grid2(L32.ny, L32.nx), dim3(64, 4), 0, s, L32.ny,
                        L32.nx, x, ld_x, H.x, out);
@@ -1709,7 +1716,9 @@ int mixed_solve(Plan& P, const TB* rhs, int64_t ld_rhs, double* x, int64_t ld_x,
   while (true) {
     const int no = first ? outer_pred : 1;
     if (first) {
-      rc = mixed_enqueue<TB>(P, rhs, ld_rhs, x, ld_x, rtol, atol, no, inner, inner_rtol, slot, s,
+      std::vector<int> sched(no, inner);
+      rc = mixed_enqueue<TB>(P, rhs, ld_rhs, x, ld_x, rtol, atol, no, sched.data(), inner_rtol,
+                             slot, s,
                              pf, ctx);
     } else {
       // Continue: do not reset the outer scalars (keeps the outer count).
@@ -1991,6 +2000,15 @@ int plan_max_inner(const Plan& P, int slot) {
   const PcgScalars& h = P.hsc[2 * slot];
   return std::max(h.max_inner, h.iter);
 }
+int plan_inner_iters(const Plan& P, int slot, int* per_outer, int n) {
+  const PcgScalars& h = P.hsc[2 * slot];
+  const PcgScalars& o = P.hsc[2 * slot + 1];
+  const int no = std::min(o.outer, n);
+  for (int k = 0; k < no; ++k) per_outer[k] = (k < 4) ? h.it_outer[k] : 0;
+  // the last step is never followed by a reset: its count is still in iter
+  if (no >= 1 && o.outer <= n) per_outer[no - 1] = h.iter;
+  return no;
+}
 int plan_kind(const Plan& P) { return P.kind; }
 
 template int pcg_enqueue<float>(Plan&, const float*, int64_t, float*, int64_t, double, double,
@@ -1998,9 +2016,10 @@ template int pcg_enqueue<float>(Plan&, const float*, int64_t, float*, int64_t, d
 template int pcg_enqueue<double>(Plan&, const double*, int64_t, double*, int64_t, double, double,
                                  bool, int, int, hipStream_t);
 template int mixed_enqueue<float>(Plan&, const float*, int64_t, double*, int64_t, double, double,
-                                  int, int, double, int, hipStream_t, PrecondFn, void*);
+                                  int, const int*, double, int, hipStream_t, PrecondFn, void*);
 template int mixed_enqueue<double>(Plan&, const double*, int64_t, double*, int64_t, double,
-                                   double, int, int, double, int, hipStream_t, PrecondFn, void*);
+                                   double, int, const int*, double, int, hipStream_t, PrecondFn,
+                                   void*);
 template int certify_enqueue<double>(Plan&, const double*, int64_t, const double*, int64_t,
                                      double, double, int, hipStream_t);
 template int certify_enqueue<float>(Plan&, const float*, int64_t, const float*, int64_t, double,

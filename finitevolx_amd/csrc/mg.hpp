@@ -25,6 +25,7 @@ struct PcgScalars {
   int outer;      // outer iterations done
   int total;      // inner iterations summed over refinement steps
   int max_inner;  // largest inner iteration count of one refinement step
+  int it_outer[4];  // inner iterations of refinement steps 0..3 (last solve)
   int pad;
 };
 
@@ -52,7 +53,7 @@ int pcg_solve(Plan& P, const T* rhs, int64_t ld_rhs, T* x, int64_t ld_x, double 
               double atol, int max_iters, bool use_x0, CgInfoC* info, int slot, hipStream_t s);
 template <typename TB>
 int mixed_enqueue(Plan& P, const TB* rhs, int64_t ld_rhs, double* x, int64_t ld_x, double rtol,
-                  double atol, int n_outer, int n_inner, double inner_rtol, int slot,
+                  double atol, int n_outer, const int* n_inner, double inner_rtol, int slot,
                   hipStream_t s, PrecondFn pf = nullptr, void* ctx = nullptr);
 // Synchronous mixed solve.  With ``pf`` the inner FP32 solves use it; an
 // inner solve that hits ``max_inner`` without reaching its tolerance makes the
@@ -90,8 +91,11 @@ float* fine_work(Plan& P, int k);
 int plan_nx(const Plan& P);
 int plan_ny(const Plan& P);
 // Largest inner iteration count of one refinement step of the last
-// synchronous mixed solve on ``slot``.
+// synchronous mixed solve on ``slot``; plan_inner_iters fills per_outer[o]
+// (o < n) with refinement step o's inner iterations and returns the number
+// of steps.
 int plan_max_inner(const Plan& P, int slot);
+int plan_inner_iters(const Plan& P, int slot, int* per_outer, int n);
 // Learned preconditioner application (precond.hip); sweeps as in
 // fvx_mg_precond_solve.
 int learned_precond_apply(Plan& P, void* net, double scale, int sweeps, const float* r, float* z,

@@ -176,9 +176,12 @@ struct QGPlan {
   hipStream_t cap = nullptr;
   hipGraph_t graph[2] = {nullptr, nullptr};
   hipGraphExec_t exec[2] = {nullptr, nullptr};
+  static constexpr int kMaxOuter = 8;
   int b_iters = 0, b_outer = 0, b_inner = 0;  // budgets the graphs hold
+  int b_inner_o[kMaxOuter] = {};              // inner budget per refinement step
   int obs_iters = 0, obs_outer = 0;           // calibration observations
   int obs_inner = 0;                          // largest inner solve (mixed)
+  int obs_inner_o[kMaxOuter] = {};            // per refinement step (mixed)
   bool calibrated = false;
   int recaptures = 0;
   // Learned preconditioner for the inner FP32 solves (mixed mode only).
@@ -243,7 +246,13 @@ static int qg_step_impl(QGPlan& Q, TS* q, double* psi, int nsteps, QGInfoC* info
                              Q.max_iters, Q.inner_rtol, &ci, 0, s,
                              Q.net ? &net_precond : nullptr, Q.net ? &Q : nullptr);
         Q.fallbacks += ci.pad;
-        if (!rc) Q.obs_inner = std::max(Q.obs_inner, plan_max_inner(*Q.mg, 0));
+        if (!rc) {
+          Q.obs_inner = std::max(Q.obs_inner, plan_max_inner(*Q.mg, 0));
+          int per[QGPlan::kMaxOuter];
+          const int no = plan_inner_iters(*Q.mg, 0, per, QGPlan::kMaxOuter);
+          for (int o = 0; o < no; ++o) Q.obs_inner_o[o] = std::max(Q.obs_inner_o[o], per[o]);
+          Q.obs_outer = std::max(Q.obs_outer, ci.outer);
+        }
       } else {
         if constexpr (std::is_same<TS, double>::value) {
           // Recursive residual to rtol/2 so the certified FP64 true residual
@@ -289,7 +298,7 @@ static int qg_step_enqueue(QGPlan& Q, const TS* q_in, TS* q_out, hipStream_t s) 
     int rc;
     if (Q.solve == kSolveMixed) {
       rc = mixed_enqueue<TS>(*Q.mg, qs + off, Nx, psi + off, Nx, Q.rtol, 0.0, Q.b_outer,
-                             Q.b_inner, Q.inner_rtol, k, s, Q.net ? &net_precond : nullptr,
+                             Q.b_inner_o, Q.inner_rtol, k, s, Q.net ? &net_precond : nullptr,
                              Q.net ? &Q : nullptr);
     } else {
       if constexpr (std::is_same<TS, double>::value) {
@@ -379,6 +388,8 @@ static int qg_run_graph(QGPlan& Q, TS* q, double* psi, int nsteps, QGInfoC* info
   if (!Q.calibrated) {
     // One synchronous step to observe iteration counts (advances q in place).
     Q.obs_inner = 0;
+    Q.obs_outer = 0;
+    for (int o = 0; o < QGPlan::kMaxOuter; ++o) Q.obs_inner_o[o] = 0;
     QGInfoC ci;
     int rc = qg_step_impl<TS>(Q, q, Q.psi, 1, &ci, s);
     if (rc) return rc;
@@ -389,7 +400,7 @@ static int qg_run_graph(QGPlan& Q, TS* q, double* psi, int nsteps, QGInfoC* info
     acc.max_rel_residual = std::max(acc.max_rel_residual, ci.max_rel_residual);
     if (!ci.converged) acc.converged = 0;
     Q.obs_iters = ci.max_iterations;
-    Q.obs_outer = std::max(1, (ci.outer + ci.solves - 1) / std::max(1, ci.solves));
+    if (Q.solve != kSolveMixed) Q.obs_outer = 1;
     done_steps = 1;
   }
   if (done_steps < nsteps) {
@@ -398,10 +409,16 @@ static int qg_run_graph(QGPlan& Q, TS* q, double* psi, int nsteps, QGInfoC* info
     int want_outer = std::max(1, Q.obs_outer);
     // Inner budget per refinement sweep: the largest inner solve seen.
     int want_inner = std::max(2, (Q.solve == kSolveMixed ? Q.obs_inner : Q.obs_iters) + 1);
+    want_outer = std::min(want_outer, QGPlan::kMaxOuter);
     if (!Q.calibrated || Q.exec[0] == nullptr) {
       Q.b_iters = want_iters;
       Q.b_outer = want_outer;
       Q.b_inner = want_inner;
+      // Per refinement step: what that step took in the calibration solves,
+      // plus one (steps never observed get the largest inner budget).
+      for (int o = 0; o < QGPlan::kMaxOuter; ++o)
+        Q.b_inner_o[o] = Q.obs_inner_o[o] > 0 ? std::min(Q.max_iters, Q.obs_inner_o[o] + 1)
+                                              : want_inner;
       int rc = capture_graphs<TS>(Q);
       if (rc) return rc;
       Q.calibrated = true;
@@ -446,7 +463,9 @@ static int qg_run_graph(QGPlan& Q, TS* q, double* psi, int nsteps, QGInfoC* info
       acc.retries += 1;
       Q.b_iters = std::min(Q.max_iters, Q.b_iters + 2);
       Q.b_inner = std::min(Q.max_iters, Q.b_inner + 1);
-      Q.b_outer = std::min(Q.max_outer, Q.b_outer + 1);
+      for (int o = 0; o < QGPlan::kMaxOuter; ++o)
+        Q.b_inner_o[o] = std::min(Q.max_iters, Q.b_inner_o[o] + 1);
+      Q.b_outer = std::min(std::min(Q.max_outer, QGPlan::kMaxOuter), Q.b_outer + 1);
       int rc = capture_graphs<TS>(Q);
       if (rc) return rc;
     }
@@ -516,8 +535,25 @@ int fvx_qg_plan_create(int ny, int nx, double Lx, double Ly, double beta, double
     }
   }
   void* mg = nullptr;
-  int rc = fvx_mg_plan_create(m.data(), nullptr, ny, nx, Q->dx, Q->dy, Q->lam, 0, n_pre, n_post,
-                              n_coarse, omega, 1 /*dirichlet*/,
+  // Hierarchy depth: coarsen until the cell size reaches 2 Ld.  Below that the
+  // operator is dominated by 1/Ld^2 (lambda h^2 >= 4) and the damped-Jacobi
+  // bottom solve already converges; deeper levels only add latency-bound
+  // launches without changing the PCG iteration count.
+  int n_levels = 0;  // 0 = as deep as the grid halves
+  if (Q->lam > 0.0) {
+    const double h = std::min(Q->dx, Q->dy);
+    const double ratio = 2.0 / (std::sqrt(Q->lam) * h);  // 2 Ld / h
+    n_levels = std::max(2, 1 + (int)std::ceil(std::log2(std::max(ratio, 1.0))));
+    int deepest = 1, hy = ny, hx = nx;  // the halving rule of multigrid.py:50-76
+    while (hy % 2 == 0 && hx % 2 == 0 && hy / 2 >= 8 && hx / 2 >= 8) {
+      hy /= 2;
+      hx /= 2;
+      ++deepest;
+    }
+    n_levels = std::min(n_levels, deepest);
+  }
+  int rc = fvx_mg_plan_create(m.data(), nullptr, ny, nx, Q->dx, Q->dy, Q->lam, n_levels, n_pre,
+                              n_post, n_coarse, omega, 1 /*dirichlet*/,
                               solve_mode == kSolveMixed ? 3264 : 64, &mg);
   if (rc) {
     delete Q;
