@@ -958,8 +958,11 @@ __global__ __launch_bounds__(256) void pcg_dir_kernel(LevelDev<T> L, const T* __
     }
     if (col) {
       const T qv = (lap - L.lam * pc) * mval<T>(L.mask, k);
-      p_out[k] = pc;
-      q[k] = qv;
+      // Streaming stores: both arrays are consumed by the next kernel from
+      // HBM anyway (they exceed L2), and write-allocation here measured as
+      // extra fetch traffic.
+      __builtin_nontemporal_store(pc, &p_out[k]);
+      __builtin_nontemporal_store(qv, &q[k]);
       pq += (double)pc * qv;
     }
     ps = pc;
