@@ -169,8 +169,8 @@ __global__ __launch_bounds__(kFinishThreads) void reduce_finish_kernel(
 // Operator at a cell of an LDS tile (u, cx, cy indexed by tile coordinates).
 // ---------------------------------------------------------------------------
 template <typename T, int KIND, int W>
-__device__ __forceinline__ T tile_Au(const T* U, const T* CX, const T* CY, const T* M, int y,
-                                     int x, T idx2, T idy2, T lam) {
+__device__ __forceinline__ T tile_Au(const T* U, const T* CX, const T* CY, T m, int y, int x,
+                                     T idx2, T idy2, T lam) {
   const int k = y * W + x;
   const T u = U[k];
   T lap;
@@ -180,7 +180,7 @@ __device__ __forceinline__ T tile_Au(const T* U, const T* CX, const T* CY, const
     lap = (CX[k] * (U[k + 1] - u) - CX[k - 1] * (u - U[k - 1])) * idx2 +
           (CY[k] * (U[k + W] - u) - CY[k - W] * (u - U[k - W])) * idy2;
   }
-  return (lap - lam * u) * M[k];
+  return (lap - lam * u) * m;
 }
 
 // Operator at a cell of a global level array (zero outside the array).
@@ -288,7 +288,10 @@ __device__ __forceinline__ T prolong_at(const LevelDev<T>& C, int j, int i) {
   return w / (d > T(1) ? d : T(1));
 }
 
-template <typename T, int KIND, int NS, int MODE, bool ZERO>
+// MK: the level has a land mask (staged in LDS); without one a cell's mask is
+// just "inside the level", computed on the fly, which saves LDS (and raises
+// occupancy) for all-wet levels.
+template <typename T, int KIND, int NS, int MODE, bool ZERO, bool MK>
 __global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
   // The convergence guard is checked after the halo loads are issued, so its
   // round trip overlaps theirs.
@@ -297,7 +300,7 @@ __global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
   constexpr int W = kTx + 2 * H, Hh = kTy + 2 * H, N = W * Hh;
   __shared__ T sU[2][N];
   __shared__ T sB[N];
-  __shared__ T sM[N];
+  __shared__ T sM[MK ? N : 1];
   __shared__ T sCX[KIND == kNoflux ? N : 1];
   __shared__ T sCY[KIND == kNoflux ? N : 1];
   __shared__ T sI[KIND == kNoflux ? N : 1];
@@ -314,7 +317,7 @@ __global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
   // ci0 .. ci0 + CW - 1 of the coarse level (zero outside it).
   constexpr int CW = kTx / 2 + 4, CH = kTy / 2 + 4;
   __shared__ T sCu[kUpMode ? CW * CH : 1];
-  __shared__ T sCm[kUpMode ? CW * CH : 1];
+  __shared__ T sCm[(kUpMode && MK) ? CW * CH : 1];
   const int cj0 = ((j0 - H) >> 1) - 1, ci0 = ((i0 - H) >> 1) - 1;
   T rU[kUpMode ? IT : 1], rM[kUpMode ? IT : 1];
 
@@ -329,7 +332,7 @@ __global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
     const bool in = (gj >= 0 && gj < L.ny && gi >= 0 && gi < L.nx);
     const int g = in ? gj * L.nx + gi : 0;
     const T m = in ? mval<T>(L.mask, g) : T(0);
-    sM[k] = m;
+    if constexpr (MK) sM[k] = m;
     sB[k] = in ? a.b[g] : T(0);
     if constexpr (KIND == kNoflux) {
       sCX[k] = in ? L.cx[g] : T(0);
@@ -355,7 +358,7 @@ __global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
       const bool in = (J >= 0 && J < C.ny && I >= 0 && I < C.nx);
       const int g = in ? J * C.nx + I : 0;
       const T m = in ? mval<T>(C.mask, g) : T(0);
-      sCm[k] = m;
+      if constexpr (MK) sCm[k] = m;
       sCu[k] = in ? C.u[g] * m : T(0);
     }
     __syncthreads();
@@ -371,7 +374,16 @@ __global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
       const int sj = (gj & 1) ? CW : -CW, si = (gi & 1) ? 1 : -1;
       const int q = cy * CW + cx;
       const T w = T(9) * sCu[q] + T(3) * sCu[q + si] + T(3) * sCu[q + sj] + sCu[q + sj + si];
-      const T d = T(9) * sCm[q] + T(3) * sCm[q + si] + T(3) * sCm[q + sj] + sCm[q + sj + si];
+      T d;
+      if constexpr (MK) {
+        d = T(9) * sCm[q] + T(3) * sCm[q + si] + T(3) * sCm[q + sj] + sCm[q + sj + si];
+      } else {  // coarse mask = inside the coarse level
+        const int J = gj >> 1, I = gi >> 1, Jn = J + ((gj & 1) ? 1 : -1),
+                  In = I + ((gi & 1) ? 1 : -1);
+        const T mjc = (J >= 0 && J < a.C.ny) ? T(1) : T(0), mjn = (Jn >= 0 && Jn < a.C.ny) ? T(1) : T(0);
+        const T mic = (I >= 0 && I < a.C.nx) ? T(1) : T(0), min_ = (In >= 0 && In < a.C.nx) ? T(1) : T(0);
+        d = T(9) * mjc * mic + T(3) * mjc * min_ + T(3) * mjn * mic + mjn * min_;
+      }
       const T dd = d > T(1) ? d : T(1);
       T e;
       if constexpr (std::is_same<T, float>::value) e = w * __builtin_amdgcn_rcpf(dd);
@@ -383,6 +395,15 @@ __global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
   if (skip) return;
   __syncthreads();
 
+  // Mask of tile cell (y, x).
+  auto Mq = [&](int y, int x) -> T {
+    if constexpr (MK) {
+      return sM[y * W + x];
+    } else {
+      const int gj = j0 - H + y, gi = i0 - H + x;
+      return (gj >= 0 && gj < L.ny && gi >= 0 && gi < L.nx) ? T(1) : T(0);
+    }
+  };
   const T om = a.omega;
   int cur = 0;
 #pragma unroll
@@ -393,15 +414,16 @@ __global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
     for (int k = tid; k < ny_ * nx_; k += kTileThreads) {
       const int y = lo + k / nx_, x = lo + k % nx_;
       const int q = y * W + x;
+      const T mq = Mq(y, x);
       T inv;
-      if constexpr (KIND == kDirichlet) inv = L.inv_d * sM[q];
+      if constexpr (KIND == kDirichlet) inv = L.inv_d * mq;
       else inv = sI[q];
       T un;
       if (ZERO && s == 1 && MODE != kUp && MODE != kUpDots) {
-        un = (om * inv * sB[q]) * sM[q];
+        un = (om * inv * sB[q]) * mq;
       } else {
-        const T au = tile_Au<T, KIND, W>(sU[cur], sCX, sCY, sM, y, x, L.idx2, L.idy2, L.lam);
-        un = (sU[cur][q] + om * inv * (sB[q] - au)) * sM[q];
+        const T au = tile_Au<T, KIND, W>(sU[cur], sCX, sCY, mq, y, x, L.idx2, L.idy2, L.lam);
+        un = (sU[cur][q] + om * inv * (sB[q] - au)) * mq;
       }
       sU[cur ^ 1][q] = un;
     }
@@ -415,8 +437,9 @@ __global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
     for (int k = tid; k < kTx * kTy; k += kTileThreads) {
       const int y = H + k / kTx, x = H + k % kTx;
       const int q = y * W + x;
-      const T au = tile_Au<T, KIND, W>(sU[cur], sCX, sCY, sM, y, x, L.idx2, L.idy2, L.lam);
-      R[q] = (sB[q] - au) * sM[q];
+      const T mq = Mq(y, x);
+      const T au = tile_Au<T, KIND, W>(sU[cur], sCX, sCY, mq, y, x, L.idx2, L.idy2, L.lam);
+      R[q] = (sB[q] - au) * mq;
     }
     __syncthreads();
     // 2x2 mask-weighted restriction (multigrid.py:464-507).
@@ -427,8 +450,9 @@ __global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
       if (J >= C.ny || I >= C.nx) continue;
       const int y = H + 2 * cy_, x = H + 2 * cx_;
       const int q00 = y * W + x, q10 = q00 + W, q01 = q00 + 1, q11 = q00 + W + 1;
-      const T s4 = R[q00] * sM[q00] + R[q10] * sM[q10] + R[q01] * sM[q01] + R[q11] * sM[q11];
-      const T cnt = sM[q00] + sM[q10] + sM[q01] + sM[q11];
+      const T m00 = Mq(y, x), m10 = Mq(y + 1, x), m01 = Mq(y, x + 1), m11 = Mq(y + 1, x + 1);
+      const T s4 = R[q00] * m00 + R[q10] * m10 + R[q01] * m01 + R[q11] * m11;
+      const T cnt = m00 + m10 + m01 + m11;
       const int64_t g = (int64_t)J * C.nx + I;
       C.b[g] = (s4 / (cnt > T(1) ? cnt : T(1))) * mval<T>(C.mask, g);
     }
@@ -898,11 +922,18 @@ __global__ __launch_bounds__(256) void pcg_init_kernel(LevelDev<T> L, const T* _
 }
 
 // PCG: p_out = z + beta p_in ; q = A p_out ; partial p.q -> alpha = rho / pq.
-// Each wave owns a 64-column strip of kDirRows rows and slides a three-row
-// register window down it (every row of z, p read once per strip); east/west
-// neighbours come from lane shuffles, the strip's edge columns from memory.
-constexpr int kDirRows = 8;
+// Each wave owns a 64-column strip of kStencilRows rows and slides a
+// three-row register window down it (every row of z, p read once per strip);
+// east/west neighbours come from lane shuffles, a strip's two edge columns
+// from memory.  The four waves of a block sit side by side (256 columns), so
+// three of their four edge loads hit the cache lines their neighbour wave on
+// the same CU just read.
+constexpr int kDirRows = 8;       // rows per wave of the pointwise update kernel
+constexpr int kStencilRows = 16;  // rows per wave of the direction kernel
 static inline dim3 grid_dir(int ny, int nx) { return dim3(cdiv(nx, 64), cdiv(ny, 4 * kDirRows)); }
+static inline dim3 grid_stencil(int ny, int nx) {
+  return dim3(cdiv(nx, 256), cdiv(ny, kStencilRows));
+}
 
 template <typename T, int KIND>
 __global__ __launch_bounds__(256) void pcg_dir_kernel(LevelDev<T> L, const T* __restrict__ z,
@@ -911,8 +942,8 @@ __global__ __launch_bounds__(256) void pcg_dir_kernel(LevelDev<T> L, const T* __
                                                       double* partials, PcgScalars* sc) {
   const int done = sc->done;  // checked once the first rows are in flight
   const int lane = threadIdx.x;
-  const int i = blockIdx.x * 64 + lane;
-  const int j0 = (blockIdx.y * 4 + threadIdx.y) * kDirRows;  // uniform within a wave
+  const int i = (blockIdx.x * 4 + threadIdx.y) * 64 + lane;
+  const int j0 = blockIdx.y * kStencilRows;  // uniform within a wave
   const bool first = sc->first;
   const T beta = first ? T(0) : (T)sc->beta;
   const int nx = L.nx, ny = L.ny;
@@ -936,7 +967,7 @@ __global__ __launch_bounds__(256) void pcg_dir_kernel(LevelDev<T> L, const T* __
   T ps = Prow(j0 - 1), pc = Prow(j0);
   T cys = CY(j0 - 1);
   if (done) return;
-  for (int r = 0; r < kDirRows; ++r) {
+  for (int r = 0; r < kStencilRows; ++r) {
     const int j = j0 + r;
     if (j >= ny) break;
     const T pn = Prow(j + 1);
@@ -958,11 +989,8 @@ __global__ __launch_bounds__(256) void pcg_dir_kernel(LevelDev<T> L, const T* __
     }
     if (col) {
       const T qv = (lap - L.lam * pc) * mval<T>(L.mask, k);
-      // Streaming stores: both arrays are consumed by the next kernel from
-      // HBM anyway (they exceed L2), and write-allocation here measured as
-      // extra fetch traffic.
-      __builtin_nontemporal_store(pc, &p_out[k]);
-      __builtin_nontemporal_store(qv, &q[k]);
+      p_out[k] = pc;
+      q[k] = qv;
       pq += (double)pc * qv;
     }
     ps = pc;
@@ -1345,8 +1373,12 @@ static void launch_tile(const Plan& P, const LevelDev<T>& L, const LevelDev<T>* 
   a.sc = sc;
   a.tiles_x = cdiv(L.nx, kTx);
   a.ntiles = a.tiles_x * cdiv(L.ny, kTy);
-  hipLaunchKernelGGL((mg_tile_kernel<T, KIND, NS, MODE, ZERO>), dim3(a.ntiles), dim3(kTileThreads),
-                     0, s, a);
+  if (L.mask)
+    hipLaunchKernelGGL((mg_tile_kernel<T, KIND, NS, MODE, ZERO, true>), dim3(a.ntiles),
+                       dim3(kTileThreads), 0, s, a);
+  else
+    hipLaunchKernelGGL((mg_tile_kernel<T, KIND, NS, MODE, ZERO, false>), dim3(a.ntiles),
+                       dim3(kTileThreads), 0, s, a);
   if constexpr (MODE == kUpDots)
     if (finish)
       hipLaunchKernelGGL((reduce_finish_kernel<2, kRedUpDots>), dim3(1), dim3(kFinishThreads), 0,
@@ -1561,11 +1593,11 @@ static int pcg_iters(Plan& P, Hier<T>& H, T* x, int64_t ld_x, int n, PcgScalars*
                          H.z, H.r, H.q, P.partials, sc);
       finish2<2, kRedDots>(P, L.ny, L.nx, sc, s);
     }
-    hipLaunchKernelGGL((pcg_dir_kernel<T, KIND>), grid_dir(L.ny, L.nx), dim3(64, 4), 0, s, L,
+    hipLaunchKernelGGL((pcg_dir_kernel<T, KIND>), grid_stencil(L.ny, L.nx), dim3(64, 4), 0, s, L,
                        H.z, H.p0, H.p1, H.q, P.partials, sc);
     hipLaunchKernelGGL((reduce_finish_kernel<1, kRedDir>), dim3(1), dim3(kFinishThreads), 0, s,
-                       P.partials, (int)(grid_dir(L.ny, L.nx).x * grid_dir(L.ny, L.nx).y), sc,
-                       0.0, 0.0);
+                       P.partials, (int)(grid_stencil(L.ny, L.nx).x * grid_stencil(L.ny, L.nx).y),
+                       sc, 0.0, 0.0);
     std::swap(H.p0, H.p1);
     hipLaunchKernelGGL(pcg_update_kernel<T>, grid_dir(L.ny, L.nx), dim3(64, 4), 0, s, L.ny, L.nx,
                        x, ld_x, H.p0, H.r, H.q, P.partials, sc);
@@ -1965,8 +1997,8 @@ static int kernel_bench_impl(Plan& P, int which, int reps, hipStream_t s) {
   }
   for (int k = 0; k < reps; ++k) {
     if (which == 0) {
-      hipLaunchKernelGGL((pcg_dir_kernel<T, KIND>), grid_dir(L.ny, L.nx), dim3(64, 4), 0, s, L,
-                         H.z, H.p0, H.p1, H.q, P.partials, sc);
+      hipLaunchKernelGGL((pcg_dir_kernel<T, KIND>), grid_stencil(L.ny, L.nx), dim3(64, 4), 0, s,
+                         L, H.z, H.p0, H.p1, H.q, P.partials, sc);
     } else if (which == 1) {
       hipLaunchKernelGGL(pcg_update_kernel<T>, grid_dir(L.ny, L.nx), dim3(64, 4), 0, s, L.ny,
                          L.nx, H.x, (int64_t)L.nx, H.p0, H.r, H.q, P.partials, sc);
