@@ -929,7 +929,7 @@ __global__ __launch_bounds__(256) void pcg_init_kernel(LevelDev<T> L, const T* _
 // three of their four edge loads hit the cache lines their neighbour wave on
 // the same CU just read.
 constexpr int kDirRows = 8;       // rows per wave of the pointwise update kernel
-constexpr int kStencilRows = 16;  // rows per wave of the direction kernel
+constexpr int kStencilRows = 8;   // rows per wave of the direction kernel
 static inline dim3 grid_dir(int ny, int nx) { return dim3(cdiv(nx, 64), cdiv(ny, 4 * kDirRows)); }
 static inline dim3 grid_stencil(int ny, int nx) {
   return dim3(cdiv(nx, 256), cdiv(ny, kStencilRows));
