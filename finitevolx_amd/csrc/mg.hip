@@ -23,7 +23,6 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
-#include <type_traits>
 #include <vector>
 
 #include "fvx_common.hpp"
@@ -311,10 +310,6 @@ __global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
   const int i0 = (t % a.tiles_x) * kTx, j0 = (t / a.tiles_x) * kTy;
   const int tid = threadIdx.x;
 
-  // Tiles whose halo lies inside the level need no range checks (all-wet
-  // levels: their mask is 1 everywhere); uniform per block.
-  const bool inner_tile = !MK && j0 - H >= 0 && j0 + kTy + H <= L.ny && i0 - H >= 0 &&
-                          i0 + kTx + H <= L.nx;
   constexpr bool kUpMode = (MODE == kUp || MODE == kUpDots);
   constexpr int IT = (N + kTileThreads - 1) / kTileThreads;
   // Level l+1's iterate (times its mask) and mask around this tile for the
@@ -328,22 +323,13 @@ __global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
 
   // Fully unrolled (the trip count is a compile-time constant) so the
   // global loads of all iterations are in flight together; 32-bit offsets.
-  // Tile coordinates advance incrementally (kTileThreads = dY W + dX).
-  constexpr int dY = kTileThreads / W, dX = kTileThreads % W;
-  int ty_ = tid / W, tx_ = tid % W;
 #pragma unroll
   for (int it = 0; it < IT; ++it) {
     const int k = tid + it * kTileThreads;
     if (k >= N) break;
-    const int y = ty_, x = tx_;
-    tx_ += dX;
-    ty_ += dY;
-    if (tx_ >= W) {
-      tx_ -= W;
-      ++ty_;
-    }
+    const int y = k / W, x = k % W;
     const int gj = j0 - H + y, gi = i0 - H + x;
-    const bool in = inner_tile || (gj >= 0 && gj < L.ny && gi >= 0 && gi < L.nx);
+    const bool in = (gj >= 0 && gj < L.ny && gi >= 0 && gi < L.nx);
     const int g = in ? gj * L.nx + gi : 0;
     const T m = in ? mval<T>(L.mask, g) : T(0);
     if constexpr (MK) sM[k] = m;
@@ -420,24 +406,15 @@ __global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
   };
   const T om = a.omega;
   int cur = 0;
-  auto sweep = [&](auto inner, int s) {
+#pragma unroll
+  for (int s = 1; s <= NS; ++s) {
     // Valid region shrinks by one cell per sweep.
     const int lo = s, hiy = Hh - s, hix = W - s;
     const int ny_ = hiy - lo, nx_ = hix - lo;
-    const int sdY = kTileThreads / nx_, sdX = kTileThreads % nx_;
-    int sy = tid / nx_, sx = tid % nx_;
     for (int k = tid; k < ny_ * nx_; k += kTileThreads) {
-      const int y = lo + sy, x = lo + sx;
-      sx += sdX;
-      sy += sdY;
-      if (sx >= nx_) {
-        sx -= nx_;
-        ++sy;
-      }
+      const int y = lo + k / nx_, x = lo + k % nx_;
       const int q = y * W + x;
-      T mq;
-      if constexpr (decltype(inner)::value) mq = T(1);
-      else mq = Mq(y, x);
+      const T mq = Mq(y, x);
       T inv;
       if constexpr (KIND == kDirichlet) inv = L.inv_d * mq;
       else inv = sI[q];
@@ -450,11 +427,6 @@ __global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
       }
       sU[cur ^ 1][q] = un;
     }
-  };
-#pragma unroll
-  for (int s = 1; s <= NS; ++s) {
-    if (inner_tile) sweep(std::true_type{}, s);
-    else sweep(std::false_type{}, s);
     __syncthreads();
     cur ^= 1;
   }
@@ -1030,28 +1002,18 @@ __global__ __launch_bounds__(256) void pcg_dir_kernel(LevelDev<T> L, const T* __
 }
 
 // PCG: x += alpha p ; r -= alpha q ; ||r||^2 -> convergence flag.  Each wave
-// updates a 64-column strip of kDirRows rows (grid_dir).  alpha = rho / p.Ap
-// is reduced here from the direction kernel's partials (dir_partials,
-// dir_nblk of them), by every block in the same fixed order, so the
-// direction kernel needs no finishing launch; block (0, 0) records p.Ap and
-// alpha for the host and the next kernels.
+// updates a 64-column strip of kDirRows rows (grid_dir), so the finishing
+// reduction has as few partials as the direction kernel's.
 template <typename T>
 __global__ __launch_bounds__(256) void pcg_update_kernel(int ny, int nx, T* __restrict__ x,
                                                          int64_t ld_x, const T* __restrict__ p,
                                                          T* __restrict__ r,
                                                          const T* __restrict__ q,
-                                                         const double* __restrict__ dir_partials,
-                                                         int dir_nblk, double* partials,
-                                                         PcgScalars* sc) {
-  __shared__ double sred[4];
-  __shared__ double salpha;
+                                                         double* partials, PcgScalars* sc) {
   const int done = sc->done;
-  const double rz = sc->rz;
-  const int tid = lin_tid();
   const int i = blockIdx.x * 64 + threadIdx.x;
   const int j0 = (blockIdx.y * 4 + threadIdx.y) * kDirRows;
-  double pqp = 0.0;
-  for (int b = tid; b < dir_nblk; b += 256) pqp += dir_partials[b];
+  const T alpha = (T)sc->alpha;
   double rr = 0.0;
   // All loads first (their round trip overlaps the scalars'), then the guard.
   T xv[kDirRows], pv[kDirRows], rv0[kDirRows], qv[kDirRows];
@@ -1066,20 +1028,6 @@ __global__ __launch_bounds__(256) void pcg_update_kernel(int ny, int nx, T* __re
     qv[t] = in ? q[k] : T(0);
   }
   if (done) return;
-  pqp = wave_sum(pqp);
-  if ((tid & 63) == 0) sred[tid >> 6] = pqp;
-  __syncthreads();
-  if (tid == 0) {
-    const double pq = sred[0] + sred[1] + sred[2] + sred[3];
-    salpha = rz / pq;
-    if (blockIdx.x == 0 && blockIdx.y == 0) {
-      sc->pq = pq;
-      sc->alpha = salpha;
-      sc->first = 0;
-    }
-  }
-  __syncthreads();
-  const T alpha = (T)salpha;
 #pragma unroll
   for (int t = 0; t < kDirRows; ++t) {
     const int j = j0 + t;
@@ -1225,7 +1173,6 @@ struct Plan {
   Hier<double> h64;
   Hier<float> h32;
   double* partials = nullptr;
-  double* partials_alt = nullptr;  // second partials region (read while partials is written)
   PcgScalars* sc = nullptr;   // kSlots pairs: [2k] inner, [2k+1] outer
   PcgScalars* hsc = nullptr;  // pinned host mirror
   int last_iters = 4, last_outer = 2, last_inner = 4;
@@ -1646,13 +1593,14 @@ static int pcg_iters(Plan& P, Hier<T>& H, T* x, int64_t ld_x, int n, PcgScalars*
                          H.z, H.r, H.q, P.partials, sc);
       finish2<2, kRedDots>(P, L.ny, L.nx, sc, s);
     }
-    const dim3 gs = grid_stencil(L.ny, L.nx);
-    hipLaunchKernelGGL((pcg_dir_kernel<T, KIND>), gs, dim3(64, 4), 0, s, L, H.z, H.p0, H.p1, H.q,
-                       P.partials_alt, sc);
+    hipLaunchKernelGGL((pcg_dir_kernel<T, KIND>), grid_stencil(L.ny, L.nx), dim3(64, 4), 0, s, L,
+                       H.z, H.p0, H.p1, H.q, P.partials, sc);
+    hipLaunchKernelGGL((reduce_finish_kernel<1, kRedDir>), dim3(1), dim3(kFinishThreads), 0, s,
+                       P.partials, (int)(grid_stencil(L.ny, L.nx).x * grid_stencil(L.ny, L.nx).y),
+                       sc, 0.0, 0.0);
     std::swap(H.p0, H.p1);
     hipLaunchKernelGGL(pcg_update_kernel<T>, grid_dir(L.ny, L.nx), dim3(64, 4), 0, s, L.ny, L.nx,
-                       x, ld_x, H.p0, H.r, H.q, P.partials_alt, (int)(gs.x * gs.y), P.partials,
-                       sc);
+                       x, ld_x, H.p0, H.r, H.q, P.partials, sc);
     hipLaunchKernelGGL((reduce_finish_kernel<1, kRedUpdate>), dim3(1), dim3(kFinishThreads), 0, s,
                        P.partials, (int)(grid_dir(L.ny, L.nx).x * grid_dir(L.ny, L.nx).y), sc,
                        0.0, 0.0);
@@ -2050,12 +1998,10 @@ static int kernel_bench_impl(Plan& P, int which, int reps, hipStream_t s) {
   for (int k = 0; k < reps; ++k) {
     if (which == 0) {
       hipLaunchKernelGGL((pcg_dir_kernel<T, KIND>), grid_stencil(L.ny, L.nx), dim3(64, 4), 0, s,
-                         L, H.z, H.p0, H.p1, H.q, P.partials_alt, sc);
+                         L, H.z, H.p0, H.p1, H.q, P.partials, sc);
     } else if (which == 1) {
-      const dim3 gs = grid_stencil(L.ny, L.nx);
       hipLaunchKernelGGL(pcg_update_kernel<T>, grid_dir(L.ny, L.nx), dim3(64, 4), 0, s, L.ny,
-                         L.nx, H.x, (int64_t)L.nx, H.p0, H.r, H.q, P.partials_alt,
-                         (int)(gs.x * gs.y), P.partials, sc);
+                         L.nx, H.x, (int64_t)L.nx, H.p0, H.r, H.q, P.partials, sc);
     } else {
       // The last kernel of a preconditioning V-cycle on level 0: prolongation
       // of level 1's iterate, correction, two sweeps, flexible-PCG dots.
@@ -2198,7 +2144,6 @@ int fvx_mg_plan_create(const double* mask, const double* coeff, int ny, int nx, 
   if (!rc) {
     const size_t maxblk = (size_t)cdiv(nx, 64) * cdiv(ny, 4) + 64;
     hipError_t e = hipMalloc(&P->partials, maxblk * 4 * sizeof(double));
-    P->partials_alt = P->partials + 2 * maxblk;
     if (e == hipSuccess) e = hipMalloc(&P->sc, 2 * kSlots * sizeof(PcgScalars));
     if (e == hipSuccess) e = hipMemset(P->sc, 0, 2 * kSlots * sizeof(PcgScalars));
     if (e == hipSuccess)
