@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "fvx_common.hpp"
@@ -310,6 +311,10 @@ __global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
   const int i0 = (t % a.tiles_x) * kTx, j0 = (t / a.tiles_x) * kTy;
   const int tid = threadIdx.x;
 
+  // Tiles whose halo lies inside the level need no range checks (all-wet
+  // levels: their mask is 1 everywhere); uniform per block.
+  const bool inner_tile = !MK && j0 - H >= 0 && j0 + kTy + H <= L.ny && i0 - H >= 0 &&
+                          i0 + kTx + H <= L.nx;
   constexpr bool kUpMode = (MODE == kUp || MODE == kUpDots);
   constexpr int IT = (N + kTileThreads - 1) / kTileThreads;
   // Level l+1's iterate (times its mask) and mask around this tile for the
@@ -323,13 +328,22 @@ __global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
 
   // Fully unrolled (the trip count is a compile-time constant) so the
   // global loads of all iterations are in flight together; 32-bit offsets.
+  // Tile coordinates advance incrementally (kTileThreads = dY W + dX).
+  constexpr int dY = kTileThreads / W, dX = kTileThreads % W;
+  int ty_ = tid / W, tx_ = tid % W;
 #pragma unroll
   for (int it = 0; it < IT; ++it) {
     const int k = tid + it * kTileThreads;
     if (k >= N) break;
-    const int y = k / W, x = k % W;
+    const int y = ty_, x = tx_;
+    tx_ += dX;
+    ty_ += dY;
+    if (tx_ >= W) {
+      tx_ -= W;
+      ++ty_;
+    }
     const int gj = j0 - H + y, gi = i0 - H + x;
-    const bool in = (gj >= 0 && gj < L.ny && gi >= 0 && gi < L.nx);
+    const bool in = inner_tile || (gj >= 0 && gj < L.ny && gi >= 0 && gi < L.nx);
     const int g = in ? gj * L.nx + gi : 0;
     const T m = in ? mval<T>(L.mask, g) : T(0);
     if constexpr (MK) sM[k] = m;
@@ -406,15 +420,24 @@ __global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
   };
   const T om = a.omega;
   int cur = 0;
-#pragma unroll
-  for (int s = 1; s <= NS; ++s) {
+  auto sweep = [&](auto inner, int s) {
     // Valid region shrinks by one cell per sweep.
     const int lo = s, hiy = Hh - s, hix = W - s;
     const int ny_ = hiy - lo, nx_ = hix - lo;
+    const int sdY = kTileThreads / nx_, sdX = kTileThreads % nx_;
+    int sy = tid / nx_, sx = tid % nx_;
     for (int k = tid; k < ny_ * nx_; k += kTileThreads) {
-      const int y = lo + k / nx_, x = lo + k % nx_;
+      const int y = lo + sy, x = lo + sx;
+      sx += sdX;
+      sy += sdY;
+      if (sx >= nx_) {
+        sx -= nx_;
+        ++sy;
+      }
       const int q = y * W + x;
-      const T mq = Mq(y, x);
+      T mq;
+      if constexpr (decltype(inner)::value) mq = T(1);
+      else mq = Mq(y, x);
       T inv;
       if constexpr (KIND == kDirichlet) inv = L.inv_d * mq;
       else inv = sI[q];
@@ -427,6 +450,11 @@ __global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
       }
       sU[cur ^ 1][q] = un;
     }
+  };
+#pragma unroll
+  for (int s = 1; s <= NS; ++s) {
+    if (inner_tile) sweep(std::true_type{}, s);
+    else sweep(std::false_type{}, s);
     __syncthreads();
     cur ^= 1;
   }
@@ -1002,8 +1030,9 @@ __global__ __launch_bounds__(256) void pcg_dir_kernel(LevelDev<T> L, const T* __
 }
 
 // PCG: x += alpha p ; r -= alpha q ; ||r||^2 -> convergence flag.  Each wave
-// updates a 64-column strip of kDirRows rows (grid_dir), so the finishing
-// reduction has as few partials as the direction kernel's.
+// updates a 64-column strip of kDirRows rows (grid_dir).  (Reducing the
+// direction kernel's partials here in every block instead of in a finishing
+// launch measured no faster: the reduction moved onto each block's path.)
 template <typename T>
 __global__ __launch_bounds__(256) void pcg_update_kernel(int ny, int nx, T* __restrict__ x,
                                                          int64_t ld_x, const T* __restrict__ p,
@@ -1593,11 +1622,11 @@ static int pcg_iters(Plan& P, Hier<T>& H, T* x, int64_t ld_x, int n, PcgScalars*
                          H.z, H.r, H.q, P.partials, sc);
       finish2<2, kRedDots>(P, L.ny, L.nx, sc, s);
     }
-    hipLaunchKernelGGL((pcg_dir_kernel<T, KIND>), grid_stencil(L.ny, L.nx), dim3(64, 4), 0, s, L,
-                       H.z, H.p0, H.p1, H.q, P.partials, sc);
+    const dim3 gs = grid_stencil(L.ny, L.nx);
+    hipLaunchKernelGGL((pcg_dir_kernel<T, KIND>), gs, dim3(64, 4), 0, s, L, H.z, H.p0, H.p1, H.q,
+                       P.partials, sc);
     hipLaunchKernelGGL((reduce_finish_kernel<1, kRedDir>), dim3(1), dim3(kFinishThreads), 0, s,
-                       P.partials, (int)(grid_stencil(L.ny, L.nx).x * grid_stencil(L.ny, L.nx).y),
-                       sc, 0.0, 0.0);
+                       P.partials, (int)(gs.x * gs.y), sc, 0.0, 0.0);
     std::swap(H.p0, H.p1);
     hipLaunchKernelGGL(pcg_update_kernel<T>, grid_dir(L.ny, L.nx), dim3(64, 4), 0, s, L.ny, L.nx,
                        x, ld_x, H.p0, H.r, H.q, P.partials, sc);
