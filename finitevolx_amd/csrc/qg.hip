@@ -451,7 +451,13 @@ static int qg_run_graph(QGPlan& Q, TS* q, double* psi, int nsteps, QGInfoC* info
         if (rc) return rc;
         continue;
       }
-      if (Q.recaptures > 50 || Q.b_iters >= Q.max_iters || Q.b_outer >= Q.max_outer) {
+      // Give up widening once the budget that can still grow is at its cap
+      // (FP64: PCG iterations; mixed: refinement steps and inner iterations).
+      const bool capped = (Q.solve == kSolveMixed)
+                              ? (Q.b_outer >= std::min(Q.max_outer, QGPlan::kMaxOuter) &&
+                                 Q.b_inner >= Q.max_iters)
+                              : (Q.b_iters >= Q.max_iters);
+      if (Q.recaptures > 50 || capped) {
         acc.converged = 0;
         acc.solves += st.solves;
         acc.max_rel_residual = std::max(acc.max_rel_residual, st.max_rel_residual);
@@ -462,9 +468,11 @@ static int qg_run_graph(QGPlan& Q, TS* q, double* psi, int nsteps, QGInfoC* info
       Q.recaptures += 1;
       acc.retries += 1;
       Q.b_iters = std::min(Q.max_iters, Q.b_iters + 2);
-      Q.b_inner = std::min(Q.max_iters, Q.b_inner + 1);
+      // widen by a quarter (at least one iteration) so long budgets converge
+      // in a few recaptures
+      Q.b_inner = std::min(Q.max_iters, Q.b_inner + std::max(1, Q.b_inner / 4));
       for (int o = 0; o < QGPlan::kMaxOuter; ++o)
-        Q.b_inner_o[o] = std::min(Q.max_iters, Q.b_inner_o[o] + 1);
+        Q.b_inner_o[o] = std::min(Q.max_iters, Q.b_inner_o[o] + std::max(1, Q.b_inner_o[o] / 4));
       Q.b_outer = std::min(std::min(Q.max_outer, QGPlan::kMaxOuter), Q.b_outer + 1);
       int rc = capture_graphs<TS>(Q);
       if (rc) return rc;

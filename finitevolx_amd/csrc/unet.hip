@@ -243,15 +243,25 @@ __global__ void conv_first_kernel(const float* __restrict__ r, const float* __re
     nb[t] = (y >= 0 && y < H && x >= 0 && x < W) ? r[(int64_t)y * W + x] : 0.f;
   }
   const int64_t p = (int64_t)j * W + i;
-  for (int c = 0; c < C; ++c) {
-    float s = 0.f;
+  // 8 channels per 16-byte store (C is a multiple of 16).
+  for (int c8 = 0; c8 < C; c8 += 8) {
+    uint32_t pk[4];
 #pragma unroll
-    for (int t = 0; t < 9; ++t) s += w[t * C + c] * nb[t];
-    out[p * C + c] = f2bf(fmaxf(s, 0.f));
+    for (int h = 0; h < 4; ++h) {
+      float s0 = 0.f, s1 = 0.f;
+#pragma unroll
+      for (int t = 0; t < 9; ++t) {
+        s0 += w[t * C + c8 + 2 * h] * nb[t];
+        s1 += w[t * C + c8 + 2 * h + 1] * nb[t];
+      }
+      pk[h] = (uint32_t)f2bf(fmaxf(s0, 0.f)) | ((uint32_t)f2bf(fmaxf(s1, 0.f)) << 16);
+    }
+    *reinterpret_cast<uint4*>(out + p * C + c8) = make_uint4(pk[0], pk[1], pk[2], pk[3]);
   }
 }
 
 // y[p] = scale * sum_tap sum_c w[tap][c] d[p+tap][c]; optional mask multiply.
+// Each pixel's channels are read 8 at a time (16-byte loads).
 template <typename TO>
 __global__ void conv_last_kernel(const uint16_t* __restrict__ d, const float* __restrict__ w,
                                  int C, TO* __restrict__ y, int64_t ld_y, int H, int W,
@@ -261,11 +271,20 @@ __global__ void conv_last_kernel(const uint16_t* __restrict__ d, const float* __
   const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
   if (i >= W || j >= H) return;
   float s = 0.f;
+#pragma unroll
   for (int t = 0; t < 9; ++t) {
     const int yy = j + t / 3 - 1, xx = i + t % 3 - 1;
     if (yy < 0 || yy >= H || xx < 0 || xx >= W) continue;
     const uint16_t* dp = d + ((int64_t)yy * W + xx) * C;
-    for (int c = 0; c < C; ++c) s += w[t * C + c] * bf2f(dp[c]);
+    for (int c8 = 0; c8 < C; c8 += 8) {
+      const uint4 v = *reinterpret_cast<const uint4*>(dp + c8);
+      const uint32_t u[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int h = 0; h < 4; ++h) {
+        s += w[t * C + c8 + 2 * h] * bf2f((uint16_t)(u[h] & 0xffffu));
+        s += w[t * C + c8 + 2 * h + 1] * bf2f((uint16_t)(u[h] >> 16));
+      }
+    }
   }
   if (scale) s *= *scale;
   if (mask && !mask[(int64_t)j * W + i]) s = 0.f;
