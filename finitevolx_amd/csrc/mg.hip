@@ -315,6 +315,9 @@ __global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
   // levels: their mask is 1 everywhere); uniform per block.
   const bool inner_tile = !MK && j0 - H >= 0 && j0 + kTy + H <= L.ny && i0 - H >= 0 &&
                           i0 + kTx + H <= L.nx;
+  // ... and whose prolongation stencil stays inside the coarse level.
+  const bool inner_coarse = !MK && j0 - H >= 1 && j0 + kTy + H <= L.ny - 1 && i0 - H >= 1 &&
+                            i0 + kTx + H <= L.nx - 1;
   constexpr bool kUpMode = (MODE == kUp || MODE == kUpDots);
   constexpr int IT = (N + kTileThreads - 1) / kTileThreads;
   // Level l+1's iterate (times its mask) and mask around this tile for the
@@ -391,6 +394,8 @@ __global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
       T d;
       if constexpr (MK) {
         d = T(9) * sCm[q] + T(3) * sCm[q + si] + T(3) * sCm[q + sj] + sCm[q + sj + si];
+      } else if (inner_coarse) {  // all four coarse cells exist: weights sum to 16
+        d = T(16);
       } else {  // coarse mask = inside the coarse level
         const int J = gj >> 1, I = gi >> 1, Jn = J + ((gj & 1) ? 1 : -1),
                   In = I + ((gi & 1) ? 1 : -1);
@@ -400,7 +405,8 @@ __global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
       }
       const T dd = d > T(1) ? d : T(1);
       T e;
-      if constexpr (std::is_same<T, float>::value) e = w * __builtin_amdgcn_rcpf(dd);
+      if (!MK && inner_coarse) e = w * T(0.0625);  // exact: w / 16
+      else if constexpr (std::is_same<T, float>::value) e = w * __builtin_amdgcn_rcpf(dd);
       else e = w / dd;
       const T m = rM[it];
       sU[0][k] = (rU[it] + e * m) * m;
