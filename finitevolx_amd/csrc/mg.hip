@@ -69,36 +69,38 @@ __device__ __forceinline__ int lin_tid() {
 
 template <int NV, int kThreads>
 __device__ __forceinline__ void block_partials(const double (&v)[NV], double* partials) {
+  static_assert(NV <= 64, "one summing thread per value");
   __shared__ double red[kThreads / 64][NV];
   const int tid = lin_tid();
   const int lane = tid & 63, wid = tid >> 6;
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
-    const double s = wave_sum(v[k]);
-    if (lane == 0) red[wid][k] = s;
+    const double s = wave_sum_last(v[k]);
+    if (lane == 63) red[wid][k] = s;
   }
   __syncthreads();
-  if (tid == 0) {
+  if (tid < NV) {
+    // Value-major: value k of block b at partials[k * nblocks + b], so the
+    // finishing kernel's reads are coalesced.  One thread per value sums the
+    // waves in order.
+    const int64_t nblk = (int64_t)gridDim.x * gridDim.y;
     const int64_t bid = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
+    double s = 0.0;
 #pragma unroll
-    for (int k = 0; k < NV; ++k) {
-      double s = 0.0;
-      for (int w = 0; w < kThreads / 64; ++w) s += red[w][k];
-      partials[bid * NV + k] = s;
-    }
+    for (int w = 0; w < kThreads / 64; ++w) s += red[w][tid];
+    partials[tid * nblk + bid] = s;
   }
 }
 
 // Reduction sites (which scalars the finishing kernel updates).
 enum RedSite : int {
-  kRedUpDots = 0,  // z.r, z.q_prev -> flexible beta, rho
+  kRedCG = 0,      // the nine PCG dots -> beta, p.Ap, alpha, predicted ||r||^2
   kRedInit = 1,    // ||b||^2, ||r0||^2 -> tolerance, flags
-  kRedDir = 2,     // p.Ap -> alpha
-  kRedUpdate = 3,  // ||r||^2 -> convergence
   kRedRefine = 4,  // FP64 refinement residual
-  kRedDots = 5,    // as kRedUpDots, from the separate dots kernel
   kRedTrue = 6,    // FP64 certification residual
 };
+// Order of the PCG dots (z = M r, a = A z, q = q_{k-1} = A p_{k-1}).
+enum CgDot : int { kZR, kZQ, kZA, kRR, kRA, kRQ, kAA, kAQ, kQQ, kNumCgDots };
 constexpr int kFinishThreads = 1024;
 
 template <int NV, int SITE>
@@ -109,36 +111,89 @@ __global__ __launch_bounds__(kFinishThreads) void reduce_finish_kernel(
   int skip = 0;
   if constexpr (SITE == kRedRefine) skip = sc->out_done;
   else if constexpr (SITE != kRedInit && SITE != kRedTrue) skip = sc->done;
+  // The PCG step's previous-iteration scalars, in the same round trip.
+  int first = 1;
+  double pq_prev = 0.0, tol2 = 0.0;
+  if constexpr (SITE == kRedCG) {
+    first = sc->first;
+    pq_prev = sc->pq;
+    tol2 = sc->tol2;
+  }
   __shared__ double red[kFinishThreads / 64][NV];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   double acc[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) acc[k] = 0.0;
-#pragma unroll 8
+#pragma unroll 4
   for (int b = tid; b < nblk; b += kFinishThreads) {
 #pragma unroll
-    for (int k = 0; k < NV; ++k) acc[k] += partials[(int64_t)b * NV + k];
+    for (int k = 0; k < NV; ++k) acc[k] += partials[(int64_t)k * nblk + b];
   }
-  if (skip) return;
+  if (skip) {
+    // done == 2: the last update has been applied; stop it being applied again.
+    if constexpr (SITE == kRedCG)
+      if (tid == 0 && skip == 2) sc->done = 1;
+    return;
+  }
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
-    const double s = wave_sum(acc[k]);
-    if (lane == 0) red[wid][k] = s;
+    const double s = wave_sum_last(acc[k]);
+    if (lane == 63) red[wid][k] = s;
+  }
+  __syncthreads();
+  // One thread per value sums the waves in order (the LDS reads of all values
+  // overlap), then thread 0 applies the site's update.
+  __shared__ double tot[NV];
+  if (tid < NV) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < kFinishThreads / 64; ++w) s += red[w][tid];
+    tot[tid] = s;
   }
   __syncthreads();
   if (tid != 0) return;
   double out[NV];
 #pragma unroll
-  for (int k = 0; k < NV; ++k) {
-    double s = 0.0;
-    for (int w = 0; w < kFinishThreads / 64; ++w) s += red[w][k];
-    out[k] = s;
-  }
-  if constexpr (SITE == kRedUpDots || SITE == kRedDots) {
-    // Flexible (Polak-Ribiere) beta: z_{k+1}.(r_{k+1} - r_k) / rho_k with
-    // r_{k+1} - r_k = -alpha q_k.
-    sc->beta = sc->first ? 0.0 : (-sc->alpha * out[1]) / sc->rz;
-    sc->rz = out[0];
+  for (int k = 0; k < NV; ++k) out[k] = tot[k];
+  if constexpr (SITE == kRedCG) {
+    // One flexible-PCG step from the dots of z = M r_k and a = A z_k (the
+    // preconditioned CG that solve_cg, iterative.py:48-141, takes from
+    // lineax.CG, with the Polak-Ribiere beta of flexible PCG):
+    //   beta  = -z.q_{k-1} / p_{k-1}.q_{k-1}   (Polak-Ribiere: z.(r_k - r_{k-1}) / rho_{k-1})
+    //   p_k = z + beta p_{k-1},  q_k = A p_k = a + beta q_{k-1}
+    //   p_k.q_k = z.a + 2 beta z.q_{k-1} + beta^2 p_{k-1}.q_{k-1}   (A symmetric)
+    //   alpha = r_k.z / p_k.q_k
+    //   ||r_{k+1}||^2 = ||r_k - alpha q_k||^2 expanded in the same dots.
+    // The update kernel applies the step; when the predicted residual meets
+    // the tolerance, done = 2 lets that last update run and stops the rest.
+    double beta = first ? 0.0 : -out[kZQ] / pq_prev;
+    double pq = out[kZA] + beta * (2.0 * out[kZQ] + beta * (first ? 0.0 : pq_prev));
+    // p.Ap = z.Az (1 - cos^2) with cos the A-angle between z and p_{k-1}:
+    // when that nearly cancels, restart from p = z.
+    if (beta != 0.0 && !(fabs(pq) > 1e-6 * fabs(out[kZA]))) {
+      beta = 0.0;
+      pq = out[kZA];
+    }
+    // (A and M may have opposite signs -- PCG is invariant to that.)  Zero or
+    // non-finite: breakdown, no update; the caller certifies the residual.
+    if (!(fabs(pq) > 0.0 && fabs(pq) < HUGE_VAL)) {
+      sc->done = 1;
+      return;
+    }
+    const double alpha = out[kZR] / pq;
+    const double rq = out[kRA] + beta * out[kRQ];
+    const double qq = out[kAA] + beta * (2.0 * out[kAQ] + beta * out[kQQ]);
+    const double rr = out[kRR] - alpha * (2.0 * rq - alpha * qq);
+    sc->beta = beta;
+    sc->pq = pq;
+    sc->alpha = alpha;
+    sc->rz = out[kZR];
+    sc->zq = out[kZQ];
+    sc->rr = rr;
+    sc->first = 0;
+    sc->iter += 1;
+    sc->total += 1;
+    if (rr <= tol2) sc->done = 2;
   } else if constexpr (SITE == kRedInit) {
     sc->bb = out[0];
     sc->rr = out[1];
@@ -149,15 +204,6 @@ __global__ __launch_bounds__(kFinishThreads) void reduce_finish_kernel(
     sc->total = 0;
     sc->alpha = 0.0;
     sc->beta = 0.0;
-  } else if constexpr (SITE == kRedDir) {
-    sc->pq = out[0];
-    sc->alpha = sc->rz / out[0];
-    sc->first = 0;
-  } else if constexpr (SITE == kRedUpdate) {
-    sc->rr = out[0];
-    sc->iter += 1;
-    sc->total += 1;
-    if (out[0] <= sc->tol2) sc->done = 1;
   } else {  // kRedRefine, kRedTrue
     sc->bb = out[0];
     sc->out_rr = out[1];
@@ -248,9 +294,11 @@ __device__ __forceinline__ T glob_inv(const LevelDev<T>& L, int64_t k) {
 //   0 SMOOTH   : NS sweeps                                  -> u_out
 //   1 DOWN     : NS sweeps, residual, restriction           -> u_out, C.b
 //   2 UP       : u += P(C.u), NS sweeps                     -> u_out
-//   3 UP_DOTS  : as UP, plus z.r and z.w partial dots for flexible PCG
+//   3 UP_CG    : as UP with one more halo ring, then a = A z on the tile
+//                interior -> u_out (z), az_out (a), and the nine PCG dots of
+//                z, a, r (= b) and q_{k-1} (= w)
 // ---------------------------------------------------------------------------
-enum TileMode : int { kSmooth = 0, kDown = 1, kUp = 2, kUpDots = 3 };
+enum TileMode : int { kSmooth = 0, kDown = 1, kUp = 2, kUpCG = 3 };
 
 constexpr int kTx = 64, kTy = 16, kTileThreads = 256;
 
@@ -261,6 +309,7 @@ struct TileArgs {
   const T* b;
   T* u_out;
   const T* w;
+  T* az_out;
   T omega;
   double* partials;
   PcgScalars* sc;
@@ -297,7 +346,7 @@ __global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
   // The convergence guard is checked after the halo loads are issued, so its
   // round trip overlaps theirs.
   const bool skip = a.sc && a.sc->done;
-  constexpr int H = NS + (MODE == kDown ? 1 : 0);
+  constexpr int H = NS + ((MODE == kDown || MODE == kUpCG) ? 1 : 0);
   constexpr int W = kTx + 2 * H, Hh = kTy + 2 * H, N = W * Hh;
   __shared__ T sU[2][N];
   __shared__ T sB[N];
@@ -318,16 +367,36 @@ __global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
   // ... and whose prolongation stencil stays inside the coarse level.
   const bool inner_coarse = !MK && j0 - H >= 1 && j0 + kTy + H <= L.ny - 1 && i0 - H >= 1 &&
                             i0 + kTx + H <= L.nx - 1;
-  constexpr bool kUpMode = (MODE == kUp || MODE == kUpDots);
+  constexpr bool kUpMode = (MODE == kUp || MODE == kUpCG);
   constexpr int IT = (N + kTileThreads - 1) / kTileThreads;
   // Level l+1's iterate (times its mask) and mask around this tile for the
   // prolongation, staged in LDS: rows cj0 .. cj0 + CH - 1, columns
-  // ci0 .. ci0 + CW - 1 of the coarse level (zero outside it).
-  constexpr int CW = kTx / 2 + 4, CH = kTy / 2 + 4;
-  __shared__ T sCu[kUpMode ? CW * CH : 1];
-  __shared__ T sCm[(kUpMode && MK) ? CW * CH : 1];
+  // ci0 .. ci0 + CW - 1 of the coarse level (zero outside it).  With j0, i0
+  // even the fine rows j0 - H .. j0 + kTy + H - 1 touch coarse rows
+  // (j0 - H) / 2 - 1 .. (j0 + kTy + H - 1) / 2 + 1 (floor division).
+  constexpr int CW = kTx / 2 + 2 * ((H + 1) / 2) + 2, CH = kTy / 2 + 2 * ((H + 1) / 2) + 2;
+  // Staged in the spare sweep buffer sU[1]: the first sweep writes it only
+  // after the barrier that ends the prolongation (keeps the level-0 tile's
+  // LDS small enough for eight blocks per CU).
+  static_assert(2 * CW * CH <= N, "coarse staging must fit in a sweep buffer");
+  T* const sCu = sU[1];
+  T* const sCm = sU[1] + CW * CH;
   const int cj0 = ((j0 - H) >> 1) - 1, ci0 = ((i0 - H) >> 1) - 1;
   T rU[kUpMode ? IT : 1], rM[kUpMode ? IT : 1];
+  // q_{k-1} on the tile interior (kTx * kTy = 4 kTileThreads cells), loaded
+  // up front so its round trip overlaps the sweeps.
+  constexpr int kIn = kTx * kTy / kTileThreads;
+  static_assert(kTx * kTy == kIn * kTileThreads, "tile interior must split evenly");
+  T rQ[MODE == kUpCG ? kIn : 1];
+  if constexpr (MODE == kUpCG) {
+    const bool first = a.sc->first;
+#pragma unroll
+    for (int t = 0; t < kIn; ++t) {
+      const int k = tid + t * kTileThreads;
+      const int gj = j0 + k / kTx, gi = i0 + k % kTx;
+      rQ[t] = (!first && gj < L.ny && gi < L.nx) ? a.w[gj * L.nx + gi] : T(0);
+    }
+  }
 
   // Fully unrolled (the trip count is a compile-time constant) so the
   // global loads of all iterations are in flight together; 32-bit offsets.
@@ -448,7 +517,7 @@ __global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
       if constexpr (KIND == kDirichlet) inv = L.inv_d * mq;
       else inv = sI[q];
       T un;
-      if (ZERO && s == 1 && MODE != kUp && MODE != kUpDots) {
+      if (ZERO && s == 1 && !kUpMode) {
         un = (om * inv * sB[q]) * mq;
       } else {
         const T au = tile_Au<T, KIND, W>(sU[cur], sCX, sCY, mq, y, x, L.idx2, L.idy2, L.lam);
@@ -492,26 +561,46 @@ __global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
     }
   }
 
-  // Write the tile interior.
-  double d0 = 0.0, d1 = 0.0;
+  if constexpr (MODE == kUpCG) {
+    // z and a = A z on the tile interior (the sweeps left one valid ring
+    // around it), and the PCG dots in FP64.
+    double d[kNumCgDots];
 #pragma unroll
-  for (int k = tid; k < kTx * kTy; k += kTileThreads) {
-    const int y = H + k / kTx, x = H + k % kTx;
-    const int gj = j0 + k / kTx, gi = i0 + k % kTx;
-    if (gj < L.ny && gi < L.nx) {
-      const int q = y * W + x;
-      const int64_t g = (int64_t)gj * L.nx + gi;
-      const T z = sU[cur][q];
-      a.u_out[g] = z;
-      if constexpr (MODE == kUpDots) {
-        d0 += (double)z * (double)sB[q];
-        if (!a.sc->first) d1 += (double)z * (double)a.w[g];
+    for (int k = 0; k < kNumCgDots; ++k) d[k] = 0.0;
+#pragma unroll
+    for (int t = 0; t < kIn; ++t) {
+      const int k = tid + t * kTileThreads;
+      const int y = H + k / kTx, x = H + k % kTx;
+      const int gj = j0 + k / kTx, gi = i0 + k % kTx;
+      if (gj < L.ny && gi < L.nx) {
+        const int q = y * W + x;
+        const int g = gj * L.nx + gi;
+        const T z = sU[cur][q];
+        const T mq = inner_tile ? T(1) : Mq(y, x);
+        const T az = tile_Au<T, KIND, W>(sU[cur], sCX, sCY, mq, y, x, L.idx2, L.idy2, L.lam);
+        a.u_out[g] = z;
+        a.az_out[g] = az;
+        const double zd = z, rd = sB[q], ad = az, qd = rQ[t];
+        d[kZR] += zd * rd;
+        d[kZQ] += zd * qd;
+        d[kZA] += zd * ad;
+        d[kRR] += rd * rd;
+        d[kRA] += rd * ad;
+        d[kRQ] += rd * qd;
+        d[kAA] += ad * ad;
+        d[kAQ] += ad * qd;
+        d[kQQ] += qd * qd;
       }
     }
-  }
-  if constexpr (MODE == kUpDots) {
-    double v[2] = {d0, d1};
-    block_partials<2, kTileThreads>(v, a.partials);
+    block_partials<kNumCgDots, kTileThreads>(d, a.partials);
+  } else {
+    // Write the tile interior.
+#pragma unroll
+    for (int k = tid; k < kTx * kTy; k += kTileThreads) {
+      const int y = H + k / kTx, x = H + k % kTx;
+      const int gj = j0 + k / kTx, gi = i0 + k % kTx;
+      if (gj < L.ny && gi < L.nx) a.u_out[(int64_t)gj * L.nx + gi] = sU[cur][y * W + x];
+    }
   }
 }
 
@@ -955,127 +1044,135 @@ __global__ __launch_bounds__(256) void pcg_init_kernel(LevelDev<T> L, const T* _
   block_partials<2, 256>(v, partials);
 }
 
-// PCG: p_out = z + beta p_in ; q = A p_out ; partial p.q -> alpha = rho / pq.
+// PCG with a preconditioner other than the V-cycle: a = A z and the PCG dots
+// (as the V-cycle's last kernel forms them).
 // Each wave owns a 64-column strip of kStencilRows rows and slides a
-// three-row register window down it (every row of z, p read once per strip);
+// three-row register window down it (every row of z read once per strip);
 // east/west neighbours come from lane shuffles, a strip's two edge columns
 // from memory.  The four waves of a block sit side by side (256 columns), so
 // three of their four edge loads hit the cache lines their neighbour wave on
 // the same CU just read.
 constexpr int kDirRows = 8;       // rows per wave of the pointwise update kernel
-constexpr int kStencilRows = 8;   // rows per wave of the direction kernel
+constexpr int kStencilRows = 8;   // rows per wave of the stencil kernel
 static inline dim3 grid_dir(int ny, int nx) { return dim3(cdiv(nx, 64), cdiv(ny, 4 * kDirRows)); }
 static inline dim3 grid_stencil(int ny, int nx) {
   return dim3(cdiv(nx, 256), cdiv(ny, kStencilRows));
 }
 
 template <typename T, int KIND>
-__global__ __launch_bounds__(256) void pcg_dir_kernel(LevelDev<T> L, const T* __restrict__ z,
-                                                      const T* __restrict__ p_in,
-                                                      T* __restrict__ p_out, T* __restrict__ q,
-                                                      double* partials, PcgScalars* sc) {
+__global__ __launch_bounds__(256) void pcg_az_dots_kernel(LevelDev<T> L, const T* __restrict__ z,
+                                                          const T* __restrict__ r,
+                                                          const T* __restrict__ q,
+                                                          T* __restrict__ az, double* partials,
+                                                          PcgScalars* sc) {
   const int done = sc->done;  // checked once the first rows are in flight
   const int lane = threadIdx.x;
   const int i = (blockIdx.x * 4 + threadIdx.y) * 64 + lane;
   const int j0 = blockIdx.y * kStencilRows;  // uniform within a wave
   const bool first = sc->first;
-  const T beta = first ? T(0) : (T)sc->beta;
   const int nx = L.nx, ny = L.ny;
   const bool col = i < nx;
-  // p at (row j, this lane's column); rows outside the level are zero
+  // z at (row j, this lane's column); rows outside the level are zero
   // (j is uniform across the wave, col per lane).  32-bit offsets.
-  auto Prow = [&](int j) -> T {
-    if (j < 0 || j >= ny || !col) return T(0);
-    const int k = j * nx + i;
-    return first ? z[k] : z[k] + beta * p_in[k];
-  };
-  auto Pat = [&](int j, int ii) -> T {  // edge lanes' outer neighbours
-    if (ii < 0 || ii >= nx) return T(0);
-    const int k = j * nx + ii;
-    return first ? z[k] : z[k] + beta * p_in[k];
+  auto Zrow = [&](int j) -> T { return (j < 0 || j >= ny || !col) ? T(0) : z[j * nx + i]; };
+  auto Zat = [&](int j, int ii) -> T {  // edge lanes' outer neighbours
+    return (ii < 0 || ii >= nx) ? T(0) : z[j * nx + ii];
   };
   auto CY = [&](int j) -> T {  // north-face coefficient of row j
     return (KIND == kNoflux && j >= 0 && j < ny && col) ? L.cy[j * nx + i] : T(0);
   };
-  double pq = 0.0;
-  T ps = Prow(j0 - 1), pc = Prow(j0);
+  double d[kNumCgDots];
+#pragma unroll
+  for (int k = 0; k < kNumCgDots; ++k) d[k] = 0.0;
+  T zs = Zrow(j0 - 1), zc = Zrow(j0);
   T cys = CY(j0 - 1);
   if (done) return;
-  for (int r = 0; r < kStencilRows; ++r) {
-    const int j = j0 + r;
+  for (int rw = 0; rw < kStencilRows; ++rw) {
+    const int j = j0 + rw;
     if (j >= ny) break;
-    const T pn = Prow(j + 1);
-    T w = __shfl_up(pc, 1, 64);
-    T e = __shfl_down(pc, 1, 64);
-    if (lane == 0) w = Pat(j, i - 1);
-    if (lane == 63) e = Pat(j, i + 1);
+    const T zn = Zrow(j + 1);
+    T w = __shfl_up(zc, 1, 64);
+    T e = __shfl_down(zc, 1, 64);
+    if (lane == 0) w = Zat(j, i - 1);
+    if (lane == 63) e = Zat(j, i + 1);
     T lap;
     T cyn = T(0);
     const int k = j * nx + i;
     if constexpr (KIND == kDirichlet) {
-      lap = (e + w - T(2) * pc) * L.idx2 + (pn + ps - T(2) * pc) * L.idy2;
+      lap = (e + w - T(2) * zc) * L.idx2 + (zn + zs - T(2) * zc) * L.idy2;
     } else {
       const T cxe = col ? L.cx[k] : T(0);
       T cxw = __shfl_up(cxe, 1, 64);
       if (lane == 0) cxw = (i > 0) ? L.cx[k - 1] : T(0);
       cyn = CY(j);
-      lap = (cxe * (e - pc) - cxw * (pc - w)) * L.idx2 + (cyn * (pn - pc) - cys * (pc - ps)) * L.idy2;
+      lap = (cxe * (e - zc) - cxw * (zc - w)) * L.idx2 + (cyn * (zn - zc) - cys * (zc - zs)) * L.idy2;
     }
     if (col) {
-      const T qv = (lap - L.lam * pc) * mval<T>(L.mask, k);
-      p_out[k] = pc;
-      q[k] = qv;
-      pq += (double)pc * qv;
+      const T av = (lap - L.lam * zc) * mval<T>(L.mask, k);
+      az[k] = av;
+      const double zd = zc, rd = r[k], ad = av, qd = first ? 0.0 : (double)q[k];
+      d[kZR] += zd * rd;
+      d[kZQ] += zd * qd;
+      d[kZA] += zd * ad;
+      d[kRR] += rd * rd;
+      d[kRA] += rd * ad;
+      d[kRQ] += rd * qd;
+      d[kAA] += ad * ad;
+      d[kAQ] += ad * qd;
+      d[kQQ] += qd * qd;
     }
-    ps = pc;
-    pc = pn;
+    zs = zc;
+    zc = zn;
     cys = cyn;
   }
-  double v[1] = {pq};
-  block_partials<1, 256>(v, partials);
+  block_partials<kNumCgDots, 256>(d, partials);
 }
 
-// PCG: x += alpha p ; r -= alpha q ; ||r||^2 -> convergence flag.  Each wave
-// updates a 64-column strip of kDirRows rows (grid_dir).  (Reducing the
-// direction kernel's partials here in every block instead of in a finishing
-// launch measured no faster: the reduction moved onto each block's path.)
+// PCG step with the scalars of the finishing kernel (kRedCG):
+//   p = z + beta p ; q = a + beta q ; x += alpha p ; r -= alpha q
+// in place, pointwise.  Each wave updates a 64-column strip of kDirRows rows
+// (grid_dir).  Runs unless done == 1 (done == 2: this is the last update).
 template <typename T>
-__global__ __launch_bounds__(256) void pcg_update_kernel(int ny, int nx, T* __restrict__ x,
-                                                         int64_t ld_x, const T* __restrict__ p,
-                                                         T* __restrict__ r,
-                                                         const T* __restrict__ q,
-                                                         double* partials, PcgScalars* sc) {
+__global__ __launch_bounds__(256) void pcg_cg_update_kernel(int ny, int nx, T* __restrict__ x,
+                                                            int64_t ld_x, const T* __restrict__ z,
+                                                            const T* __restrict__ az,
+                                                            T* __restrict__ p, T* __restrict__ q,
+                                                            T* __restrict__ r, PcgScalars* sc) {
   const int done = sc->done;
   const int i = blockIdx.x * 64 + threadIdx.x;
   const int j0 = (blockIdx.y * 4 + threadIdx.y) * kDirRows;
-  const T alpha = (T)sc->alpha;
-  double rr = 0.0;
+  const double beta64 = sc->beta;
+  const T alpha = (T)sc->alpha, beta = (T)beta64;
   // All loads first (their round trip overlaps the scalars'), then the guard.
-  T xv[kDirRows], pv[kDirRows], rv0[kDirRows], qv[kDirRows];
+  T xv[kDirRows], zv[kDirRows], av[kDirRows], pv[kDirRows], qv[kDirRows], rv[kDirRows];
 #pragma unroll
   for (int t = 0; t < kDirRows; ++t) {
     const int j = j0 + t;
     const bool in = i < nx && j < ny;
     const int64_t k = (int64_t)j * nx + i;
     xv[t] = in ? x[(int64_t)j * ld_x + i] : T(0);
+    zv[t] = in ? z[k] : T(0);
+    av[t] = in ? az[k] : T(0);
     pv[t] = in ? p[k] : T(0);
-    rv0[t] = in ? r[k] : T(0);
     qv[t] = in ? q[k] : T(0);
+    rv[t] = in ? r[k] : T(0);
   }
-  if (done) return;
+  if (done == 1) return;
+  // beta == 0 on the first step: p_{k-1}, q_{k-1} are not read into it.
+  const bool restart = beta64 == 0.0;
 #pragma unroll
   for (int t = 0; t < kDirRows; ++t) {
     const int j = j0 + t;
     if (i < nx && j < ny) {
       const int64_t k = (int64_t)j * nx + i;
-      x[(int64_t)j * ld_x + i] = xv[t] + alpha * pv[t];
-      const T rv = rv0[t] - alpha * qv[t];
-      r[k] = rv;
-      rr += (double)rv * rv;
+      const T pn = restart ? zv[t] : zv[t] + beta * pv[t];
+      const T qn = restart ? av[t] : av[t] + beta * qv[t];
+      p[k] = pn;
+      q[k] = qn;
+      x[(int64_t)j * ld_x + i] = xv[t] + alpha * pn;
+      r[k] = rv[t] - alpha * qn;
     }
   }
-  double v[1] = {rr};
-  block_partials<1, 256>(v, partials);
 }
 
 // Mixed precision: r32 = (b64 - A64 x64) m in FP64, ||r||^2 and ||b||^2 in
@@ -1196,7 +1293,8 @@ struct Hier {
   std::vector<LevelDev<T>> lv;
   std::vector<void*> allocs;
   // PCG vectors on level 0
-  T *r = nullptr, *z = nullptr, *p0 = nullptr, *p1 = nullptr, *q = nullptr, *x = nullptr;
+  // r residual, z = M r, az = A z, p direction, q = A p, x iterate
+  T *r = nullptr, *z = nullptr, *p = nullptr, *az = nullptr, *q = nullptr, *x = nullptr;
 };
 
 struct Plan {
@@ -1365,8 +1463,8 @@ static int upload_hier(Plan& P, Hier<T>& H, const std::vector<HostLevel>& hl) {
   }
   H.r = (T*)v[0];
   H.z = (T*)v[1];
-  H.p0 = (T*)v[2];
-  H.p1 = (T*)v[3];
+  H.p = (T*)v[2];
+  H.az = (T*)v[3];
   H.q = (T*)v[4];
   H.x = (T*)v[5];
   return kOk;
@@ -1394,7 +1492,7 @@ Hier<float>& hier<float>(Plan& P) { return P.h32; }
 template <typename T, int KIND, int NS, int MODE, bool ZERO>
 static void launch_tile(const Plan& P, const LevelDev<T>& L, const LevelDev<T>* C, const T* u_in,
                         const T* b, T* u_out, const T* w, PcgScalars* sc, hipStream_t s,
-                        bool finish = true) {
+                        T* az_out = nullptr, bool finish = true) {
   TileArgs<T> a;
   std::memset(&a, 0, sizeof(a));
   a.L = L;
@@ -1403,6 +1501,7 @@ static void launch_tile(const Plan& P, const LevelDev<T>& L, const LevelDev<T>* 
   a.b = b;
   a.u_out = u_out;
   a.w = w;
+  a.az_out = az_out;
   a.omega = (T)P.omega;
   a.partials = P.partials;
   a.sc = sc;
@@ -1414,20 +1513,20 @@ static void launch_tile(const Plan& P, const LevelDev<T>& L, const LevelDev<T>* 
   else
     hipLaunchKernelGGL((mg_tile_kernel<T, KIND, NS, MODE, ZERO, false>), dim3(a.ntiles),
                        dim3(kTileThreads), 0, s, a);
-  if constexpr (MODE == kUpDots)
+  if constexpr (MODE == kUpCG)
     if (finish)
-      hipLaunchKernelGGL((reduce_finish_kernel<2, kRedUpDots>), dim3(1), dim3(kFinishThreads), 0,
-                         s, P.partials, a.ntiles, sc, 0.0, 0.0);
+      hipLaunchKernelGGL((reduce_finish_kernel<kNumCgDots, kRedCG>), dim3(1), dim3(kFinishThreads),
+                         0, s, P.partials, a.ntiles, sc, 0.0, 0.0);
 }
 
-// Level-0 up kernel of a preconditioning V-cycle with the flexible-PCG dots
-// (two post-sweeps), as the V-cycle launches it, without its finishing
-// kernel (profiling hook).
+// Level-0 up kernel of a preconditioning V-cycle with the PCG dots (two
+// post-sweeps), as the V-cycle launches it, without its finishing kernel
+// (profiling hook).
 template <typename T, int KIND>
-static void launch_vcycle_up_dots(const Plan& P, const LevelDev<T>& L, const LevelDev<T>& C,
-                                  const T* u, const T* b, T* out, const T* w, PcgScalars* sc,
-                                  hipStream_t s) {
-  launch_tile<T, KIND, 2, kUpDots, false>(P, L, &C, u, b, out, w, sc, s, false);
+static void launch_vcycle_up_cg(const Plan& P, const LevelDev<T>& L, const LevelDev<T>& C,
+                                const T* u, const T* b, T* out, const T* w, T* az,
+                                PcgScalars* sc, hipStream_t s) {
+  launch_tile<T, KIND, 2, kUpCG, false>(P, L, &C, u, b, out, w, sc, s, az, false);
 }
 
 // Finishing kernel of a reduction over the blocks of grid2(ny, nx).
@@ -1465,32 +1564,13 @@ static T* smooth_chunks(const Plan& P, const LevelDev<T>& L, const T* u_in, bool
   return dst;
 }
 
-// Flexible-PCG dot products z.r and z.q_prev when they cannot be fused into
-// the last V-cycle kernel.
-template <typename T>
-__global__ __launch_bounds__(256) void pcg_dots_kernel(int ny, int nx, const T* __restrict__ z,
-                                                       const T* __restrict__ r,
-                                                       const T* __restrict__ q,
-                                                       double* partials, PcgScalars* sc) {
-  const int done = sc->done;
-  const int first = sc->first;
-  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
-  double d0 = 0.0, d1 = 0.0;
-  if (i < nx && j < ny) {
-    const int64_t k = (int64_t)j * nx + i;
-    d0 = (double)z[k] * r[k];
-    if (!first) d1 = (double)z[k] * q[k];
-  }
-  if (done) return;
-  double v[2] = {d0, d1};
-  block_partials<2, 256>(v, partials);
-}
-
-// One V-cycle z = V(u0 or 0, rhs) at level 0.  Returns 1 in *fused when the
-// flexible-PCG dots against rhs and w were formed by the last kernel.
+// One V-cycle z = V(u0 or 0, rhs) at level 0.  With ``az`` given, the last
+// kernel also writes az = A z and the PCG dots against rhs and w (kUpCG) when
+// it can (two or fewer post-sweeps on a tiled level 0); *fused says whether
+// it did.
 template <typename T, int KIND>
-static int v_cycle_impl(Plan& P, const T* u0, const T* rhs, T* out, bool want_dots,
-                        const T* w, PcgScalars* sc, hipStream_t s, bool* fused) {
+static int v_cycle_impl(Plan& P, const T* u0, const T* rhs, T* out, T* az, const T* w,
+                        PcgScalars* sc, hipStream_t s, bool* fused) {
   Hier<T>& H = hier<T>(P);
   const int nl = P.nl, lc = P.lc;
   if (fused) *fused = false;
@@ -1576,12 +1656,12 @@ static int v_cycle_impl(Plan& P, const T* u0, const T* rhs, T* out, bool want_do
     const int npost = P.n_post;
     const int fz = npost >= 2 ? 2 : npost;
     const int tail = npost - fz;
-    const bool dk = want_dots && l == 0 && tail == 0;
+    const bool dk = az && l == 0 && tail == 0;
     T* dst = (l == 0 && tail == 0) ? out : L.t;
     if (dk) {
-      if (fz == 2) launch_tile<T, KIND, 2, kUpDots, false>(P, L, &C, L.u, b, dst, w, sc, s);
-      else if (fz == 1) launch_tile<T, KIND, 1, kUpDots, false>(P, L, &C, L.u, b, dst, w, sc, s);
-      else launch_tile<T, KIND, 0, kUpDots, false>(P, L, &C, L.u, b, dst, w, sc, s);
+      if (fz == 2) launch_tile<T, KIND, 2, kUpCG, false>(P, L, &C, L.u, b, dst, w, sc, s, az);
+      else if (fz == 1) launch_tile<T, KIND, 1, kUpCG, false>(P, L, &C, L.u, b, dst, w, sc, s, az);
+      else launch_tile<T, KIND, 0, kUpCG, false>(P, L, &C, L.u, b, dst, w, sc, s, az);
       if (fused) *fused = true;
     } else {
       if (fz == 2) launch_tile<T, KIND, 2, kUp, false>(P, L, &C, L.u, b, dst, nullptr, sc, s);
@@ -1601,14 +1681,17 @@ static int v_cycle_impl(Plan& P, const T* u0, const T* rhs, T* out, bool want_do
 template <typename T>
 static int v_cycle(Plan& P, const T* u0, const T* rhs, T* out, PcgScalars* sc, hipStream_t s) {
   return P.kind == kDirichlet
-             ? v_cycle_impl<T, kDirichlet>(P, u0, rhs, out, false, nullptr, sc, s, nullptr)
-             : v_cycle_impl<T, kNoflux>(P, u0, rhs, out, false, nullptr, sc, s, nullptr);
+             ? v_cycle_impl<T, kDirichlet>(P, u0, rhs, out, nullptr, nullptr, sc, s, nullptr)
+             : v_cycle_impl<T, kNoflux>(P, u0, rhs, out, nullptr, nullptr, sc, s, nullptr);
 }
 
 static dim3 grid2(int ny, int nx) { return dim3(cdiv(nx, 64), cdiv(ny, 4)); }
 
 // Enqueue ``n`` flexible-PCG iterations (device-guarded) on hierarchy H.
-// Precondition: H.r holds the residual and sc->first / tol2 are set.
+// Precondition: H.r holds the residual and sc->first / tol2 are set.  One
+// iteration is z = M r with az = A z and the PCG dots (fused into the
+// V-cycle's last kernel, else a stencil kernel), the finishing kernel that
+// turns the dots into the step's scalars, and the pointwise update.
 template <typename T, int KIND>
 static int pcg_iters(Plan& P, Hier<T>& H, T* x, int64_t ld_x, int n, PcgScalars* sc,
                      hipStream_t s, PrecondFn pf = nullptr, void* ctx = nullptr) {
@@ -1618,32 +1701,25 @@ static int pcg_iters(Plan& P, Hier<T>& H, T* x, int64_t ld_x, int n, PcgScalars*
     int rc;
     if constexpr (std::is_same<T, float>::value) {
       if (pf) rc = pf(ctx, H.r, H.z, L.ny, L.nx, &sc->done, s);
-      else rc = v_cycle_impl<T, KIND>(P, nullptr, H.r, H.z, true, H.q, sc, s, &fused);
+      else rc = v_cycle_impl<T, KIND>(P, nullptr, H.r, H.z, H.az, H.q, sc, s, &fused);
     } else {
-      rc = v_cycle_impl<T, KIND>(P, nullptr, H.r, H.z, true, H.q, sc, s, &fused);
+      rc = v_cycle_impl<T, KIND>(P, nullptr, H.r, H.z, H.az, H.q, sc, s, &fused);
     }
     if (rc) return rc;
     if (!fused) {
-      hipLaunchKernelGGL(pcg_dots_kernel<T>, grid2(L.ny, L.nx), dim3(64, 4), 0, s, L.ny, L.nx,
-                         H.z, H.r, H.q, P.partials, sc);
-      finish2<2, kRedDots>(P, L.ny, L.nx, sc, s);
+      const dim3 gs = grid_stencil(L.ny, L.nx);
+      hipLaunchKernelGGL((pcg_az_dots_kernel<T, KIND>), gs, dim3(64, 4), 0, s, L, H.z, H.r, H.q,
+                         H.az, P.partials, sc);
+      hipLaunchKernelGGL((reduce_finish_kernel<kNumCgDots, kRedCG>), dim3(1),
+                         dim3(kFinishThreads), 0, s, P.partials, (int)(gs.x * gs.y), sc, 0.0,
+                         0.0);
     }
-    const dim3 gs = grid_stencil(L.ny, L.nx);
-    hipLaunchKernelGGL((pcg_dir_kernel<T, KIND>), gs, dim3(64, 4), 0, s, L, H.z, H.p0, H.p1, H.q,
-                       P.partials, sc);
-    hipLaunchKernelGGL((reduce_finish_kernel<1, kRedDir>), dim3(1), dim3(kFinishThreads), 0, s,
-                       P.partials, (int)(gs.x * gs.y), sc, 0.0, 0.0);
-    std::swap(H.p0, H.p1);
-    hipLaunchKernelGGL(pcg_update_kernel<T>, grid_dir(L.ny, L.nx), dim3(64, 4), 0, s, L.ny, L.nx,
-                       x, ld_x, H.p0, H.r, H.q, P.partials, sc);
-    hipLaunchKernelGGL((reduce_finish_kernel<1, kRedUpdate>), dim3(1), dim3(kFinishThreads), 0, s,
-                       P.partials, (int)(grid_dir(L.ny, L.nx).x * grid_dir(L.ny, L.nx).y), sc,
-                       0.0, 0.0);
+    hipLaunchKernelGGL(pcg_cg_update_kernel<T>, grid_dir(L.ny, L.nx), dim3(64, 4), 0, s, L.ny,
+                       L.nx, x, ld_x, H.z, H.az, H.p, H.q, H.r, sc);
   }
   FVX_LAUNCH_CHECK();
   return kOk;
 }
-
 
 template <typename T>
 __global__ __launch_bounds__(256) void zero_pitched_kernel(int ny, int nx, T* __restrict__ x,
@@ -1882,8 +1958,8 @@ __global__ __launch_bounds__(kFinishThreads) void energy_finish_kernel(
   double a0 = 0.0, a1 = 0.0;
 #pragma unroll 8
   for (int b = tid; b < nblk; b += kFinishThreads) {
-    a0 += partials[2 * (int64_t)b];
-    a1 += partials[2 * (int64_t)b + 1];
+    a0 += partials[b];
+    a1 += partials[(int64_t)nblk + b];
   }
   a0 = wave_sum(a0);
   a1 = wave_sum(a1);
@@ -2014,7 +2090,7 @@ static int kernel_bench_impl(Plan& P, int which, int reps, hipStream_t s) {
   const LevelDev<T>& L = H.lv[0];
   PcgScalars* sc = &P.sc[2 * (kSlots - 1)];
   if (reps == 0) {  // prepare: zero the vectors, arm the scalars
-    for (T* v : {H.r, H.z, H.p0, H.p1, H.q, H.x})
+    for (T* v : {H.r, H.z, H.p, H.az, H.q, H.x})
       hipLaunchKernelGGL(zero_pitched_kernel<T>, grid2(L.ny, L.nx), dim3(64, 4), 0, s, L.ny, L.nx,
                          v, (int64_t)L.nx);
     if (P.nl >= 2) {
@@ -2032,16 +2108,16 @@ static int kernel_bench_impl(Plan& P, int which, int reps, hipStream_t s) {
   }
   for (int k = 0; k < reps; ++k) {
     if (which == 0) {
-      hipLaunchKernelGGL((pcg_dir_kernel<T, KIND>), grid_stencil(L.ny, L.nx), dim3(64, 4), 0, s,
-                         L, H.z, H.p0, H.p1, H.q, P.partials, sc);
+      hipLaunchKernelGGL((pcg_az_dots_kernel<T, KIND>), grid_stencil(L.ny, L.nx), dim3(64, 4), 0,
+                         s, L, H.z, H.r, H.q, H.az, P.partials, sc);
     } else if (which == 1) {
-      hipLaunchKernelGGL(pcg_update_kernel<T>, grid_dir(L.ny, L.nx), dim3(64, 4), 0, s, L.ny,
-                         L.nx, H.x, (int64_t)L.nx, H.p0, H.r, H.q, P.partials, sc);
+      hipLaunchKernelGGL(pcg_cg_update_kernel<T>, grid_dir(L.ny, L.nx), dim3(64, 4), 0, s, L.ny,
+                         L.nx, H.x, (int64_t)L.nx, H.z, H.az, H.p, H.q, H.r, sc);
     } else {
       // The last kernel of a preconditioning V-cycle on level 0: prolongation
-      // of level 1's iterate, correction, two sweeps, flexible-PCG dots.
+      // of level 1's iterate, correction, two sweeps, a = A z, PCG dots.
       LevelDev<T> C = H.lv[1];
-      launch_vcycle_up_dots<T, KIND>(P, L, C, H.p0, H.r, H.z, H.q, sc, s);
+      launch_vcycle_up_cg<T, KIND>(P, L, C, H.p, H.r, H.z, H.q, H.az, sc, s);
     }
   }
   FVX_LAUNCH_CHECK();
@@ -2127,6 +2203,9 @@ int fvx_mg_plan_create(const double* mask, const double* coeff, int ny, int nx, 
   using namespace fvx;
   FVX_CHECK_ARG(mask && plan_out, "null argument");
   FVX_CHECK_ARG(ny >= 1 && nx >= 1, "empty grid");
+  // The level kernels index cells (plus a halo row) with 32-bit offsets.
+  FVX_CHECK_ARG((int64_t)(ny + 8) * (nx + 8) < ((int64_t)1 << 31),
+                "grid too large: at most 2^31 cells including halo");
   FVX_CHECK_ARG(kind == kNoflux || kind == kDirichlet, "unknown operator kind %d", kind);
   FVX_CHECK_ARG(precision == 32 || precision == 64 || precision == 3264,
                 "precision must be 32, 64 or 3264 (mixed)");
@@ -2177,8 +2256,13 @@ int fvx_mg_plan_create(const double* mask, const double* coeff, int ny, int nx, 
     P->has32 = true;
   }
   if (!rc) {
-    const size_t maxblk = (size_t)cdiv(nx, 64) * cdiv(ny, 4) + 64;
-    hipError_t e = hipMalloc(&P->partials, maxblk * 4 * sizeof(double));
+    // Partial sums: two per block of grid2 (residual, init, refinement and
+    // energy dots), kNumCgDots per level-0 tile or stencil block (PCG dots).
+    const size_t n2 = (size_t)2 * cdiv(nx, 64) * cdiv(ny, 4);
+    const size_t ncg = (size_t)kNumCgDots * std::max<size_t>(
+        (size_t)cdiv(nx, kTx) * cdiv(ny, kTy), (size_t)cdiv(nx, 256) * cdiv(ny, kStencilRows));
+    const size_t npart = std::max(n2, ncg) + 64;
+    hipError_t e = hipMalloc(&P->partials, npart * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&P->sc, 2 * kSlots * sizeof(PcgScalars));
     if (e == hipSuccess) e = hipMemset(P->sc, 0, 2 * kSlots * sizeof(PcgScalars));
     if (e == hipSuccess)
@@ -2352,18 +2436,19 @@ int fvx_mg_mixed_solve(void* plan, const double* rhs, int64_t ld_rhs, double* x,
 
 // Profiling hook (no reference counterpart): launch ``reps`` back-to-back
 // instances of one fine-level PCG kernel on the plan's own vectors with a
-// scalar block that never converges.  which: 0 pcg_dir (p = z + beta p,
-// q = A p, p.q), 1 pcg_update (x += alpha p, r -= alpha q, r.r); bits 32 or
-// 64 select the hierarchy; which 2 is the level-0 up kernel of a
-// preconditioning V-cycle (prolongation of level 1, correction, two sweeps,
-// flexible-PCG dots).  reps = 0 prepares (zeroes the vectors, arms the
-// scalars) and must precede timed calls.
+// scalar block that never converges.  which: 0 pcg_az_dots (a = A z and the
+// PCG dots, the path of preconditioners other than the V-cycle),
+// 1 pcg_cg_update (p = z + beta p, q = a + beta q, x += alpha p,
+// r -= alpha q), 2 the level-0 up kernel of a preconditioning V-cycle
+// (prolongation of level 1, correction, two sweeps, a = A z, PCG dots); bits
+// 32 or 64 select the hierarchy.  reps = 0 prepares (zeroes the vectors, arms
+// the scalars) and must precede timed calls.
 int fvx_mg_kernel_bench(void* plan, int which, int bits, int reps, void* stream) {
   using namespace fvx;
   FVX_CHECK_ARG(plan, "null plan");
   Plan* P = (Plan*)plan;
   FVX_CHECK_ARG(which >= 0 && which <= 2,
-                "which must be 0 (pcg_dir), 1 (pcg_update) or 2 (V-cycle up + dots)");
+                "which must be 0 (pcg_az_dots), 1 (pcg_cg_update) or 2 (V-cycle up + dots)");
   FVX_CHECK_ARG((bits == 32 && P->has32) || (bits == 64 && P->has64),
                 "plan has no hierarchy of that precision");
   FVX_CHECK_ARG(reps >= 0, "negative reps");
