@@ -68,22 +68,27 @@ __device__ __forceinline__ float wet(const uint8_t* m, int64_t idx) {
   return m ? (m[idx] ? 1.0f : 0.0f) : 1.0f;
 }
 
-// One DPP lane move of a double (two 32-bit moves); lanes without a source
-// read 0.
+// DPP lane moves; lanes without a source read 0 (bound_ctrl), so the sums
+// below need no initialised destination.
 template <int kCtrl>
 __device__ __forceinline__ double dpp_move(double v) {
   const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, kCtrl, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), kCtrl, 0xf, 0xf, false);
+  const int lo = __builtin_amdgcn_mov_dpp((int)b, kCtrl, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), kCtrl, 0xf, 0xf, true);
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <int kCtrl>
+__device__ __forceinline__ float dpp_move(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), kCtrl, 0xf, 0xf, true));
 }
 
 // Sum over the 64 lanes of a wavefront with DPP lane moves (VALU only, no
 // LDS round trips); the total is valid in lane 63.  Pairs and quads
 // (quad_perm), rows of 16 (row_shr 4, 8: lane 15 of each row holds the row
 // sum), then rows 0+1 into lane 31 and rows 2+3 into lane 63 (row_bcast 15)
-// and lane 31 into lane 63 (row_bcast 31).
-__device__ __forceinline__ double wave_sum_last(double v) {
+// and lane 31 into lane 63 (row_bcast 31).  Other lanes hold partial sums.
+template <typename T>
+__device__ __forceinline__ T wave_sum_last(T v) {
   v += dpp_move<0xb1>(v);   // quad_perm [1, 0, 3, 2]
   v += dpp_move<0x4e>(v);   // quad_perm [2, 3, 0, 1]
   v += dpp_move<0x114>(v);  // row_shr 4

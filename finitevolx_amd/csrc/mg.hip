@@ -67,16 +67,16 @@ __device__ __forceinline__ int lin_tid() {
   return threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
 }
 
-template <int NV, int kThreads>
-__device__ __forceinline__ void block_partials(const double (&v)[NV], double* partials) {
+template <int NV, int kThreads, typename A>
+__device__ __forceinline__ void block_partials(const A (&v)[NV], double* partials) {
   static_assert(NV <= 64, "one summing thread per value");
   __shared__ double red[kThreads / 64][NV];
   const int tid = lin_tid();
   const int lane = tid & 63, wid = tid >> 6;
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
-    const double s = wave_sum_last(v[k]);
-    if (lane == 63) red[wid][k] = s;
+    const A s = wave_sum_last(v[k]);
+    if (lane == 63) red[wid][k] = (double)s;
   }
   __syncthreads();
   if (tid < NV) {
@@ -563,10 +563,13 @@ __global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
 
   if constexpr (MODE == kUpCG) {
     // z and a = A z on the tile interior (the sweeps left one valid ring
-    // around it), and the PCG dots in FP64.
-    double d[kNumCgDots];
+    // around it), and the PCG dots: summed in the level's precision within a
+    // wave (FP32 on the FP32 hierarchy: its vectors carry FP32 rounding
+    // already, and the scalars only steer the iteration -- the FP64
+    // refinement certifies the solution), in FP64 across waves and blocks.
+    T d[kNumCgDots];
 #pragma unroll
-    for (int k = 0; k < kNumCgDots; ++k) d[k] = 0.0;
+    for (int k = 0; k < kNumCgDots; ++k) d[k] = T(0);
 #pragma unroll
     for (int t = 0; t < kIn; ++t) {
       const int k = tid + t * kTileThreads;
@@ -580,7 +583,7 @@ __global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
         const T az = tile_Au<T, KIND, W>(sU[cur], sCX, sCY, mq, y, x, L.idx2, L.idy2, L.lam);
         a.u_out[g] = z;
         a.az_out[g] = az;
-        const double zd = z, rd = sB[q], ad = az, qd = rQ[t];
+        const T zd = z, rd = sB[q], ad = az, qd = rQ[t];
         d[kZR] += zd * rd;
         d[kZQ] += zd * qd;
         d[kZA] += zd * ad;
