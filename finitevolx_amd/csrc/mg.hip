@@ -608,6 +608,274 @@ __global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
 }
 
 // ---------------------------------------------------------------------------
+// Wave-tile smoother kernel for FP32 levels without a land mask and with the
+// Dirichlet operator (constant coefficients): the same modes and arithmetic
+// as mg_tile_kernel, laid out for the wavefront.  One wavefront owns a tile
+// of 64 columns x (wt_rows + 2H) rows including its halo, each lane one
+// column of u and b in registers.  East/west neighbours come from DPP wave
+// shifts, north/south from the neighbouring registers, so the sweeps use no
+// LDS, no barriers and no index arithmetic.  The tile interior is
+// (64 - 2H) x wt_rows; four independent tiles per 256-thread block.
+// ---------------------------------------------------------------------------
+
+// Value of lane + 1 (wave_shl 1) and lane - 1 (wave_shr 1); 0 where there is
+// none (lane 63, lane 0).
+__device__ __forceinline__ float lane_east(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xf, 0xf, true));
+}
+__device__ __forceinline__ float lane_west(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
+}
+
+template <int NS, int MODE>
+constexpr int wt_halo() {
+  return NS + ((MODE == kDown || MODE == kUpCG) ? 1 : 0);
+}
+// Interior rows per tile (even, for the 2x2 restriction).  Short tiles keep
+// the registers per lane low enough for 4-8 waves per SIMD, which hides the
+// load latency better than the halo they save costs (measured at 2048^2:
+// 8-10 rows beat 12-16).
+template <int MODE>
+constexpr int wt_rows() {
+  return MODE == kUpCG ? 10 : 8;
+}
+
+template <int NS, int MODE, bool ZERO>
+__global__ __launch_bounds__(256) void mg_wave_tile_kernel(TileArgs<float> a) {
+  using T = float;
+  constexpr int H = wt_halo<NS, MODE>();
+  constexpr int WTR = wt_rows<MODE>();
+  constexpr int R = WTR + 2 * H;  // rows j0 - H .. j0 + WTR + H - 1
+  constexpr bool kUpMode = (MODE == kUp || MODE == kUpCG);
+  const bool skip = a.sc && a.sc->done;  // checked once the loads are in flight
+  const LevelDev<T>& L = a.L;
+  const int lane = threadIdx.x & 63;
+  // Tile of this wavefront, made scalar so that every per-tile quantity and
+  // branch below is uniform (lane moves are only valid with all lanes on).
+  const int tile =
+      __builtin_amdgcn_readfirstlane(xcd_swizzle(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6));
+  const bool active = tile < a.ntiles;
+  const int tyi = active ? tile / a.tiles_x : 0;
+  const int i0 = (active ? tile - tyi * a.tiles_x : 0) * (64 - 2 * H);
+  const int j0 = tyi * WTR;
+  const int gi = i0 - H + lane;
+  const bool colin = active && gi >= 0 && gi < L.nx;
+  // Wave-uniform: every cell of the tile lies inside the level (mask 1).
+  const bool inner = active && j0 - H >= 0 && j0 - H + R <= L.ny && i0 - H >= 0 &&
+                     i0 - H + 64 <= L.nx;
+  // Interior columns of this lane, inside the level.
+  const bool lin = colin && lane >= H && lane < 64 - H;
+
+  // Loads from clamped (always valid) cells, zero selected outside the level:
+  // no branch per row, 32-bit offsets from the array base.
+  const int gic = min(max(gi, 0), L.nx - 1);
+  T u[R], b[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int gj = j0 - H + r;
+    const bool in = colin && gj >= 0 && gj < L.ny;
+    const unsigned g = (unsigned)(min(max(gj, 0), L.ny - 1) * L.nx + gic);
+    const T bv = a.b[g];
+    b[r] = in ? bv : T(0);
+    if constexpr (ZERO) {
+      u[r] = T(0);
+    } else {
+      const T uv = a.u_in[g];
+      u[r] = in ? uv : T(0);  // u * mask (multigrid.py:383), mask 1 inside
+    }
+  }
+  const bool first = (MODE == kUpCG) ? a.sc->first : true;
+  // Level l+1's iterate at this lane's coarse column I = gi / 2 and at its
+  // neighbour I +- 1 on the side of gi, over the coarse rows the tile touches:
+  // index p <-> coarse row j0 / 2 + floor(-H / 2) - 1 + p.
+  constexpr int RH = (-H) >> 1;
+  constexpr int NCR = ((R - 1 - H) >> 1) - RH + 3;
+  // With H odd, lane 0 has gi odd and lane 63 gi even, so every lane's
+  // neighbour column I +- 1 is the column I of the lane beside it (east for
+  // gi odd, west for gi even): one lane move instead of a second load.
+  constexpr bool kShareN = (H & 1) != 0;
+  T cI[kUpMode ? NCR : 1], cN[(kUpMode && !kShareN) ? NCR : 1];
+  const int I = gi >> 1, In = I + ((gi & 1) ? 1 : -1);
+  if constexpr (kUpMode) {
+    const LevelDev<T>& C = a.C;
+    const int cr0 = (j0 >> 1) + RH - 1;
+    const bool ci = active && I >= 0 && I < C.nx, cn = active && In >= 0 && In < C.nx;
+    const int Ic = min(max(I, 0), C.nx - 1), Inc = min(max(In, 0), C.nx - 1);
+#pragma unroll
+    for (int p = 0; p < NCR; ++p) {
+      const int J = cr0 + p;
+      const bool rin = J >= 0 && J < C.ny;
+      const unsigned row = (unsigned)(min(max(J, 0), C.ny - 1) * C.nx);
+      const T vi = C.u[row + Ic];
+      cI[p] = (rin && ci) ? vi : T(0);
+      if constexpr (!kShareN) {
+        const T vn = C.u[row + Inc];
+        cN[p] = (rin && cn) ? vn : T(0);
+      }
+    }
+  }
+  // Column In of coarse row p (zero outside the coarse level).
+  auto cNat = [&](int p) -> T {
+    if constexpr (kShareN) {
+      // Both moves with every lane active, then a select: a lane move inside
+      // a branch on the lane's parity would read the other parity's lanes
+      // while they are disabled (and get 0).
+      const T e = lane_east(cI[p]), w = lane_west(cI[p]);
+      const T v = (gi & 1) ? e : w;
+      return (In >= 0 && In < a.C.nx) ? v : T(0);
+    } else {
+      return cN[p];
+    }
+  };
+  if (skip) return;
+
+  const T om = a.omega, idx2 = L.idx2, idy2 = L.idy2, lam = L.lam, invd = L.inv_d;
+  // PCG dots (UP_CG), reduced after the interior/edge branch.
+  T d[kNumCgDots];
+#pragma unroll
+  for (int k = 0; k < kNumCgDots; ++k) d[k] = T(0);
+  // Cell mask of tile row r (1 inside the level).
+  auto Mr = [&](int r) -> T {
+    const int gj = j0 - H + r;
+    return (colin && gj >= 0 && gj < L.ny) ? T(1) : T(0);
+  };
+  auto body = [&](auto INNER) {
+    constexpr bool kIn = decltype(INNER)::value;
+    auto mask = [&](int r) -> T {
+      if constexpr (kIn) return T(1);
+      else return Mr(r);
+    };
+    // (A u)(r) for the tile's current u (tile_Au with neighbours from DPP).
+    auto Au = [&](int r, T m) -> T {
+      const T c = u[r];
+      const T e = lane_east(c), w = lane_west(c);
+      const T lap = (e + w - T(2) * c) * idx2 + (u[r + 1] + u[r - 1] - T(2) * c) * idy2;
+      return (lap - lam * c) * m;
+    };
+    // u = (u + P e_c) * mask  (multigrid.py:875-876), _prolongate
+    // (multigrid.py:510-603).
+    if constexpr (kUpMode) {
+      const LevelDev<T>& C = a.C;
+      // every fine cell's four coarse cells exist: the weights sum to 16
+      const bool inner_c = kIn && j0 - H >= 1 && j0 - H + R <= L.ny - 1 && i0 - H >= 1 &&
+                           i0 - H + 64 <= L.nx - 1;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int p = ((r - H) >> 1) - RH + 1;
+        const int pn = p + (((r + H) & 1) ? 1 : -1);
+        const T w = T(9) * cI[p] + T(3) * cNat(p) + T(3) * cI[pn] + cNat(pn);
+        if (inner_c) {
+          u[r] = u[r] + w * T(0.0625);  // exact: w / 16; mask 1
+        } else {
+          const int gj = j0 - H + r;
+          const int J = gj >> 1, Jn = J + ((gj & 1) ? 1 : -1);
+          const T mjc = (J >= 0 && J < C.ny) ? T(1) : T(0);
+          const T mjn = (Jn >= 0 && Jn < C.ny) ? T(1) : T(0);
+          const T mic = (I >= 0 && I < C.nx) ? T(1) : T(0);
+          const T min_ = (In >= 0 && In < C.nx) ? T(1) : T(0);
+          const T d = T(9) * mjc * mic + T(3) * mjc * min_ + T(3) * mjn * mic + mjn * min_;
+          const T e = w * __builtin_amdgcn_rcpf(d > T(1) ? d : T(1));
+          const T m = mask(r);
+          u[r] = (u[r] + e * m) * m;
+        }
+      }
+    }
+    // Damped-Jacobi sweeps (multigrid.py:408-461); the valid region shrinks
+    // by one row and one lane per sweep.  prev keeps row r - 1's old value.
+#pragma unroll
+    for (int sw = 1; sw <= NS; ++sw) {
+      T prev = u[sw - 1];
+#pragma unroll
+      for (int r = sw; r < R - sw; ++r) {
+        const T c = u[r];
+        const T m = mask(r);
+        const T inv = invd * m;
+        T un;
+        if (ZERO && sw == 1 && !kUpMode) {
+          un = (om * inv * b[r]) * m;
+        } else {
+          const T e = lane_east(c), w = lane_west(c);
+          const T lap = (e + w - T(2) * c) * idx2 + (u[r + 1] + prev - T(2) * c) * idy2;
+          const T au = (lap - lam * c) * m;
+          un = (c + om * inv * (b[r] - au)) * m;
+        }
+        prev = c;
+        u[r] = un;
+      }
+    }
+
+    if constexpr (MODE == kDown) {
+      // r = (b - A u) * m on the interior rows, 2x2 mask-weighted restriction
+      // (multigrid.py:464-507): rows summed in registers, the column pair
+      // (gi even, gi + 1) through one lane shift.
+      const LevelDev<T>& C = a.C;
+      const bool even = lin && ((gi & 1) == 0);
+#pragma unroll
+      for (int k2 = 0; k2 < WTR / 2; ++k2) {
+        const int r0 = H + 2 * k2;
+        const T m0 = mask(r0), m1 = mask(r0 + 1);
+        const T res0 = (b[r0] - Au(r0, m0)) * m0, res1 = (b[r0 + 1] - Au(r0 + 1, m1)) * m1;
+        const T s2 = res0 * m0 + res1 * m1;
+        const T c2 = m0 + m1;
+        const T s4 = s2 + lane_east(s2);
+        const T cnt = c2 + lane_east(c2);
+        const int J = (j0 >> 1) + k2;
+        if (even && J < C.ny && I < C.nx) {
+          const T mC = T(1);  // the coarse level of an all-wet level is all wet
+          C.b[J * C.nx + I] = (s4 / (cnt > T(1) ? cnt : T(1))) * mC;
+        }
+      }
+    }
+    if constexpr (MODE == kUpCG) {
+      // z, a = A z on the interior and the PCG dots (see mg_tile_kernel).
+      // q_{k-1} is read only here (a compiler barrier keeps its loads from
+      // being hoisted to the kernel start, where u, b and the coarse rows
+      // already fill the registers).
+      asm volatile("" ::: "memory");
+      T rq[WTR];
+#pragma unroll
+      for (int t = 0; t < WTR; ++t) {
+        const int gj = j0 + t;
+        const T v = a.w[(unsigned)(min(gj, L.ny - 1) * L.nx + gic)];
+        rq[t] = (!first && lin && gj < L.ny) ? v : T(0);
+      }
+#pragma unroll
+      for (int t = 0; t < WTR; ++t) {
+        const int r = H + t;
+        const int gj = j0 + t;
+        const T m = mask(r);
+        const T az = Au(r, m);
+        if (lin && gj < L.ny) {
+          const int g = gj * L.nx + gi;
+          const T z = u[r];
+          a.u_out[g] = z;
+          a.az_out[g] = az;
+          const T rd = b[r], qd = rq[t];
+          d[kZR] += z * rd;
+          d[kZQ] += z * qd;
+          d[kZA] += z * az;
+          d[kRR] += rd * rd;
+          d[kRA] += rd * az;
+          d[kRQ] += rd * qd;
+          d[kAA] += az * az;
+          d[kAQ] += az * qd;
+          d[kQQ] += qd * qd;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int t = 0; t < WTR; ++t) {
+        const int gj = j0 + t;
+        if (lin && gj < L.ny) a.u_out[gj * L.nx + gi] = u[H + t];
+      }
+    }
+  };
+  if (inner) body(std::true_type{});
+  else body(std::false_type{});
+  if constexpr (MODE == kUpCG) block_partials<kNumCgDots, 256>(d, a.partials);
+}
+
+// ---------------------------------------------------------------------------
 // Single-workgroup coarse V-cycle (levels lc .. nL-1) on global arrays that
 // stay resident in L2.  Starts from a zero guess at level lc (preconditioner
 // use) and writes the result to levels[lc].u.
@@ -1508,6 +1776,21 @@ static void launch_tile(const Plan& P, const LevelDev<T>& L, const LevelDev<T>* 
   a.omega = (T)P.omega;
   a.partials = P.partials;
   a.sc = sc;
+  if constexpr (std::is_same<T, float>::value && KIND == kDirichlet) {
+    if (!L.mask && (!C || !C->mask)) {
+      // all-wet FP32 level: one wavefront per tile (mg_wave_tile_kernel)
+      constexpr int H = wt_halo<NS, MODE>();
+      a.tiles_x = cdiv(L.nx, 64 - 2 * H);
+      a.ntiles = a.tiles_x * cdiv(L.ny, wt_rows<MODE>());
+      const int nblk = cdiv(a.ntiles, 4);
+      hipLaunchKernelGGL((mg_wave_tile_kernel<NS, MODE, ZERO>), dim3(nblk), dim3(256), 0, s, a);
+      if constexpr (MODE == kUpCG)
+        if (finish)
+          hipLaunchKernelGGL((reduce_finish_kernel<kNumCgDots, kRedCG>), dim3(1),
+                             dim3(kFinishThreads), 0, s, P.partials, nblk, sc, 0.0, 0.0);
+      return;
+    }
+  }
   a.tiles_x = cdiv(L.nx, kTx);
   a.ntiles = a.tiles_x * cdiv(L.ny, kTy);
   if (L.mask)

@@ -92,6 +92,52 @@ def test_v_cycle_matches_oracle(ny, nx, kind, masked, varcoef, npre, npost):
     assert_close_scaled(to_np(got32), ref0, 1e-4, "V(0,b) f32")
 
 
+@pytest.mark.parametrize("ny,nx", [(256, 192), (200, 168), (96, 520)])
+@pytest.mark.parametrize("npre,npost", [(2, 2), (1, 3), (6, 6)])
+def test_all_wet_dirichlet_fp32_v_cycle_matches_oracle(ny, nx, npre, npost):
+    """FP32 levels without a land mask under the Dirichlet operator run the
+    wavefront-tile kernels (registers + lane moves); sizes whose tile counts
+    are ragged and hierarchies deep enough to tile several levels, from a zero
+    and from a nonzero initial guess."""
+    need_gpu()
+    m = np.ones((ny, nx))
+    dx, dy, lam = 1.0 / nx, 1.3 / ny, 10.0
+    levels = fvx.build_multigrid(m, dx, dy, lam, kind="dirichlet")
+    rng = np.random.default_rng(21)
+    b = rng.standard_normal((ny, nx))
+    u0 = rng.standard_normal((ny, nx))
+    kw = dict(n_pre=npre, n_post=npost, n_coarse=30, omega=0.9)
+    mg32 = S.build_multigrid_solver(m, dx, dy, lam, kind="dirichlet", precision="float32", **kw)
+    got0 = mg32.v_cycle(None, torch.tensor(b, dtype=torch.float32))
+    assert_close_scaled(to_np(got0), fvx.v_cycle(levels, np.zeros_like(b), b, **kw), 1e-5,
+                        "V(0,b) f32")
+    got1 = mg32.v_cycle(torch.tensor(u0, dtype=torch.float32), torch.tensor(b, dtype=torch.float32))
+    assert_close_scaled(to_np(got1), fvx.v_cycle(levels, u0, b, **kw), 1e-5, "V(u0,b) f32")
+
+
+@pytest.mark.parametrize("ny,nx", [(256, 192), (200, 168)])
+def test_all_wet_dirichlet_fp32_pcg_iterates_match_oracle(ny, nx):
+    """The first flexible-PCG iterates x_1..x_3 of the FP32 solver (V-cycle
+    whose last kernel also forms A z and the dot products) against the
+    oracle's PCG with the same V(2,2) preconditioner: any error in z, A z or
+    the step scalars shows at the 1e-2 level, FP32 rounding at 1e-6."""
+    need_gpu()
+    m = np.ones((ny, nx))
+    dx, dy, lam = 1.0 / nx, 1.3 / ny, 10.0
+    levels = fvx.build_multigrid(m, dx, dy, lam, kind="dirichlet")
+    b = np.random.default_rng(22).standard_normal((ny, nx))
+    kw = dict(n_pre=2, n_post=2, n_coarse=4, omega=0.8)
+    M = lambda r: fvx.v_cycle(levels, np.zeros_like(r), r, **kw)
+    A = lambda u: fvx.apply_operator(u, levels[0])
+    mg32 = S.build_multigrid_solver(m, dx, dy, lam, kind="dirichlet", precision="float32", **kw)
+    for k in (1, 2, 3):
+        xr, _, _, _ = fvx.pcg(A, b, precond=M, rtol=1e-12, atol=0.0, max_steps=k)
+        x, info = mg32.pcg(torch.tensor(b, dtype=torch.float32), rtol=1e-12, atol=0.0,
+                           max_steps=k)
+        assert info.iterations == k
+        assert_close_scaled(to_np(x).astype(np.float64), xr, 1e-4, f"x_{k}")
+
+
 def test_solver_call_and_reference_thresholds():
     """Mirrors reference tests/test_multigrid.py:222-301 thresholds."""
     need_gpu()
