@@ -1409,12 +1409,14 @@ __global__ __launch_bounds__(256) void pcg_cg_update_kernel(int ny, int nx, T* _
                                                             const T* __restrict__ az,
                                                             T* __restrict__ p, T* __restrict__ q,
                                                             T* __restrict__ r, PcgScalars* sc) {
-  const int done = sc->done;
+  // The guard first: budgeted iterations after convergence then cost no
+  // traffic (this kernel is bandwidth-bound, so the flag's round trip hides
+  // behind the other blocks' loads).
+  if (sc->done == 1) return;
   const int i = blockIdx.x * 64 + threadIdx.x;
   const int j0 = (blockIdx.y * 4 + threadIdx.y) * kDirRows;
   const double beta64 = sc->beta;
   const T alpha = (T)sc->alpha, beta = (T)beta64;
-  // All loads first (their round trip overlaps the scalars'), then the guard.
   T xv[kDirRows], zv[kDirRows], av[kDirRows], pv[kDirRows], qv[kDirRows], rv[kDirRows];
 #pragma unroll
   for (int t = 0; t < kDirRows; ++t) {
@@ -1428,7 +1430,6 @@ __global__ __launch_bounds__(256) void pcg_cg_update_kernel(int ny, int nx, T* _
     qv[t] = in ? q[k] : T(0);
     rv[t] = in ? r[k] : T(0);
   }
-  if (done == 1) return;
   // beta == 0 on the first step: p_{k-1}, q_{k-1} are not read into it.
   const bool restart = beta64 == 0.0;
 #pragma unroll
