@@ -633,11 +633,11 @@ constexpr int wt_halo() {
 }
 // Interior rows per tile (even, for the 2x2 restriction).  Short tiles keep
 // the registers per lane low enough for 4-8 waves per SIMD, which hides the
-// load latency better than the halo they save costs (measured at 2048^2:
-// 8-10 rows beat 12-16).
+// load latency better than the halo they save costs (measured at 2048^2,
+// every mode: 8 rows beat 6, 10, 12 and 16).
 template <int MODE>
 constexpr int wt_rows() {
-  return MODE == kUpCG ? 10 : 8;
+  return 8;
 }
 
 template <int NS, int MODE, bool ZERO>
