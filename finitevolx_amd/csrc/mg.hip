@@ -2429,6 +2429,14 @@ int plan_levels(const Plan& P) { return P.nl; }
 bool plan_has_mixed(const Plan& P) { return P.has32 && P.has64; }
 int plan_nx(const Plan& P) { return P.has64 ? P.h64.lv[0].nx : P.h32.lv[0].nx; }
 int plan_ny(const Plan& P) { return P.has64 ? P.h64.lv[0].ny : P.h32.lv[0].ny; }
+void plan_schedule(const Plan& P, int* outer, int* inner) {
+  *outer = P.last_outer;
+  *inner = P.last_inner;
+}
+void plan_set_schedule(Plan& P, int outer, int inner) {
+  P.last_outer = std::max(1, outer);
+  P.last_inner = std::max(1, inner);
+}
 int plan_max_inner(const Plan& P, int slot) {
   const PcgScalars& h = P.hsc[2 * slot];
   return std::max(h.max_inner, h.iter);

@@ -95,6 +95,10 @@ int plan_ny(const Plan& P);
 // (o < n) with refinement step o's inner iterations and returns the number
 // of steps.
 int plan_max_inner(const Plan& P, int slot);
+// The synchronous mixed solve's opening schedule (refinement steps x inner
+// iterations, each capped by the call's limits); it adapts to the last solve.
+void plan_schedule(const Plan& P, int* outer, int* inner);
+void plan_set_schedule(Plan& P, int outer, int inner);
 int plan_inner_iters(const Plan& P, int slot, int* per_outer, int n);
 // Learned preconditioner application (precond.hip); sweeps as in
 // fvx_mg_precond_solve.

@@ -183,6 +183,7 @@ struct QGPlan {
   int obs_inner = 0;                          // largest inner solve (mixed)
   int obs_inner_o[kMaxOuter] = {};            // per refinement step (mixed)
   bool calibrated = false;
+  int eager_steps = 0;  // synchronous steps run before calibration
   int recaptures = 0;
   // Learned preconditioner for the inner FP32 solves (mixed mode only).
   void* net = nullptr;
@@ -385,43 +386,63 @@ static int qg_run_graph(QGPlan& Q, TS* q, double* psi, int nsteps, QGInfoC* info
   FVX_HIP(hipMemcpyAsync(Q.psi, psi, N * sizeof(double), hipMemcpyDeviceToDevice, s));
   int done_steps = 0;
   TS* cur = q;
-  if (!Q.calibrated) {
-    // One synchronous step to observe iteration counts (advances q in place).
-    Q.obs_inner = 0;
-    Q.obs_outer = 0;
-    for (int o = 0; o < QGPlan::kMaxOuter; ++o) Q.obs_inner_o[o] = 0;
-    QGInfoC ci;
-    int rc = qg_step_impl<TS>(Q, q, Q.psi, 1, &ci, s);
-    if (rc) return rc;
+  auto absorb = [&](const QGInfoC& ci) {
     acc.solves += ci.solves;
     acc.iterations += ci.iterations;
     acc.max_iterations = std::max(acc.max_iterations, ci.max_iterations);
     acc.outer += ci.outer;
     acc.max_rel_residual = std::max(acc.max_rel_residual, ci.max_rel_residual);
     if (!ci.converged) acc.converged = 0;
+  };
+  // Calibration: synchronous steps observe the iteration counts, then the
+  // step graphs are captured with those budgets (also within a one-step
+  // call, so that later calls replay graphs only).  The first steps usually
+  // start from a cold psi (zero or stale) and need more iterations than the
+  // run will (at 2048^2: 10.7, 9.3, 9.0 per solve, then 8.3), so the first
+  // kObserveStep - 1 steps run unobserved; the next one is observed (mixed
+  // precision: with unchunked inner solves, so each refinement step shows
+  // the iterations it needs from a warm start).
+  constexpr int kObserveStep = 3;
+  while (!Q.calibrated && done_steps < nsteps) {
+    const bool observe = Q.eager_steps >= kObserveStep - 1;
+    int so = 0, si = 0;
+    if (observe) {
+      Q.obs_inner = 0;
+      Q.obs_outer = 0;
+      for (int o = 0; o < QGPlan::kMaxOuter; ++o) Q.obs_inner_o[o] = 0;
+      if (Q.solve == kSolveMixed) {
+        plan_schedule(*Q.mg, &so, &si);
+        plan_set_schedule(*Q.mg, 1, Q.max_iters);
+      }
+    }
+    QGInfoC ci;
+    int rc = qg_step_impl<TS>(Q, q, Q.psi, 1, &ci, s);
+    if (observe && Q.solve == kSolveMixed) plan_set_schedule(*Q.mg, so, si);
+    if (rc) return rc;
+    absorb(ci);
+    ++done_steps;
+    ++Q.eager_steps;
+    if (!observe) continue;
     Q.obs_iters = ci.max_iterations;
     if (Q.solve != kSolveMixed) Q.obs_outer = 1;
-    done_steps = 1;
+    // Budgets: observed + 1 inner iteration of margin.
+    Q.b_iters = std::max(2, Q.obs_iters + 1);
+    Q.b_outer = std::min(std::max(1, Q.obs_outer), QGPlan::kMaxOuter);
+    // Inner budget per refinement sweep: the largest inner solve seen.
+    Q.b_inner = std::max(2, (Q.solve == kSolveMixed ? Q.obs_inner : Q.obs_iters) + 1);
+    // Per refinement step: what that step took in the observed solves, plus
+    // one (steps never observed get the largest inner budget).
+    for (int o = 0; o < QGPlan::kMaxOuter; ++o)
+      Q.b_inner_o[o] = Q.obs_inner_o[o] > 0 ? std::min(Q.max_iters, Q.obs_inner_o[o] + 1)
+                                            : Q.b_inner;
+    rc = capture_graphs<TS>(Q);
+    if (rc) return rc;
+    Q.calibrated = true;
   }
   if (done_steps < nsteps) {
-    // Budgets: observed + 1 inner iteration of margin.
-    int want_iters = std::max(2, Q.obs_iters + 1);
-    int want_outer = std::max(1, Q.obs_outer);
-    // Inner budget per refinement sweep: the largest inner solve seen.
-    int want_inner = std::max(2, (Q.solve == kSolveMixed ? Q.obs_inner : Q.obs_iters) + 1);
-    want_outer = std::min(want_outer, QGPlan::kMaxOuter);
-    if (!Q.calibrated || Q.exec[0] == nullptr) {
-      Q.b_iters = want_iters;
-      Q.b_outer = want_outer;
-      Q.b_inner = want_inner;
-      // Per refinement step: what that step took in the calibration solves,
-      // plus one (steps never observed get the largest inner budget).
-      for (int o = 0; o < QGPlan::kMaxOuter; ++o)
-        Q.b_inner_o[o] = Q.obs_inner_o[o] > 0 ? std::min(Q.max_iters, Q.obs_inner_o[o] + 1)
-                                              : want_inner;
+    if (Q.exec[0] == nullptr) {
       int rc = capture_graphs<TS>(Q);
       if (rc) return rc;
-      Q.calibrated = true;
     }
     FVX_HIP(hipMemcpyAsync(Q.qa, q, N * sizeof(TS), hipMemcpyDeviceToDevice, s));
     int g = 0;  // exec[0]: qa -> qb
