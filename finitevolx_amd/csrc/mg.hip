@@ -1851,6 +1851,11 @@ static T* smooth_chunks(const Plan& P, const LevelDev<T>& L, const T* u_in, bool
   return dst;
 }
 
+// Damped-Jacobi sweeps fused into one level kernel (with the residual and
+// restriction on the down leg, the prolongation on the up leg); more sweeps
+// per side run as separate smoothing kernels first (down) or after (up).
+constexpr int kMaxFuse = 3;
+
 // One V-cycle z = V(u0 or 0, rhs) at level 0.  With ``az`` given, the last
 // kernel also writes az = A z and the PCG dots against rhs and w (kUpCG) when
 // it can (two or fewer post-sweeps on a tiled level 0); *fused says whether
@@ -1902,12 +1907,15 @@ static int v_cycle_impl(Plan& P, const T* u0, const T* rhs, T* out, T* az, const
     bool zero = (l > 0) || (u0 == nullptr);
     const T* src = (l == 0) ? u0 : nullptr;
     const int npre = P.n_pre;
-    const int fz = npre >= 2 ? 2 : npre;  // sweeps fused with residual+restriction
+    const int fz = npre < kMaxFuse ? npre : kMaxFuse;  // fused with residual + restriction
     if (npre - fz > 0) {
       src = smooth_chunks<T, KIND>(P, L, src, zero, b, npre - fz, L.t, L.t2, nullptr, sc, s);
       zero = false;
     }
-    if (fz == 2) {
+    if (fz == 3) {
+      if (zero) launch_tile<T, KIND, 3, kDown, true>(P, L, &C, src, b, L.u, nullptr, sc, s);
+      else launch_tile<T, KIND, 3, kDown, false>(P, L, &C, src, b, L.u, nullptr, sc, s);
+    } else if (fz == 2) {
       if (zero) launch_tile<T, KIND, 2, kDown, true>(P, L, &C, src, b, L.u, nullptr, sc, s);
       else launch_tile<T, KIND, 2, kDown, false>(P, L, &C, src, b, L.u, nullptr, sc, s);
     } else if (fz == 1) {
@@ -1941,17 +1949,19 @@ static int v_cycle_impl(Plan& P, const T* u0, const T* rhs, T* out, T* az, const
     C.u = cur;
     const T* b = (l == 0) ? rhs : L.b;
     const int npost = P.n_post;
-    const int fz = npost >= 2 ? 2 : npost;
+    const int fz = npost < kMaxFuse ? npost : kMaxFuse;
     const int tail = npost - fz;
     const bool dk = az && l == 0 && tail == 0;
     T* dst = (l == 0 && tail == 0) ? out : L.t;
     if (dk) {
-      if (fz == 2) launch_tile<T, KIND, 2, kUpCG, false>(P, L, &C, L.u, b, dst, w, sc, s, az);
+      if (fz == 3) launch_tile<T, KIND, 3, kUpCG, false>(P, L, &C, L.u, b, dst, w, sc, s, az);
+      else if (fz == 2) launch_tile<T, KIND, 2, kUpCG, false>(P, L, &C, L.u, b, dst, w, sc, s, az);
       else if (fz == 1) launch_tile<T, KIND, 1, kUpCG, false>(P, L, &C, L.u, b, dst, w, sc, s, az);
       else launch_tile<T, KIND, 0, kUpCG, false>(P, L, &C, L.u, b, dst, w, sc, s, az);
       if (fused) *fused = true;
     } else {
-      if (fz == 2) launch_tile<T, KIND, 2, kUp, false>(P, L, &C, L.u, b, dst, nullptr, sc, s);
+      if (fz == 3) launch_tile<T, KIND, 3, kUp, false>(P, L, &C, L.u, b, dst, nullptr, sc, s);
+      else if (fz == 2) launch_tile<T, KIND, 2, kUp, false>(P, L, &C, L.u, b, dst, nullptr, sc, s);
       else if (fz == 1) launch_tile<T, KIND, 1, kUp, false>(P, L, &C, L.u, b, dst, nullptr, sc, s);
       else launch_tile<T, KIND, 0, kUp, false>(P, L, &C, L.u, b, dst, nullptr, sc, s);
     }

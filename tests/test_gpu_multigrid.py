@@ -116,17 +116,18 @@ def test_all_wet_dirichlet_fp32_v_cycle_matches_oracle(ny, nx, npre, npost):
 
 
 @pytest.mark.parametrize("ny,nx", [(256, 192), (200, 168)])
-def test_all_wet_dirichlet_fp32_pcg_iterates_match_oracle(ny, nx):
+@pytest.mark.parametrize("nsw", [2, 3])
+def test_all_wet_dirichlet_fp32_pcg_iterates_match_oracle(ny, nx, nsw):
     """The first flexible-PCG iterates x_1..x_3 of the FP32 solver (V-cycle
-    whose last kernel also forms A z and the dot products) against the
-    oracle's PCG with the same V(2,2) preconditioner: any error in z, A z or
+    whose last kernel also forms A z and the dot products, 2 or 3 fused
+    sweeps) against the oracle's PCG with the same V-cycle: any error in z, A z or
     the step scalars shows at the 1e-2 level, FP32 rounding at 1e-6."""
     need_gpu()
     m = np.ones((ny, nx))
     dx, dy, lam = 1.0 / nx, 1.3 / ny, 10.0
     levels = fvx.build_multigrid(m, dx, dy, lam, kind="dirichlet")
     b = np.random.default_rng(22).standard_normal((ny, nx))
-    kw = dict(n_pre=2, n_post=2, n_coarse=4, omega=0.8)
+    kw = dict(n_pre=nsw, n_post=nsw, n_coarse=4, omega=0.85)
     M = lambda r: fvx.v_cycle(levels, np.zeros_like(r), r, **kw)
     A = lambda u: fvx.apply_operator(u, levels[0])
     mg32 = S.build_multigrid_solver(m, dx, dy, lam, kind="dirichlet", precision="float32", **kw)
