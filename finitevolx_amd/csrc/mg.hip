@@ -632,12 +632,13 @@ constexpr int wt_halo() {
   return NS + ((MODE == kDown || MODE == kUpCG) ? 1 : 0);
 }
 // Interior rows per tile (even, for the 2x2 restriction).  Short tiles keep
-// the registers per lane low enough for 4-8 waves per SIMD, which hides the
-// load latency better than the halo they save costs (measured at 2048^2,
-// every mode: 8 rows beat 6, 10, 12 and 16).
+// the registers per lane low enough for 3-8 waves per SIMD, which hides the
+// load latency better than the halo they save costs.  Measured at 2048^2
+// with three fused sweeps: 8 rows for the down and up legs (against 6, 10,
+// 12, 16), 10 for the up leg with the PCG dots (against 6 and 8).
 template <int MODE>
 constexpr int wt_rows() {
-  return 8;
+  return MODE == kUpCG ? 10 : 8;
 }
 
 template <int NS, int MODE, bool ZERO>
@@ -1449,42 +1450,73 @@ __global__ __launch_bounds__(256) void pcg_cg_update_kernel(int ny, int nx, T* _
 
 // Mixed precision: r32 = (b64 - A64 x64) m in FP64, ||r||^2 and ||b||^2 in
 // FP64; the inner FP32 solve starts from e = 0 so e is cleared here too.
+// Each wave owns a 64-column strip of kResRows rows and slides a three-row
+// register window of x down it (every row of x read once per strip);
+// east/west neighbours come from lane shuffles, a strip's two edge columns
+// from memory.  Grid: grid_res (cdiv(nx, 64) x cdiv(ny, 4 kResRows)).
+constexpr int kResRows = 8;
+static inline dim3 grid_res(int ny, int nx) { return dim3(cdiv(nx, 64), cdiv(ny, 4 * kResRows)); }
+
 template <int KIND, typename TB>
 __global__ __launch_bounds__(256) void refine_residual_kernel(
     LevelDev<double> L, const TB* __restrict__ b, int64_t ld_b, const double* __restrict__ x,
     int64_t ld_x, float* __restrict__ r32, float* __restrict__ e32, double rtol2, double atol2,
     double* partials, PcgScalars* sc) {
-  const int done = sc->out_done;
-  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+  const int done = sc->out_done;  // checked once the first rows are in flight
+  const int lane = threadIdx.x;
+  const int i = blockIdx.x * 64 + lane;
+  const int j0 = (blockIdx.y * 4 + threadIdx.y) * kResRows;  // uniform within a wave
+  const int nx = L.nx, ny = L.ny;
+  const bool col = i < nx;
+  // x at (row j, column ii), masked for the Dirichlet operator (the reference
+  // applies the stencil to u * mask, multigrid.py:383); zero outside.
+  auto Xat = [&](int j, int ii) -> double {
+    if (j < 0 || j >= ny || ii < 0 || ii >= nx) return 0.0;
+    const double v = x[(int64_t)j * ld_x + ii];
+    return KIND == kDirichlet ? v * mval<double>(L.mask, (int64_t)j * nx + ii) : v;
+  };
+  auto CY = [&](int j) -> double {  // north-face coefficient of row j
+    return (KIND == kNoflux && j >= 0 && j < ny && col) ? L.cy[(int64_t)j * nx + i] : 0.0;
+  };
+  double xs = col ? Xat(j0 - 1, i) : 0.0, xc = col ? Xat(j0, i) : 0.0;
+  double cys = CY(j0 - 1);
   double bb = 0.0, rr = 0.0;
-  if (i < L.nx && j < L.ny) {
-    const int64_t k = (int64_t)j * L.nx + i;
-    const int64_t kx = (int64_t)j * ld_x + i;
-    const double m = mval<double>(L.mask, k);
-    auto X = [&](int64_t kk, int64_t km) {
-      return KIND == kDirichlet ? x[kk] * mval<double>(L.mask, km) : x[kk];
-    };
-    const double c = x[kx] * m;
-    const double e = (i + 1 < L.nx) ? X(kx + 1, k + 1) : 0.0;
-    const double w = (i > 0) ? X(kx - 1, k - 1) : 0.0;
-    const double n = (j + 1 < L.ny) ? X(kx + ld_x, k + L.nx) : 0.0;
-    const double s = (j > 0) ? X(kx - ld_x, k - L.nx) : 0.0;
+  for (int rw = 0; rw < kResRows; ++rw) {
+    const int j = j0 + rw;
+    if (j >= ny) break;
+    const double xn = col ? Xat(j + 1, i) : 0.0;
+    double w = __shfl_up(xc, 1, 64);
+    double e = __shfl_down(xc, 1, 64);
+    if (lane == 0) w = Xat(j, i - 1);
+    if (lane == 63) e = Xat(j, i + 1);
+    const int64_t k = (int64_t)j * nx + i;
+    const double m = col ? mval<double>(L.mask, k) : 0.0;
+    // centre u * mask; for the no-flux operator x itself is unmasked above
+    const double c = (KIND == kDirichlet) ? xc : xc * m;
     double lap;
+    double cyn = 0.0;
     if constexpr (KIND == kDirichlet) {
-      lap = (e + w - 2.0 * c) * L.idx2 + (n + s - 2.0 * c) * L.idy2;
+      lap = (e + w - 2.0 * c) * L.idx2 + (xn + xs - 2.0 * c) * L.idy2;
     } else {
-      const double cxe = L.cx[k], cxw = (i > 0) ? L.cx[k - 1] : 0.0;
-      const double cyn = L.cy[k], cys = (j > 0) ? L.cy[k - L.nx] : 0.0;
-      lap = (cxe * (e - c) - cxw * (c - w)) * L.idx2 + (cyn * (n - c) - cys * (c - s)) * L.idy2;
+      const double cxe = col ? L.cx[k] : 0.0;
+      double cxw = __shfl_up(cxe, 1, 64);
+      if (lane == 0) cxw = (i > 0 && i - 1 < nx) ? L.cx[k - 1] : 0.0;
+      cyn = CY(j);
+      lap = (cxe * (e - c) - cxw * (c - w)) * L.idx2 + (cyn * (xn - c) - cys * (c - xs)) * L.idy2;
     }
-    const double bv = (double)b[(int64_t)j * ld_b + i] * m;
-    const double rv = (bv - (lap - L.lam * c) * m) * m;
-    if (!done) {
-      r32[k] = (float)rv;
-      e32[k] = 0.0f;
+    if (col) {
+      const double bv = (double)b[(int64_t)j * ld_b + i] * m;
+      const double rv = (bv - (lap - L.lam * c) * m) * m;
+      if (!done) {
+        r32[k] = (float)rv;
+        e32[k] = 0.0f;
+      }
+      bb += bv * bv;
+      rr += rv * rv;
     }
-    bb = bv * bv;
-    rr = rv * rv;
+    xs = xc;
+    xc = xn;
+    cys = cyn;
   }
   if (done) return;
   double v[2] = {bb, rr};
@@ -1495,10 +1527,25 @@ __global__ __launch_bounds__(256) void refine_update_kernel(int ny, int nx, doub
                                                             int64_t ld_x,
                                                             const float* __restrict__ e,
                                                             PcgScalars* sc) {
+  // x += e over kDirRows rows per wave (grid_dir), loads first.
   if (sc->out_done) return;
-  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
-  if (i < nx && j < ny) x[(int64_t)j * ld_x + i] += (double)e[(int64_t)j * nx + i];
-  if (i == 0 && j == 0) sc->outer += 1;
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int j0 = (blockIdx.y * 4 + threadIdx.y) * kDirRows;
+  double xv[kDirRows];
+  float ev[kDirRows];
+#pragma unroll
+  for (int t = 0; t < kDirRows; ++t) {
+    const int j = j0 + t;
+    const bool in = i < nx && j < ny;
+    xv[t] = in ? x[(int64_t)j * ld_x + i] : 0.0;
+    ev[t] = in ? e[(int64_t)j * nx + i] : 0.0f;
+  }
+#pragma unroll
+  for (int t = 0; t < kDirRows; ++t) {
+    const int j = j0 + t;
+    if (i < nx && j < ny) x[(int64_t)j * ld_x + i] = xv[t] + (double)ev[t];
+  }
+  if (i == 0 && j0 == 0) sc->outer += 1;
 }
 
 // Reset an inner solve on an already-formed residual r (mixed precision):
@@ -1806,14 +1853,23 @@ static void launch_tile(const Plan& P, const LevelDev<T>& L, const LevelDev<T>* 
                          0, s, P.partials, a.ntiles, sc, 0.0, 0.0);
 }
 
-// Level-0 up kernel of a preconditioning V-cycle with the PCG dots (two
-// post-sweeps), as the V-cycle launches it, without its finishing kernel
-// (profiling hook).
+// Damped-Jacobi sweeps fused into one level kernel (with the residual and
+// restriction on the down leg, the prolongation on the up leg); more sweeps
+// per side run as separate smoothing kernels first (down) or after (up).
+constexpr int kMaxFuse = 3;
+
+// Level-0 up kernel of a preconditioning V-cycle with the PCG dots, with the
+// plan's fused post-sweeps, as the V-cycle launches it, without its
+// finishing kernel (profiling hook).
 template <typename T, int KIND>
 static void launch_vcycle_up_cg(const Plan& P, const LevelDev<T>& L, const LevelDev<T>& C,
                                 const T* u, const T* b, T* out, const T* w, T* az,
                                 PcgScalars* sc, hipStream_t s) {
-  launch_tile<T, KIND, 2, kUpCG, false>(P, L, &C, u, b, out, w, sc, s, az, false);
+  const int fz = P.n_post < kMaxFuse ? P.n_post : kMaxFuse;
+  if (fz == 3) launch_tile<T, KIND, 3, kUpCG, false>(P, L, &C, u, b, out, w, sc, s, az, false);
+  else if (fz == 2) launch_tile<T, KIND, 2, kUpCG, false>(P, L, &C, u, b, out, w, sc, s, az, false);
+  else if (fz == 1) launch_tile<T, KIND, 1, kUpCG, false>(P, L, &C, u, b, out, w, sc, s, az, false);
+  else launch_tile<T, KIND, 0, kUpCG, false>(P, L, &C, u, b, out, w, sc, s, az, false);
 }
 
 // Finishing kernel of a reduction over the blocks of grid2(ny, nx).
@@ -1850,11 +1906,6 @@ static T* smooth_chunks(const Plan& P, const LevelDev<T>& L, const T* u_in, bool
   }
   return dst;
 }
-
-// Damped-Jacobi sweeps fused into one level kernel (with the residual and
-// restriction on the down leg, the prolongation on the up leg); more sweeps
-// per side run as separate smoothing kernels first (down) or after (up).
-constexpr int kMaxFuse = 3;
 
 // One V-cycle z = V(u0 or 0, rhs) at level 0.  With ``az`` given, the last
 // kernel also writes az = A z and the PCG dots against rhs and w (kUpCG) when
@@ -2106,6 +2157,25 @@ int pcg_solve(Plan& P, const T* rhs, int64_t ld_rhs, T* x, int64_t ld_x, double 
   return kOk;
 }
 
+// FP64 refinement residual of x into the FP32 hierarchy's r (and e = 0), with
+// its finishing reduction into the outer scalars ``out``.
+template <typename TB>
+static void launch_refine_residual(Plan& P, const TB* rhs, int64_t ld_rhs, const double* x,
+                                   int64_t ld_x, double rtol, double atol, PcgScalars* out,
+                                   hipStream_t s) {
+  Hier<float>& H = P.h32;
+  const auto& L64 = P.h64.lv[0];
+  const dim3 g = grid_res(L64.ny, L64.nx);
+  if (P.kind == kDirichlet)
+    hipLaunchKernelGGL((refine_residual_kernel<kDirichlet, TB>), g, dim3(64, 4), 0, s, L64, rhs,
+                       ld_rhs, x, ld_x, H.r, H.x, rtol * rtol, atol * atol, P.partials, out);
+  else
+    hipLaunchKernelGGL((refine_residual_kernel<kNoflux, TB>), g, dim3(64, 4), 0, s, L64, rhs,
+                       ld_rhs, x, ld_x, H.r, H.x, rtol * rtol, atol * atol, P.partials, out);
+  hipLaunchKernelGGL((reduce_finish_kernel<2, kRedRefine>), dim3(1), dim3(kFinishThreads), 0, s,
+                     P.partials, (int)(g.x * g.y), out, rtol * rtol, atol * atol);
+}
+
 // Enqueue the mixed-precision schedule: n_outer x [FP64 residual, FP32 PCG
 // (n_inner[o] guarded iterations), FP64 update] and a closing FP64 residual.
 template <typename TB>
@@ -2119,15 +2189,7 @@ int mixed_enqueue(Plan& P, const TB* rhs, int64_t ld_rhs, double* x, int64_t ld_
   PcgScalars* out = &P.sc[2 * slot + 1];
   hipLaunchKernelGGL(scalars_reset_kernel, dim3(1), dim3(64), 0, s, in, 2);
   auto residual = [&]() {
-    if (P.kind == kDirichlet)
-      hipLaunchKernelGGL((refine_residual_kernel<kDirichlet, TB>), grid2(L64.ny, L64.nx),
-                         dim3(64, 4), 0, s, L64, rhs, ld_rhs, x, ld_x, H.r, H.x, rtol * rtol,
-                         atol * atol, P.partials, out);
-    else
-      hipLaunchKernelGGL((refine_residual_kernel<kNoflux, TB>), grid2(L64.ny, L64.nx),
-                         dim3(64, 4), 0, s, L64, rhs, ld_rhs, x, ld_x, H.r, H.x, rtol * rtol,
-                         atol * atol, P.partials, out);
-    finish2<2, kRedRefine>(P, L64.ny, L64.nx, out, s, rtol * rtol, atol * atol);
+    launch_refine_residual<TB>(P, rhs, ld_rhs, x, ld_x, rtol, atol, out, s);
   };
   for (int o = 0; o < n_outer; ++o) {
     residual();
@@ -2136,7 +2198,7 @@ int mixed_enqueue(Plan& P, const TB* rhs, int64_t ld_rhs, double* x, int64_t ld_
                  ? pcg_iters<float, kDirichlet>(P, H, H.x, L32.nx, n_inner[o], in, s, pf, ctx)
                  : pcg_iters<float, kNoflux>(P, H, H.x, L32.nx, n_inner[o], in, s, pf, ctx);
     if (rc) return rc;
-    hipLaunchKernelGGL(refine_update_kernel, grid2(L32.ny, L32.nx), dim3(64, 4), 0, s, L32.ny,
+    hipLaunchKernelGGL(refine_update_kernel, grid_dir(L32.ny, L32.nx), dim3(64, 4), 0, s, L32.ny,
                        L32.nx, x, ld_x, H.x, out);
   }
   residual();
@@ -2174,18 +2236,9 @@ int mixed_solve(Plan& P, const TB* rhs, int64_t ld_rhs, double* x, int64_t ld_x,
                ? pcg_iters<float, kDirichlet>(P, H, H.x, L32.nx, inner, in, s, pf, ctx)
                : pcg_iters<float, kNoflux>(P, H, H.x, L32.nx, inner, in, s, pf, ctx);
       if (rc) return rc;
-      hipLaunchKernelGGL(refine_update_kernel, grid2(L32.ny, L32.nx), dim3(64, 4), 0, s, L32.ny,
+      hipLaunchKernelGGL(refine_update_kernel, grid_dir(L32.ny, L32.nx), dim3(64, 4), 0, s, L32.ny,
                          L32.nx, x, ld_x, H.x, out);
-      const auto& L64 = P.h64.lv[0];
-      if (P.kind == kDirichlet)
-        hipLaunchKernelGGL((refine_residual_kernel<kDirichlet, TB>), grid2(L64.ny, L64.nx),
-                           dim3(64, 4), 0, s, L64, rhs, ld_rhs, x, ld_x, H.r, H.x, rtol * rtol,
-                           atol * atol, P.partials, out);
-      else
-        hipLaunchKernelGGL((refine_residual_kernel<kNoflux, TB>), grid2(L64.ny, L64.nx),
-                           dim3(64, 4), 0, s, L64, rhs, ld_rhs, x, ld_x, H.r, H.x, rtol * rtol,
-                           atol * atol, P.partials, out);
-      finish2<2, kRedRefine>(P, L64.ny, L64.nx, out, s, rtol * rtol, atol * atol);
+      launch_refine_residual<TB>(P, rhs, ld_rhs, x, ld_x, rtol, atol, out, s);
       FVX_LAUNCH_CHECK();
     }
     if (rc) return rc;
