@@ -850,7 +850,7 @@ __global__ __launch_bounds__(256) void mg_wave_tile_kernel(TileArgs<float> a) {
           const int g = gj * L.nx + gi;
           const T z = u[r];
           a.u_out[g] = z;
-          a.az_out[g] = az;
+          if (a.az_out) a.az_out[g] = az;  // else the update kernel recomputes it
           const T rd = b[r], qd = rq[t];
           d[kZR] += z * rd;
           d[kZQ] += z * qd;
@@ -1326,6 +1326,7 @@ __global__ __launch_bounds__(256) void pcg_init_kernel(LevelDev<T> L, const T* _
 // the same CU just read.
 constexpr int kDirRows = 8;       // rows per wave of the pointwise update kernel
 constexpr int kStencilRows = 8;   // rows per wave of the stencil kernel
+static_assert(kDirRows == kStencilRows, "pcg_cg_update_az_kernel runs on grid_stencil");
 static inline dim3 grid_dir(int ny, int nx) { return dim3(cdiv(nx, 64), cdiv(ny, 4 * kDirRows)); }
 static inline dim3 grid_stencil(int ny, int nx) {
   return dim3(cdiv(nx, 256), cdiv(ny, kStencilRows));
@@ -1440,6 +1441,70 @@ __global__ __launch_bounds__(256) void pcg_cg_update_kernel(int ny, int nx, T* _
       const int64_t k = (int64_t)j * nx + i;
       const T pn = restart ? zv[t] : zv[t] + beta * pv[t];
       const T qn = restart ? av[t] : av[t] + beta * qv[t];
+      p[k] = pn;
+      q[k] = qn;
+      x[(int64_t)j * ld_x + i] = xv[t] + alpha * pn;
+      r[k] = rv[t] - alpha * qn;
+    }
+  }
+}
+
+// The same PCG step for FP32 levels without a land mask under the Dirichlet
+// operator, whose wavefront-tile up leg does not store a = A z: a is formed
+// here from z (three-row register window per 64-column strip, east/west
+// neighbours through lane moves, the strip's edge columns from memory; the
+// four waves of a block sit side by side as in grid_stencil, so most edge
+// loads hit lines a neighbouring wave just read), which saves writing and
+// reading one array per iteration.
+template <typename T>
+__global__ __launch_bounds__(256) void pcg_cg_update_az_kernel(LevelDev<T> L, T* __restrict__ x,
+                                                               int64_t ld_x,
+                                                               const T* __restrict__ z,
+                                                               T* __restrict__ p,
+                                                               T* __restrict__ q,
+                                                               T* __restrict__ r,
+                                                               PcgScalars* sc) {
+  if (sc->done == 1) return;  // see pcg_cg_update_kernel
+  const int lane = threadIdx.x;
+  const int i = (blockIdx.x * 4 + threadIdx.y) * 64 + lane;
+  const int j0 = blockIdx.y * kDirRows;  // grid_stencil with kDirRows == kStencilRows
+  const int nx = L.nx, ny = L.ny;
+  const bool col = i < nx;
+  const double beta64 = sc->beta;
+  const T alpha = (T)sc->alpha, beta = (T)beta64;
+  auto Zat = [&](int j, int ii) -> T {
+    return (j >= 0 && j < ny && ii >= 0 && ii < nx) ? z[(int64_t)j * nx + ii] : T(0);
+  };
+  T zr[kDirRows + 2], zw[kDirRows], ze[kDirRows];
+  T xv[kDirRows], pv[kDirRows], qv[kDirRows], rv[kDirRows];
+#pragma unroll
+  for (int t = 0; t < kDirRows + 2; ++t) zr[t] = col ? Zat(j0 - 1 + t, i) : T(0);
+#pragma unroll
+  for (int t = 0; t < kDirRows; ++t) {
+    const int j = j0 + t;
+    const bool in = col && j < ny;
+    const int64_t k = (int64_t)j * nx + i;
+    xv[t] = in ? x[(int64_t)j * ld_x + i] : T(0);
+    pv[t] = in ? p[k] : T(0);
+    qv[t] = in ? q[k] : T(0);
+    rv[t] = in ? r[k] : T(0);
+    zw[t] = (lane == 0) ? Zat(j, i - 1) : T(0);
+    ze[t] = (lane == 63) ? Zat(j, i + 1) : T(0);
+  }
+  const bool restart = beta64 == 0.0;
+#pragma unroll
+  for (int t = 0; t < kDirRows; ++t) {
+    const int j = j0 + t;
+    const T c = zr[t + 1];
+    T w = lane_west(c), e = lane_east(c);
+    if (lane == 0) w = zw[t];
+    if (lane == 63) e = ze[t];
+    const T lap = (e + w - T(2) * c) * L.idx2 + (zr[t + 2] + zr[t] - T(2) * c) * L.idy2;
+    const T av = lap - L.lam * c;  // mask 1 inside the level
+    if (col && j < ny) {
+      const int64_t k = (int64_t)j * nx + i;
+      const T pn = restart ? c : c + beta * pv[t];
+      const T qn = restart ? av : av + beta * qv[t];
       p[k] = pn;
       q[k] = qn;
       x[(int64_t)j * ld_x + i] = xv[t] + alpha * pn;
@@ -1808,6 +1873,17 @@ Hier<double>& hier<double>(Plan& P) { return P.h64; }
 template <>
 Hier<float>& hier<float>(Plan& P) { return P.h32; }
 
+// Levels the wavefront-tile kernels run: FP32, Dirichlet operator, no land
+// mask (nor on the coarser level: the restriction of an all-wet mask is all
+// wet).  Their up leg with the PCG dots does not store a = A z; the PCG
+// update kernel of such a level recomputes it (pcg_cg_update_az_kernel).
+template <typename T, int KIND>
+static bool wave_tiled(const LevelDev<T>& L, const LevelDev<T>* C) {
+  if constexpr (std::is_same<T, float>::value && KIND == kDirichlet)
+    return !L.mask && (!C || !C->mask);
+  return false;
+}
+
 template <typename T, int KIND, int NS, int MODE, bool ZERO>
 static void launch_tile(const Plan& P, const LevelDev<T>& L, const LevelDev<T>* C, const T* u_in,
                         const T* b, T* u_out, const T* w, PcgScalars* sc, hipStream_t s,
@@ -1825,8 +1901,9 @@ static void launch_tile(const Plan& P, const LevelDev<T>& L, const LevelDev<T>* 
   a.partials = P.partials;
   a.sc = sc;
   if constexpr (std::is_same<T, float>::value && KIND == kDirichlet) {
-    if (!L.mask && (!C || !C->mask)) {
+    if (wave_tiled<T, KIND>(L, C)) {
       // all-wet FP32 level: one wavefront per tile (mg_wave_tile_kernel)
+      a.az_out = nullptr;  // recomputed by pcg_cg_update_az_kernel
       constexpr int H = wt_halo<NS, MODE>();
       a.tiles_x = cdiv(L.nx, 64 - 2 * H);
       a.ntiles = a.tiles_x * cdiv(L.ny, wt_rows<MODE>());
@@ -2062,8 +2139,17 @@ static int pcg_iters(Plan& P, Hier<T>& H, T* x, int64_t ld_x, int n, PcgScalars*
                          dim3(kFinishThreads), 0, s, P.partials, (int)(gs.x * gs.y), sc, 0.0,
                          0.0);
     }
-    hipLaunchKernelGGL(pcg_cg_update_kernel<T>, grid_dir(L.ny, L.nx), dim3(64, 4), 0, s, L.ny,
-                       L.nx, x, ld_x, H.z, H.az, H.p, H.q, H.r, sc);
+    bool recompute = false;
+    if constexpr (std::is_same<T, float>::value)
+      recompute = fused && H.lv.size() > 1 && wave_tiled<T, KIND>(L, &H.lv[1]);
+    if (recompute) {
+      if constexpr (std::is_same<T, float>::value)
+        hipLaunchKernelGGL(pcg_cg_update_az_kernel<T>, grid_stencil(L.ny, L.nx), dim3(64, 4), 0, s, L,
+                           x, ld_x, H.z, H.p, H.q, H.r, sc);
+    } else {
+      hipLaunchKernelGGL(pcg_cg_update_kernel<T>, grid_dir(L.ny, L.nx), dim3(64, 4), 0, s, L.ny,
+                         L.nx, x, ld_x, H.z, H.az, H.p, H.q, H.r, sc);
+    }
   }
   FVX_LAUNCH_CHECK();
   return kOk;
@@ -2461,8 +2547,18 @@ static int kernel_bench_impl(Plan& P, int which, int reps, hipStream_t s) {
       hipLaunchKernelGGL((pcg_az_dots_kernel<T, KIND>), grid_stencil(L.ny, L.nx), dim3(64, 4), 0,
                          s, L, H.z, H.r, H.q, H.az, P.partials, sc);
     } else if (which == 1) {
-      hipLaunchKernelGGL(pcg_cg_update_kernel<T>, grid_dir(L.ny, L.nx), dim3(64, 4), 0, s, L.ny,
-                         L.nx, H.x, (int64_t)L.nx, H.z, H.az, H.p, H.q, H.r, sc);
+      // the update kernel the PCG loop runs on this level
+      bool recompute = false;
+      if constexpr (std::is_same<T, float>::value)
+        recompute = P.nl > 1 && wave_tiled<T, KIND>(L, &H.lv[1]);
+      if (recompute) {
+        if constexpr (std::is_same<T, float>::value)
+          hipLaunchKernelGGL(pcg_cg_update_az_kernel<T>, grid_stencil(L.ny, L.nx), dim3(64, 4), 0, s,
+                             L, H.x, (int64_t)L.nx, H.z, H.p, H.q, H.r, sc);
+      } else {
+        hipLaunchKernelGGL(pcg_cg_update_kernel<T>, grid_dir(L.ny, L.nx), dim3(64, 4), 0, s,
+                           L.ny, L.nx, H.x, (int64_t)L.nx, H.z, H.az, H.p, H.q, H.r, sc);
+      }
     } else {
       // The last kernel of a preconditioning V-cycle on level 0: prolongation
       // of level 1's iterate, correction, two sweeps, a = A z, PCG dots.
