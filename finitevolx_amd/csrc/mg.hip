@@ -632,13 +632,13 @@ constexpr int wt_halo() {
   return NS + ((MODE == kDown || MODE == kUpCG) ? 1 : 0);
 }
 // Interior rows per tile (even, for the 2x2 restriction).  Short tiles keep
-// the registers per lane low enough for 3-8 waves per SIMD, which hides the
+// the registers per lane low enough for 6-8 waves per SIMD, which hides the
 // load latency better than the halo they save costs.  Measured at 2048^2
 // with three fused sweeps: 8 rows for the down and up legs (against 6, 10,
-// 12, 16), 10 for the up leg with the PCG dots (against 6 and 8).
+// 12, 16) and for the up leg with the PCG dots (against 6 and 10).
 template <int MODE>
 constexpr int wt_rows() {
-  return MODE == kUpCG ? 10 : 8;
+  return 8;
 }
 
 template <int NS, int MODE, bool ZERO>
@@ -661,9 +661,6 @@ __global__ __launch_bounds__(256) void mg_wave_tile_kernel(TileArgs<float> a) {
   const int j0 = tyi * WTR;
   const int gi = i0 - H + lane;
   const bool colin = active && gi >= 0 && gi < L.nx;
-  // Wave-uniform: every cell of the tile lies inside the level (mask 1).
-  const bool inner = active && j0 - H >= 0 && j0 - H + R <= L.ny && i0 - H >= 0 &&
-                     i0 - H + 64 <= L.nx;
   // Interior columns of this lane, inside the level.
   const bool lin = colin && lane >= H && lane < 64 - H;
 
@@ -691,11 +688,10 @@ __global__ __launch_bounds__(256) void mg_wave_tile_kernel(TileArgs<float> a) {
   // index p <-> coarse row j0 / 2 + floor(-H / 2) - 1 + p.
   constexpr int RH = (-H) >> 1;
   constexpr int NCR = ((R - 1 - H) >> 1) - RH + 3;
-  // With H odd, lane 0 has gi odd and lane 63 gi even, so every lane's
-  // neighbour column I +- 1 is the column I of the lane beside it (east for
-  // gi odd, west for gi even): one lane move instead of a second load.
-  constexpr bool kShareN = (H & 1) != 0;
-  T cI[kUpMode ? NCR : 1], cN[(kUpMode && !kShareN) ? NCR : 1];
+  // (Each lane loads both coarse columns: taking the neighbour column from
+  // the lane beside it, possible for odd H, held more registers than the
+  // loads it saved.)
+  T cI[kUpMode ? NCR : 1], cN[kUpMode ? NCR : 1];
   const int I = gi >> 1, In = I + ((gi & 1) ? 1 : -1);
   if constexpr (kUpMode) {
     const LevelDev<T>& C = a.C;
@@ -707,172 +703,148 @@ __global__ __launch_bounds__(256) void mg_wave_tile_kernel(TileArgs<float> a) {
       const int J = cr0 + p;
       const bool rin = J >= 0 && J < C.ny;
       const unsigned row = (unsigned)(min(max(J, 0), C.ny - 1) * C.nx);
-      const T vi = C.u[row + Ic];
+      const T vi = C.u[row + Ic], vn = C.u[row + Inc];
       cI[p] = (rin && ci) ? vi : T(0);
-      if constexpr (!kShareN) {
-        const T vn = C.u[row + Inc];
-        cN[p] = (rin && cn) ? vn : T(0);
-      }
+      cN[p] = (rin && cn) ? vn : T(0);
     }
   }
-  // Column In of coarse row p (zero outside the coarse level).
-  auto cNat = [&](int p) -> T {
-    if constexpr (kShareN) {
-      // Both moves with every lane active, then a select: a lane move inside
-      // a branch on the lane's parity would read the other parity's lanes
-      // while they are disabled (and get 0).
-      const T e = lane_east(cI[p]), w = lane_west(cI[p]);
-      const T v = (gi & 1) ? e : w;
-      return (In >= 0 && In < a.C.nx) ? v : T(0);
-    } else {
-      return cN[p];
-    }
-  };
   if (skip) return;
 
   const T om = a.omega, idx2 = L.idx2, idy2 = L.idy2, lam = L.lam, invd = L.inv_d;
-  // PCG dots (UP_CG), reduced after the interior/edge branch.
+  // One code path for every tile.  Cells outside the level (tile rows above
+  // or below it, lanes left or right of it) are set to zero after the
+  // prolongation and after each sweep, which is what the reference's
+  // multiplication by the mask does there (mask 0 outside, 1 inside an
+  // all-wet level); inside, the mask is 1 and drops out of every formula.
+  // (A separate all-inside path doubled the kernel and its registers.)
+  auto row_in = [&](int r) -> bool {
+    const int gj = j0 - H + r;
+    return gj >= 0 && gj < L.ny;
+  };
+  auto keep = [&](int r, T v) -> T { return (colin && row_in(r)) ? v : T(0); };
+  // (A u)(r) for the tile's current u, inside the level.
+  auto Au = [&](int r) -> T {
+    const T c = u[r];
+    const T e = lane_east(c), w = lane_west(c);
+    const T lap = (e + w - T(2) * c) * idx2 + (u[r + 1] + u[r - 1] - T(2) * c) * idy2;
+    return lap - lam * c;
+  };
+  // u = (u + P e_c) * mask  (multigrid.py:875-876), _prolongate
+  // (multigrid.py:510-603).  The weight normaliser is separable: with the
+  // coarse row J = gj / 2, its neighbour Jn on the side of gj, and the same
+  // for columns, d = 9 mJ mI + 3 mJ mIn + 3 mJn mI + mJn mIn
+  //                = (3 mJ + mJn) (3 mI + mIn),
+  // 16 wherever all four coarse cells exist; where the row or column factor
+  // is zero, so is every weighted coarse value.
+  if constexpr (kUpMode) {
+    const LevelDev<T>& C = a.C;
+    const T cf = T(3) * T(I >= 0 && I < C.nx) + T(In >= 0 && In < C.nx);
+    const T lanef = __builtin_amdgcn_rcpf(cf > T(1) ? cf : T(1));
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int p = ((r - H) >> 1) - RH + 1;
+      const int pn = p + (((r + H) & 1) ? 1 : -1);
+      const T w = T(9) * cI[p] + T(3) * cN[p] + T(3) * cI[pn] + cN[pn];
+      const int gj = j0 - H + r;
+      const int J = gj >> 1, Jn = J + ((gj & 1) ? 1 : -1);
+      const int rf = 3 * (J >= 0 && J < C.ny) + (Jn >= 0 && Jn < C.ny);  // wave-uniform
+      const T rowf = rf == 4 ? T(0.25) : (rf == 3 ? T(1) / T(3) : (rf == 1 ? T(1) : T(0)));
+      u[r] = keep(r, u[r] + w * (rowf * lanef));
+    }
+  }
+  // Damped-Jacobi sweeps (multigrid.py:408-461); the valid region shrinks
+  // by one row and one lane per sweep.  prev keeps row r - 1's old value.
+  // u + om (b - A u) / diag with A u = idx2 (e + w) + idy2 (n + s) - (2 idx2
+  // + 2 idy2 + lam) u and 1 / diag = invd:
+  const T kb = om * invd, kx = -kb * idx2, ky = -kb * idy2;
+  const T k0 = T(1) + kb * (T(2) * idx2 + T(2) * idy2 + lam);
+#pragma unroll
+  for (int sw = 1; sw <= NS; ++sw) {
+    T prev = u[sw - 1];
+#pragma unroll
+    for (int r = sw; r < R - sw; ++r) {
+      const T c = u[r];
+      T un;
+      if (ZERO && sw == 1 && !kUpMode) {
+        un = om * invd * b[r];  // b is zero outside the level
+      } else {
+        // c + om / diag * (b - A u), expanded into one coefficient per term
+        const T e = lane_east(c), w = lane_west(c);
+        un = keep(r, k0 * c + kb * b[r] + kx * (e + w) + ky * (u[r + 1] + prev));
+      }
+      prev = c;
+      u[r] = un;
+    }
+  }
+
+  // PCG dots (UP_CG), reduced at the end.
   T d[kNumCgDots];
 #pragma unroll
   for (int k = 0; k < kNumCgDots; ++k) d[k] = T(0);
-  // Cell mask of tile row r (1 inside the level).
-  auto Mr = [&](int r) -> T {
-    const int gj = j0 - H + r;
-    return (colin && gj >= 0 && gj < L.ny) ? T(1) : T(0);
-  };
-  auto body = [&](auto INNER) {
-    constexpr bool kIn = decltype(INNER)::value;
-    auto mask = [&](int r) -> T {
-      if constexpr (kIn) return T(1);
-      else return Mr(r);
-    };
-    // (A u)(r) for the tile's current u (tile_Au with neighbours from DPP).
-    auto Au = [&](int r, T m) -> T {
-      const T c = u[r];
-      const T e = lane_east(c), w = lane_west(c);
-      const T lap = (e + w - T(2) * c) * idx2 + (u[r + 1] + u[r - 1] - T(2) * c) * idy2;
-      return (lap - lam * c) * m;
-    };
-    // u = (u + P e_c) * mask  (multigrid.py:875-876), _prolongate
-    // (multigrid.py:510-603).
-    if constexpr (kUpMode) {
-      const LevelDev<T>& C = a.C;
-      // every fine cell's four coarse cells exist: the weights sum to 16
-      const bool inner_c = kIn && j0 - H >= 1 && j0 - H + R <= L.ny - 1 && i0 - H >= 1 &&
-                           i0 - H + 64 <= L.nx - 1;
+  if constexpr (MODE == kDown) {
+    // r = (b - A u) * m on the interior rows, 2x2 mask-weighted restriction
+    // (multigrid.py:464-507): rows summed in registers, the column pair
+    // (gi even, gi + 1) through one lane shift.
+    const LevelDev<T>& C = a.C;
+    const bool even = lin && ((gi & 1) == 0);
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const int p = ((r - H) >> 1) - RH + 1;
-        const int pn = p + (((r + H) & 1) ? 1 : -1);
-        const T w = T(9) * cI[p] + T(3) * cNat(p) + T(3) * cI[pn] + cNat(pn);
-        if (inner_c) {
-          u[r] = u[r] + w * T(0.0625);  // exact: w / 16; mask 1
-        } else {
-          const int gj = j0 - H + r;
-          const int J = gj >> 1, Jn = J + ((gj & 1) ? 1 : -1);
-          const T mjc = (J >= 0 && J < C.ny) ? T(1) : T(0);
-          const T mjn = (Jn >= 0 && Jn < C.ny) ? T(1) : T(0);
-          const T mic = (I >= 0 && I < C.nx) ? T(1) : T(0);
-          const T min_ = (In >= 0 && In < C.nx) ? T(1) : T(0);
-          const T d = T(9) * mjc * mic + T(3) * mjc * min_ + T(3) * mjn * mic + mjn * min_;
-          const T e = w * __builtin_amdgcn_rcpf(d > T(1) ? d : T(1));
-          const T m = mask(r);
-          u[r] = (u[r] + e * m) * m;
-        }
+    for (int k2 = 0; k2 < WTR / 2; ++k2) {
+      const int r0 = H + 2 * k2;
+      const T m0 = keep(r0, T(1)), m1 = keep(r0 + 1, T(1));
+      const T res0 = (b[r0] - Au(r0)) * m0, res1 = (b[r0 + 1] - Au(r0 + 1)) * m1;
+      const T s2 = res0 + res1;
+      const T c2 = m0 + m1;
+      const T s4 = s2 + lane_east(s2);
+      const T cnt = c2 + lane_east(c2);
+      const int J = (j0 >> 1) + k2;
+      if (even && J < C.ny && I < C.nx) {
+        // the coarse level of an all-wet level is all wet (mask 1)
+        C.b[J * C.nx + I] = s4 / (cnt > T(1) ? cnt : T(1));
       }
     }
-    // Damped-Jacobi sweeps (multigrid.py:408-461); the valid region shrinks
-    // by one row and one lane per sweep.  prev keeps row r - 1's old value.
+  }
+  if constexpr (MODE == kUpCG) {
+    // z, a = A z on the interior and the PCG dots (see mg_tile_kernel).
+    // q_{k-1} is read only here (a compiler barrier keeps its loads from
+    // being hoisted to the kernel start, where u, b and the coarse rows
+    // already fill the registers).
+    asm volatile("" ::: "memory");
+    T rq[WTR];
 #pragma unroll
-    for (int sw = 1; sw <= NS; ++sw) {
-      T prev = u[sw - 1];
+    for (int t = 0; t < WTR; ++t) {
+      const int gj = j0 + t;
+      const T v = a.w[(unsigned)(min(gj, L.ny - 1) * L.nx + gic)];
+      rq[t] = (!first && lin && gj < L.ny) ? v : T(0);
+    }
 #pragma unroll
-      for (int r = sw; r < R - sw; ++r) {
-        const T c = u[r];
-        const T m = mask(r);
-        const T inv = invd * m;
-        T un;
-        if (ZERO && sw == 1 && !kUpMode) {
-          un = (om * inv * b[r]) * m;
-        } else {
-          const T e = lane_east(c), w = lane_west(c);
-          const T lap = (e + w - T(2) * c) * idx2 + (u[r + 1] + prev - T(2) * c) * idy2;
-          const T au = (lap - lam * c) * m;
-          un = (c + om * inv * (b[r] - au)) * m;
-        }
-        prev = c;
-        u[r] = un;
+    for (int t = 0; t < WTR; ++t) {
+      const int r = H + t;
+      const int gj = j0 + t;
+      const T az = Au(r);
+      if (lin && gj < L.ny) {
+        const int g = gj * L.nx + gi;
+        const T z = u[r];
+        a.u_out[g] = z;
+        if (a.az_out) a.az_out[g] = az;  // else the update kernel recomputes it
+        const T rd = b[r], qd = rq[t];
+        d[kZR] += z * rd;
+        d[kZQ] += z * qd;
+        d[kZA] += z * az;
+        d[kRR] += rd * rd;
+        d[kRA] += rd * az;
+        d[kRQ] += rd * qd;
+        d[kAA] += az * az;
+        d[kAQ] += az * qd;
+        d[kQQ] += qd * qd;
       }
     }
-
-    if constexpr (MODE == kDown) {
-      // r = (b - A u) * m on the interior rows, 2x2 mask-weighted restriction
-      // (multigrid.py:464-507): rows summed in registers, the column pair
-      // (gi even, gi + 1) through one lane shift.
-      const LevelDev<T>& C = a.C;
-      const bool even = lin && ((gi & 1) == 0);
+  } else {
 #pragma unroll
-      for (int k2 = 0; k2 < WTR / 2; ++k2) {
-        const int r0 = H + 2 * k2;
-        const T m0 = mask(r0), m1 = mask(r0 + 1);
-        const T res0 = (b[r0] - Au(r0, m0)) * m0, res1 = (b[r0 + 1] - Au(r0 + 1, m1)) * m1;
-        const T s2 = res0 * m0 + res1 * m1;
-        const T c2 = m0 + m1;
-        const T s4 = s2 + lane_east(s2);
-        const T cnt = c2 + lane_east(c2);
-        const int J = (j0 >> 1) + k2;
-        if (even && J < C.ny && I < C.nx) {
-          const T mC = T(1);  // the coarse level of an all-wet level is all wet
-          C.b[J * C.nx + I] = (s4 / (cnt > T(1) ? cnt : T(1))) * mC;
-        }
-      }
+    for (int t = 0; t < WTR; ++t) {
+      const int gj = j0 + t;
+      if (lin && gj < L.ny) a.u_out[gj * L.nx + gi] = u[H + t];
     }
-    if constexpr (MODE == kUpCG) {
-      // z, a = A z on the interior and the PCG dots (see mg_tile_kernel).
-      // q_{k-1} is read only here (a compiler barrier keeps its loads from
-      // being hoisted to the kernel start, where u, b and the coarse rows
-      // already fill the registers).
-      asm volatile("" ::: "memory");
-      T rq[WTR];
-#pragma unroll
-      for (int t = 0; t < WTR; ++t) {
-        const int gj = j0 + t;
-        const T v = a.w[(unsigned)(min(gj, L.ny - 1) * L.nx + gic)];
-        rq[t] = (!first && lin && gj < L.ny) ? v : T(0);
-      }
-#pragma unroll
-      for (int t = 0; t < WTR; ++t) {
-        const int r = H + t;
-        const int gj = j0 + t;
-        const T m = mask(r);
-        const T az = Au(r, m);
-        if (lin && gj < L.ny) {
-          const int g = gj * L.nx + gi;
-          const T z = u[r];
-          a.u_out[g] = z;
-          if (a.az_out) a.az_out[g] = az;  // else the update kernel recomputes it
-          const T rd = b[r], qd = rq[t];
-          d[kZR] += z * rd;
-          d[kZQ] += z * qd;
-          d[kZA] += z * az;
-          d[kRR] += rd * rd;
-          d[kRA] += rd * az;
-          d[kRQ] += rd * qd;
-          d[kAA] += az * az;
-          d[kAQ] += az * qd;
-          d[kQQ] += qd * qd;
-        }
-      }
-    } else {
-#pragma unroll
-      for (int t = 0; t < WTR; ++t) {
-        const int gj = j0 + t;
-        if (lin && gj < L.ny) a.u_out[gj * L.nx + gi] = u[H + t];
-      }
-    }
-  };
-  if (inner) body(std::true_type{});
-  else body(std::false_type{});
+  }
   if constexpr (MODE == kUpCG) block_partials<kNumCgDots, 256>(d, a.partials);
 }
 
