@@ -298,7 +298,24 @@ __device__ __forceinline__ T glob_inv(const LevelDev<T>& L, int64_t k) {
 //                interior -> u_out (z), az_out (a), and the nine PCG dots of
 //                z, a, r (= b) and q_{k-1} (= w)
 // ---------------------------------------------------------------------------
-enum TileMode : int { kSmooth = 0, kDown = 1, kUp = 2, kUpCG = 3 };
+enum TileMode : int { kSmooth = 0, kDown = 1, kUp = 2, kUpCG = 3, kDownUpd = 4 };
+
+// The PCG step that the level-0 down kernel of the next V-cycle applies
+// first (kDownUpd; wavefront tiles only): with a = A z,
+// q_out = a + beta q_in, r_out = r_in - alpha q_out, p = z + beta p,
+// x += alpha p.  r_in and q_in are read on the tile's halo as well, so they
+// cannot be the arrays written.
+template <typename T>
+struct PcgUpdArgs {
+  const T* z;
+  T* p;
+  T* x;
+  int64_t ld_x;
+  const T* q_in;
+  T* q_out;
+  const T* r_in;  // passed to the kernel as its b
+  T* r_out;
+};
 
 constexpr int kTx = 64, kTy = 16, kTileThreads = 256;
 
@@ -314,6 +331,7 @@ struct TileArgs {
   double* partials;
   PcgScalars* sc;
   int tiles_x, ntiles;
+  PcgUpdArgs<T> upd;  // kDownUpd; b is r_in
 };
 
 template <typename T, int KIND>
@@ -627,9 +645,30 @@ __device__ __forceinline__ float lane_west(float v) {
   return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
 }
 
+// Buffer accesses: a 4-register descriptor per array (base, byte size), a
+// wave-uniform row offset in a scalar register and the lane's column offset,
+// so no per-access 64-bit address arithmetic; accesses past the array's end
+// read 0 / are dropped.  Arrays must be under 2^31 bytes.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const float* p, int n) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(p), (short)0, n * 4, 0x00020000);
+}
+__device__ __forceinline__ float buf_load(__amdgpu_buffer_rsrc_t r, unsigned col, int row_off) {
+  return __int_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, col * 4u, row_off * 4, 0));
+}
+__device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, unsigned col, int row_off,
+                                          float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_int(v), r, col * 4u, row_off * 4, 0);
+}
+
 template <int NS, int MODE>
 constexpr int wt_halo() {
-  return NS + ((MODE == kDown || MODE == kUpCG) ? 1 : 0);
+  return NS + ((MODE == kDown || MODE == kDownUpd || MODE == kUpCG) ? 1 : 0);
+}
+// Lanes of halo on each side: one more for kDownUpd, whose a = A z is wrong
+// on the wave's two edge lanes (their outer neighbours are in no lane).
+template <int NS, int MODE>
+constexpr int wt_lane_halo() {
+  return wt_halo<NS, MODE>() + (MODE == kDownUpd ? 1 : 0);
 }
 // Interior rows per tile (even, for the 2x2 restriction).  Short tiles keep
 // the registers per lane low enough for 6-8 waves per SIMD, which hides the
@@ -645,10 +684,15 @@ template <int NS, int MODE, bool ZERO>
 __global__ __launch_bounds__(256) void mg_wave_tile_kernel(TileArgs<float> a) {
   using T = float;
   constexpr int H = wt_halo<NS, MODE>();
+  constexpr int HL = wt_lane_halo<NS, MODE>();
   constexpr int WTR = wt_rows<MODE>();
   constexpr int R = WTR + 2 * H;  // rows j0 - H .. j0 + WTR + H - 1
   constexpr bool kUpMode = (MODE == kUp || MODE == kUpCG);
-  const bool skip = a.sc && a.sc->done;  // checked once the loads are in flight
+  constexpr bool kDownMode = (MODE == kDown || MODE == kDownUpd);
+  constexpr bool kUpd = MODE == kDownUpd;
+  // Stop flag, checked once the loads are in flight (kDownUpd: 2 = apply the
+  // PCG step, then stop).
+  const int flag = a.sc ? a.sc->done : 0;
   const LevelDev<T>& L = a.L;
   const int lane = threadIdx.x & 63;
   // Tile of this wavefront, made scalar so that every per-tile quantity and
@@ -657,71 +701,181 @@ __global__ __launch_bounds__(256) void mg_wave_tile_kernel(TileArgs<float> a) {
       __builtin_amdgcn_readfirstlane(xcd_swizzle(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6));
   const bool active = tile < a.ntiles;
   const int tyi = active ? tile / a.tiles_x : 0;
-  const int i0 = (active ? tile - tyi * a.tiles_x : 0) * (64 - 2 * H);
+  const int i0 = (active ? tile - tyi * a.tiles_x : 0) * (64 - 2 * HL);
   const int j0 = tyi * WTR;
-  const int gi = i0 - H + lane;
+  const int gi = i0 - HL + lane;
   const bool colin = active && gi >= 0 && gi < L.nx;
   // Interior columns of this lane, inside the level.
-  const bool lin = colin && lane >= H && lane < 64 - H;
-
-  // Loads from clamped (always valid) cells, zero selected outside the level:
-  // no branch per row, 32-bit offsets from the array base.
-  const int gic = min(max(gi, 0), L.nx - 1);
-  T u[R], b[R];
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const int gj = j0 - H + r;
-    const bool in = colin && gj >= 0 && gj < L.ny;
-    const unsigned g = (unsigned)(min(max(gj, 0), L.ny - 1) * L.nx + gic);
-    const T bv = a.b[g];
-    b[r] = in ? bv : T(0);
-    if constexpr (ZERO) {
-      u[r] = T(0);
-    } else {
-      const T uv = a.u_in[g];
-      u[r] = in ? uv : T(0);  // u * mask (multigrid.py:383), mask 1 inside
-    }
-  }
-  const bool first = (MODE == kUpCG) ? a.sc->first : true;
-  // Level l+1's iterate at this lane's coarse column I = gi / 2 and at its
-  // neighbour I +- 1 on the side of gi, over the coarse rows the tile touches:
-  // index p <-> coarse row j0 / 2 + floor(-H / 2) - 1 + p.
+  const bool lin = colin && lane >= HL && lane < 64 - HL;
+  const int I = gi >> 1, In = I + ((gi & 1) ? 1 : -1);
+  // Coarse rows cr0 .. cr0 + NCR - 1 hold level l+1's iterate at this lane's
+  // coarse column I = gi / 2 (cI) and at its neighbour In = I +- 1 on the
+  // side of gi (cN), for every fine row of the tile.
   constexpr int RH = (-H) >> 1;
   constexpr int NCR = ((R - 1 - H) >> 1) - RH + 3;
-  // (Each lane loads both coarse columns: taking the neighbour column from
-  // the lane beside it, possible for odd H, held more registers than the
-  // loads it saved.)
-  T cI[kUpMode ? NCR : 1], cN[kUpMode ? NCR : 1];
-  const int I = gi >> 1, In = I + ((gi & 1) ? 1 : -1);
-  if constexpr (kUpMode) {
-    const LevelDev<T>& C = a.C;
-    const int cr0 = (j0 >> 1) + RH - 1;
-    const bool ci = active && I >= 0 && I < C.nx, cn = active && In >= 0 && In < C.nx;
-    const int Ic = min(max(I, 0), C.nx - 1), Inc = min(max(In, 0), C.nx - 1);
-#pragma unroll
-    for (int p = 0; p < NCR; ++p) {
-      const int J = cr0 + p;
-      const bool rin = J >= 0 && J < C.ny;
-      const unsigned row = (unsigned)(min(max(J, 0), C.ny - 1) * C.nx);
-      const T vi = C.u[row + Ic], vn = C.u[row + Inc];
-      cI[p] = (rin && ci) ? vi : T(0);
-      cN[p] = (rin && cn) ? vn : T(0);
-    }
-  }
-  if (skip) return;
-
-  const T om = a.omega, idx2 = L.idx2, idy2 = L.idy2, lam = L.lam, invd = L.inv_d;
-  // One code path for every tile.  Cells outside the level (tile rows above
-  // or below it, lanes left or right of it) are set to zero after the
-  // prolongation and after each sweep, which is what the reference's
-  // multiplication by the mask does there (mask 0 outside, 1 inside an
-  // all-wet level); inside, the mask is 1 and drops out of every formula.
-  // (A separate all-inside path doubled the kernel and its registers.)
+  const int cr0 = (j0 >> 1) + RH - 1;
+  // Cells outside the level are zero (the reference multiplies by the mask,
+  // 0 outside and 1 inside an all-wet level).  Lanes outside are handled by
+  // lane selects and zero update coefficients; rows outside exist only in the
+  // tiles at the top and bottom of the level (wave-uniform), which zero them
+  // in a branch of their own, so that no other tile pays per-row conditions.
+  constexpr int HZ = kUpd ? 1 : 0;  // rows of z beyond the tile (kDownUpd)
+  const bool edge = j0 - H - HZ < 0 || j0 - H + R + HZ > L.ny ||
+                    (kUpMode && (cr0 < 0 || cr0 + NCR > a.C.ny));
   auto row_in = [&](int r) -> bool {
     const int gj = j0 - H + r;
     return gj >= 0 && gj < L.ny;
   };
-  auto keep = [&](int r, T v) -> T { return (colin && row_in(r)) ? v : T(0); };
+
+  // Loads from clamped (always valid) cells, 32-bit offsets.
+  const unsigned gic = (unsigned)min(max(gi, 0), L.nx - 1);
+  const int nL = L.ny * L.nx;
+  T u[R], b[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int gjc = min(max(j0 - H + r, 0), L.ny - 1);
+    const T bv = buf_load(buf_rsrc(a.b, nL), gic, gjc * L.nx);
+    b[r] = colin ? bv : T(0);
+    if constexpr (ZERO) {
+      u[r] = T(0);
+    } else {
+      const T uv = buf_load(buf_rsrc(a.u_in, nL), gic, gjc * L.nx);
+      u[r] = colin ? uv : T(0);  // u * mask (multigrid.py:383)
+    }
+  }
+  const bool first = (MODE == kUpCG) ? a.sc->first : true;
+  // (Each lane loads both coarse columns: taking the neighbour column from
+  // the lane beside it, possible for odd H, held more registers than the
+  // loads it saved.)
+  T cI[kUpMode ? NCR : 1], cN[kUpMode ? NCR : 1];
+  if constexpr (kUpMode) {
+    const LevelDev<T>& C = a.C;
+    const bool ci = active && I >= 0 && I < C.nx, cn = active && In >= 0 && In < C.nx;
+    const unsigned Ic = (unsigned)min(max(I, 0), C.nx - 1);
+    const unsigned Inc = (unsigned)min(max(In, 0), C.nx - 1);
+#pragma unroll
+    for (int p = 0; p < NCR; ++p) {
+      const int crow = min(max(cr0 + p, 0), C.ny - 1) * C.nx;
+      const T vi = buf_load(buf_rsrc(C.u, C.ny * C.nx), Ic, crow);
+      const T vn = buf_load(buf_rsrc(C.u, C.ny * C.nx), Inc, crow);
+      cI[p] = ci ? vi : T(0);
+      cN[p] = cn ? vn : T(0);
+    }
+  }
+  // kDownUpd: z on rows -1 .. R of the tile, q_in on its rows.
+  T zr[kUpd ? R + 2 : 1], qv[kUpd ? R : 1];
+  if constexpr (kUpd) {
+#pragma unroll
+    for (int t = 0; t < R + 2; ++t) {
+      const int gjc = min(max(j0 - H - 1 + t, 0), L.ny - 1);
+      const T v = buf_load(buf_rsrc(a.upd.z, nL), gic, gjc * L.nx);
+      zr[t] = colin ? v : T(0);
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int gjc = min(max(j0 - H + r, 0), L.ny - 1);
+      const T v = buf_load(buf_rsrc(a.upd.q_in, nL), gic, gjc * L.nx);
+      qv[r] = colin ? v : T(0);
+    }
+  }
+  if (kUpd ? flag == 1 : flag != 0) return;
+  // Row factor of the prolongation weights (below): 1/4 where both coarse
+  // rows exist, 0 on rows outside the level.
+  T rowf[kUpMode ? R : 1];
+#pragma unroll
+  for (int r = 0; r < (kUpMode ? R : 1); ++r) rowf[r] = T(0.25);
+  if (edge) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (!row_in(r)) {
+        u[r] = T(0);
+        b[r] = T(0);
+        if constexpr (kUpd) qv[r] = T(0);
+      }
+    }
+    if constexpr (kUpd) {
+#pragma unroll
+      for (int t = 0; t < R + 2; ++t) {
+        const int gj = j0 - H - 1 + t;
+        if (gj < 0 || gj >= L.ny) zr[t] = T(0);
+      }
+    }
+    if constexpr (kUpMode) {
+      const LevelDev<T>& C = a.C;
+#pragma unroll
+      for (int p = 0; p < NCR; ++p) {
+        const int J = cr0 + p;
+        if (J < 0 || J >= C.ny) {
+          cI[p] = T(0);
+          cN[p] = T(0);
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int gj = j0 - H + r;
+        const int J = gj >> 1, Jn = J + ((gj & 1) ? 1 : -1);
+        const int rf = 3 * (J >= 0 && J < C.ny) + (Jn >= 0 && Jn < C.ny);
+        rowf[r] = !row_in(r) ? T(0)
+                  : rf == 4  ? T(0.25)
+                  : rf == 3  ? T(1) / T(3)
+                  : rf == 1  ? T(1)
+                             : T(0);
+      }
+    }
+  }
+  // Rows outside the level back to zero after a sweep (edge tiles only).
+  auto zero_outer_rows = [&]() {
+    if (edge) {
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        if (!row_in(r)) u[r] = T(0);
+    }
+  };
+
+  const T om = a.omega, idx2 = L.idx2, idy2 = L.idy2, lam = L.lam, invd = L.inv_d;
+  if constexpr (kUpd) {
+    // The previous PCG step (pcg_cg_update_kernel's arithmetic) on every
+    // cell of the tile, so that the sweeps below start from r_{k+1}:
+    // a = A z, q_k = a + beta q_{k-1}, r_{k+1} = r_k - alpha q_k; beta == 0
+    // (the first step or a restart) does not read p_{k-1}, q_{k-1}.
+    const double beta64 = a.sc->beta;
+    const T alpha = (T)a.sc->alpha, beta = (T)beta64;
+    const bool restart = beta64 == 0.0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const T c = zr[r + 1];
+      const T e = lane_east(c), w = lane_west(c);
+      const T av = (e + w - T(2) * c) * idx2 + (zr[r + 2] + zr[r] - T(2) * c) * idy2 - lam * c;
+      const T qn = restart ? av : av + beta * qv[r];
+      qv[r] = qn;
+      b[r] = colin ? b[r] - alpha * qn : T(0);  // zero outside the level
+    }
+    if (edge) {
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        if (!row_in(r)) b[r] = T(0);
+    }
+    // Store q_k and r_{k+1}; p_k = z + beta p_{k-1}, x += alpha p_k, on the
+    // tile interior (in place: no other tile reads p or x).
+    if (lin) {
+#pragma unroll
+      for (int t = 0; t < WTR; ++t) {
+        const int gj = j0 + t;
+        if (gj < L.ny) {
+          const int r = H + t;
+          buf_store(buf_rsrc(a.upd.q_out, nL), (unsigned)gi, gj * L.nx, qv[r]);
+          buf_store(buf_rsrc(a.upd.r_out, nL), (unsigned)gi, gj * L.nx, b[r]);
+          const int64_t k = (int64_t)gj * L.nx + gi;
+          const T zc = zr[r + 1];
+          const T pn = restart ? zc : zc + beta * a.upd.p[k];
+          a.upd.p[k] = pn;
+          T* xp = a.upd.x + (int64_t)gj * a.upd.ld_x + gi;
+          *xp = *xp + alpha * pn;
+        }
+      }
+    }
+    if (flag != 0) return;  // flag 2: that was the last step
+  }
   // (A u)(r) for the tile's current u, inside the level.
   auto Au = [&](int r) -> T {
     const T c = u[r];
@@ -739,25 +893,23 @@ __global__ __launch_bounds__(256) void mg_wave_tile_kernel(TileArgs<float> a) {
   if constexpr (kUpMode) {
     const LevelDev<T>& C = a.C;
     const T cf = T(3) * T(I >= 0 && I < C.nx) + T(In >= 0 && In < C.nx);
-    const T lanef = __builtin_amdgcn_rcpf(cf > T(1) ? cf : T(1));
+    const T lanef = colin ? __builtin_amdgcn_rcpf(cf > T(1) ? cf : T(1)) : T(0);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int p = ((r - H) >> 1) - RH + 1;
       const int pn = p + (((r + H) & 1) ? 1 : -1);
       const T w = T(9) * cI[p] + T(3) * cN[p] + T(3) * cI[pn] + cN[pn];
-      const int gj = j0 - H + r;
-      const int J = gj >> 1, Jn = J + ((gj & 1) ? 1 : -1);
-      const int rf = 3 * (J >= 0 && J < C.ny) + (Jn >= 0 && Jn < C.ny);  // wave-uniform
-      const T rowf = rf == 4 ? T(0.25) : (rf == 3 ? T(1) / T(3) : (rf == 1 ? T(1) : T(0)));
-      u[r] = keep(r, u[r] + w * (rowf * lanef));
+      u[r] = u[r] + w * (rowf[r] * lanef);
     }
   }
   // Damped-Jacobi sweeps (multigrid.py:408-461); the valid region shrinks
   // by one row and one lane per sweep.  prev keeps row r - 1's old value.
   // u + om (b - A u) / diag with A u = idx2 (e + w) + idy2 (n + s) - (2 idx2
-  // + 2 idy2 + lam) u and 1 / diag = invd:
-  const T kb = om * invd, kx = -kb * idx2, ky = -kb * idy2;
-  const T k0 = T(1) + kb * (T(2) * idx2 + T(2) * idy2 + lam);
+  // + 2 idy2 + lam) u and 1 / diag = invd, one coefficient per term; the
+  // coefficients are zero on lanes outside the level.
+  const T lk = colin ? T(1) : T(0);
+  const T kb = om * invd * lk, kx = -om * invd * idx2 * lk, ky = -om * invd * idy2 * lk;
+  const T k0 = (T(1) + om * invd * (T(2) * idx2 + T(2) * idy2 + lam)) * lk;
 #pragma unroll
   for (int sw = 1; sw <= NS; ++sw) {
     T prev = u[sw - 1];
@@ -766,22 +918,22 @@ __global__ __launch_bounds__(256) void mg_wave_tile_kernel(TileArgs<float> a) {
       const T c = u[r];
       T un;
       if (ZERO && sw == 1 && !kUpMode) {
-        un = om * invd * b[r];  // b is zero outside the level
+        un = kb * b[r];
       } else {
-        // c + om / diag * (b - A u), expanded into one coefficient per term
         const T e = lane_east(c), w = lane_west(c);
-        un = keep(r, k0 * c + kb * b[r] + kx * (e + w) + ky * (u[r + 1] + prev));
+        un = k0 * c + kb * b[r] + kx * (e + w) + ky * (u[r + 1] + prev);
       }
       prev = c;
       u[r] = un;
     }
+    zero_outer_rows();
   }
 
   // PCG dots (UP_CG), reduced at the end.
   T d[kNumCgDots];
 #pragma unroll
   for (int k = 0; k < kNumCgDots; ++k) d[k] = T(0);
-  if constexpr (MODE == kDown) {
+  if constexpr (kDownMode) {
     // r = (b - A u) * m on the interior rows, 2x2 mask-weighted restriction
     // (multigrid.py:464-507): rows summed in registers, the column pair
     // (gi even, gi + 1) through one lane shift.
@@ -790,7 +942,8 @@ __global__ __launch_bounds__(256) void mg_wave_tile_kernel(TileArgs<float> a) {
 #pragma unroll
     for (int k2 = 0; k2 < WTR / 2; ++k2) {
       const int r0 = H + 2 * k2;
-      const T m0 = keep(r0, T(1)), m1 = keep(r0 + 1, T(1));
+      const T m0 = (colin && row_in(r0)) ? T(1) : T(0);
+      const T m1 = (colin && row_in(r0 + 1)) ? T(1) : T(0);
       const T res0 = (b[r0] - Au(r0)) * m0, res1 = (b[r0 + 1] - Au(r0 + 1)) * m1;
       const T s2 = res0 + res1;
       const T c2 = m0 + m1;
@@ -799,7 +952,8 @@ __global__ __launch_bounds__(256) void mg_wave_tile_kernel(TileArgs<float> a) {
       const int J = (j0 >> 1) + k2;
       if (even && J < C.ny && I < C.nx) {
         // the coarse level of an all-wet level is all wet (mask 1)
-        C.b[J * C.nx + I] = s4 / (cnt > T(1) ? cnt : T(1));
+        buf_store(buf_rsrc(C.b, C.ny * C.nx), (unsigned)I, J * C.nx,
+                  s4 / (cnt > T(1) ? cnt : T(1)));
       }
     }
   }
@@ -812,37 +966,48 @@ __global__ __launch_bounds__(256) void mg_wave_tile_kernel(TileArgs<float> a) {
     T rq[WTR];
 #pragma unroll
     for (int t = 0; t < WTR; ++t) {
-      const int gj = j0 + t;
-      const T v = a.w[(unsigned)(min(gj, L.ny - 1) * L.nx + gic)];
-      rq[t] = (!first && lin && gj < L.ny) ? v : T(0);
+      const T v = buf_load(buf_rsrc(a.w, nL), gic, min(j0 + t, L.ny - 1) * L.nx);
+      rq[t] = (!first && lin) ? v : T(0);
     }
+    if (edge) {
+#pragma unroll
+      for (int t = 0; t < WTR; ++t)
+        if (!row_in(H + t)) rq[t] = T(0);
+    }
+    // Every product below has a factor that is zero outside the tile
+    // interior (rd, qd, or az): z itself is only stored.
 #pragma unroll
     for (int t = 0; t < WTR; ++t) {
       const int r = H + t;
-      const int gj = j0 + t;
-      const T az = Au(r);
-      if (lin && gj < L.ny) {
-        const int g = gj * L.nx + gi;
-        const T z = u[r];
-        a.u_out[g] = z;
-        if (a.az_out) a.az_out[g] = az;  // else the update kernel recomputes it
-        const T rd = b[r], qd = rq[t];
-        d[kZR] += z * rd;
-        d[kZQ] += z * qd;
-        d[kZA] += z * az;
-        d[kRR] += rd * rd;
-        d[kRA] += rd * az;
-        d[kRQ] += rd * qd;
-        d[kAA] += az * az;
-        d[kAQ] += az * qd;
-        d[kQQ] += qd * qd;
+      const T azv = Au(r);
+      const T az = (lin && (!edge || row_in(r))) ? azv : T(0);
+      const T z = u[r], rd = lin ? b[r] : T(0), qd = rq[t];
+      d[kZR] += z * rd;
+      d[kZQ] += z * qd;
+      d[kZA] += z * az;
+      d[kRR] += rd * rd;
+      d[kRA] += rd * az;
+      d[kRQ] += rd * qd;
+      d[kAA] += az * az;
+      d[kAQ] += az * qd;
+      d[kQQ] += qd * qd;
+    }
+    // z only: the PCG update kernel forms A z again from it (a.az_out is
+    // not used on this path).
+    if (lin) {
+#pragma unroll
+      for (int t = 0; t < WTR; ++t) {
+        const int gj = j0 + t;
+        if (gj < L.ny) buf_store(buf_rsrc(a.u_out, nL), (unsigned)gi, gj * L.nx, u[H + t]);
       }
     }
   } else {
+    if (lin) {
 #pragma unroll
-    for (int t = 0; t < WTR; ++t) {
-      const int gj = j0 + t;
-      if (lin && gj < L.ny) a.u_out[gj * L.nx + gi] = u[H + t];
+      for (int t = 0; t < WTR; ++t) {
+        const int gj = j0 + t;
+        if (gj < L.ny) buf_store(buf_rsrc(a.u_out, nL), (unsigned)gi, gj * L.nx, u[H + t]);
+      }
     }
   }
   if constexpr (MODE == kUpCG) block_partials<kNumCgDots, 256>(d, a.partials);
@@ -1429,13 +1594,11 @@ __global__ __launch_bounds__(256) void pcg_cg_update_kernel(int ny, int nx, T* _
 // loads hit lines a neighbouring wave just read), which saves writing and
 // reading one array per iteration.
 template <typename T>
-__global__ __launch_bounds__(256) void pcg_cg_update_az_kernel(LevelDev<T> L, T* __restrict__ x,
-                                                               int64_t ld_x,
-                                                               const T* __restrict__ z,
-                                                               T* __restrict__ p,
-                                                               T* __restrict__ q,
-                                                               T* __restrict__ r,
-                                                               PcgScalars* sc) {
+__global__ __launch_bounds__(256) void pcg_cg_update_az_kernel(
+    LevelDev<T> L, T* __restrict__ x, int64_t ld_x, const T* __restrict__ z, T* __restrict__ p,
+    const T* q, T* q_out, const T* r, T* r_out, PcgScalars* sc) {
+  // q_out, r_out: q and r in place, or other arrays (each cell reads only
+  // its own q and r).
   if (sc->done == 1) return;  // see pcg_cg_update_kernel
   const int lane = threadIdx.x;
   const int i = (blockIdx.x * 4 + threadIdx.y) * 64 + lane;
@@ -1478,9 +1641,9 @@ __global__ __launch_bounds__(256) void pcg_cg_update_az_kernel(LevelDev<T> L, T*
       const T pn = restart ? c : c + beta * pv[t];
       const T qn = restart ? av : av + beta * qv[t];
       p[k] = pn;
-      q[k] = qn;
+      q_out[k] = qn;
       x[(int64_t)j * ld_x + i] = xv[t] + alpha * pn;
-      r[k] = rv[t] - alpha * qn;
+      r_out[k] = rv[t] - alpha * qn;
     }
   }
 }
@@ -1651,6 +1814,9 @@ struct Hier {
   // PCG vectors on level 0
   // r residual, z = M r, az = A z, p direction, q = A p, x iterate
   T *r = nullptr, *z = nullptr, *p = nullptr, *az = nullptr, *q = nullptr, *x = nullptr;
+  // second r and q (FP32): the PCG step fused into the next V-cycle's down
+  // leg reads r_k, q_{k-1} on its tiles' halos and writes r_{k+1}, q_k here
+  T *r2 = nullptr, *q2 = nullptr;
 };
 
 struct Plan {
@@ -1811,8 +1977,9 @@ static int upload_hier(Plan& P, Hier<T>& H, const std::vector<HostLevel>& hl) {
     L.t3 = (T*)w[4];
   }
   const size_t N0 = (size_t)hl[0].ny * hl[0].nx;
-  void* v[6];
-  for (int k = 0; k < 6; ++k) {
+  void* v[8] = {};
+  const int nvec = std::is_same<T, float>::value ? 8 : 6;
+  for (int k = 0; k < nvec; ++k) {
     int rc = alloc(N0 * sizeof(T), &v[k]);
     if (rc) return rc;
     FVX_HIP(hipMemset(v[k], 0, N0 * sizeof(T)));
@@ -1823,6 +1990,8 @@ static int upload_hier(Plan& P, Hier<T>& H, const std::vector<HostLevel>& hl) {
   H.az = (T*)v[3];
   H.q = (T*)v[4];
   H.x = (T*)v[5];
+  H.r2 = (T*)v[6];
+  H.q2 = (T*)v[7];
   return kOk;
 }
 
@@ -1851,15 +2020,17 @@ Hier<float>& hier<float>(Plan& P) { return P.h32; }
 // update kernel of such a level recomputes it (pcg_cg_update_az_kernel).
 template <typename T, int KIND>
 static bool wave_tiled(const LevelDev<T>& L, const LevelDev<T>* C) {
+  // (its buffer accesses address each array in 32-bit byte offsets)
   if constexpr (std::is_same<T, float>::value && KIND == kDirichlet)
-    return !L.mask && (!C || !C->mask);
+    return !L.mask && (!C || !C->mask) && (int64_t)L.ny * L.nx < (int64_t(1) << 29);
   return false;
 }
 
 template <typename T, int KIND, int NS, int MODE, bool ZERO>
 static void launch_tile(const Plan& P, const LevelDev<T>& L, const LevelDev<T>* C, const T* u_in,
                         const T* b, T* u_out, const T* w, PcgScalars* sc, hipStream_t s,
-                        T* az_out = nullptr, bool finish = true) {
+                        T* az_out = nullptr, bool finish = true,
+                        const PcgUpdArgs<T>* upd = nullptr) {
   TileArgs<T> a;
   std::memset(&a, 0, sizeof(a));
   a.L = L;
@@ -1876,8 +2047,9 @@ static void launch_tile(const Plan& P, const LevelDev<T>& L, const LevelDev<T>* 
     if (wave_tiled<T, KIND>(L, C)) {
       // all-wet FP32 level: one wavefront per tile (mg_wave_tile_kernel)
       a.az_out = nullptr;  // recomputed by pcg_cg_update_az_kernel
-      constexpr int H = wt_halo<NS, MODE>();
-      a.tiles_x = cdiv(L.nx, 64 - 2 * H);
+      if (upd) a.upd = *upd;
+      constexpr int HL = wt_lane_halo<NS, MODE>();
+      a.tiles_x = cdiv(L.nx, 64 - 2 * HL);
       a.ntiles = a.tiles_x * cdiv(L.ny, wt_rows<MODE>());
       const int nblk = cdiv(a.ntiles, 4);
       hipLaunchKernelGGL((mg_wave_tile_kernel<NS, MODE, ZERO>), dim3(nblk), dim3(256), 0, s, a);
@@ -1888,18 +2060,20 @@ static void launch_tile(const Plan& P, const LevelDev<T>& L, const LevelDev<T>* 
       return;
     }
   }
-  a.tiles_x = cdiv(L.nx, kTx);
-  a.ntiles = a.tiles_x * cdiv(L.ny, kTy);
-  if (L.mask)
-    hipLaunchKernelGGL((mg_tile_kernel<T, KIND, NS, MODE, ZERO, true>), dim3(a.ntiles),
-                       dim3(kTileThreads), 0, s, a);
-  else
-    hipLaunchKernelGGL((mg_tile_kernel<T, KIND, NS, MODE, ZERO, false>), dim3(a.ntiles),
-                       dim3(kTileThreads), 0, s, a);
-  if constexpr (MODE == kUpCG)
-    if (finish)
-      hipLaunchKernelGGL((reduce_finish_kernel<kNumCgDots, kRedCG>), dim3(1), dim3(kFinishThreads),
-                         0, s, P.partials, a.ntiles, sc, 0.0, 0.0);
+  if constexpr (MODE != kDownUpd) {
+    a.tiles_x = cdiv(L.nx, kTx);
+    a.ntiles = a.tiles_x * cdiv(L.ny, kTy);
+    if (L.mask)
+      hipLaunchKernelGGL((mg_tile_kernel<T, KIND, NS, MODE, ZERO, true>), dim3(a.ntiles),
+                         dim3(kTileThreads), 0, s, a);
+    else
+      hipLaunchKernelGGL((mg_tile_kernel<T, KIND, NS, MODE, ZERO, false>), dim3(a.ntiles),
+                         dim3(kTileThreads), 0, s, a);
+    if constexpr (MODE == kUpCG)
+      if (finish)
+        hipLaunchKernelGGL((reduce_finish_kernel<kNumCgDots, kRedCG>), dim3(1), dim3(kFinishThreads),
+                           0, s, P.partials, a.ntiles, sc, 0.0, 0.0);
+  }
 }
 
 // Damped-Jacobi sweeps fused into one level kernel (with the residual and
@@ -1962,7 +2136,8 @@ static T* smooth_chunks(const Plan& P, const LevelDev<T>& L, const T* u_in, bool
 // it did.
 template <typename T, int KIND>
 static int v_cycle_impl(Plan& P, const T* u0, const T* rhs, T* out, T* az, const T* w,
-                        PcgScalars* sc, hipStream_t s, bool* fused) {
+                        PcgScalars* sc, hipStream_t s, bool* fused,
+                        const PcgUpdArgs<T>* upd = nullptr) {
   Hier<T>& H = hier<T>(P);
   const int nl = P.nl, lc = P.lc;
   if (fused) *fused = false;
@@ -2008,6 +2183,19 @@ static int v_cycle_impl(Plan& P, const T* u0, const T* rhs, T* out, T* az, const
     const T* src = (l == 0) ? u0 : nullptr;
     const int npre = P.n_pre;
     const int fz = npre < kMaxFuse ? npre : kMaxFuse;  // fused with residual + restriction
+    if (l == 0 && upd) {
+      // The previous PCG step, then this down leg from r_{k+1} = rhs
+      // (pcg_iters checks that the plan allows it).
+      if (fz == 3) launch_tile<T, KIND, 3, kDownUpd, true>(P, L, &C, nullptr, upd->r_in, L.u,
+                                                           nullptr, sc, s, nullptr, true, upd);
+      else if (fz == 2) launch_tile<T, KIND, 2, kDownUpd, true>(P, L, &C, nullptr, upd->r_in, L.u,
+                                                                nullptr, sc, s, nullptr, true, upd);
+      else if (fz == 1) launch_tile<T, KIND, 1, kDownUpd, true>(P, L, &C, nullptr, upd->r_in, L.u,
+                                                                nullptr, sc, s, nullptr, true, upd);
+      else launch_tile<T, KIND, 0, kDownUpd, true>(P, L, &C, nullptr, upd->r_in, L.u, nullptr, sc,
+                                                   s, nullptr, true, upd);
+      continue;
+    }
     if (npre - fz > 0) {
       src = smooth_chunks<T, KIND>(P, L, src, zero, b, npre - fz, L.t, L.t2, nullptr, sc, s);
       zero = false;
@@ -2093,6 +2281,41 @@ template <typename T, int KIND>
 static int pcg_iters(Plan& P, Hier<T>& H, T* x, int64_t ld_x, int n, PcgScalars* sc,
                      hipStream_t s, PrecondFn pf = nullptr, void* ctx = nullptr) {
   const LevelDev<T>& L = H.lv[0];
+  // All-wet FP32 level 0 under the V-cycle with fused sweeps: each PCG step
+  // after the first is applied by the next V-cycle's level-0 down kernel
+  // (kDownUpd), which reads r_k, q_{k-1} from one pair of arrays and writes
+  // r_{k+1}, q_k to the other; the last step runs on its own and leaves r, q
+  // in H.r, H.q.
+  bool fuse_step = false;
+  if constexpr (std::is_same<T, float>::value && KIND == kDirichlet)
+    fuse_step = !pf && P.lc >= 1 && P.nl > 1 && wave_tiled<T, KIND>(L, &H.lv[1]) &&
+                P.n_pre <= kMaxFuse && P.n_post <= kMaxFuse && H.r2 && H.q2;
+  if (fuse_step) {
+    if constexpr (std::is_same<T, float>::value && KIND == kDirichlet) {
+      T *rc = H.r, *qc = H.q, *rn = H.r2, *qn = H.q2;
+      for (int k = 0; k < n; ++k) {
+        bool fused = false;
+        int rc_;
+        if (k == 0) {
+          rc_ = v_cycle_impl<T, KIND>(P, nullptr, rc, H.z, H.az, qc, sc, s, &fused);
+        } else {
+          PcgUpdArgs<T> u{H.z, H.p, x, ld_x, qc, qn, rc, rn};
+          rc_ = v_cycle_impl<T, KIND>(P, nullptr, rn, H.z, H.az, qn, sc, s, &fused, &u);
+          std::swap(rc, rn);
+          std::swap(qc, qn);
+        }
+        if (rc_) return rc_;
+        if (!fused) {
+          set_error("internal: the fused PCG step needs the V-cycle's up leg with the dots");
+          return kInvalidArgument;
+        }
+      }
+      hipLaunchKernelGGL(pcg_cg_update_az_kernel<T>, grid_stencil(L.ny, L.nx), dim3(64, 4), 0, s,
+                         L, x, ld_x, H.z, H.p, qc, H.q, rc, H.r, sc);
+    }
+    FVX_LAUNCH_CHECK();
+    return kOk;
+  }
   for (int k = 0; k < n; ++k) {
     bool fused = false;
     int rc;
@@ -2117,7 +2340,7 @@ static int pcg_iters(Plan& P, Hier<T>& H, T* x, int64_t ld_x, int n, PcgScalars*
     if (recompute) {
       if constexpr (std::is_same<T, float>::value)
         hipLaunchKernelGGL(pcg_cg_update_az_kernel<T>, grid_stencil(L.ny, L.nx), dim3(64, 4), 0, s, L,
-                           x, ld_x, H.z, H.p, H.q, H.r, sc);
+                           x, ld_x, H.z, H.p, H.q, H.q, H.r, H.r, sc);
     } else {
       hipLaunchKernelGGL(pcg_cg_update_kernel<T>, grid_dir(L.ny, L.nx), dim3(64, 4), 0, s, L.ny,
                          L.nx, x, ld_x, H.z, H.az, H.p, H.q, H.r, sc);
@@ -2498,9 +2721,10 @@ static int kernel_bench_impl(Plan& P, int which, int reps, hipStream_t s) {
   const LevelDev<T>& L = H.lv[0];
   PcgScalars* sc = &P.sc[2 * (kSlots - 1)];
   if (reps == 0) {  // prepare: zero the vectors, arm the scalars
-    for (T* v : {H.r, H.z, H.p, H.az, H.q, H.x})
-      hipLaunchKernelGGL(zero_pitched_kernel<T>, grid2(L.ny, L.nx), dim3(64, 4), 0, s, L.ny, L.nx,
-                         v, (int64_t)L.nx);
+    for (T* v : {H.r, H.z, H.p, H.az, H.q, H.x, H.r2, H.q2})
+      if (v)
+        hipLaunchKernelGGL(zero_pitched_kernel<T>, grid2(L.ny, L.nx), dim3(64, 4), 0, s, L.ny,
+                           L.nx, v, (int64_t)L.nx);
     if (P.nl >= 2) {
       const LevelDev<T>& C = H.lv[1];
       hipLaunchKernelGGL(zero_pitched_kernel<T>, grid2(C.ny, C.nx), dim3(64, 4), 0, s, C.ny, C.nx,
@@ -2510,8 +2734,15 @@ static int kernel_bench_impl(Plan& P, int which, int reps, hipStream_t s) {
     FVX_LAUNCH_CHECK();
     return kOk;
   }
-  if (which == 2 && P.nl < 2) {
-    set_error("the prolongation kernel needs two levels");
+  if (which >= 2 && P.nl < 2) {
+    set_error("the level-0 V-cycle kernels need two levels");
+    return kInvalidArgument;
+  }
+  bool step_fusable = false;
+  if constexpr (std::is_same<T, float>::value && KIND == kDirichlet)
+    step_fusable = wave_tiled<T, KIND>(L, &H.lv[1]) && P.n_pre <= kMaxFuse && H.r2 && H.q2;
+  if (which == 3 && !step_fusable) {
+    set_error("the down kernel with the PCG step needs an all-wet FP32 Dirichlet level 0");
     return kInvalidArgument;
   }
   for (int k = 0; k < reps; ++k) {
@@ -2526,16 +2757,30 @@ static int kernel_bench_impl(Plan& P, int which, int reps, hipStream_t s) {
       if (recompute) {
         if constexpr (std::is_same<T, float>::value)
           hipLaunchKernelGGL(pcg_cg_update_az_kernel<T>, grid_stencil(L.ny, L.nx), dim3(64, 4), 0, s,
-                             L, H.x, (int64_t)L.nx, H.z, H.p, H.q, H.r, sc);
+                             L, H.x, (int64_t)L.nx, H.z, H.p, H.q, H.q, H.r, H.r, sc);
       } else {
         hipLaunchKernelGGL(pcg_cg_update_kernel<T>, grid_dir(L.ny, L.nx), dim3(64, 4), 0, s,
                            L.ny, L.nx, H.x, (int64_t)L.nx, H.z, H.az, H.p, H.q, H.r, sc);
       }
-    } else {
+    } else if (which == 2) {
       // The last kernel of a preconditioning V-cycle on level 0: prolongation
       // of level 1's iterate, correction, two sweeps, a = A z, PCG dots.
       LevelDev<T> C = H.lv[1];
       launch_vcycle_up_cg<T, KIND>(P, L, C, H.p, H.r, H.z, H.q, H.az, sc, s);
+    } else {
+      // The first kernel of a preconditioning V-cycle on level 0 with the
+      // previous PCG step applied first (r, q read; r2, q2 written).
+      PcgUpdArgs<T> u{H.z, H.p, H.x, (int64_t)L.nx, H.q, H.q2, H.r, H.r2};
+      const LevelDev<T>& C = H.lv[1];
+      const int fz = P.n_pre < kMaxFuse ? P.n_pre : kMaxFuse;
+      if (fz == 3) launch_tile<T, KIND, 3, kDownUpd, true>(P, L, &C, nullptr, H.r, L.u, nullptr,
+                                                           sc, s, nullptr, true, &u);
+      else if (fz == 2) launch_tile<T, KIND, 2, kDownUpd, true>(P, L, &C, nullptr, H.r, L.u,
+                                                                nullptr, sc, s, nullptr, true, &u);
+      else if (fz == 1) launch_tile<T, KIND, 1, kDownUpd, true>(P, L, &C, nullptr, H.r, L.u,
+                                                                nullptr, sc, s, nullptr, true, &u);
+      else launch_tile<T, KIND, 0, kDownUpd, true>(P, L, &C, nullptr, H.r, L.u, nullptr, sc, s,
+                                                   nullptr, true, &u);
     }
   }
   FVX_LAUNCH_CHECK();
@@ -2873,8 +3118,9 @@ int fvx_mg_kernel_bench(void* plan, int which, int bits, int reps, void* stream)
   using namespace fvx;
   FVX_CHECK_ARG(plan, "null plan");
   Plan* P = (Plan*)plan;
-  FVX_CHECK_ARG(which >= 0 && which <= 2,
-                "which must be 0 (pcg_az_dots), 1 (pcg_cg_update) or 2 (V-cycle up + dots)");
+  FVX_CHECK_ARG(which >= 0 && which <= 3,
+                "which must be 0 (pcg_az_dots), 1 (pcg_cg_update), 2 (V-cycle up + dots) or "
+                "3 (V-cycle down + PCG step)");
   FVX_CHECK_ARG((bits == 32 && P->has32) || (bits == 64 && P->has64),
                 "plan has no hierarchy of that precision");
   FVX_CHECK_ARG(reps >= 0, "negative reps");

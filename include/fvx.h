@@ -207,9 +207,10 @@ int fvx_mg_precond_solve(void* plan, int precond, void* net, double out_scale, i
  * vectors, never converging.  which 0 = matvec a = A z with the PCG dots
  * (preconditioners other than the V-cycle), 1 = update (p = z + beta p,
  * q = a + beta q, x += alpha p, r -= alpha q), 2 = the level-0 up kernel of
- * a preconditioning V-cycle (prolongation of level 1, correction, two
- * sweeps, a = A z, PCG dots); bits 32/64 picks the hierarchy; reps = 0
- * prepares and must come first.                                            */
+ * a preconditioning V-cycle (prolongation of level 1, correction, the
+ * post-sweeps, a = A z, PCG dots), 3 = the level-0 down kernel with the
+ * previous PCG step applied first (all-wet FP32 Dirichlet plans); bits
+ * 32/64 picks the hierarchy; reps = 0 prepares and must come first.        */
 int fvx_mg_kernel_bench(void* plan, int which, int bits, int reps, void* stream);
 
 /* ---- Spectral rectangular Helmholtz solvers: solve_helmholtz_{dst,dct,fft}
