@@ -629,11 +629,11 @@ __global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
 // Wave-tile smoother kernel for FP32 levels without a land mask and with the
 // Dirichlet operator (constant coefficients): the same modes and arithmetic
 // as mg_tile_kernel, laid out for the wavefront.  One wavefront owns a tile
-// of 64 columns x (wt_rows + 2H) rows including its halo, each lane one
+// of 64 columns x (ROWS + 2H) rows including its halo, each lane one
 // column of u and b in registers.  East/west neighbours come from DPP wave
 // shifts, north/south from the neighbouring registers, so the sweeps use no
 // LDS, no barriers and no index arithmetic.  The tile interior is
-// (64 - 2H) x wt_rows; four independent tiles per 256-thread block.
+// (64 - 2H) x ROWS; four independent tiles per 256-thread block.
 // ---------------------------------------------------------------------------
 
 // Value of lane + 1 (wave_shl 1) and lane - 1 (wave_shr 1); 0 where there is
@@ -670,22 +670,21 @@ template <int NS, int MODE>
 constexpr int wt_lane_halo() {
   return wt_halo<NS, MODE>() + (MODE == kDownUpd ? 1 : 0);
 }
-// Interior rows per tile (even, for the 2x2 restriction).  Short tiles keep
-// the registers per lane low enough for 6-8 waves per SIMD, which hides the
-// load latency better than the halo they save costs.  Measured at 2048^2
-// with three fused sweeps: 8 rows for the down and up legs (against 6, 10,
-// 12, 16) and for the up leg with the PCG dots (against 6 and 10).
-template <int MODE>
-constexpr int wt_rows() {
-  return 8;
-}
+// Interior rows per tile (ROWS; even, for the 2x2 restriction).  Short tiles
+// keep the registers per lane low enough for 6-8 waves per SIMD, which hides
+// the load latency better than the halo they save costs: measured at 2048^2
+// with three fused sweeps, 8 rows (against 6, 10, 12, 16).  On small levels,
+// whose kernels last about as long as one wave's chain of loads and
+// arithmetic, tiles of 2 rows shorten that chain.
+constexpr int kTileRows = 8, kShortTileRows = 2;
+constexpr int64_t kShortTileMaxCells = 512 * 512;
 
-template <int NS, int MODE, bool ZERO>
+template <int NS, int MODE, bool ZERO, int ROWS>
 __global__ __launch_bounds__(256) void mg_wave_tile_kernel(TileArgs<float> a) {
   using T = float;
   constexpr int H = wt_halo<NS, MODE>();
   constexpr int HL = wt_lane_halo<NS, MODE>();
-  constexpr int WTR = wt_rows<MODE>();
+  constexpr int WTR = ROWS;
   constexpr int R = WTR + 2 * H;  // rows j0 - H .. j0 + WTR + H - 1
   constexpr bool kUpMode = (MODE == kUp || MODE == kUpCG);
   constexpr bool kDownMode = (MODE == kDown || MODE == kDownUpd);
@@ -2049,10 +2048,22 @@ static void launch_tile(const Plan& P, const LevelDev<T>& L, const LevelDev<T>* 
       a.az_out = nullptr;  // recomputed by pcg_cg_update_az_kernel
       if (upd) a.upd = *upd;
       constexpr int HL = wt_lane_halo<NS, MODE>();
+      constexpr bool kMayShort = MODE == kSmooth || MODE == kDown || MODE == kUp;
+      const bool short_tiles = kMayShort && (int64_t)L.ny * L.nx <= kShortTileMaxCells;
       a.tiles_x = cdiv(L.nx, 64 - 2 * HL);
-      a.ntiles = a.tiles_x * cdiv(L.ny, wt_rows<MODE>());
+      a.ntiles = a.tiles_x * cdiv(L.ny, short_tiles ? kShortTileRows : kTileRows);
       const int nblk = cdiv(a.ntiles, 4);
-      hipLaunchKernelGGL((mg_wave_tile_kernel<NS, MODE, ZERO>), dim3(nblk), dim3(256), 0, s, a);
+      if constexpr (kMayShort) {
+        if (short_tiles)
+          hipLaunchKernelGGL((mg_wave_tile_kernel<NS, MODE, ZERO, kShortTileRows>), dim3(nblk),
+                             dim3(256), 0, s, a);
+        else
+          hipLaunchKernelGGL((mg_wave_tile_kernel<NS, MODE, ZERO, kTileRows>), dim3(nblk),
+                             dim3(256), 0, s, a);
+      } else {
+        hipLaunchKernelGGL((mg_wave_tile_kernel<NS, MODE, ZERO, kTileRows>), dim3(nblk),
+                           dim3(256), 0, s, a);
+      }
       if constexpr (MODE == kUpCG)
         if (finish)
           hipLaunchKernelGGL((reduce_finish_kernel<kNumCgDots, kRedCG>), dim3(1),
