@@ -68,6 +68,30 @@ __device__ __forceinline__ float wet(const uint8_t* m, int64_t idx) {
   return m ? (m[idx] ? 1.0f : 0.0f) : 1.0f;
 }
 
+// Value of lane + 1 (wave_shl 1) and lane - 1 (wave_shr 1); 0 where there is
+// none (lane 63, lane 0).
+__device__ __forceinline__ float lane_east(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xf, 0xf, true));
+}
+__device__ __forceinline__ float lane_west(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
+}
+
+// Buffer accesses: a 4-register descriptor per array (base, byte size), a
+// wave-uniform row offset in a scalar register and the lane's column offset,
+// so no per-access 64-bit address arithmetic; accesses past the array's end
+// read 0 / are dropped.  Arrays must be under 2^31 bytes.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const float* p, int n) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(p), (short)0, n * 4, 0x00020000);
+}
+__device__ __forceinline__ float buf_load(__amdgpu_buffer_rsrc_t r, unsigned col, int row_off) {
+  return __int_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, col * 4u, row_off * 4, 0));
+}
+__device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, unsigned col, int row_off,
+                                          float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_int(v), r, col * 4u, row_off * 4, 0);
+}
+
 // DPP lane moves; lanes without a source read 0 (bound_ctrl), so the sums
 // below need no initialised destination.
 template <int kCtrl>

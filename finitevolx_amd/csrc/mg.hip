@@ -297,6 +297,9 @@ __device__ __forceinline__ T glob_inv(const LevelDev<T>& L, int64_t k) {
 //   3 UP_CG    : as UP with one more halo ring, then a = A z on the tile
 //                interior -> u_out (z), az_out (a), and the nine PCG dots of
 //                z, a, r (= b) and q_{k-1} (= w)
+//   4 DOWN_STEP: (wavefront tiles only) the previous PCG step on the tile,
+//                then DOWN from zero on the new residual -> upd.q_out,
+//                upd.r_out, upd.p, upd.x, u_out, C.b
 // ---------------------------------------------------------------------------
 enum TileMode : int { kSmooth = 0, kDown = 1, kUp = 2, kUpCG = 3, kDownUpd = 4 };
 
@@ -636,30 +639,6 @@ __global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
 // (64 - 2H) x ROWS; four independent tiles per 256-thread block.
 // ---------------------------------------------------------------------------
 
-// Value of lane + 1 (wave_shl 1) and lane - 1 (wave_shr 1); 0 where there is
-// none (lane 63, lane 0).
-__device__ __forceinline__ float lane_east(float v) {
-  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xf, 0xf, true));
-}
-__device__ __forceinline__ float lane_west(float v) {
-  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
-}
-
-// Buffer accesses: a 4-register descriptor per array (base, byte size), a
-// wave-uniform row offset in a scalar register and the lane's column offset,
-// so no per-access 64-bit address arithmetic; accesses past the array's end
-// read 0 / are dropped.  Arrays must be under 2^31 bytes.
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const float* p, int n) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(p), (short)0, n * 4, 0x00020000);
-}
-__device__ __forceinline__ float buf_load(__amdgpu_buffer_rsrc_t r, unsigned col, int row_off) {
-  return __int_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, col * 4u, row_off * 4, 0));
-}
-__device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, unsigned col, int row_off,
-                                          float v) {
-  __builtin_amdgcn_raw_buffer_store_b32(__float_as_int(v), r, col * 4u, row_off * 4, 0);
-}
-
 template <int NS, int MODE>
 constexpr int wt_halo() {
   return NS + ((MODE == kDown || MODE == kDownUpd || MODE == kUpCG) ? 1 : 0);
@@ -678,6 +657,12 @@ constexpr int wt_lane_halo() {
 // arithmetic, tiles of 2 rows shorten that chain.
 constexpr int kTileRows = 8, kShortTileRows = 2;
 constexpr int64_t kShortTileMaxCells = 512 * 512;
+// Rows of the level-0 PCG kernels (up leg with the dots, down leg with the
+// PCG step).
+template <int MODE>
+constexpr int wt_pcg_rows() {
+  return MODE == kUpCG ? 8 : 8;
+}
 
 template <int NS, int MODE, bool ZERO, int ROWS>
 __global__ __launch_bounds__(256) void mg_wave_tile_kernel(TileArgs<float> a) {
@@ -1680,12 +1665,14 @@ __global__ __launch_bounds__(256) void refine_residual_kernel(
   double xs = col ? Xat(j0 - 1, i) : 0.0, xc = col ? Xat(j0, i) : 0.0;
   double cys = CY(j0 - 1);
   double bb = 0.0, rr = 0.0;
+#pragma unroll
   for (int rw = 0; rw < kResRows; ++rw) {
     const int j = j0 + rw;
-    if (j >= ny) break;
+    if (j >= ny) break;  // wave-uniform
     const double xn = col ? Xat(j + 1, i) : 0.0;
-    double w = __shfl_up(xc, 1, 64);
-    double e = __shfl_down(xc, 1, 64);
+    // lane - 1 and lane + 1 by DPP wave shifts (every lane active here)
+    double w = dpp_move<0x138>(xc);
+    double e = dpp_move<0x130>(xc);
     if (lane == 0) w = Xat(j, i - 1);
     if (lane == 63) e = Xat(j, i + 1);
     const int64_t k = (int64_t)j * nx + i;
@@ -1698,7 +1685,7 @@ __global__ __launch_bounds__(256) void refine_residual_kernel(
       lap = (e + w - 2.0 * c) * L.idx2 + (xn + xs - 2.0 * c) * L.idy2;
     } else {
       const double cxe = col ? L.cx[k] : 0.0;
-      double cxw = __shfl_up(cxe, 1, 64);
+      double cxw = dpp_move<0x138>(cxe);
       if (lane == 0) cxw = (i > 0 && i - 1 < nx) ? L.cx[k - 1] : 0.0;
       cyn = CY(j);
       lap = (cxe * (e - c) - cxw * (c - w)) * L.idx2 + (cyn * (xn - c) - cys * (c - xs)) * L.idy2;
@@ -2051,7 +2038,8 @@ static void launch_tile(const Plan& P, const LevelDev<T>& L, const LevelDev<T>* 
       constexpr bool kMayShort = MODE == kSmooth || MODE == kDown || MODE == kUp;
       const bool short_tiles = kMayShort && (int64_t)L.ny * L.nx <= kShortTileMaxCells;
       a.tiles_x = cdiv(L.nx, 64 - 2 * HL);
-      a.ntiles = a.tiles_x * cdiv(L.ny, short_tiles ? kShortTileRows : kTileRows);
+      constexpr int kLongRows = kMayShort ? kTileRows : wt_pcg_rows<MODE>();
+      a.ntiles = a.tiles_x * cdiv(L.ny, short_tiles ? kShortTileRows : kLongRows);
       const int nblk = cdiv(a.ntiles, 4);
       if constexpr (kMayShort) {
         if (short_tiles)
@@ -2061,7 +2049,7 @@ static void launch_tile(const Plan& P, const LevelDev<T>& L, const LevelDev<T>* 
           hipLaunchKernelGGL((mg_wave_tile_kernel<NS, MODE, ZERO, kTileRows>), dim3(nblk),
                              dim3(256), 0, s, a);
       } else {
-        hipLaunchKernelGGL((mg_wave_tile_kernel<NS, MODE, ZERO, kTileRows>), dim3(nblk),
+        hipLaunchKernelGGL((mg_wave_tile_kernel<NS, MODE, ZERO, kLongRows>), dim3(nblk),
                            dim3(256), 0, s, a);
       }
       if constexpr (MODE == kUpCG)

@@ -136,12 +136,136 @@ __global__ __launch_bounds__(kQThreads) void qg_stage_kernel(
   }
 }
 
+// The same stage for an FP32 state without a land mask, laid out for the
+// wavefront: one wavefront per tile of 64 columns (60 written, two halo lanes
+// on each side) x kQWRows rows; each lane holds its column of q (rows
+// j0-2 .. j0+R+1) and psi (rows j0-1 .. j0+R) in registers, east/west
+// neighbours come from DPP lane shifts, north/south from the neighbouring
+// registers: no LDS, no barriers.  Arithmetic as in qg_stage_kernel.
+constexpr int kQWRows = 8;
+
+template <typename TP>
+__global__ __launch_bounds__(256) void qg_stage_wave_kernel(
+    const float* __restrict__ qs, const TP* __restrict__ psi, const float* __restrict__ qn,
+    float* __restrict__ out, int Ny, int Nx, QGStageParams p, int tiles_x, int ntiles) {
+  using T = float;
+  constexpr int R = kQWRows;
+  const int lane = threadIdx.x & 63;
+  const int tile =
+      __builtin_amdgcn_readfirstlane(xcd_swizzle(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6));
+  const bool active = tile < ntiles;
+  const int tyi = active ? tile / tiles_x : 0;
+  const int i0 = (active ? tile - tyi * tiles_x : 0) * 60;
+  const int j0 = tyi * R;
+  const int gi = i0 - 2 + lane;
+  const bool colin = active && gi >= 0 && gi < Nx;
+  const unsigned gic = (unsigned)min(max(gi, 0), Nx - 1);
+  const int n = Ny * Nx;
+  // q rows j0-2+t (t = 0 .. R+3), psi rows j0-1+t (t = 0 .. R+1); zero
+  // outside the array.
+  T q[R + 4], f[R + 2];
+#pragma unroll
+  for (int t = 0; t < R + 4; ++t) {
+    const int gj = j0 - 2 + t;
+    const T v = buf_load(buf_rsrc(qs, n), gic, min(max(gj, 0), Ny - 1) * Nx);
+    q[t] = (colin && gj >= 0 && gj < Ny) ? v : T(0);
+  }
+#pragma unroll
+  for (int t = 0; t < R + 2; ++t) {
+    const int gj = j0 - 1 + t;
+    const TP v = psi[(int64_t)min(max(gj, 0), Ny - 1) * Nx + gic];
+    f[t] = (colin && gj >= 0 && gj < Ny) ? (T)v : T(0);
+  }
+  // beta (y - y0) at cell centres y_j = (j - 1/2) dy for rows j0-1 .. j0+R and
+  // F = -F0 sin(kF y) for rows j0 .. j0+R-1: one row per lane, then read back
+  // as wave-uniform values.
+  const int gjl = j0 - 1 + lane;
+  const float by_l = (float)(p.beta * ((gjl - 0.5) * p.dy - p.y0));
+  const float F_l = (lane >= 1 && lane <= R) ? (float)(-p.F0 * sin(p.kF * ((gjl - 0.5) * p.dy)))
+                                             : 0.0f;
+  auto row_val = [](float v, int l) -> T {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+  };
+  const T idx = (T)p.idx, idy = (T)p.idy;
+  const bool ci = gi >= 1 && gi <= Nx - 2;  // interior column
+  const bool cE = gi <= Nx - 3, cW = gi >= 2;
+  // First no-flux Laplacian on rows j0-1 .. j0+R (zero outside the interior);
+  // q's east/west neighbours kept for the Jacobian.
+  T l1[R + 2], qE[R + 2], qW[R + 2];
+#pragma unroll
+  for (int t = 0; t < R + 2; ++t) {
+    const int qi = t + 1, gj = j0 - 1 + t;
+    const T c = q[qi];
+    const T e = lane_east(c), w = lane_west(c);
+    qE[t] = e;
+    qW[t] = w;
+    const T fe = cE ? (e - c) * idx : T(0);
+    const T fw = cW ? (c - w) * idx : T(0);
+    const T fn = (gj <= Ny - 3) ? (q[qi + 1] - c) * idy : T(0);
+    const T fs = (gj >= 2) ? (c - q[qi - 1]) * idy : T(0);
+    const bool in = ci && gj >= 1 && gj <= Ny - 2;
+    l1[t] = in ? (fe - fw) * idx + (fn - fs) * idy : T(0);
+  }
+  T fE[R + 2], fW[R + 2];
+#pragma unroll
+  for (int t = 0; t < R + 2; ++t) {
+    fE[t] = lane_east(f[t]);
+    fW[t] = lane_west(f[t]);
+  }
+  const T nu4 = (T)p.nu4, inv12 = (T)p.inv12;
+  const T a = (T)p.a, b = (T)p.b, cdt = (T)p.cdt;
+  const bool lout = colin && lane >= 2 && lane < 62;
+#pragma unroll
+  for (int t = 0; t < R; ++t) {
+    const int y = t + 1;  // f, l1, qE, qW, by index of this row
+    const int qy = t + 2;  // q index of this row
+    const int gj = j0 + t;
+    // Arakawa J(psi, q + beta (y - y0)).
+    const T bN = row_val(by_l, y + 1), bC = row_val(by_l, y), bS = row_val(by_l, y - 1);
+    const T fe_ = fE[y], fw_ = fW[y], fn_ = f[y + 1], fs_ = f[y - 1];
+    const T fNE = fE[y + 1], fNW = fW[y + 1], fSE = fE[y - 1], fSW = fW[y - 1];
+    const T gE = qE[y] + bC, gW = qW[y] + bC;
+    const T gN = q[qy + 1] + bN, gS = q[qy - 1] + bS;
+    const T gNE = qE[y + 1] + bN, gNW = qW[y + 1] + bN;
+    const T gSE = qE[y - 1] + bS, gSW = qW[y - 1] + bS;
+    const T jpp = (fe_ - fw_) * (gN - gS) - (fn_ - fs_) * (gE - gW);
+    const T jpx = fe_ * (gNE - gSE) - fw_ * (gNW - gSW) - fn_ * (gNE - gNW) + fs_ * (gSE - gSW);
+    const T jxp = gN * (fNE - fNW) - gS * (fSE - fSW) - gE * (fNE - fSE) + gW * (fNW - fSW);
+    const T J = ((jpp + jpx) + jxp) * inv12;
+    // Second no-flux Laplacian -> biharmonic tendency -nu4 D(D q).
+    const T c = l1[y];
+    const T le = lane_east(c), lw = lane_west(c);
+    const T fe = cE ? (le - c) * idx : T(0);
+    const T fw = cW ? (c - lw) * idx : T(0);
+    const T fn = (gj <= Ny - 3) ? (l1[y + 1] - c) * idy : T(0);
+    const T fs = (gj >= 2) ? (c - l1[y - 1]) * idy : T(0);
+    const T bih = -nu4 * ((fe - fw) * idx + (fn - fs) * idy);
+    const T rhs = (-J + row_val(F_l, y)) + bih;
+    if (lout && gj < Ny) {
+      const int64_t g = (int64_t)gj * Nx + gi;
+      T res = T(0);
+      if (ci && gj >= 1 && gj <= Ny - 2)
+        res = (a != T(0) ? a * qn[g] : T(0)) + b * q[qy] + cdt * rhs;
+      out[g] = res;
+    }
+  }
+}
+
 template <typename TS, typename TP>
 int qg_stage(const TS* qs, const TP* psi, const TS* qn, TS* out, const uint8_t* mask, int Ny,
              int Nx, const QGStageParams& p, hipStream_t s) {
   FVX_CHECK_ARG(qs && psi && qn && out, "null field pointer");
   FVX_CHECK_ARG(Ny >= 3 && Nx >= 3, "array too small");
   FVX_CHECK_ARG(out != qs && out != (const TS*)psi, "out must not alias q_s or psi");
+  if constexpr (std::is_same<TS, float>::value) {
+    if (!mask && (int64_t)Ny * Nx < (int64_t(1) << 29)) {
+      const int wtx = cdiv(Nx, 60), ntiles = wtx * cdiv(Ny, kQWRows);
+      hipLaunchKernelGGL((qg_stage_wave_kernel<TP>), dim3(cdiv(ntiles, 4)), dim3(256), 0, s, qs,
+                         psi, qn, out, Ny, Nx, p, wtx, ntiles);
+      FVX_LAUNCH_CHECK();
+      return kOk;
+    }
+  }
   const int tx = cdiv(Nx, kQTx), ty = cdiv(Ny, kQTy);
   if (mask)
     hipLaunchKernelGGL((qg_stage_kernel<TS, TP, true>), dim3(tx * ty), dim3(kQThreads), 0, s, qs,
