@@ -1209,8 +1209,10 @@ __global__ __launch_bounds__(kCoarseThreads) void mg_coarse_lds_kernel(CoarseArg
   CoarseLds<T, KIND> S{reinterpret_cast<T*>(coarse_smem)};
   T* sm = S.sm;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-  int ou[16], ot[16], ob[16];
-  {
+  // LDS offsets of each level's u, t, b (a table in LDS: indexed by the
+  // runtime level, a private array would live in scratch memory).
+  __shared__ int ou[16], ot[16], ob[16];
+  if (threadIdx.x == 0) {
     int o = 0;
     for (int l = a.lc; l < a.nl; ++l) {
       const int n = a.lv[l].ny * a.lv[l].nx;
@@ -1220,6 +1222,7 @@ __global__ __launch_bounds__(kCoarseThreads) void mg_coarse_lds_kernel(CoarseArg
       o += 3 * n;
     }
   }
+  __syncthreads();
   const T om = a.omega;
   // Level lc's right-hand side (and masked guess) from global memory.
   {
@@ -1324,7 +1327,9 @@ size_t coarse_lds_bytes(const CoarseArgs<T>& a) {
   for (int l = a.lc; l < a.nl; ++l) n += 3 * (size_t)a.lv[l].ny * a.lv[l].nx;
   return n * sizeof(T);
 }
-constexpr size_t kCoarseLdsMax = 160 * 1024;
+// Dynamic LDS available to the coarse kernel: 160 KB less its static offset
+// table (3 x 16 ints).
+constexpr size_t kCoarseLdsMax = 160 * 1024 - 3 * 16 * sizeof(int);
 
 template <typename T, int KIND>
 static void launch_coarse(const CoarseArgs<T>& ca, hipStream_t s) {
