@@ -80,7 +80,9 @@ __device__ __forceinline__ float lane_west(float v) {
 // Buffer accesses: a 4-register descriptor per array (base, byte size), a
 // wave-uniform row offset in a scalar register and the lane's column offset,
 // so no per-access 64-bit address arithmetic; accesses past the array's end
-// read 0 / are dropped.  Arrays must be under 2^31 bytes.
+// read 0 / are dropped.  Arrays must be under 2^31 bytes.  The last
+// descriptor word, 0x00020000, is the CDNA (gfx9-family) setting for raw
+// 32-bit data (data format 32, no swizzle, no index stride).
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const float* p, int n) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(p), (short)0, n * 4, 0x00020000);
 }
