@@ -522,7 +522,7 @@ static int qg_run_graph(QGPlan& Q, TS* q, double* psi, int nsteps, QGInfoC* info
   // step graphs are captured with those budgets (also within a one-step
   // call, so that later calls replay graphs only).  The first steps usually
   // start from a cold psi (zero or stale) and need more iterations than the
-  // run will (at 2048^2: 10.7, 9.3, 9.0 per solve, then 8.3), so the first
+  // run will, so the first
   // kObserveStep - 1 steps run unobserved; the next one is observed (mixed
   // precision: with unchunked inner solves, so each refinement step shows
   // the iterations it needs from a warm start).
@@ -554,10 +554,16 @@ static int qg_run_graph(QGPlan& Q, TS* q, double* psi, int nsteps, QGInfoC* info
     Q.b_outer = std::min(std::max(1, Q.obs_outer), QGPlan::kMaxOuter);
     // Inner budget per refinement sweep: the largest inner solve seen.
     Q.b_inner = std::max(2, (Q.solve == kSolveMixed ? Q.obs_inner : Q.obs_iters) + 1);
-    // Per refinement step: what that step took in the observed solves, plus
-    // one (steps never observed get the largest inner budget).
+    // Per refinement step: what that step took in the observed solves (steps
+    // never observed get the largest inner budget).  Multigrid-preconditioned
+    // inner solves take the same count step after step, so they get no
+    // margin: every iteration slot a converged solve leaves unused still
+    // costs its launches (~30 us at 2048^2), and a step whose solve misses
+    // its budget is redone with a wider one.  The learned preconditioner's
+    // longer, less regular counts keep one iteration of margin.
+    const int margin = Q.net ? 1 : 0;
     for (int o = 0; o < QGPlan::kMaxOuter; ++o)
-      Q.b_inner_o[o] = Q.obs_inner_o[o] > 0 ? std::min(Q.max_iters, Q.obs_inner_o[o] + 1)
+      Q.b_inner_o[o] = Q.obs_inner_o[o] > 0 ? std::min(Q.max_iters, Q.obs_inner_o[o] + margin)
                                             : Q.b_inner;
     rc = capture_graphs<TS>(Q);
     if (rc) return rc;
