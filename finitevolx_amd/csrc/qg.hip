@@ -549,19 +549,19 @@ static int qg_run_graph(QGPlan& Q, TS* q, double* psi, int nsteps, QGInfoC* info
     if (!observe) continue;
     Q.obs_iters = ci.max_iterations;
     if (Q.solve != kSolveMixed) Q.obs_outer = 1;
-    // Budgets: observed + 1 inner iteration of margin.
-    Q.b_iters = std::max(2, Q.obs_iters + 1);
+    // Budgets: the observed counts (see the margin below).
+    const int margin = Q.net ? 1 : 0;
+    Q.b_iters = std::max(2, Q.obs_iters + margin);
     Q.b_outer = std::min(std::max(1, Q.obs_outer), QGPlan::kMaxOuter);
     // Inner budget per refinement sweep: the largest inner solve seen.
     Q.b_inner = std::max(2, (Q.solve == kSolveMixed ? Q.obs_inner : Q.obs_iters) + 1);
     // Per refinement step: what that step took in the observed solves (steps
     // never observed get the largest inner budget).  Multigrid-preconditioned
-    // inner solves take the same count step after step, so they get no
+    // solves take the same count step after step, so they get no
     // margin: every iteration slot a converged solve leaves unused still
     // costs its launches (~30 us at 2048^2), and a step whose solve misses
     // its budget is redone with a wider one.  The learned preconditioner's
     // longer, less regular counts keep one iteration of margin.
-    const int margin = Q.net ? 1 : 0;
     for (int o = 0; o < QGPlan::kMaxOuter; ++o)
       Q.b_inner_o[o] = Q.obs_inner_o[o] > 0 ? std::min(Q.max_iters, Q.obs_inner_o[o] + margin)
                                             : Q.b_inner;
