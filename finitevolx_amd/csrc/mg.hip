@@ -1739,6 +1739,47 @@ __global__ __launch_bounds__(256) void refine_update_kernel(int ny, int nx, doub
   if (i == 0 && j0 == 0) sc->outer += 1;
 }
 
+// The same update when the inner solve's last PCG step is still pending (its
+// V-cycles applied every other step inside their first kernel): the inner
+// correction is e + alpha (z + beta p), formed in FP32 exactly as the step
+// kernel forms x, then added in FP64 -- the step's r, q and p are not needed
+// after an inner solve.  Skips the step when the inner solve stopped with it
+// already applied (done == 1).
+__global__ __launch_bounds__(256) void refine_update_step_kernel(
+    int ny, int nx, double* __restrict__ x, int64_t ld_x, const float* __restrict__ e,
+    const float* __restrict__ z, const float* __restrict__ p, const PcgScalars* in,
+    PcgScalars* sc) {
+  if (sc->out_done) return;
+  const bool step = in->done != 1;
+  const double beta64 = in->beta;
+  const float alpha = (float)in->alpha, beta = (float)beta64;
+  const bool restart = beta64 == 0.0;
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int j0 = (blockIdx.y * 4 + threadIdx.y) * kDirRows;
+  double xv[kDirRows];
+  float ev[kDirRows], zv[kDirRows], pv[kDirRows];
+#pragma unroll
+  for (int t = 0; t < kDirRows; ++t) {
+    const int j = j0 + t;
+    const bool in_ = i < nx && j < ny;
+    const int64_t k = (int64_t)j * nx + i;
+    xv[t] = in_ ? x[(int64_t)j * ld_x + i] : 0.0;
+    ev[t] = in_ ? e[k] : 0.0f;
+    zv[t] = (in_ && step) ? z[k] : 0.0f;
+    pv[t] = (in_ && step && !restart) ? p[k] : 0.0f;
+  }
+#pragma unroll
+  for (int t = 0; t < kDirRows; ++t) {
+    const int j = j0 + t;
+    if (i < nx && j < ny) {
+      float ef = ev[t];
+      if (step) ef = ef + alpha * (restart ? zv[t] : zv[t] + beta * pv[t]);
+      x[(int64_t)j * ld_x + i] = xv[t] + (double)ef;
+    }
+  }
+  if (i == 0 && j0 == 0) sc->outer += 1;
+}
+
 // Reset an inner solve on an already-formed residual r (mixed precision):
 // tol from ||r0||, flags cleared.  Single thread.
 __global__ void inner_reset_kernel(PcgScalars* inner, const PcgScalars* outer, double inner_rtol) {
@@ -2283,8 +2324,10 @@ static dim3 grid2(int ny, int nx) { return dim3(cdiv(nx, 64), cdiv(ny, 4)); }
 // turns the dots into the step's scalars, and the pointwise update.
 template <typename T, int KIND>
 static int pcg_iters(Plan& P, Hier<T>& H, T* x, int64_t ld_x, int n, PcgScalars* sc,
-                     hipStream_t s, PrecondFn pf = nullptr, void* ctx = nullptr) {
+                     hipStream_t s, PrecondFn pf = nullptr, void* ctx = nullptr,
+                     bool* defer_last = nullptr) {
   const LevelDev<T>& L = H.lv[0];
+  if (defer_last) *defer_last = false;
   // All-wet FP32 level 0 under the V-cycle with fused sweeps: each PCG step
   // after the first is applied by the next V-cycle's level-0 down kernel
   // (kDownUpd), which reads r_k, q_{k-1} from one pair of arrays and writes
@@ -2314,8 +2357,12 @@ static int pcg_iters(Plan& P, Hier<T>& H, T* x, int64_t ld_x, int n, PcgScalars*
           return kInvalidArgument;
         }
       }
-      hipLaunchKernelGGL(pcg_cg_update_az_kernel<T>, grid_stencil(L.ny, L.nx), dim3(64, 4), 0, s,
-                         L, x, ld_x, H.z, H.p, qc, H.q, rc, H.r, sc);
+      // The last step: here, or (defer_last) by the caller, which needs only
+      // its x (refine_update_step_kernel).
+      if (defer_last) *defer_last = true;
+      else
+        hipLaunchKernelGGL(pcg_cg_update_az_kernel<T>, grid_stencil(L.ny, L.nx), dim3(64, 4), 0,
+                           s, L, x, ld_x, H.z, H.p, qc, H.q, rc, H.r, sc);
     }
     FVX_LAUNCH_CHECK();
     return kOk;
@@ -2463,6 +2510,19 @@ static void launch_refine_residual(Plan& P, const TB* rhs, int64_t ld_rhs, const
 
 // Enqueue the mixed-precision schedule: n_outer x [FP64 residual, FP32 PCG
 // (n_inner[o] guarded iterations), FP64 update] and a closing FP64 residual.
+// x += inner correction (with the inner solve's last PCG step when deferred).
+static void launch_refine_update(Plan& P, double* x, int64_t ld_x, bool deferred,
+                                 const PcgScalars* in, PcgScalars* out, hipStream_t s) {
+  Hier<float>& H = P.h32;
+  const auto& L32 = H.lv[0];
+  if (deferred)
+    hipLaunchKernelGGL(refine_update_step_kernel, grid_dir(L32.ny, L32.nx), dim3(64, 4), 0, s,
+                       L32.ny, L32.nx, x, ld_x, H.x, H.z, H.p, in, out);
+  else
+    hipLaunchKernelGGL(refine_update_kernel, grid_dir(L32.ny, L32.nx), dim3(64, 4), 0, s, L32.ny,
+                       L32.nx, x, ld_x, H.x, out);
+}
+
 template <typename TB>
 int mixed_enqueue(Plan& P, const TB* rhs, int64_t ld_rhs, double* x, int64_t ld_x, double rtol,
                   double atol, int n_outer, const int* n_inner, double inner_rtol, int slot,
@@ -2479,12 +2539,14 @@ int mixed_enqueue(Plan& P, const TB* rhs, int64_t ld_rhs, double* x, int64_t ld_
   for (int o = 0; o < n_outer; ++o) {
     residual();
     hipLaunchKernelGGL(inner_reset_kernel, dim3(1), dim3(1), 0, s, in, out, inner_rtol);
+    bool deferred = false;
     int rc = (P.kind == kDirichlet)
-                 ? pcg_iters<float, kDirichlet>(P, H, H.x, L32.nx, n_inner[o], in, s, pf, ctx)
-                 : pcg_iters<float, kNoflux>(P, H, H.x, L32.nx, n_inner[o], in, s, pf, ctx);
+                 ? pcg_iters<float, kDirichlet>(P, H, H.x, L32.nx, n_inner[o], in, s, pf, ctx,
+                                                &deferred)
+                 : pcg_iters<float, kNoflux>(P, H, H.x, L32.nx, n_inner[o], in, s, pf, ctx,
+                                             &deferred);
     if (rc) return rc;
-    hipLaunchKernelGGL(refine_update_kernel, grid_dir(L32.ny, L32.nx), dim3(64, 4), 0, s, L32.ny,
-                       L32.nx, x, ld_x, H.x, out);
+    launch_refine_update(P, x, ld_x, deferred, in, out, s);
   }
   residual();
   FVX_LAUNCH_CHECK();
@@ -2517,12 +2579,12 @@ int mixed_solve(Plan& P, const TB* rhs, int64_t ld_rhs, double* x, int64_t ld_x,
       PcgScalars* in = &P.sc[2 * slot];
       PcgScalars* out = &P.sc[2 * slot + 1];
       hipLaunchKernelGGL(inner_reset_kernel, dim3(1), dim3(1), 0, s, in, out, inner_rtol);
+      bool deferred = false;
       rc = (P.kind == kDirichlet)
-               ? pcg_iters<float, kDirichlet>(P, H, H.x, L32.nx, inner, in, s, pf, ctx)
-               : pcg_iters<float, kNoflux>(P, H, H.x, L32.nx, inner, in, s, pf, ctx);
+               ? pcg_iters<float, kDirichlet>(P, H, H.x, L32.nx, inner, in, s, pf, ctx, &deferred)
+               : pcg_iters<float, kNoflux>(P, H, H.x, L32.nx, inner, in, s, pf, ctx, &deferred);
       if (rc) return rc;
-      hipLaunchKernelGGL(refine_update_kernel, grid_dir(L32.ny, L32.nx), dim3(64, 4), 0, s, L32.ny,
-                         L32.nx, x, ld_x, H.x, out);
+      launch_refine_update(P, x, ld_x, deferred, in, out, s);
       launch_refine_residual<TB>(P, rhs, ld_rhs, x, ld_x, rtol, atol, out, s);
       FVX_LAUNCH_CHECK();
     }
