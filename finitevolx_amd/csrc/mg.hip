@@ -97,19 +97,45 @@ enum RedSite : int {
   kRedCG = 0,      // the nine PCG dots -> beta, p.Ap, alpha, predicted ||r||^2
   kRedInit = 1,    // ||b||^2, ||r0||^2 -> tolerance, flags
   kRedRefine = 4,  // FP64 refinement residual
+  kRedRefineReset = 5,  // the same, then the inner solve's reset (sc - 1)
   kRedTrue = 6,    // FP64 certification residual
 };
+
+// Reset of an inner FP32 solve on an already-formed refinement residual:
+// tolerance from ||r0||, flags cleared; single thread.  The refinement step
+// that has just finished keeps its iteration count.  Reduce the residual by
+// inner_rtol, but no further than half the outer target: later refinement
+// steps then stop as soon as the FP64 residual can certify (FP32 rounding of
+// the correction is ~eps32 kappa ||r||, far below that target once ||r|| is
+// small).
+__device__ __forceinline__ void inner_reset(PcgScalars* inner, const PcgScalars* outer,
+                                            double inner_rtol) {
+  if (outer->outer >= 1 && outer->outer <= 4) inner->it_outer[outer->outer - 1] = inner->iter;
+  if (outer->out_done) {
+    inner->done = 1;
+    return;
+  }
+  inner->max_inner = max(inner->max_inner, inner->iter);
+  inner->rr = outer->out_rr;
+  inner->tol2 = fmax(inner_rtol * inner_rtol * outer->out_rr, 0.25 * outer->out_tol2);
+  inner->done = 0;
+  inner->first = 1;
+  inner->iter = 0;
+}
 // Order of the PCG dots (z = M r, a = A z, q = q_{k-1} = A p_{k-1}).
 enum CgDot : int { kZR, kZQ, kZA, kRR, kRA, kRQ, kAA, kAQ, kQQ, kNumCgDots };
 constexpr int kFinishThreads = 1024;
 
 template <int NV, int SITE>
 __global__ __launch_bounds__(kFinishThreads) void reduce_finish_kernel(
-    const double* __restrict__ partials, int nblk, PcgScalars* sc, double rtol2, double atol2) {
+    const double* __restrict__ partials, int nblk, PcgScalars* sc, double rtol2, double atol2,
+    double inner_rtol = 0.0) {
+  // kRedRefineReset: the inner solve's scalars sit just before the outer ones.
+  constexpr bool kReset = SITE == kRedRefineReset;
   // The same guards as the producing kernels, read together with the
   // partials so that the two memory round trips overlap.
   int skip = 0;
-  if constexpr (SITE == kRedRefine) skip = sc->out_done;
+  if constexpr (SITE == kRedRefine || kReset) skip = sc->out_done;
   else if constexpr (SITE != kRedInit && SITE != kRedTrue) skip = sc->done;
   // The PCG step's previous-iteration scalars, in the same round trip.
   int first = 1;
@@ -133,6 +159,8 @@ __global__ __launch_bounds__(kFinishThreads) void reduce_finish_kernel(
     // done == 2: the last update has been applied; stop it being applied again.
     if constexpr (SITE == kRedCG)
       if (tid == 0 && skip == 2) sc->done = 1;
+    if constexpr (kReset)
+      if (tid == 0) inner_reset(sc - 1, sc, inner_rtol);
     return;
   }
 #pragma unroll
@@ -204,11 +232,12 @@ __global__ __launch_bounds__(kFinishThreads) void reduce_finish_kernel(
     sc->total = 0;
     sc->alpha = 0.0;
     sc->beta = 0.0;
-  } else {  // kRedRefine, kRedTrue
+  } else {  // kRedRefine, kRedRefineReset, kRedTrue
     sc->bb = out[0];
     sc->out_rr = out[1];
     sc->out_tol2 = fmax(rtol2 * out[0], atol2);
     sc->out_done = out[1] <= sc->out_tol2 ? 1 : 0;
+    if constexpr (kReset) inner_reset(sc - 1, sc, inner_rtol);
   }
 }
 
@@ -1783,22 +1812,7 @@ __global__ __launch_bounds__(256) void refine_update_step_kernel(
 // Reset an inner solve on an already-formed residual r (mixed precision):
 // tol from ||r0||, flags cleared.  Single thread.
 __global__ void inner_reset_kernel(PcgScalars* inner, const PcgScalars* outer, double inner_rtol) {
-  // Refinement step outer->outer - 1 has just finished: keep its count.
-  if (outer->outer >= 1 && outer->outer <= 4) inner->it_outer[outer->outer - 1] = inner->iter;
-  if (outer->out_done) {
-    inner->done = 1;
-    return;
-  }
-  inner->max_inner = max(inner->max_inner, inner->iter);
-  inner->rr = outer->out_rr;
-  // Reduce the residual by inner_rtol, but no further than half the outer
-  // target: later refinement steps then stop as soon as the FP64 residual
-  // can certify (FP32 rounding of the correction is ~eps32 kappa ||r||,
-  // far below that target once ||r|| is small).
-  inner->tol2 = fmax(inner_rtol * inner_rtol * outer->out_rr, 0.25 * outer->out_tol2);
-  inner->done = 0;
-  inner->first = 1;
-  inner->iter = 0;
+  inner_reset(inner, outer, inner_rtol);
 }
 
 // Certification: ||b - A x||^2 and ||b||^2 in FP64 into ``res`` (out_rr, bb,
@@ -2494,7 +2508,8 @@ int pcg_solve(Plan& P, const T* rhs, int64_t ld_rhs, T* x, int64_t ld_x, double 
 template <typename TB>
 static void launch_refine_residual(Plan& P, const TB* rhs, int64_t ld_rhs, const double* x,
                                    int64_t ld_x, double rtol, double atol, PcgScalars* out,
-                                   hipStream_t s) {
+                                   hipStream_t s, bool reset_inner = false,
+                                   double inner_rtol = 0.0) {
   Hier<float>& H = P.h32;
   const auto& L64 = P.h64.lv[0];
   const dim3 g = grid_res(L64.ny, L64.nx);
@@ -2504,8 +2519,15 @@ static void launch_refine_residual(Plan& P, const TB* rhs, int64_t ld_rhs, const
   else
     hipLaunchKernelGGL((refine_residual_kernel<kNoflux, TB>), g, dim3(64, 4), 0, s, L64, rhs,
                        ld_rhs, x, ld_x, H.r, H.x, rtol * rtol, atol * atol, P.partials, out);
-  hipLaunchKernelGGL((reduce_finish_kernel<2, kRedRefine>), dim3(1), dim3(kFinishThreads), 0, s,
-                     P.partials, (int)(g.x * g.y), out, rtol * rtol, atol * atol);
+  // with reset_inner, the same kernel resets the inner solve (out - 1) on
+  // this residual
+  if (reset_inner)
+    hipLaunchKernelGGL((reduce_finish_kernel<2, kRedRefineReset>), dim3(1), dim3(kFinishThreads),
+                       0, s, P.partials, (int)(g.x * g.y), out, rtol * rtol, atol * atol,
+                       inner_rtol);
+  else
+    hipLaunchKernelGGL((reduce_finish_kernel<2, kRedRefine>), dim3(1), dim3(kFinishThreads), 0,
+                       s, P.partials, (int)(g.x * g.y), out, rtol * rtol, atol * atol, 0.0);
 }
 
 // Enqueue the mixed-precision schedule: n_outer x [FP64 residual, FP32 PCG
@@ -2533,12 +2555,13 @@ int mixed_enqueue(Plan& P, const TB* rhs, int64_t ld_rhs, double* x, int64_t ld_
   PcgScalars* in = &P.sc[2 * slot];
   PcgScalars* out = &P.sc[2 * slot + 1];
   hipLaunchKernelGGL(scalars_reset_kernel, dim3(1), dim3(64), 0, s, in, 2);
-  auto residual = [&]() {
-    launch_refine_residual<TB>(P, rhs, ld_rhs, x, ld_x, rtol, atol, out, s);
+  // out == in + 1: the residual's finishing kernel also resets the inner solve
+  auto residual = [&](bool reset_inner) {
+    launch_refine_residual<TB>(P, rhs, ld_rhs, x, ld_x, rtol, atol, out, s, reset_inner,
+                               inner_rtol);
   };
   for (int o = 0; o < n_outer; ++o) {
-    residual();
-    hipLaunchKernelGGL(inner_reset_kernel, dim3(1), dim3(1), 0, s, in, out, inner_rtol);
+    residual(true);
     bool deferred = false;
     int rc = (P.kind == kDirichlet)
                  ? pcg_iters<float, kDirichlet>(P, H, H.x, L32.nx, n_inner[o], in, s, pf, ctx,
@@ -2548,7 +2571,7 @@ int mixed_enqueue(Plan& P, const TB* rhs, int64_t ld_rhs, double* x, int64_t ld_
     if (rc) return rc;
     launch_refine_update(P, x, ld_x, deferred, in, out, s);
   }
-  residual();
+  residual(false);
   FVX_LAUNCH_CHECK();
   return kOk;
 }
