@@ -115,12 +115,14 @@ def test_all_wet_dirichlet_fp32_v_cycle_matches_oracle(ny, nx, npre, npost):
     assert_close_scaled(to_np(got1), fvx.v_cycle(levels, u0, b, **kw), 1e-5, "V(u0,b) f32")
 
 
-@pytest.mark.parametrize("ny,nx", [(256, 192), (200, 168)])
+@pytest.mark.parametrize("ny,nx", [(256, 192), (200, 168), (96, 520)])
 @pytest.mark.parametrize("nsw", [2, 3])
 def test_all_wet_dirichlet_fp32_pcg_iterates_match_oracle(ny, nx, nsw):
-    """The first flexible-PCG iterates x_1..x_3 of the FP32 solver (V-cycle
-    whose last kernel also forms A z and the dot products, 2 or 3 fused
-    sweeps) against the oracle's PCG with the same V-cycle: any error in z, A z or
+    """The first flexible-PCG iterates x_1..x_4 of the FP32 solver (V-cycle
+    whose last kernel also forms A z and the dot products, and whose first
+    kernel applies the previous step; 2 or 3 fused sweeps; widths that are
+    not multiples of the 54- and 56-column tile interiors) against the
+    oracle's PCG with the same V-cycle: any error in z, A z, the fused step or
     the step scalars shows at the 1e-2 level, FP32 rounding at 1e-6."""
     need_gpu()
     m = np.ones((ny, nx))
@@ -131,7 +133,7 @@ def test_all_wet_dirichlet_fp32_pcg_iterates_match_oracle(ny, nx, nsw):
     M = lambda r: fvx.v_cycle(levels, np.zeros_like(r), r, **kw)
     A = lambda u: fvx.apply_operator(u, levels[0])
     mg32 = S.build_multigrid_solver(m, dx, dy, lam, kind="dirichlet", precision="float32", **kw)
-    for k in (1, 2, 3):
+    for k in (1, 2, 3, 4):
         xr, _, _, _ = fvx.pcg(A, b, precond=M, rtol=1e-12, atol=0.0, max_steps=k)
         x, info = mg32.pcg(torch.tensor(b, dtype=torch.float32), rtol=1e-12, atol=0.0,
                            max_steps=k)
