@@ -249,3 +249,27 @@ def test_large_bottom_level_uses_coarse_kernel():
     b = np.random.default_rng(0).standard_normal((n, n))
     ref = fvx.v_cycle(lev, np.zeros_like(b), b, n_pre=2, n_post=2, n_coarse=10)
     assert_close_scaled(to_np(mg.v_cycle(None, b)), ref, 1e-10, "V 130")
+
+
+def test_kernel_bench_down_step_mode():
+    """fvx_mg_kernel_bench mode 3 (the level-0 down kernel with the previous
+    PCG step) runs on an all-wet FP32 Dirichlet plan and is refused, with the
+    library's invalid-argument error, on a plan with a land mask."""
+    need_gpu()
+    from finitevolx_amd import _lib
+
+    _lib.extra_signatures({"fvx_mg_kernel_bench": [_lib._vp, _lib._i, _lib._i, _lib._i, _lib._vp]})
+    ny = nx = 256
+    kw = dict(n_pre=3, n_post=3, n_coarse=4, omega=0.85)
+    wet = S.build_multigrid_solver(np.ones((ny, nx)), 1.0 / nx, 1.0 / ny, 10.0, kind="dirichlet",
+                                   precision="mixed", **kw)
+    for reps in (0, 3):
+        _lib.call("fvx_mg_kernel_bench", wet._h, 3, 32, reps, _lib.stream())
+    torch.cuda.synchronize()
+    m = np.ones((ny, nx))
+    m[:, :9] = 0.0
+    masked = S.build_multigrid_solver(m, 1.0 / nx, 1.0 / ny, 10.0, kind="dirichlet",
+                                      precision="mixed", **kw)
+    _lib.call("fvx_mg_kernel_bench", masked._h, 3, 32, 0, _lib.stream())
+    with pytest.raises(ValueError):
+        _lib.call("fvx_mg_kernel_bench", masked._h, 3, 32, 1, _lib.stream())
