@@ -2468,7 +2468,8 @@ int certify_enqueue(Plan& P, const T* rhs, int64_t ld_rhs, const T* x, int64_t l
 // Synchronous PCG: predicted chunk, then single iterations until converged.
 template <typename T>
 int pcg_solve(Plan& P, const T* rhs, int64_t ld_rhs, T* x, int64_t ld_x, double rtol,
-              double atol, int max_iters, bool use_x0, CgInfoC* info, int slot, hipStream_t s) {
+              double atol, int max_iters, bool use_x0, CgInfoC* info, int slot, hipStream_t s,
+              double cert_rtol) {
   Hier<T>& H = hier<T>(P);
   PcgScalars* sc = &P.sc[2 * slot];
   int chunk = std::max(1, std::min(P.last_iters, max_iters));
@@ -2486,7 +2487,8 @@ int pcg_solve(Plan& P, const T* rhs, int64_t ld_rhs, T* x, int64_t ld_x, double 
     done_iters += 1;
   }
   P.last_iters = std::max(1, P.hsc[2 * slot].iter);
-  rc = certify_enqueue<T>(P, rhs, ld_rhs, x, ld_x, rtol, atol, slot, s);
+  rc = certify_enqueue<T>(P, rhs, ld_rhs, x, ld_x, cert_rtol < 0 ? rtol : cert_rtol, atol, slot,
+                          s);
   if (rc) return rc;
   FVX_HIP(hipMemcpyAsync(&P.hsc[2 * slot], &P.sc[2 * slot], 2 * sizeof(PcgScalars),
                          hipMemcpyDeviceToHost, s));
@@ -2931,9 +2933,9 @@ template int certify_enqueue<double>(Plan&, const double*, int64_t, const double
 template int certify_enqueue<float>(Plan&, const float*, int64_t, const float*, int64_t, double,
                                     double, int, hipStream_t);
 template int pcg_solve<double>(Plan&, const double*, int64_t, double*, int64_t, double, double,
-                               int, bool, CgInfoC*, int, hipStream_t);
+                               int, bool, CgInfoC*, int, hipStream_t, double);
 template int pcg_solve<float>(Plan&, const float*, int64_t, float*, int64_t, double, double, int,
-                              bool, CgInfoC*, int, hipStream_t);
+                              bool, CgInfoC*, int, hipStream_t, double);
 template int mixed_solve<float>(Plan&, const float*, int64_t, double*, int64_t, double, double,
                                 int, int, double, CgInfoC*, int, hipStream_t, PrecondFn, void*);
 template int mixed_solve<double>(Plan&, const double*, int64_t, double*, int64_t, double, double,

@@ -49,8 +49,11 @@ template <typename T>
 int certify_enqueue(Plan& P, const T* rhs, int64_t ld_rhs, const T* x, int64_t ld_x, double rtol,
                     double atol, int slot, hipStream_t s);
 template <typename T>
+// rtol: the recursive residual's target; cert_rtol (< 0: rtol) the tolerance
+// the FP64 true residual is certified against.
 int pcg_solve(Plan& P, const T* rhs, int64_t ld_rhs, T* x, int64_t ld_x, double rtol,
-              double atol, int max_iters, bool use_x0, CgInfoC* info, int slot, hipStream_t s);
+              double atol, int max_iters, bool use_x0, CgInfoC* info, int slot, hipStream_t s,
+              double cert_rtol = -1.0);
 template <typename TB>
 int mixed_enqueue(Plan& P, const TB* rhs, int64_t ld_rhs, double* x, int64_t ld_x, double rtol,
                   double atol, int n_outer, const int* n_inner, double inner_rtol, int slot,

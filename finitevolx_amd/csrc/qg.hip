@@ -136,19 +136,52 @@ __global__ __launch_bounds__(kQThreads) void qg_stage_kernel(
   }
 }
 
-// The same stage for an FP32 state without a land mask, laid out for the
-// wavefront: one wavefront per tile of 64 columns (60 written, two halo lanes
-// on each side) x kQWRows rows; each lane holds its column of q (rows
-// j0-2 .. j0+R+1) and psi (rows j0-1 .. j0+R) in registers, east/west
-// neighbours come from DPP lane shifts, north/south from the neighbouring
-// registers: no LDS, no barriers.  Arithmetic as in qg_stage_kernel.
+// The same stage laid out for the wavefront: one wavefront per tile of 64
+// columns (60 written, two halo lanes on each side) x kQWRows rows; each lane
+// holds its column of q (rows j0-2 .. j0+R+1) and psi (rows j0-1 .. j0+R) in
+// registers, east/west neighbours come from DPP lane shifts, north/south from
+// the neighbouring registers: no LDS, no barriers.  Arithmetic as in
+// qg_stage_kernel.  TS is the state's storage type, TC the type the tendency
+// is computed in; the stage combination a q_n + b q_s + c dt rhs is formed in
+// TS from the stored q_s, so an FP64 state with an FP32 tendency rounds the
+// tendency only, never the state.
 constexpr int kQWRows = 8;
 
-template <typename TP>
+template <typename T>
+__device__ __forceinline__ T lane_e(T v) {
+  return dpp_move<0x130>(v);  // wave_shl 1: value of lane + 1, 0 in lane 63
+}
+template <typename T>
+__device__ __forceinline__ T lane_w(T v) {
+  return dpp_move<0x138>(v);  // wave_shr 1: value of lane - 1, 0 in lane 0
+}
+template <typename T>
+__device__ __forceinline__ T read_lane(T v, int l) {
+  if constexpr (sizeof(T) == 4) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+  } else {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+  }
+}
+// Element (row_off + col) of a state array of n elements; FP32 arrays through
+// a buffer descriptor (scalar row offset), FP64 through a plain load.
+__device__ __forceinline__ float load_state(const float* a, int n, unsigned col, int row_off) {
+  return buf_load(buf_rsrc(a, n), col, row_off);
+}
+__device__ __forceinline__ double load_state(const double* a, int n, unsigned col, int row_off) {
+  (void)n;
+  return a[(int64_t)row_off + col];
+}
+
+template <typename TS, typename TC, typename TP, bool MASKED>
 __global__ __launch_bounds__(256) void qg_stage_wave_kernel(
-    const float* __restrict__ qs, const TP* __restrict__ psi, const float* __restrict__ qn,
-    float* __restrict__ out, int Ny, int Nx, QGStageParams p, int tiles_x, int ntiles) {
-  using T = float;
+    const TS* __restrict__ qs, const TP* __restrict__ psi, const TS* __restrict__ qn,
+    TS* __restrict__ out, const uint8_t* __restrict__ mask, int Ny, int Nx, QGStageParams p,
+    int tiles_x, int ntiles) {
+  using T = TC;
   constexpr int R = kQWRows;
   const int lane = threadIdx.x & 63;
   const int tile =
@@ -162,13 +195,16 @@ __global__ __launch_bounds__(256) void qg_stage_wave_kernel(
   const unsigned gic = (unsigned)min(max(gi, 0), Nx - 1);
   const int n = Ny * Nx;
   // q rows j0-2+t (t = 0 .. R+3), psi rows j0-1+t (t = 0 .. R+1); zero
-  // outside the array.
+  // outside the array.  qc keeps the stored q_s of the R output rows.
   T q[R + 4], f[R + 2];
+  TS qc[R];
 #pragma unroll
   for (int t = 0; t < R + 4; ++t) {
     const int gj = j0 - 2 + t;
-    const T v = buf_load(buf_rsrc(qs, n), gic, min(max(gj, 0), Ny - 1) * Nx);
-    q[t] = (colin && gj >= 0 && gj < Ny) ? v : T(0);
+    TS v = load_state(qs, n, gic, min(max(gj, 0), Ny - 1) * Nx);
+    v = (colin && gj >= 0 && gj < Ny) ? v : TS(0);
+    q[t] = (T)v;
+    if (t >= 2 && t < R + 2) qc[t - 2] = v;
   }
 #pragma unroll
   for (int t = 0; t < R + 2; ++t) {
@@ -180,12 +216,8 @@ __global__ __launch_bounds__(256) void qg_stage_wave_kernel(
   // F = -F0 sin(kF y) for rows j0 .. j0+R-1: one row per lane, then read back
   // as wave-uniform values.
   const int gjl = j0 - 1 + lane;
-  const float by_l = (float)(p.beta * ((gjl - 0.5) * p.dy - p.y0));
-  const float F_l = (lane >= 1 && lane <= R) ? (float)(-p.F0 * sin(p.kF * ((gjl - 0.5) * p.dy)))
-                                             : 0.0f;
-  auto row_val = [](float v, int l) -> T {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
-  };
+  const T by_l = (T)(p.beta * ((gjl - 0.5) * p.dy - p.y0));
+  const T F_l = (lane >= 1 && lane <= R) ? (T)(-p.F0 * sin(p.kF * ((gjl - 0.5) * p.dy))) : T(0);
   const T idx = (T)p.idx, idy = (T)p.idy;
   const bool ci = gi >= 1 && gi <= Nx - 2;  // interior column
   const bool cE = gi <= Nx - 3, cW = gi >= 2;
@@ -196,7 +228,7 @@ __global__ __launch_bounds__(256) void qg_stage_wave_kernel(
   for (int t = 0; t < R + 2; ++t) {
     const int qi = t + 1, gj = j0 - 1 + t;
     const T c = q[qi];
-    const T e = lane_east(c), w = lane_west(c);
+    const T e = lane_e(c), w = lane_w(c);
     qE[t] = e;
     qW[t] = w;
     const T fe = cE ? (e - c) * idx : T(0);
@@ -209,11 +241,11 @@ __global__ __launch_bounds__(256) void qg_stage_wave_kernel(
   T fE[R + 2], fW[R + 2];
 #pragma unroll
   for (int t = 0; t < R + 2; ++t) {
-    fE[t] = lane_east(f[t]);
-    fW[t] = lane_west(f[t]);
+    fE[t] = lane_e(f[t]);
+    fW[t] = lane_w(f[t]);
   }
   const T nu4 = (T)p.nu4, inv12 = (T)p.inv12;
-  const T a = (T)p.a, b = (T)p.b, cdt = (T)p.cdt;
+  const TS a = (TS)p.a, b = (TS)p.b, cdt = (TS)p.cdt;
   const bool lout = colin && lane >= 2 && lane < 62;
 #pragma unroll
   for (int t = 0; t < R; ++t) {
@@ -221,7 +253,7 @@ __global__ __launch_bounds__(256) void qg_stage_wave_kernel(
     const int qy = t + 2;  // q index of this row
     const int gj = j0 + t;
     // Arakawa J(psi, q + beta (y - y0)).
-    const T bN = row_val(by_l, y + 1), bC = row_val(by_l, y), bS = row_val(by_l, y - 1);
+    const T bN = read_lane(by_l, y + 1), bC = read_lane(by_l, y), bS = read_lane(by_l, y - 1);
     const T fe_ = fE[y], fw_ = fW[y], fn_ = f[y + 1], fs_ = f[y - 1];
     const T fNE = fE[y + 1], fNW = fW[y + 1], fSE = fE[y - 1], fSW = fW[y - 1];
     const T gE = qE[y] + bC, gW = qW[y] + bC;
@@ -234,37 +266,54 @@ __global__ __launch_bounds__(256) void qg_stage_wave_kernel(
     const T J = ((jpp + jpx) + jxp) * inv12;
     // Second no-flux Laplacian -> biharmonic tendency -nu4 D(D q).
     const T c = l1[y];
-    const T le = lane_east(c), lw = lane_west(c);
+    const T le = lane_e(c), lw = lane_w(c);
     const T fe = cE ? (le - c) * idx : T(0);
     const T fw = cW ? (c - lw) * idx : T(0);
     const T fn = (gj <= Ny - 3) ? (l1[y + 1] - c) * idy : T(0);
     const T fs = (gj >= 2) ? (c - l1[y - 1]) * idy : T(0);
     const T bih = -nu4 * ((fe - fw) * idx + (fn - fs) * idy);
-    const T rhs = (-J + row_val(F_l, y)) + bih;
+    const T rhs = (-J + read_lane(F_l, y)) + bih;
     if (lout && gj < Ny) {
       const int64_t g = (int64_t)gj * Nx + gi;
-      T res = T(0);
-      if (ci && gj >= 1 && gj <= Ny - 2)
-        res = (a != T(0) ? a * qn[g] : T(0)) + b * q[qy] + cdt * rhs;
+      TS res = TS(0);
+      if (ci && gj >= 1 && gj <= Ny - 2) {
+        TS r = (TS)rhs;
+        if constexpr (MASKED) r = mask[g] ? r : TS(0);
+        res = (a != TS(0) ? a * qn[g] : TS(0)) + b * qc[t] + cdt * r;
+      }
       out[g] = res;
     }
   }
 }
 
+// tend32: compute the tendency of an FP64 state in FP32 (FP32 states always
+// use FP32).
 template <typename TS, typename TP>
 int qg_stage(const TS* qs, const TP* psi, const TS* qn, TS* out, const uint8_t* mask, int Ny,
-             int Nx, const QGStageParams& p, hipStream_t s) {
+             int Nx, const QGStageParams& p, hipStream_t s, bool tend32 = false) {
   FVX_CHECK_ARG(qs && psi && qn && out, "null field pointer");
   FVX_CHECK_ARG(Ny >= 3 && Nx >= 3, "array too small");
   FVX_CHECK_ARG(out != qs && out != (const TS*)psi, "out must not alias q_s or psi");
-  if constexpr (std::is_same<TS, float>::value) {
-    if (!mask && (int64_t)Ny * Nx < (int64_t(1) << 29)) {
-      const int wtx = cdiv(Nx, 60), ntiles = wtx * cdiv(Ny, kQWRows);
-      hipLaunchKernelGGL((qg_stage_wave_kernel<TP>), dim3(cdiv(ntiles, 4)), dim3(256), 0, s, qs,
-                         psi, qn, out, Ny, Nx, p, wtx, ntiles);
-      FVX_LAUNCH_CHECK();
-      return kOk;
+  if ((int64_t)Ny * Nx < (int64_t(1) << 29)) {
+    const int wtx = cdiv(Nx, 60), ntiles = wtx * cdiv(Ny, kQWRows);
+    const dim3 grid(cdiv(ntiles, 4)), block(256);
+    if (std::is_same<TS, float>::value || tend32) {
+      if (mask)
+        hipLaunchKernelGGL((qg_stage_wave_kernel<TS, float, TP, true>), grid, block, 0, s, qs,
+                           psi, qn, out, mask, Ny, Nx, p, wtx, ntiles);
+      else
+        hipLaunchKernelGGL((qg_stage_wave_kernel<TS, float, TP, false>), grid, block, 0, s, qs,
+                           psi, qn, out, mask, Ny, Nx, p, wtx, ntiles);
+    } else if constexpr (std::is_same<TS, double>::value) {
+      if (mask)
+        hipLaunchKernelGGL((qg_stage_wave_kernel<TS, double, TP, true>), grid, block, 0, s, qs,
+                           psi, qn, out, mask, Ny, Nx, p, wtx, ntiles);
+      else
+        hipLaunchKernelGGL((qg_stage_wave_kernel<TS, double, TP, false>), grid, block, 0, s, qs,
+                           psi, qn, out, mask, Ny, Nx, p, wtx, ntiles);
     }
+    FVX_LAUNCH_CHECK();
+    return kOk;
   }
   const int tx = cdiv(Nx, kQTx), ty = cdiv(Ny, kQTy);
   if (mask)
@@ -287,6 +336,7 @@ struct QGPlan {
   int ny, nx, Ny, Nx;
   double Lx, Ly, dx, dy, beta, lam, nu4, F0, dt, rtol, inner_rtol;
   int state;  // 32 or 64
+  bool tend32 = false;  // FP64 state with the tendency computed in FP32
   int solve;  // kSolveF64 or kSolveMixed
   int max_iters, max_outer;
   uint8_t* mask = nullptr;  // full-array wet mask (device), null = basin
@@ -381,9 +431,10 @@ static int qg_step_impl(QGPlan& Q, TS* q, double* psi, int nsteps, QGInfoC* info
       } else {
         if constexpr (std::is_same<TS, double>::value) {
           // Recursive residual to rtol/2 so the certified FP64 true residual
-          // meets rtol despite recurrence drift.
+          // meets rtol despite recurrence drift; certified at rtol, as the
+          // graph path certifies.
           rc = pcg_solve<double>(*Q.mg, qs + off, Nx, psi + off, Nx, 0.5 * Q.rtol, 0.0,
-                                 Q.max_iters, true, &ci, 0, s);
+                                 Q.max_iters, true, &ci, 0, s, Q.rtol);
         } else {
           set_error("FP64 MG-PCG needs an FP64 state");
           return kInvalidArgument;
@@ -398,7 +449,8 @@ static int qg_step_impl(QGPlan& Q, TS* q, double* psi, int nsteps, QGInfoC* info
       const double rel = ci.rhs_norm > 0 ? ci.residual_norm / ci.rhs_norm : ci.residual_norm;
       acc.max_rel_residual = std::max(acc.max_rel_residual, rel);
       rc = qg_stage<TS, double>(qs, psi, q, out, Q.mask, Q.Ny, Q.Nx,
-                                stage_params(Q, coef[k][0], coef[k][1], coef[k][2]), s);
+                                stage_params(Q, coef[k][0], coef[k][1], coef[k][2]), s,
+                                Q.tend32);
       if (rc) return rc;
     }
   }
@@ -437,7 +489,7 @@ static int qg_step_enqueue(QGPlan& Q, const TS* q_in, TS* q_out, hipStream_t s) 
     }
     if (rc) return rc;
     rc = qg_stage<TS, double>(qs, psi, q_in, out, Q.mask, Q.Ny, Q.Nx,
-                              stage_params(Q, coef[k][0], coef[k][1], coef[k][2]), s);
+                              stage_params(Q, coef[k][0], coef[k][1], coef[k][2]), s, Q.tend32);
     if (rc) return rc;
   }
   return kOk;
@@ -659,11 +711,12 @@ int fvx_qg_plan_create(int ny, int nx, double Lx, double Ly, double beta, double
   using namespace fvx;
   FVX_CHECK_ARG(plan_out, "null plan_out");
   FVX_CHECK_ARG(ny >= 8 && nx >= 8, "interior must be at least 8 x 8");
-  FVX_CHECK_ARG(state_precision == 32 || state_precision == 64, "state precision 32 or 64");
+  FVX_CHECK_ARG(state_precision == 32 || state_precision == 64 || state_precision == 6432,
+                "state precision 32, 64 or 6432 (FP64 state, FP32 tendency)");
   FVX_CHECK_ARG(solve_mode == kSolveF64 || solve_mode == kSolveMixed,
                 "solve mode must be 64 (FP64 MG-PCG) or 3264 (mixed)");
-  FVX_CHECK_ARG(!(solve_mode == kSolveF64 && state_precision == 32),
-                "the FP64 MG-PCG path needs an FP64 state");
+  FVX_CHECK_ARG(!(solve_mode == kSolveF64 && state_precision != 64),
+                "the FP64 MG-PCG path needs an FP64 state and FP64 tendency");
   FVX_CHECK_ARG(Lx > 0 && Ly > 0 && Ld > 0 && dt > 0, "physical parameters must be positive");
   QGPlan* Q = new QGPlan();
   Q->ny = ny;
@@ -681,7 +734,8 @@ int fvx_qg_plan_create(int ny, int nx, double Lx, double Ly, double beta, double
   Q->dt = dt;
   Q->rtol = rtol;
   Q->inner_rtol = inner_rtol;
-  Q->state = state_precision;
+  Q->state = state_precision == 32 ? 32 : 64;
+  Q->tend32 = state_precision == 6432;
   Q->solve = solve_mode;
   Q->max_iters = max_iters;
   Q->max_outer = max_outer;
@@ -720,7 +774,7 @@ int fvx_qg_plan_create(int ny, int nx, double Lx, double Ly, double beta, double
   }
   Q->mg = (Plan*)mg;
   const size_t N = (size_t)Q->Ny * Q->Nx;
-  const size_t es = state_precision == 64 ? 8 : 4;
+  const size_t es = Q->state == 64 ? 8 : 4;
   hipError_t e = hipMalloc(&Q->q1, N * es);
   if (e == hipSuccess) e = hipMalloc(&Q->q2, N * es);
   if (e == hipSuccess) e = hipMalloc(&Q->qa, N * es);
@@ -841,7 +895,7 @@ int fvx_qg_step_graph(void* plan, void* q, double* psi, int nsteps, fvx_qg_info*
   return qg_run_graph<float>(*Q, (float*)q, psi, nsteps, (QGInfoC*)info, s);
 }
 
-#define FVX_QG_STAGE(SUF, TS)                                                                     \
+#define FVX_QG_STAGE(SUF, TS, T32)                                                                \
   int fvx_qg_stage_##SUF(const TS* qs, const double* psi, const TS* qn, TS* out,                  \
                          const uint8_t* mask, int Ny, int Nx, double dx, double dy, double beta,  \
                          double Ly, double F0, double nu4, double a, double b, double cdt,        \
@@ -860,9 +914,11 @@ int fvx_qg_step_graph(void* plan, void* q, double* psi, int nsteps, fvx_qg_info*
     p.a = a;                                                                                      \
     p.b = b;                                                                                      \
     p.cdt = cdt;                                                                                  \
-    return qg_stage<TS, double>(qs, psi, qn, out, mask, Ny, Nx, p, (hipStream_t)stream);          \
+    return qg_stage<TS, double>(qs, psi, qn, out, mask, Ny, Nx, p, (hipStream_t)stream, T32);     \
   }
-FVX_QG_STAGE(f64, double)
-FVX_QG_STAGE(f32, float)
+FVX_QG_STAGE(f64, double, false)
+FVX_QG_STAGE(f32, float, true)
+// FP64 state, tendency computed in FP32, combination in FP64.
+FVX_QG_STAGE(f64t32, double, true)
 #undef FVX_QG_STAGE
 }

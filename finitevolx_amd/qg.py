@@ -41,10 +41,14 @@ _lib.extra_signatures({
     "fvx_qg_budgets": [_vp, C.POINTER(C.c_int)],
     "fvx_qg_stage_f64": [_vp, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _d, _d, _d, _d, _d, _vp],
     "fvx_qg_stage_f32": [_vp, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _d, _d, _d, _d, _d, _vp],
+    "fvx_qg_stage_f64t32": [_vp, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _d, _d, _d, _d, _d,
+                            _vp],
 })
 
-# precision name -> (state bits, solve mode)
-MODES = {"float64": (64, 64), "mixed": (32, 3264), "mixed64": (64, 3264)}
+# precision name -> (state code, solve mode); state code 6432 = FP64 state
+# with the tendency computed in FP32
+MODES = {"float64": (64, 64), "mixed": (6432, 3264), "mixed32": (32, 3264),
+         "mixed64": (64, 3264)}
 
 
 class QGInfo(dict):
@@ -54,9 +58,12 @@ class QGInfo(dict):
 class QGModel:
     """Fused GPU QG model on a closed ``ny x nx`` basin (interior cells).
 
-    ``precision``: ``"float64"`` (FP64 state, stencils and MG-PCG),
-    ``"mixed"`` (FP32 state and stencils, FP32 MG-PCG inner solves with FP64
-    refinement of psi) or ``"mixed64"`` (FP64 state, mixed solve).
+    ``precision``: ``"float64"`` (FP64 state, stencils and MG-PCG);
+    ``"mixed"`` (FP64 state, tendency stencils in FP32, FP32 MG-PCG inner
+    solves with FP64 refinement of psi); ``"mixed32"`` (as ``"mixed"`` with
+    the state stored in FP32, whose rounding each stage drifts the PV by
+    ~1e-6 relative over 100 steps); ``"mixed64"`` (as ``"mixed"`` with the
+    tendency in FP64).
     ``mask``: optional interior wet mask ``(ny, nx)``.
     """
 
@@ -73,7 +80,8 @@ class QGModel:
         self.beta, self.Ld, self.nu4, self.F0, self.dt = beta, Ld, nu4, F0, dt
         self.lambda_ = 1.0 / Ld**2
         self.precision = precision
-        self.state_bits, solve = MODES[precision]
+        self.state_code, solve = MODES[precision]
+        self.state_bits = 32 if self.state_code == 32 else 64
         self.dtype = torch.float64 if self.state_bits == 64 else torch.float32
         m = None
         self.mask = None
@@ -85,7 +93,7 @@ class QGModel:
         h = _vp()
         _lib.call("fvx_qg_plan_create", ny, nx, float(Lx), float(Ly), float(beta), float(Ld),
                   float(nu4), float(F0), float(dt), None if m is None else m.ctypes.data,
-                  self.state_bits, solve, n_pre, n_post, n_coarse, float(omega), float(rtol),
+                  self.state_code, solve, n_pre, n_post, n_coarse, float(omega), float(rtol),
                   float(inner_rtol), int(max_iters), int(max_outer), C.byref(h))
         self._h = h
         full = np.zeros((ny + 2, nx + 2), dtype=np.uint8)
@@ -168,7 +176,8 @@ class QGModel:
         psi = _lib.as_field(psi, torch.float64)
         out = torch.empty_like(q_s)
         Ny, Nx = self.shape
-        _lib.call(f"fvx_qg_stage_{_lib.suffix(q_s)}", q_s.data_ptr(), psi.data_ptr(),
+        suf = "f64t32" if self.state_code == 6432 else _lib.suffix(q_s)
+        _lib.call(f"fvx_qg_stage_{suf}", q_s.data_ptr(), psi.data_ptr(),
                   q_n.data_ptr(), out.data_ptr(), _lib.ptr(self._mask_full), Ny, Nx, self.dx,
                   self.dy, self.beta, self.Ly, self.F0, self.nu4, float(a), float(b),
                   float(c) * self.dt, _lib.stream())
