@@ -27,7 +27,10 @@ namespace fvx {
 template <typename T>
 int unet_apply(void* net, const T* r, int64_t ld_r, T* z, int64_t ld_z, int H, int W,
                double out_scale, const uint8_t* mask, hipStream_t s, const int* guard);
-}
+template <typename T>
+int lincomb3(int64_t n, double a, const T* x, double b, const T* y, double c, const T* z,
+             T* out, hipStream_t s);
+}  // namespace fvx
 
 extern "C" int fvx_mg_plan_create(const double*, const double*, int, int, double, double, double,
                                   int, int, int, int, double, int, int, void**);
@@ -346,7 +349,16 @@ struct QGPlan {
   // Graph replay: ping-pong state buffers, each graph = one full step.
   void* qa = nullptr;
   void* qb = nullptr;
-  double* psi = nullptr;
+  // Streamfunctions of the three stages of the last step and of the one
+  // before (four arrays B0..B3, rotating with period two): stage k of a step
+  // solves into its own array, warm-started from an extrapolation of the
+  // previous stages (stage_guess).
+  double* ps[4] = {nullptr, nullptr, nullptr, nullptr};
+  int parity = 0;          // 0: B0 holds the last step's stage-0 psi, 1: B3
+  bool fresh = true;       // no step taken since the history was reset
+  const double* last_psi = nullptr;  // caller's psi written by the last call
+  int cap_parity = 0;      // parity graph 0 was captured with
+  int warm = 1;            // 1: extrapolated warm starts, 0: previous stage's psi
   hipStream_t cap = nullptr;
   hipGraph_t graph[2] = {nullptr, nullptr};
   hipGraphExec_t exec[2] = {nullptr, nullptr};
@@ -397,9 +409,59 @@ static QGStageParams stage_params(const QGPlan& Q, double a, double b, double c)
   return p;
 }
 
+// Warm start of stage k's Helmholtz solve in a step taken with parity e,
+// formed in the array that stage solves into.  psi is linear in q, and the
+// stage states of rk3_ssp_step sit at t_n, t_n + dt and t_n + dt/2, so with
+// P, N the stage-0 solutions of the previous and this step and B1, B2 the
+// arrays of stages 1 and 2 (holding the previous step's until overwritten):
+//   stage 0 (t_n + dt after the previous step):  -P/3 + 2 B1/3 + 2 B2/3
+//   stage 1 (t_n + dt):                           2 N - P
+//   stage 2 (t_n + dt/2):                         (N + B1) / 2
+// each exact when psi varies linearly in time.  The first step after a reset
+// has no P: stage 1 starts from N.  warm == 0 starts every stage from the
+// previous stage's solution.  Returns the stage's streamfunction array.
+static double* stage_guess(QGPlan& Q, int k, int e, hipStream_t s, int* rc) {
+  const int64_t n = (int64_t)Q.Ny * Q.Nx;
+  double* P = Q.ps[e == 0 ? 0 : 3];
+  double* N = Q.ps[e == 0 ? 3 : 0];
+  double* B1 = Q.ps[1];
+  double* B2 = Q.ps[2];
+  const bool extrap = Q.warm == 1 && !Q.fresh;
+  if (k == 0) {
+    *rc = extrap ? lincomb3<double>(n, -1.0 / 3.0, P, 2.0 / 3.0, B1, 2.0 / 3.0, B2, N, s)
+                 : lincomb3<double>(n, 1.0, B2, 0.0, nullptr, 0.0, nullptr, N, s);
+    return N;
+  }
+  if (k == 1) {
+    *rc = extrap ? lincomb3<double>(n, 2.0, N, -1.0, P, 0.0, nullptr, B1, s)
+                 : lincomb3<double>(n, 1.0, N, 0.0, nullptr, 0.0, nullptr, B1, s);
+    return B1;
+  }
+  *rc = extrap ? lincomb3<double>(n, 0.5, N, 0.5, B1, 0.0, nullptr, B2, s)
+               : lincomb3<double>(n, 1.0, B1, 0.0, nullptr, 0.0, nullptr, B2, s);
+  return B2;
+}
+
+// Take the caller's psi as the warm start unless it is the array the last
+// call returned (then the plan's stage history continues).
+static int sync_history(QGPlan& Q, const double* psi, hipStream_t s) {
+  if (psi == Q.last_psi && !Q.fresh) return kOk;
+  const size_t bytes = (size_t)Q.Ny * Q.Nx * sizeof(double);
+  for (int b = 0; b < 4; ++b)
+    FVX_HIP(hipMemcpyAsync(Q.ps[b], psi, bytes, hipMemcpyDeviceToDevice, s));
+  Q.fresh = true;
+  return kOk;
+}
+
+static int return_psi(QGPlan& Q, double* psi, hipStream_t s) {
+  FVX_HIP(hipMemcpyAsync(psi, Q.ps[2], (size_t)Q.Ny * Q.Nx * sizeof(double),
+                         hipMemcpyDeviceToDevice, s));
+  Q.last_psi = psi;
+  return kOk;
+}
+
 template <typename TS>
-static int qg_step_impl(QGPlan& Q, TS* q, double* psi, int nsteps, QGInfoC* info,
-                        hipStream_t s) {
+static int qg_step_impl(QGPlan& Q, TS* q, int nsteps, QGInfoC* info, hipStream_t s) {
   const int Nx = Q.Nx;
   const int64_t off = (int64_t)Nx + 1;  // interior origin
   TS* q1 = (TS*)Q.q1;
@@ -415,7 +477,9 @@ static int qg_step_impl(QGPlan& Q, TS* q, double* psi, int nsteps, QGInfoC* info
       TS* out = (k == 0) ? q1 : (k == 1 ? q2 : q);
       CgInfoC ci;
       std::memset(&ci, 0, sizeof(ci));
-      int rc;
+      int rc = kOk;
+      double* psi = stage_guess(Q, k, Q.parity, s, &rc);
+      if (rc) return rc;
       if (Q.solve == kSolveMixed) {
         rc = mixed_solve<TS>(*Q.mg, qs + off, Nx, psi + off, Nx, Q.rtol, 0.0, Q.max_outer,
                              Q.max_iters, Q.inner_rtol, &ci, 0, s,
@@ -453,26 +517,30 @@ static int qg_step_impl(QGPlan& Q, TS* q, double* psi, int nsteps, QGInfoC* info
                                 Q.tend32);
       if (rc) return rc;
     }
+    Q.parity ^= 1;
+    Q.fresh = false;
   }
   if (info) *info = acc;
   return kOk;
 }
 
-// Enqueue one whole step q_in -> q_out (all kernels, no host work): the
-// sequence a graph captures.  Solves use the plan's budgets and slots 0..2.
+// Enqueue one whole step q_in -> q_out with parity e (all kernels, no host
+// work): the sequence a graph captures.  Solves use the plan's budgets and
+// slots 0..2.
 template <typename TS>
-static int qg_step_enqueue(QGPlan& Q, const TS* q_in, TS* q_out, hipStream_t s) {
+static int qg_step_enqueue(QGPlan& Q, const TS* q_in, TS* q_out, int e, hipStream_t s) {
   const int Nx = Q.Nx;
   const int64_t off = (int64_t)Nx + 1;
   TS* q1 = (TS*)Q.q1;
   TS* q2 = (TS*)Q.q2;
-  double* psi = Q.psi;
   const double coef[3][3] = {{0.0, 1.0, 1.0}, {0.75, 0.25, 0.25},
                              {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
   for (int k = 0; k < 3; ++k) {
     const TS* qs = (k == 0) ? q_in : (k == 1 ? q1 : q2);
     TS* out = (k == 0) ? q1 : (k == 1 ? q2 : q_out);
-    int rc;
+    int rc = kOk;
+    double* psi = stage_guess(Q, k, e, s, &rc);
+    if (rc) return rc;
     if (Q.solve == kSolveMixed) {
       rc = mixed_enqueue<TS>(*Q.mg, qs + off, Nx, psi + off, Nx, Q.rtol, 0.0, Q.b_outer,
                              Q.b_inner_o, Q.inner_rtol, k, s, Q.net ? &net_precond : nullptr,
@@ -511,7 +579,7 @@ static int capture_graphs(QGPlan& Q) {
     const TS* in = (const TS*)(g == 0 ? Q.qa : Q.qb);
     TS* out = (TS*)(g == 0 ? Q.qb : Q.qa);
     FVX_HIP(hipStreamBeginCapture(Q.cap, hipStreamCaptureModeThreadLocal));
-    int rc = qg_step_enqueue<TS>(Q, in, out, Q.cap);
+    int rc = qg_step_enqueue<TS>(Q, in, out, Q.cap_parity ^ g, Q.cap);
     hipGraph_t gr = nullptr;
     hipError_t e = hipStreamEndCapture(Q.cap, &gr);
     if (rc) {
@@ -559,7 +627,7 @@ static int qg_run_graph(QGPlan& Q, TS* q, double* psi, int nsteps, QGInfoC* info
     if (info) *info = acc;
     return kOk;
   }
-  FVX_HIP(hipMemcpyAsync(Q.psi, psi, N * sizeof(double), hipMemcpyDeviceToDevice, s));
+  if (int rc = sync_history(Q, psi, s)) return rc;
   int done_steps = 0;
   TS* cur = q;
   auto absorb = [&](const QGInfoC& ci) {
@@ -592,7 +660,7 @@ static int qg_run_graph(QGPlan& Q, TS* q, double* psi, int nsteps, QGInfoC* info
       }
     }
     QGInfoC ci;
-    int rc = qg_step_impl<TS>(Q, q, Q.psi, 1, &ci, s);
+    int rc = qg_step_impl<TS>(Q, q, 1, &ci, s);
     if (observe && Q.solve == kSolveMixed) plan_set_schedule(*Q.mg, so, si);
     if (rc) return rc;
     absorb(ci);
@@ -617,17 +685,30 @@ static int qg_run_graph(QGPlan& Q, TS* q, double* psi, int nsteps, QGInfoC* info
     for (int o = 0; o < QGPlan::kMaxOuter; ++o)
       Q.b_inner_o[o] = Q.obs_inner_o[o] > 0 ? std::min(Q.max_iters, Q.obs_inner_o[o] + margin)
                                             : Q.b_inner;
+    Q.cap_parity = Q.parity;
     rc = capture_graphs<TS>(Q);
     if (rc) return rc;
     Q.calibrated = true;
   }
+  // The step graphs extrapolate warm starts from the previous step's stage
+  // solutions: after a reset of the history one step runs eagerly first.
+  if (done_steps < nsteps && Q.fresh) {
+    QGInfoC ci;
+    int rc = qg_step_impl<TS>(Q, q, 1, &ci, s);
+    if (rc) return rc;
+    absorb(ci);
+    ++done_steps;
+  }
   if (done_steps < nsteps) {
     if (Q.exec[0] == nullptr) {
+      Q.cap_parity = Q.parity;
       int rc = capture_graphs<TS>(Q);
       if (rc) return rc;
     }
-    FVX_HIP(hipMemcpyAsync(Q.qa, q, N * sizeof(TS), hipMemcpyDeviceToDevice, s));
-    int g = 0;  // exec[0]: qa -> qb
+    // exec[0]: qa -> qb, exec[1]: qb -> qa; graph g was captured for steps
+    // of parity cap_parity ^ g.
+    int g = Q.parity ^ Q.cap_parity;
+    FVX_HIP(hipMemcpyAsync(g == 0 ? Q.qa : Q.qb, q, N * sizeof(TS), hipMemcpyDeviceToDevice, s));
     while (done_steps < nsteps) {
       FVX_HIP(hipGraphLaunch(Q.exec[g], s));
       QGInfoC st;
@@ -640,6 +721,7 @@ static int qg_run_graph(QGPlan& Q, TS* q, double* psi, int nsteps, QGInfoC* info
         acc.max_rel_residual = std::max(acc.max_rel_residual, st.max_rel_residual);
         ++done_steps;
         g ^= 1;
+        Q.parity ^= 1;
         continue;
       }
       // A solve missed its tolerance within the budget: widen and redo the
@@ -650,6 +732,7 @@ static int qg_run_graph(QGPlan& Q, TS* q, double* psi, int nsteps, QGInfoC* info
         Q.net = nullptr;
         Q.fallbacks += 1;
         acc.retries += 1;
+        Q.cap_parity = Q.parity ^ g;
         int rc = capture_graphs<TS>(Q);
         if (rc) return rc;
         continue;
@@ -666,6 +749,7 @@ static int qg_run_graph(QGPlan& Q, TS* q, double* psi, int nsteps, QGInfoC* info
         acc.max_rel_residual = std::max(acc.max_rel_residual, st.max_rel_residual);
         ++done_steps;
         g ^= 1;
+        Q.parity ^= 1;
         continue;
       }
       Q.recaptures += 1;
@@ -677,13 +761,14 @@ static int qg_run_graph(QGPlan& Q, TS* q, double* psi, int nsteps, QGInfoC* info
       for (int o = 0; o < QGPlan::kMaxOuter; ++o)
         Q.b_inner_o[o] = std::min(Q.max_iters, Q.b_inner_o[o] + std::max(1, Q.b_inner_o[o] / 4));
       Q.b_outer = std::min(std::min(Q.max_outer, QGPlan::kMaxOuter), Q.b_outer + 1);
+      Q.cap_parity = Q.parity ^ g;
       int rc = capture_graphs<TS>(Q);
       if (rc) return rc;
     }
     cur = (TS*)(g == 0 ? Q.qa : Q.qb);
     FVX_HIP(hipMemcpyAsync(q, cur, N * sizeof(TS), hipMemcpyDeviceToDevice, s));
   }
-  FVX_HIP(hipMemcpyAsync(psi, Q.psi, N * sizeof(double), hipMemcpyDeviceToDevice, s));
+  if (int rc = return_psi(Q, psi, s)) return rc;
   FVX_HIP(hipStreamSynchronize(s));
   if (info) *info = acc;
   return kOk;
@@ -779,7 +864,10 @@ int fvx_qg_plan_create(int ny, int nx, double Lx, double Ly, double beta, double
   if (e == hipSuccess) e = hipMalloc(&Q->q2, N * es);
   if (e == hipSuccess) e = hipMalloc(&Q->qa, N * es);
   if (e == hipSuccess) e = hipMalloc(&Q->qb, N * es);
-  if (e == hipSuccess) e = hipMalloc(&Q->psi, N * sizeof(double));
+  for (int b = 0; b < 4 && e == hipSuccess; ++b) {
+    e = hipMalloc(&Q->ps[b], N * sizeof(double));
+    if (e == hipSuccess) e = hipMemset(Q->ps[b], 0, N * sizeof(double));
+  }
   if (e == hipSuccess) e = hipMemset(Q->qa, 0, N * es);
   if (e == hipSuccess) e = hipMemset(Q->qb, 0, N * es);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&Q->cap, hipStreamNonBlocking);
@@ -814,7 +902,7 @@ int fvx_qg_plan_destroy(void* plan) {
   (void)hipFree(Q->q2);
   (void)hipFree(Q->qa);
   (void)hipFree(Q->qb);
-  (void)hipFree(Q->psi);
+  for (int b = 0; b < 4; ++b) (void)hipFree(Q->ps[b]);
   (void)hipFree(Q->mask);
   delete Q;
   return kOk;
@@ -829,9 +917,27 @@ int fvx_qg_step(void* plan, void* q, double* psi, int nsteps, fvx_qg_info* info,
   FVX_CHECK_ARG(nsteps >= 0, "negative nsteps");
   QGPlan* Q = (QGPlan*)plan;
   hipStream_t s = (hipStream_t)stream;
-  if (Q->state == 64)
-    return qg_step_impl<double>(*Q, (double*)q, psi, nsteps, (QGInfoC*)info, s);
-  return qg_step_impl<float>(*Q, (float*)q, psi, nsteps, (QGInfoC*)info, s);
+  if (int rc = sync_history(*Q, psi, s)) return rc;
+  int rc = (Q->state == 64)
+               ? qg_step_impl<double>(*Q, (double*)q, nsteps, (QGInfoC*)info, s)
+               : qg_step_impl<float>(*Q, (float*)q, nsteps, (QGInfoC*)info, s);
+  if (rc) return rc;
+  return return_psi(*Q, psi, s);
+}
+
+// Warm start of each stage's Helmholtz solve: 1 (default) extrapolates from
+// the previous stages' solutions, 0 starts from the previous stage's
+// solution.  Invalidates captured graphs.
+int fvx_qg_set_warm_start(void* plan, int mode) {
+  using namespace fvx;
+  FVX_CHECK_ARG(plan, "null plan");
+  FVX_CHECK_ARG(mode == 0 || mode == 1, "warm-start mode must be 0 or 1");
+  QGPlan* Q = (QGPlan*)plan;
+  Q->warm = mode;
+  drop_graphs(*Q);
+  Q->calibrated = false;
+  Q->eager_steps = 0;
+  return kOk;
 }
 
 // Use a learned U-Net (fvx_unet_create) as the preconditioner of the inner

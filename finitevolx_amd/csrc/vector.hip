@@ -32,6 +32,10 @@ int lincomb3(int64_t n, double a, const T* x, double b, const T* y, double c, co
   FVX_LAUNCH_CHECK();
   return kOk;
 }
+template int lincomb3<float>(int64_t, double, const float*, double, const float*, double,
+                             const float*, float*, hipStream_t);
+template int lincomb3<double>(int64_t, double, const double*, double, const double*, double,
+                              const double*, double*, hipStream_t);
 
 // out[k, p] = sum_l C[k, l] in[l, p] for nl <= 8 layers (layer_to_mode /
 // mode_to_layer, vertical/vertical_modes.py:193-263): one pass, each

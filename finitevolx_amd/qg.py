@@ -39,6 +39,7 @@ _lib.extra_signatures({
     "fvx_qg_set_preconditioner": [_vp, _vp, _d, _i],
     "fvx_qg_fallbacks": [_vp],
     "fvx_qg_budgets": [_vp, C.POINTER(C.c_int)],
+    "fvx_qg_set_warm_start": [_vp, _i],
     "fvx_qg_stage_f64": [_vp, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _d, _d, _d, _d, _d, _vp],
     "fvx_qg_stage_f32": [_vp, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _d, _d, _d, _d, _d, _vp],
     "fvx_qg_stage_f64t32": [_vp, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _d, _d, _d, _d, _d,
@@ -123,6 +124,16 @@ class QGModel:
         self._net = net  # keep the plan alive while the model uses it
         _lib.call("fvx_qg_set_preconditioner", self._h, None if net is None else net._h,
                   float(out_scale), int(sweeps))
+
+    def set_warm_start(self, mode: str = "extrapolate"):
+        """Warm start of each stage's Helmholtz solve: ``"extrapolate"``
+        (default; linear-in-time extrapolation from the previous stages'
+        streamfunctions, exact for psi linear in time) or ``"previous"`` (the
+        previous stage's streamfunction)."""
+        modes = {"previous": 0, "extrapolate": 1}
+        if mode not in modes:
+            raise ValueError(f"warm-start mode must be one of {list(modes)}")
+        _lib.call("fvx_qg_set_warm_start", self._h, modes[mode])
 
     @property
     def budgets(self) -> dict:
