@@ -1698,6 +1698,9 @@ __global__ __launch_bounds__(256) void refine_residual_kernel(
   };
   double xs = col ? Xat(j0 - 1, i) : 0.0, xc = col ? Xat(j0, i) : 0.0;
   double cys = CY(j0 - 1);
+  // A certified solve has not moved x since the residual that certified it
+  // (its updates are skipped), so the finishing kernel keeps that result.
+  if (done) return;
   double bb = 0.0, rr = 0.0;
 #pragma unroll
   for (int rw = 0; rw < kResRows; ++rw) {
@@ -1727,10 +1730,8 @@ __global__ __launch_bounds__(256) void refine_residual_kernel(
     if (col) {
       const double bv = (double)b[(int64_t)j * ld_b + i] * m;
       const double rv = (bv - (lap - L.lam * c) * m) * m;
-      if (!done) {
-        r32[k] = (float)rv;
-        e32[k] = 0.0f;
-      }
+      r32[k] = (float)rv;
+      e32[k] = 0.0f;
       bb += bv * bv;
       rr += rv * rv;
     }
@@ -1738,7 +1739,6 @@ __global__ __launch_bounds__(256) void refine_residual_kernel(
     xc = xn;
     cys = cyn;
   }
-  if (done) return;
   double v[2] = {bb, rr};
   block_partials<2, 256>(v, partials);
 }

@@ -363,11 +363,18 @@ struct QGPlan {
   hipGraph_t graph[2] = {nullptr, nullptr};
   hipGraphExec_t exec[2] = {nullptr, nullptr};
   static constexpr int kMaxOuter = 8;
-  int b_iters = 0, b_outer = 0, b_inner = 0;  // budgets the graphs hold
-  int b_inner_o[kMaxOuter] = {};              // inner budget per refinement step
-  int obs_iters = 0, obs_outer = 0;           // calibration observations
-  int obs_inner = 0;                          // largest inner solve (mixed)
-  int obs_inner_o[kMaxOuter] = {};            // per refinement step (mixed)
+  // Iteration budgets of one solve in the step graphs: FP64 PCG iterations;
+  // mixed: refinement steps and the inner iterations of each.
+  struct Budget {
+    int iters = 0, outer = 0;
+    int inner[kMaxOuter] = {};
+  };
+  Budget kb[3];            // per RK stage, as captured
+  Budget obs[3];           // per stage: largest counts seen (eager steps)
+  Budget win[3];           // per stage: largest counts of the current window
+  int b_inner = 0;         // inner budget of refinement steps never observed
+  int win_steps = 0;       // converged graph steps in the current window
+  int win_len = 4;         // window length before budgets are tightened
   bool calibrated = false;
   int eager_steps = 0;  // synchronous steps run before calibration
   int recaptures = 0;
@@ -486,11 +493,12 @@ static int qg_step_impl(QGPlan& Q, TS* q, int nsteps, QGInfoC* info, hipStream_t
                              Q.net ? &net_precond : nullptr, Q.net ? &Q : nullptr);
         Q.fallbacks += ci.pad;
         if (!rc) {
-          Q.obs_inner = std::max(Q.obs_inner, plan_max_inner(*Q.mg, 0));
+          QGPlan::Budget& ob = Q.obs[k];
+          Q.b_inner = std::max(Q.b_inner, plan_max_inner(*Q.mg, 0));
           int per[QGPlan::kMaxOuter];
           const int no = plan_inner_iters(*Q.mg, 0, per, QGPlan::kMaxOuter);
-          for (int o = 0; o < no; ++o) Q.obs_inner_o[o] = std::max(Q.obs_inner_o[o], per[o]);
-          Q.obs_outer = std::max(Q.obs_outer, ci.outer);
+          for (int o = 0; o < no; ++o) ob.inner[o] = std::max(ob.inner[o], per[o]);
+          ob.outer = std::max(ob.outer, ci.outer);
         }
       } else {
         if constexpr (std::is_same<TS, double>::value) {
@@ -505,6 +513,7 @@ static int qg_step_impl(QGPlan& Q, TS* q, int nsteps, QGInfoC* info, hipStream_t
         }
       }
       if (rc) return rc;
+      Q.obs[k].iters = std::max(Q.obs[k].iters, ci.iterations);
       acc.solves += 1;
       acc.iterations += ci.iterations;
       acc.max_iterations = std::max(acc.max_iterations, ci.iterations);
@@ -542,13 +551,13 @@ static int qg_step_enqueue(QGPlan& Q, const TS* q_in, TS* q_out, int e, hipStrea
     double* psi = stage_guess(Q, k, e, s, &rc);
     if (rc) return rc;
     if (Q.solve == kSolveMixed) {
-      rc = mixed_enqueue<TS>(*Q.mg, qs + off, Nx, psi + off, Nx, Q.rtol, 0.0, Q.b_outer,
-                             Q.b_inner_o, Q.inner_rtol, k, s, Q.net ? &net_precond : nullptr,
+      rc = mixed_enqueue<TS>(*Q.mg, qs + off, Nx, psi + off, Nx, Q.rtol, 0.0, Q.kb[k].outer,
+                             Q.kb[k].inner, Q.inner_rtol, k, s, Q.net ? &net_precond : nullptr,
                              Q.net ? &Q : nullptr);
     } else {
       if constexpr (std::is_same<TS, double>::value) {
         rc = pcg_enqueue<double>(*Q.mg, qs + off, Nx, psi + off, Nx, 0.5 * Q.rtol, 0.0, true,
-                                 Q.b_iters, k, s);
+                                 Q.kb[k].iters, k, s);
         if (!rc) rc = certify_enqueue<double>(*Q.mg, qs + off, Nx, psi + off, Nx, Q.rtol, 0.0, k, s);
       } else {
         set_error("FP64 MG-PCG needs an FP64 state");
@@ -596,8 +605,12 @@ static int capture_graphs(QGPlan& Q) {
   return kOk;
 }
 
-// Convergence of the three solves of the last replayed step.
-static bool step_converged(QGPlan& Q, QGInfoC& acc, hipStream_t s) {
+// Convergence of the three solves of the last replayed step, and the counts
+// each stage's solve used (per refinement step for the mixed solve): a
+// refinement step's inner count is recorded when the next refinement
+// residual resets the inner solve; the last budgeted step's stays in iter.
+static bool step_converged(QGPlan& Q, QGInfoC& acc, hipStream_t s, bool stage_ok[3],
+                           QGPlan::Budget used[3]) {
   const PcgScalars* h = fetch_slots(*Q.mg, 0, 3, s);
   if (!h) return false;
   bool ok = true;
@@ -605,7 +618,19 @@ static bool step_converged(QGPlan& Q, QGInfoC& acc, hipStream_t s) {
     const PcgScalars& in = h[2 * k];
     const PcgScalars& out = h[2 * k + 1];
     const bool conv = out.out_done != 0;  // FP64 true residual (both modes)
+    stage_ok[k] = conv;
     ok = ok && conv;
+    QGPlan::Budget& u = used[k];
+    u = QGPlan::Budget();
+    if (Q.solve == kSolveMixed) {
+      u.outer = out.outer;
+      for (int o = 0; o < std::min(out.outer, QGPlan::kMaxOuter); ++o) {
+        if (o == Q.kb[k].outer - 1) u.inner[o] = in.iter;
+        else u.inner[o] = (o < 4) ? in.it_outer[o] : Q.kb[k].inner[o];
+      }
+    } else {
+      u.iters = in.iter;
+    }
     acc.solves += 1;
     acc.iterations += in.total;
     acc.max_iterations = std::max(acc.max_iterations, in.total);
@@ -614,6 +639,19 @@ static bool step_converged(QGPlan& Q, QGInfoC& acc, hipStream_t s) {
     acc.max_rel_residual = std::max(acc.max_rel_residual, rel);
   }
   return ok;
+}
+
+static void budget_max(QGPlan::Budget& a, const QGPlan::Budget& b) {
+  a.iters = std::max(a.iters, b.iters);
+  a.outer = std::max(a.outer, b.outer);
+  for (int o = 0; o < QGPlan::kMaxOuter; ++o) a.inner[o] = std::max(a.inner[o], b.inner[o]);
+}
+
+// Largest budgets over the stages (reported by fvx_qg_budgets).
+static QGPlan::Budget budget_all(const QGPlan& Q) {
+  QGPlan::Budget m;
+  for (int k = 0; k < 3; ++k) budget_max(m, Q.kb[k]);
+  return m;
 }
 
 template <typename TS>
@@ -638,22 +676,21 @@ static int qg_run_graph(QGPlan& Q, TS* q, double* psi, int nsteps, QGInfoC* info
     acc.max_rel_residual = std::max(acc.max_rel_residual, ci.max_rel_residual);
     if (!ci.converged) acc.converged = 0;
   };
+  const int outer_cap = std::min(Q.max_outer, QGPlan::kMaxOuter);
   // Calibration: synchronous steps observe the iteration counts, then the
   // step graphs are captured with those budgets (also within a one-step
   // call, so that later calls replay graphs only).  The first steps usually
   // start from a cold psi (zero or stale) and need more iterations than the
-  // run will, so the first
-  // kObserveStep - 1 steps run unobserved; the next one is observed (mixed
-  // precision: with unchunked inner solves, so each refinement step shows
-  // the iterations it needs from a warm start).
+  // run will, so the first kObserveStep - 1 steps run unobserved; the next
+  // one is observed (mixed precision: with unchunked inner solves, so each
+  // refinement step shows the iterations it needs from a warm start).
   constexpr int kObserveStep = 3;
   while (!Q.calibrated && done_steps < nsteps) {
     const bool observe = Q.eager_steps >= kObserveStep - 1;
     int so = 0, si = 0;
     if (observe) {
-      Q.obs_inner = 0;
-      Q.obs_outer = 0;
-      for (int o = 0; o < QGPlan::kMaxOuter; ++o) Q.obs_inner_o[o] = 0;
+      for (int k = 0; k < 3; ++k) Q.obs[k] = QGPlan::Budget();
+      Q.b_inner = 0;
       if (Q.solve == kSolveMixed) {
         plan_schedule(*Q.mg, &so, &si);
         plan_set_schedule(*Q.mg, 1, Q.max_iters);
@@ -667,24 +704,25 @@ static int qg_run_graph(QGPlan& Q, TS* q, double* psi, int nsteps, QGInfoC* info
     ++done_steps;
     ++Q.eager_steps;
     if (!observe) continue;
-    Q.obs_iters = ci.max_iterations;
-    if (Q.solve != kSolveMixed) Q.obs_outer = 1;
-    // Budgets: the observed counts (see the margin below).
+    // Budgets: each stage's observed counts.  Multigrid-preconditioned solves
+    // take about the same count step after step, so they get no margin:
+    // every iteration slot a converged solve leaves unused still costs its
+    // launches (~30 us at 2048^2), and a step whose solve misses its budget
+    // is redone with a wider one.  The learned preconditioner's longer, less
+    // regular counts keep one iteration of margin.  Refinement steps never
+    // observed get the largest inner count seen, plus one.
     const int margin = Q.net ? 1 : 0;
-    Q.b_iters = std::max(2, Q.obs_iters + margin);
-    Q.b_outer = std::min(std::max(1, Q.obs_outer), QGPlan::kMaxOuter);
-    // Inner budget per refinement sweep: the largest inner solve seen.
-    Q.b_inner = std::max(2, (Q.solve == kSolveMixed ? Q.obs_inner : Q.obs_iters) + 1);
-    // Per refinement step: what that step took in the observed solves (steps
-    // never observed get the largest inner budget).  Multigrid-preconditioned
-    // solves take the same count step after step, so they get no
-    // margin: every iteration slot a converged solve leaves unused still
-    // costs its launches (~30 us at 2048^2), and a step whose solve misses
-    // its budget is redone with a wider one.  The learned preconditioner's
-    // longer, less regular counts keep one iteration of margin.
-    for (int o = 0; o < QGPlan::kMaxOuter; ++o)
-      Q.b_inner_o[o] = Q.obs_inner_o[o] > 0 ? std::min(Q.max_iters, Q.obs_inner_o[o] + margin)
-                                            : Q.b_inner;
+    Q.b_inner = std::max(2, Q.b_inner + 1);
+    for (int k = 0; k < 3; ++k) {
+      const QGPlan::Budget& ob = Q.obs[k];
+      QGPlan::Budget& b = Q.kb[k];
+      b.iters = std::max(2, ob.iters + margin);
+      b.outer = std::min(std::max(1, ob.outer), QGPlan::kMaxOuter);
+      for (int o = 0; o < QGPlan::kMaxOuter; ++o)
+        b.inner[o] = ob.inner[o] > 0 ? std::min(Q.max_iters, ob.inner[o] + margin) : Q.b_inner;
+      Q.win[k] = QGPlan::Budget();
+    }
+    Q.win_steps = 0;
     Q.cap_parity = Q.parity;
     rc = capture_graphs<TS>(Q);
     if (rc) return rc;
@@ -713,7 +751,9 @@ static int qg_run_graph(QGPlan& Q, TS* q, double* psi, int nsteps, QGInfoC* info
       FVX_HIP(hipGraphLaunch(Q.exec[g], s));
       QGInfoC st;
       std::memset(&st, 0, sizeof(st));
-      if (step_converged(Q, st, s)) {
+      bool stage_ok[3];
+      QGPlan::Budget used[3];
+      if (step_converged(Q, st, s, stage_ok, used)) {
         acc.solves += st.solves;
         acc.iterations += st.iterations;
         acc.max_iterations = std::max(acc.max_iterations, st.max_iterations);
@@ -722,11 +762,50 @@ static int qg_run_graph(QGPlan& Q, TS* q, double* psi, int nsteps, QGInfoC* info
         ++done_steps;
         g ^= 1;
         Q.parity ^= 1;
+        // Tighten: after win_len converged steps, each stage's budgets come
+        // down to the largest counts its solves used in that window (the
+        // learned preconditioner keeps its margin).
+        for (int k = 0; k < 3; ++k) budget_max(Q.win[k], used[k]);
+        if (++Q.win_steps >= Q.win_len && !Q.net) {
+          bool tighter = false;
+          for (int k = 0; k < 3; ++k) {
+            QGPlan::Budget& b = Q.kb[k];
+            const QGPlan::Budget& w = Q.win[k];
+            if (Q.solve == kSolveMixed) {
+              const int outer = std::max(1, w.outer);
+              if (outer < b.outer) {
+                b.outer = outer;
+                tighter = true;
+              }
+              for (int o = 0; o < b.outer; ++o) {
+                const int inner = std::max(1, w.inner[o]);
+                if (inner < b.inner[o]) {
+                  b.inner[o] = inner;
+                  tighter = true;
+                }
+              }
+            } else if (std::max(1, w.iters) < b.iters) {
+              b.iters = std::max(1, w.iters);
+              tighter = true;
+            }
+            Q.win[k] = QGPlan::Budget();
+          }
+          Q.win_steps = 0;
+          if (tighter) {
+            Q.cap_parity = Q.parity ^ g;
+            int rc = capture_graphs<TS>(Q);
+            if (rc) return rc;
+          }
+        }
         continue;
       }
-      // A solve missed its tolerance within the budget: widen and redo the
-      // step from its (untouched) input buffer.
-      if (Q.net && (Q.b_inner >= Q.max_iters || Q.b_outer >= Q.max_outer)) {
+      // A solve missed its tolerance within the budget: widen that stage's
+      // budgets and redo the step from its (untouched) input buffer; the
+      // next tightening waits twice as long.
+      const QGPlan::Budget top = budget_all(Q);
+      int top_inner = 0;
+      for (int o = 0; o < top.outer; ++o) top_inner = std::max(top_inner, top.inner[o]);
+      if (Q.net && (top_inner >= Q.max_iters || top.outer >= Q.max_outer)) {
         // The learned preconditioner cannot meet the tolerance within the
         // caps: fall back to multigrid-PCG for the rest of the run.
         Q.net = nullptr;
@@ -739,10 +818,15 @@ static int qg_run_graph(QGPlan& Q, TS* q, double* psi, int nsteps, QGInfoC* info
       }
       // Give up widening once the budget that can still grow is at its cap
       // (FP64: PCG iterations; mixed: refinement steps and inner iterations).
-      const bool capped = (Q.solve == kSolveMixed)
-                              ? (Q.b_outer >= std::min(Q.max_outer, QGPlan::kMaxOuter) &&
-                                 Q.b_inner >= Q.max_iters)
-                              : (Q.b_iters >= Q.max_iters);
+      bool capped = true;
+      for (int k = 0; k < 3; ++k) {
+        if (stage_ok[k]) continue;
+        const QGPlan::Budget& b = Q.kb[k];
+        bool inner_capped = true;
+        for (int o = 0; o < b.outer; ++o) inner_capped = inner_capped && b.inner[o] >= Q.max_iters;
+        capped = capped && ((Q.solve == kSolveMixed) ? (b.outer >= outer_cap && inner_capped)
+                                                      : (b.iters >= Q.max_iters));
+      }
       if (Q.recaptures > 50 || capped) {
         acc.converged = 0;
         acc.solves += st.solves;
@@ -754,13 +838,19 @@ static int qg_run_graph(QGPlan& Q, TS* q, double* psi, int nsteps, QGInfoC* info
       }
       Q.recaptures += 1;
       acc.retries += 1;
-      Q.b_iters = std::min(Q.max_iters, Q.b_iters + 2);
-      // widen by a quarter (at least one iteration) so long budgets converge
-      // in a few recaptures
-      Q.b_inner = std::min(Q.max_iters, Q.b_inner + std::max(1, Q.b_inner / 4));
-      for (int o = 0; o < QGPlan::kMaxOuter; ++o)
-        Q.b_inner_o[o] = std::min(Q.max_iters, Q.b_inner_o[o] + std::max(1, Q.b_inner_o[o] / 4));
-      Q.b_outer = std::min(std::min(Q.max_outer, QGPlan::kMaxOuter), Q.b_outer + 1);
+      for (int k = 0; k < 3; ++k) {
+        if (stage_ok[k]) continue;
+        QGPlan::Budget& b = Q.kb[k];
+        b.iters = std::min(Q.max_iters, b.iters + 2);
+        // widen by a quarter (at least one iteration) so long budgets
+        // converge in a few recaptures
+        for (int o = 0; o < QGPlan::kMaxOuter; ++o)
+          b.inner[o] = std::min(Q.max_iters, b.inner[o] + std::max(1, b.inner[o] / 4));
+        b.outer = std::min(outer_cap, b.outer + 1);
+      }
+      for (int k = 0; k < 3; ++k) Q.win[k] = QGPlan::Budget();
+      Q.win_steps = 0;
+      Q.win_len = std::min(64, 2 * Q.win_len);
       Q.cap_parity = Q.parity ^ g;
       int rc = capture_graphs<TS>(Q);
       if (rc) return rc;
@@ -976,9 +1066,12 @@ int fvx_qg_budgets(void* plan, int* out) {
   using namespace fvx;
   FVX_CHECK_ARG(plan && out, "null argument");
   QGPlan* Q = (QGPlan*)plan;
-  out[0] = Q->b_iters;
-  out[1] = Q->b_inner;
-  out[2] = Q->b_outer;
+  const QGPlan::Budget b = budget_all(*Q);
+  int inner = 0;
+  for (int o = 0; o < b.outer; ++o) inner = std::max(inner, b.inner[o]);
+  out[0] = b.iters;
+  out[1] = inner;
+  out[2] = b.outer;
   out[3] = Q->recaptures;
   return kOk;
 }
