@@ -9,34 +9,43 @@ Array convention (reference ``grid/cartesian.py:56-80``): row-major
 ``[Ny, Nx] = [j, i]`` including a one-cell ghost ring; T[j,i] centre,
 U[j,i] east face, V[j,i] north face, X[j,i] NE corner.
 
-How each function is pinned (checked by ``tests/test_oracle_*.py``):
+How each function is pinned (each check would fail on a plausible planted
+mistake; ``tests/test_oracle_pins.py`` plants one per family):
 
 * stencils, Difference2D, Divergence2D, Interpolation T_to_X / V_to_T,
   Reconstruction/Advection2D (upwind1, weno5; masked and unmasked),
   Diffusion2D, BiharmonicDiffusion2D: the reference's own golden ``.npz``
-  files (copied to ``tests/golden/``) at rtol 1e-12 / atol 1e-14.
-* WENO3/WENO5/WENO-Z point kernels, TVD limiters: polynomial exactness,
-  mirror identities, limiter closed forms, TVD bounds (reference
-  ``tests/test_weno.py``, ``tests/test_limiters.py``), plus the goldens that
-  route through them.
-* naive/upwind2/upwind3/weno3/wenoz5/TVD fluxes: constant-field zero
-  tendency, linear exactness in the deep interior, mirror symmetry under
-  x -> -x, and agreement with weno5 where the tiers coincide.
+  files (copied to ``tests/golden/``) at rtol 1e-12 / atol 1e-14
+  (``tests/test_oracle_golden.py``).
+* WENO3/WENO5/WENO-Z point kernels: constant and linear exactness, the
+  left/right mirror identity, boundedness at a step, upwind selection, and
+  the formal order of accuracy on cell averages (``test_oracle_pins.py``,
+  after reference ``tests/test_weno.py``).
+* TVD limiters: the reference's closed-form values
+  (``tests/test_limiters.py:11-106``), phi(1/r) = phi(r)/r and the TVD
+  region.
+* naive/upwind1/upwind2/upwind3/weno3/wenoz5/TVD face fluxes: linear
+  exactness, order of accuracy, mirror symmetry under x -> -x, the wall
+  fallbacks; Advection2D: constant fields give zero tendency for both flow
+  signs, masked all-ocean equals unmasked, periodic rk3_ssp_step
+  convergence order (reference ``tests/test_advection.py:116-628``).
 * arakawa_jacobian: J(x,y)=1, J(xy,x^2)=-2x^2, antisymmetry, J(f,f)=0,
-  periodic conservation (reference ``tests/test_jacobian.py:93-230``).
+  constant arguments, periodic conservation of the integral, energy and
+  enstrophy (reference ``tests/test_jacobian.py:93-230``).
 * masked_laplacian and the multigrid operator: assembled as sparse
   matrices and checked for symmetry, against hand stencils, and against
   direct sparse solves; transfers exact on constants
-  (reference ``tests/test_multigrid.py:117-301``).
+  (reference ``tests/test_multigrid.py:117-301``; ``test_oracle_solvers.py``).
 * PCG / flexible PCG: the converged solution equals a direct sparse solve.
   The reference delegates CG to lineax 0.1.0 (not vendored); its iterate
   sequence and stopping rule are parity unpinned, see ``pcg``.
-* rk3_ssp_step: third-order convergence on y' = -y and exactness on
-  polynomials of degree <= 3 in t.
+* rk3_ssp_step: its stability polynomial 1 + z + z^2/2 + z^3/6 on
+  y' = lambda y, third-order convergence on y' = -y^2, and the reference's
+  1e-6 accuracy at dt = 0.01 (``tests/test_timestepping.py:60-106``).
+* enforce_periodic: equal to numpy's wrap padding of the interior.
 * vertical modes: closed forms of reference ``tests/test_vertical_modes.py``.
 * QG step (build-defined composition, no single reference function):
-  pinned through its parts plus conservation of the Arakawa term and the
-  certified FP64 Helmholtz residual.
+  pinned through its parts and against a direct-solve composition.
 """
 
 from __future__ import annotations
