@@ -25,6 +25,13 @@ _LIMITERS = {"minmod": 0, "van_leer": 1, "superbee": 2, "mc": 3}
 _MASK_DISPATCHABLE = frozenset({"weno3", "weno5", "wenoz5"}) | frozenset(_LIMITERS)
 _UNSUPPORTED = frozenset({"weno7", "weno9"})
 
+_vp, _i, _d = _lib._vp, _lib._i, _lib._d
+_lib.extra_signatures({
+    f"fvx_advection_rk3_step{v}_{s}": [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _d, _d, _d,
+                                      _vp]
+    for s in ("f32", "f64") for v in ("",)
+})
+
 
 def method_codes(method: str) -> tuple[int, int]:
     if method in _LIMITERS:
@@ -64,3 +71,27 @@ class Advection2D:
                   v.data_ptr(), out.data_ptr(), _lib.ptr(tx), _lib.ptr(ty), _lib.ptr(mh),
                   ny, nx, float(self.grid.dx), float(self.grid.dy), _lib.stream())
         return out
+
+    def rk3_step(self, h, u, v, dt: float, method: str = "weno5", ng: int = 3):
+        """One ``rk3_ssp_step`` of the doubly periodic tracer ``h``: before
+        each tendency the ghost ring of width ``ng`` is filled periodically
+        (``enforce_periodic``), and the result leaves with its ghosts filled;
+        the same as ``enforce_periodic(rk3_ssp_step(h, lambda q:
+        self(enforce_periodic(q, ng), u, v, method), dt), ng)`` as one native
+        call (``fvx_advection_rk3_step_*``).  Returns a new array."""
+        m, lim = method_codes(method)
+        h = _lib.as_field(h).clone()
+        u = _lib.as_field(u, h.dtype)
+        v = _lib.as_field(v, h.dtype)
+        if h.dim() != 2 or h.shape != u.shape or h.shape != v.shape:
+            raise ValueError("h, u, v must share one 2-D shape")
+        ny, nx = h.shape
+        tx = ty = mh = None
+        if self.mask is not None:
+            mh = self.mask.device("h")
+            if method in _MASK_DISPATCHABLE:
+                tx, ty = self.mask.device("tier_x"), self.mask.device("tier_y")
+        _lib.call(f"fvx_advection_rk3_step_{_lib.suffix(h)}", m, lim, h.data_ptr(), u.data_ptr(),
+                  v.data_ptr(), _lib.ptr(tx), _lib.ptr(ty), _lib.ptr(mh), ny, nx, int(ng),
+                  float(self.grid.dx), float(self.grid.dy), float(dt), _lib.stream())
+        return h

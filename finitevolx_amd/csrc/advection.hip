@@ -10,6 +10,10 @@
 // 2-cell output ring are applied in the same kernel.  HBM traffic per cell:
 // read h, u, v and write the tendency (+ 2 tier bytes + 1 mask byte masked).
 
+#include <algorithm>
+#include <cmath>
+#include <mutex>
+
 #include "fvx_common.hpp"
 
 namespace fvx {
@@ -316,9 +320,132 @@ int advection(int method, int lim, const T* h, const T* u, const T* v, T* out,
   return dispatch_adv<T, false>(method, lim, h, u, v, out, nullptr, nullptr, mh, ny, nx, dx, dy, s);
 }
 
+// ---------------------------------------------------------------------------
+// One rk3_ssp_step (timestepping/explicit_rk.py:74-121) of a doubly periodic
+// tracer: per stage the periodic ghost fill of width ng (boundary.py:52-78,
+// as the periodic advection tests do before each tendency), the Advection2D
+// tendency and the Shu-Osher combination; h advances in place and leaves with
+// its ghosts filled.  Scratch: the two stage states y1, y2.
+// ---------------------------------------------------------------------------
+
+// out = a x + b y + c z elementwise; out may alias x, y or z.
+template <typename T>
+__global__ __launch_bounds__(256) void rk_stage_kernel(int64_t n, T a, const T* x, T b,
+                                                       const T* y, T c, const T* z, T* out) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += stride)
+    out[k] = a * x[k] + b * y[k] + c * z[k];
+}
+
+}  // namespace fvx
+
+extern "C" int fvx_enforce_periodic_f32(float*, int, int, int, void*);
+extern "C" int fvx_enforce_periodic_f64(double*, int, int, int, void*);
+
+namespace fvx {
+
+static int periodic(float* f, int ny, int nx, int ng, hipStream_t s) {
+  return fvx_enforce_periodic_f32(f, ny, nx, ng, s);
+}
+static int periodic(double* f, int ny, int nx, int ng, hipStream_t s) {
+  return fvx_enforce_periodic_f64(f, ny, nx, ng, s);
+}
+
+template <typename T>
+static int rk_stage(int64_t n, double a, const T* x, double b, const T* y, double c, const T* z,
+                    T* out, hipStream_t s) {
+  const int blocks = (int)std::min<int64_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(rk_stage_kernel<T>, dim3(blocks), dim3(256), 0, s, n, (T)a, x, (T)b, y,
+                     (T)c, z, out);
+  FVX_LAUNCH_CHECK();
+  return kOk;
+}
+
+// Argument checks shared by both entry points (no device work).
+template <typename T>
+static int adv_rk3_check(const T* h, const T* u, const T* v, int ny, int nx, int ng, double dx,
+                         double dy, double dt) {
+  FVX_CHECK_ARG(h && u && v, "null field pointer");
+  FVX_CHECK_ARG(ng >= 1 && ny >= 3 * ng && nx >= 3 * ng && ny >= 5 && nx >= 5,
+                "ghost width %d too large for %d x %d", ng, ny, nx);
+  FVX_CHECK_ARG(dx > 0 && dy > 0, "grid spacing must be positive");
+  FVX_CHECK_ARG(std::isfinite(dt), "dt must be finite");
+  return kOk;
+}
+
+template <typename T>
+static int adv_rk3_enqueue(int method, int lim, T* h, const T* u, const T* v,
+                           const uint8_t* tier_x, const uint8_t* tier_y, const uint8_t* mh,
+                           int ny, int nx, int ng, double dx, double dy, double dt, T* y1, T* y2,
+                           hipStream_t s) {
+  const int64_t n = (int64_t)ny * nx;
+  auto L = [&](const T* q, T* k) {
+    return advection<T>(method, lim, q, u, v, k, tier_x, tier_y, mh, ny, nx, dx, dy, s);
+  };
+  int rc;
+  // k1 = L(h) into y1, y1 = h + dt k1
+  if ((rc = periodic(h, ny, nx, ng, s)) || (rc = L(h, y1)) ||
+      (rc = rk_stage<T>(n, 1.0, h, 0.0, h, dt, y1, y1, s)))
+    return rc;
+  // k2 = L(y1) into y2, y2 = 3/4 h + 1/4 y1 + 1/4 dt k2
+  if ((rc = periodic(y1, ny, nx, ng, s)) || (rc = L(y1, y2)) ||
+      (rc = rk_stage<T>(n, 0.75, h, 0.25, y1, 0.25 * dt, y2, y2, s)))
+    return rc;
+  // k3 = L(y2) into y1, h = 1/3 h + 2/3 y2 + 2/3 dt k3
+  if ((rc = periodic(y2, ny, nx, ng, s)) || (rc = L(y2, y1)) ||
+      (rc = rk_stage<T>(n, 1.0 / 3.0, h, 2.0 / 3.0, y2, 2.0 / 3.0 * dt, y1, h, s)))
+    return rc;
+  return periodic(h, ny, nx, ng, s);
+}
+
+// Stage arrays of the allocating entry point: one buffer per device, grown
+// with hipMalloc when a call needs more and kept until the process ends.
+static std::mutex g_stage_mu;
+static void* g_stage_buf[64] = {};
+static size_t g_stage_bytes[64] = {};
+
+template <typename T>
+static int adv_rk3_step(int method, int lim, T* h, const T* u, const T* v,
+                        const uint8_t* tier_x, const uint8_t* tier_y, const uint8_t* mh, int ny,
+                        int nx, int ng, double dx, double dy, double dt, hipStream_t s) {
+  if (int rc = adv_rk3_check<T>(h, u, v, ny, nx, ng, dx, dy, dt)) return rc;
+  const size_t bytes = 2 * (size_t)ny * nx * sizeof(T);
+  int dev = 0;
+  FVX_HIP(hipGetDevice(&dev));
+  FVX_CHECK_ARG(dev >= 0 && dev < 64, "device index %d out of range", dev);
+  T* buf = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_stage_mu);
+    if (g_stage_bytes[dev] < bytes) {
+      // earlier work on any stream may still use the old buffer
+      FVX_HIP(hipDeviceSynchronize());
+      if (g_stage_buf[dev]) FVX_HIP(hipFree(g_stage_buf[dev]));
+      g_stage_buf[dev] = nullptr;
+      g_stage_bytes[dev] = 0;
+      FVX_HIP(hipMalloc(&g_stage_buf[dev], bytes));
+      g_stage_bytes[dev] = bytes;
+    }
+    buf = (T*)g_stage_buf[dev];
+  }
+  return adv_rk3_enqueue<T>(method, lim, h, u, v, tier_x, tier_y, mh, ny, nx, ng, dx, dy, dt,
+                            buf, buf + (size_t)ny * nx, s);
+}
+
 }  // namespace fvx
 
 extern "C" {
+#define FVX_ADV_RK3(SUF, T)                                                                      \
+  int fvx_advection_rk3_step_##SUF(int method, int limiter, T* h, const T* u, const T* v,        \
+                                   const uint8_t* tier_x, const uint8_t* tier_y,                \
+                                   const uint8_t* mask_h, int ny, int nx, int ng, double dx,    \
+                                   double dy, double dt, void* stream) {                        \
+    return fvx::adv_rk3_step<T>(method, limiter, h, u, v, tier_x, tier_y, mask_h, ny, nx, ng,   \
+                                dx, dy, dt, (hipStream_t)stream);                               \
+  }
+FVX_ADV_RK3(f32, float)
+FVX_ADV_RK3(f64, double)
+#undef FVX_ADV_RK3
+
 int fvx_advection2d_f64(int method, int limiter, const double* h, const double* u,
                         const double* v, double* out, const uint8_t* tier_x,
                         const uint8_t* tier_y, const uint8_t* mask_h, int ny, int nx,

@@ -72,6 +72,23 @@ int fvx_advection2d_f64(int method, int limiter, const double* h, const double* 
                         const uint8_t* tier_y, const uint8_t* mask_h, int ny, int nx, double dx,
                         double dy, void* stream);
 
+/* ---- One rk3_ssp_step (timestepping/explicit_rk.py:74-121) of a doubly
+ * periodic tracer with Advection2D.__call__ tendencies (advection.py:329-420):
+ * h (ny, nx; ghost width ng) advances in place; its ghosts are filled
+ * periodically (boundary/boundary.py:52-78) before each tendency, as the
+ * reference's periodic advection tests do (tests/test_advection.py:558-628),
+ * and on exit.  Method / limiter / tiers / mask as fvx_advection2d_*.  The two
+ * stage arrays live in a per-device buffer the library allocates on first
+ * use and grows when a call needs more.                                      */
+int fvx_advection_rk3_step_f32(int method, int limiter, float* h, const float* u,
+                               const float* v, const uint8_t* tier_x, const uint8_t* tier_y,
+                               const uint8_t* mask_h, int ny, int nx, int ng, double dx,
+                               double dy, double dt, void* stream);
+int fvx_advection_rk3_step_f64(int method, int limiter, double* h, const double* u,
+                               const double* v, const uint8_t* tier_x, const uint8_t* tier_y,
+                               const uint8_t* mask_h, int ny, int nx, int ng, double dx,
+                               double dy, double dt, void* stream);
+
 /* ---- Diffusion2D (diffusion/diffusion.py:120-329) --------------------------
  * kappa: device field or NULL (then kappa0 is used everywhere).             */
 int fvx_diffusion2d_f32(const float* h, const float* kappa, double kappa0, float* out, int ny,
