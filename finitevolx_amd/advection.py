@@ -29,7 +29,7 @@ _vp, _i, _d = _lib._vp, _lib._i, _lib._d
 _lib.extra_signatures({
     f"fvx_advection_rk3_step{v}_{s}": [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _d, _d, _d,
                                       _vp]
-    for s in ("f32", "f64") for v in ("",)
+    for s in ("f32", "f64") for v in ("", "_noalloc")
 })
 
 

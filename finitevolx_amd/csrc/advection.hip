@@ -431,6 +431,53 @@ static int adv_rk3_step(int method, int lim, T* h, const T* u, const T* v,
                             buf, buf + (size_t)ny * nx, s);
 }
 
+// The allocation-free variant keeps the two stage arrays in ``scratch``, a
+// device array of kAdvScratchBytes that every GPU holds until the process
+// ends; calls on one GPU share it (overlapping calls on one GPU race).  Host
+// code never uses the array's name as an address: the first call on each
+// device looks its address up with hipGetSymbolAddress (device current) and
+// keeps it per device.
+constexpr size_t kAdvScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) unsigned char scratch[kAdvScratchBytes];
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[64] = {};
+
+static int scratch_ptr(void** out) {
+  int dev = 0;
+  FVX_HIP(hipGetDevice(&dev));
+  FVX_CHECK_ARG(dev >= 0 && dev < 64, "device index %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) {
+      set_error("hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+      return kHipError;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return kOk;
+}
+
+template <typename T>
+static int adv_rk3_step_noalloc(int method, int lim, T* h, const T* u, const T* v,
+                                const uint8_t* tier_x, const uint8_t* tier_y,
+                                const uint8_t* mh, int ny, int nx, int ng, double dx, double dy,
+                                double dt, hipStream_t s) {
+  if (int rc = adv_rk3_check<T>(h, u, v, ny, nx, ng, dx, dy, dt)) return rc;
+  const size_t bytes = 2 * (size_t)ny * nx * sizeof(T);
+  FVX_CHECK_ARG(bytes <= kAdvScratchBytes,
+                "%d x %d needs %zu bytes of stage scratch; the allocation-free step holds "
+                "%zu (two arrays of at most %zu elements)",
+                ny, nx, bytes, kAdvScratchBytes, kAdvScratchBytes / (2 * sizeof(T)));
+  void* p = nullptr;
+  if (int rc = scratch_ptr(&p)) return rc;
+  T* y1 = (T*)p;
+  return adv_rk3_enqueue<T>(method, lim, h, u, v, tier_x, tier_y, mh, ny, nx, ng, dx, dy, dt, y1,
+                            y1 + (size_t)ny * nx, s);
+}
+
 }  // namespace fvx
 
 extern "C" {
@@ -441,6 +488,14 @@ extern "C" {
                                    double dy, double dt, void* stream) {                        \
     return fvx::adv_rk3_step<T>(method, limiter, h, u, v, tier_x, tier_y, mask_h, ny, nx, ng,   \
                                 dx, dy, dt, (hipStream_t)stream);                               \
+  }                                                                                              \
+  int fvx_advection_rk3_step_noalloc_##SUF(int method, int limiter, T* h, const T* u,           \
+                                           const T* v, const uint8_t* tier_x,                   \
+                                           const uint8_t* tier_y, const uint8_t* mask_h,        \
+                                           int ny, int nx, int ng, double dx, double dy,        \
+                                           double dt, void* stream) {                           \
+    return fvx::adv_rk3_step_noalloc<T>(method, limiter, h, u, v, tier_x, tier_y, mask_h, ny,   \
+                                        nx, ng, dx, dy, dt, (hipStream_t)stream);               \
   }
 FVX_ADV_RK3(f32, float)
 FVX_ADV_RK3(f64, double)

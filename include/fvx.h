@@ -88,6 +88,23 @@ int fvx_advection_rk3_step_f64(int method, int limiter, double* h, const double*
                                const double* v, const uint8_t* tier_x, const uint8_t* tier_y,
                                const uint8_t* mask_h, int ny, int nx, int ng, double dx,
                                double dy, double dt, void* stream);
+/* The same step without any allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The two
+ * stage arrays live in a 32 MiB static device array that each GPU holds until
+ * the process ends; 2 * ny * nx * sizeof(element) above 32 MiB is refused with
+ * -1 before any device work.  Calls on one GPU share that array: calls that
+ * overlap on one GPU (other streams, other threads) race.                   */
+int fvx_advection_rk3_step_noalloc_f32(int method, int limiter, float* h, const float* u,
+                                       const float* v, const uint8_t* tier_x,
+                                       const uint8_t* tier_y, const uint8_t* mask_h, int ny,
+                                       int nx, int ng, double dx, double dy, double dt,
+                                       void* stream);
+int fvx_advection_rk3_step_noalloc_f64(int method, int limiter, double* h, const double* u,
+                                       const double* v, const uint8_t* tier_x,
+                                       const uint8_t* tier_y, const uint8_t* mask_h, int ny,
+                                       int nx, int ng, double dx, double dy, double dt,
+                                       void* stream);
 
 /* ---- Diffusion2D (diffusion/diffusion.py:120-329) --------------------------
  * kappa: device field or NULL (then kappa0 is used everywhere).             */
