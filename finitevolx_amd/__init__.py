@@ -7,7 +7,7 @@ SSP-RK3 stepping.  All compute runs in ``libfvx_hip.so``; there is no CPU
 fallback.
 """
 
-from finitevolx_amd.advection import Advection2D
+from finitevolx_amd.advection import Advection2D, Reconstruction2D
 from finitevolx_amd.boundary import enforce_periodic, pad_interior
 from finitevolx_amd.diffusion import BiharmonicDiffusion2D, Diffusion2D, diffusion_2d
 from finitevolx_amd.grid import ArakawaCGrid2D, CartesianGrid2D
@@ -39,7 +39,7 @@ from finitevolx_amd.solvers import (
     solve_poisson_fft,
     streamfunction_from_vorticity,
 )
-from finitevolx_amd.timestepping import rk3_ssp_step
+from finitevolx_amd.timestepping import euler_step, heun_step, rk3_ssp_step, rk4_step
 from finitevolx_amd.vertical import (
     build_coupling_matrix,
     decompose_vertical_modes,
@@ -48,13 +48,13 @@ from finitevolx_amd.vertical import (
 )
 
 __all__ = [
-    "Advection2D", "ArakawaCGrid2D", "BiharmonicDiffusion2D", "CGInfo", "CartesianGrid2D",
+    "Advection2D", "Reconstruction2D", "ArakawaCGrid2D", "BiharmonicDiffusion2D", "CGInfo", "CartesianGrid2D",
     "Difference2D", "Diffusion2D", "Divergence2D", "Interpolation2D", "Mask2D",
     "MultigridLevel", "MultigridSolver", "QGModel", "StencilCapability2D", "arakawa_jacobian",
     "build_coupling_matrix", "build_multigrid_solver", "decompose_vertical_modes",
     "diffusion_2d", "divergence_2d", "enforce_periodic", "interior", "layer_to_mode",
     "make_multigrid_preconditioner", "masked_laplacian", "mode_to_layer", "pad_interior",
-    "pv_inversion", "rk3_ssp_step", "solve_cg", "streamfunction_from_vorticity",
+    "pv_inversion", "euler_step", "heun_step", "rk3_ssp_step", "rk4_step", "solve_cg", "streamfunction_from_vorticity",
     "dst1_eigenvalues", "solve_helmholtz_dct", "solve_helmholtz_dst", "solve_helmholtz_fft",
     "solve_poisson_dct", "solve_poisson_dst", "solve_poisson_fft",
 ]

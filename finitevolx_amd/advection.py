@@ -95,3 +95,83 @@ class Advection2D:
                   v.data_ptr(), _lib.ptr(tx), _lib.ptr(ty), _lib.ptr(mh), ny, nx, int(ng),
                   float(self.grid.dx), float(self.grid.dy), float(dt), _lib.stream())
         return h
+
+
+_lib.extra_signatures({
+    f"fvx_face_flux_{s}": [_i, _i, _i, _vp, _vp, _vp, _vp, _i, _i, _vp] for s in ("f32", "f64")
+})
+
+
+class Reconstruction2D:
+    """Face-value reconstruction times velocity: the east- (``*_x``) and
+    north-face (``*_y``) fluxes of ``Reconstruction2D``
+    (``advection/reconstruction.py:572-1307``), each one HIP kernel, zero
+    ghost ring.  The ``*_masked`` methods pick the stencil per face from the
+    upwind cell's tier of ``mask`` (upwind1 / 3-point / 5-point scheme,
+    ``advection/flux.py:100-253``)."""
+
+    def __init__(self, grid: CartesianGrid2D):
+        self.grid = grid
+
+    def _flux(self, method, dir_, h, vel, limiter="minmod", mask=None):
+        m, lim = (8, _LIMITERS[limiter]) if method == "tvd" else (_RECON[method], 0)
+        if method == "tvd" and limiter not in _LIMITERS:
+            raise ValueError(f"Unknown limiter: {limiter!r}")
+        h = _lib.as_field(h)
+        vel = _lib.as_field(vel, h.dtype)
+        if h.dim() != 2 or h.shape != vel.shape:
+            raise ValueError("h and the face velocity must share one 2-D shape")
+        ny, nx = h.shape
+        tier = None
+        if mask is not None:
+            if tuple(mask.shape) != (ny, nx):
+                raise ValueError(f"mask shape {mask.shape} != field shape {(ny, nx)}")
+            tier = mask.device("tier_x" if dir_ == 0 else "tier_y")
+        out = torch.empty_like(h)
+        _lib.call(f"fvx_face_flux_{_lib.suffix(h)}", m, lim, dir_, h.data_ptr(), vel.data_ptr(),
+                  out.data_ptr(), _lib.ptr(tier), ny, nx, _lib.stream())
+        return out
+
+    def tvd_x(self, h, u, limiter: str = "minmod"):
+        return self._flux("tvd", 0, h, u, limiter)
+
+    def tvd_y(self, h, v, limiter: str = "minmod"):
+        return self._flux("tvd", 1, h, v, limiter)
+
+    def tvd_x_masked(self, h, u, mask: Mask2D, limiter: str = "minmod"):
+        return self._flux("tvd", 0, h, u, limiter, mask)
+
+    def tvd_y_masked(self, h, v, mask: Mask2D, limiter: str = "minmod"):
+        return self._flux("tvd", 1, h, v, limiter, mask)
+
+
+# Reconstruction codes of the face-flux kernel (FVX_ADV_* in include/fvx.h).
+_RECON = {"naive": 0, "upwind1": 1, "upwind2": 2, "upwind3": 3, "weno3": 4, "wenoz3": 5,
+          "weno5": 6, "wenoz5": 7}
+
+
+def _add_recon_methods():
+    for name in _RECON:
+        for d, axis in ((0, "x"), (1, "y")):
+            def f(self, h, vel, _n=name, _d=d):
+                return self._flux(_n, _d, h, vel)
+            f.__name__ = f"{name}_{axis}"
+            f.__doc__ = (f"{'East' if d == 0 else 'North'}-face {name} flux "
+                         f"(reconstruction.py, Reconstruction2D.{name}_{axis}).")
+            setattr(Reconstruction2D, f.__name__, f)
+    for name in ("weno5", "wenoz5"):
+        for d, axis in ((0, "x"), (1, "y")):
+            def g(self, h, vel, mask, _n=name, _d=d):
+                return self._flux(_n, _d, h, vel, mask=mask)
+            g.__name__ = f"{name}_{axis}_masked"
+            g.__doc__ = f"Tiered {name} flux (Reconstruction2D.{name}_{axis}_masked)."
+            setattr(Reconstruction2D, g.__name__, g)
+    for name in ("weno7", "weno9"):
+        for axis in ("x", "y"):
+            def n(self, *a, _n=name, **k):
+                raise NotImplementedError(f"{_n} is outside this build's hot path")
+            n.__name__ = f"{name}_{axis}"
+            setattr(Reconstruction2D, n.__name__, n)
+
+
+_add_recon_methods()

@@ -321,6 +321,113 @@ int advection(int method, int lim, const T* h, const T* u, const T* v, T* out,
 }
 
 // ---------------------------------------------------------------------------
+// Face fluxes of one direction (Reconstruction2D.<method>_x / _y and the
+// *_masked variants, reconstruction.py:582-1307): every face 1..n-2 along the
+// direction on rows (columns) 1..m-2, zero ring.  Faces the tendency never
+// reads fall back as the reference does: at the last face n-2 positive flow
+// takes the 3-point scheme in place of the 5-point one and negative flow
+// takes upwind1 (the value beyond the face) for every scheme but naive.
+// One thread per face; its six stencil values are loaded into registers
+// (zero beyond the array).
+// ---------------------------------------------------------------------------
+template <typename T, int M, int L, bool MASKED>
+__device__ __forceinline__ T face_flux_any(const T* s, int fi, int n, T vel, int tier) {
+  if (fi == n - 2 && M != kNaive) {
+    const bool pos = vel >= T(0);
+    if (!pos) {
+      if (MASKED && tier < 2) return T(0);
+      return s[1] * vel;
+    }
+    if constexpr (M == kWeno5 || M == kWenoZ5) {
+      constexpr int M3 = (M == kWeno5) ? kWeno3 : kWenoZ3;
+      if (MASKED && tier < 2) return T(0);
+      if (MASKED && tier == 2) return s[0] * vel;
+      return face_value<T, M3, L>(s, 1, fi, n, true) * vel;
+    }
+  }
+  return face_flux<T, M, L, MASKED>(s, 1, fi, n, vel, tier);
+}
+
+template <typename T, int M, int L, bool MASKED, bool YDIR>
+__global__ __launch_bounds__(256) void face_flux_kernel(const T* __restrict__ h,
+                                                        const T* __restrict__ vel,
+                                                        T* __restrict__ out,
+                                                        const uint8_t* __restrict__ tier, int ny,
+                                                        int nx) {
+  const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (i >= nx || j >= ny) return;
+  const int64_t c = (int64_t)j * nx + i;
+  const int fi = YDIR ? j : i, n = YDIR ? ny : nx;
+  const int oi = YDIR ? i : j, m = YDIR ? nx : ny;
+  const int64_t st = YDIR ? nx : 1;
+  T f = T(0);
+  if (fi >= 1 && fi <= n - 2 && oi >= 1 && oi <= m - 2) {
+    T w[6];
+#pragma unroll
+    for (int k = -2; k <= 3; ++k) {
+      const int q = fi + k;
+      w[k + 2] = (q >= 0 && q < n) ? h[c + k * st] : T(0);
+    }
+    const T v = vel[c];
+    int t = 6;
+    if constexpr (MASKED) t = (v >= T(0)) ? tier[c] : tier[c + st];
+    f = face_flux_any<T, M, L, MASKED>(w + 2, fi, n, v, t);
+  }
+  out[c] = f;
+}
+
+template <typename T, int M, int L, bool MASKED>
+static void launch_face_flux(int dir, const T* h, const T* vel, T* out, const uint8_t* tier,
+                             int ny, int nx, hipStream_t s) {
+  const dim3 g(cdiv(nx, 64), cdiv(ny, 4));
+  if (dir == 0)
+    hipLaunchKernelGGL((face_flux_kernel<T, M, L, MASKED, false>), g, dim3(256), 0, s, h, vel,
+                       out, tier, ny, nx);
+  else
+    hipLaunchKernelGGL((face_flux_kernel<T, M, L, MASKED, true>), g, dim3(256), 0, s, h, vel, out,
+                       tier, ny, nx);
+}
+
+template <typename T>
+int face_flux_call(int method, int lim, int dir, const T* h, const T* vel, T* out,
+                   const uint8_t* tier, int ny, int nx, hipStream_t s) {
+  FVX_CHECK_ARG(h && vel && out, "null field pointer");
+  FVX_CHECK_ARG(dir == 0 || dir == 1, "dir must be 0 (x, east faces) or 1 (y, north faces)");
+  FVX_CHECK_ARG(ny >= 3 && nx >= 3, "array too small (%d x %d)", ny, nx);
+  FVX_CHECK_ARG(lim >= 0 && lim <= 3, "unknown limiter %d", lim);
+#define FVX_FF(M, L)                                                                 \
+  (tier ? launch_face_flux<T, M, L, true>(dir, h, vel, out, tier, ny, nx, s)          \
+        : launch_face_flux<T, M, L, false>(dir, h, vel, out, tier, ny, nx, s))
+  if (tier) {
+    FVX_CHECK_ARG(method == kWeno3 || method == kWeno5 || method == kWenoZ5 || method == kTvd,
+                  "method %d does not support mask-based stencil dispatch", method);
+  }
+  switch (method) {
+    case kNaive: launch_face_flux<T, kNaive, 0, false>(dir, h, vel, out, nullptr, ny, nx, s); break;
+    case kUpwind1: launch_face_flux<T, kUpwind1, 0, false>(dir, h, vel, out, nullptr, ny, nx, s); break;
+    case kUpwind2: launch_face_flux<T, kUpwind2, 0, false>(dir, h, vel, out, nullptr, ny, nx, s); break;
+    case kUpwind3: launch_face_flux<T, kUpwind3, 0, false>(dir, h, vel, out, nullptr, ny, nx, s); break;
+    case kWenoZ3: launch_face_flux<T, kWenoZ3, 0, false>(dir, h, vel, out, nullptr, ny, nx, s); break;
+    case kWeno3: FVX_FF(kWeno3, 0); break;
+    case kWeno5: FVX_FF(kWeno5, 0); break;
+    case kWenoZ5: FVX_FF(kWenoZ5, 0); break;
+    case kTvd:
+      switch (lim) {
+        case kMinmod: FVX_FF(kTvd, kMinmod); break;
+        case kVanLeer: FVX_FF(kTvd, kVanLeer); break;
+        case kSuperbee: FVX_FF(kTvd, kSuperbee); break;
+        default: FVX_FF(kTvd, kMC); break;
+      }
+      break;
+    default: set_error("unknown advection method %d", method); return kInvalidArgument;
+  }
+#undef FVX_FF
+  FVX_LAUNCH_CHECK();
+  return kOk;
+}
+
+// ---------------------------------------------------------------------------
 // One rk3_ssp_step (timestepping/explicit_rk.py:74-121) of a doubly periodic
 // tracer: per stage the periodic ghost fill of width ng (boundary.py:52-78,
 // as the periodic advection tests do before each tendency), the Advection2D
@@ -500,6 +607,17 @@ extern "C" {
 FVX_ADV_RK3(f32, float)
 FVX_ADV_RK3(f64, double)
 #undef FVX_ADV_RK3
+
+int fvx_face_flux_f64(int method, int limiter, int dir, const double* h, const double* vel,
+                      double* out, const uint8_t* tier, int ny, int nx, void* stream) {
+  return fvx::face_flux_call<double>(method, limiter, dir, h, vel, out, tier, ny, nx,
+                                     (hipStream_t)stream);
+}
+int fvx_face_flux_f32(int method, int limiter, int dir, const float* h, const float* vel,
+                      float* out, const uint8_t* tier, int ny, int nx, void* stream) {
+  return fvx::face_flux_call<float>(method, limiter, dir, h, vel, out, tier, ny, nx,
+                                    (hipStream_t)stream);
+}
 
 int fvx_advection2d_f64(int method, int limiter, const double* h, const double* u,
                         const double* v, double* out, const uint8_t* tier_x,

@@ -2189,19 +2189,20 @@ static T* smooth_chunks(const Plan& P, const LevelDev<T>& L, const T* u_in, bool
   return dst;
 }
 
-// One V-cycle z = V(u0 or 0, rhs) at level 0.  With ``az`` given, the last
-// kernel also writes az = A z and the PCG dots against rhs and w (kUpCG) when
-// it can (two or fewer post-sweeps on a tiled level 0); *fused says whether
-// it did.
+// One V-cycle z = V(u0 or 0, rhs) from level l0 (rhs, u0 and out have that
+// level's shape; MultigridSolver.v_cycle(u, rhs, level_idx), multigrid.py:
+// 808-883).  With ``az`` given (level 0 only), the last kernel also writes
+// az = A z and the PCG dots against rhs and w (kUpCG) when it can (two or
+// fewer post-sweeps on a tiled level 0); *fused says whether it did.
 template <typename T, int KIND>
 static int v_cycle_impl(Plan& P, const T* u0, const T* rhs, T* out, T* az, const T* w,
                         PcgScalars* sc, hipStream_t s, bool* fused,
-                        const PcgUpdArgs<T>* upd = nullptr) {
+                        const PcgUpdArgs<T>* upd = nullptr, int l0 = 0) {
   Hier<T>& H = hier<T>(P);
   const int nl = P.nl, lc = P.lc;
   if (fused) *fused = false;
-  if (lc == 0) {
-    const LevelDev<T>& L = H.lv[0];
+  if (l0 >= lc) {
+    const LevelDev<T>& L = H.lv[l0];
     const size_t bytes = sizeof(T) * (size_t)L.ny * L.nx;
     if ((int64_t)L.ny * L.nx > kCoarseCells) {
       // One large level (grid not divisible): the V-cycle is n_coarse sweeps.
@@ -2214,13 +2215,13 @@ static int v_cycle_impl(Plan& P, const T* u0, const T* rhs, T* out, T* az, const
       FVX_LAUNCH_CHECK();
       return kOk;
     }
-    // Whole hierarchy in the single-workgroup kernel.
+    // Levels l0 .. nl-1 in the single-workgroup kernel.
     FVX_HIP(hipMemcpyAsync(L.b, rhs, bytes, hipMemcpyDeviceToDevice, s));
     if (u0) FVX_HIP(hipMemcpyAsync(L.u, u0, bytes, hipMemcpyDeviceToDevice, s));
     CoarseArgs<T> ca;
     std::memset(&ca, 0, sizeof(ca));
     for (int l = 0; l < nl; ++l) ca.lv[l] = H.lv[l];
-    ca.lc = 0;
+    ca.lc = l0;
     ca.nl = nl;
     ca.n_pre = P.n_pre;
     ca.n_post = P.n_post;
@@ -2233,13 +2234,13 @@ static int v_cycle_impl(Plan& P, const T* u0, const T* rhs, T* out, T* az, const
     FVX_LAUNCH_CHECK();
     return kOk;
   }
-  // Down leg over the tiled levels 0 .. lc-1: result iterate in L.u.
-  for (int l = 0; l < lc; ++l) {
+  // Down leg over the tiled levels l0 .. lc-1: result iterate in L.u.
+  for (int l = l0; l < lc; ++l) {
     const LevelDev<T>& L = H.lv[l];
     const LevelDev<T>& C = H.lv[l + 1];
-    const T* b = (l == 0) ? rhs : L.b;
-    bool zero = (l > 0) || (u0 == nullptr);
-    const T* src = (l == 0) ? u0 : nullptr;
+    const T* b = (l == l0) ? rhs : L.b;
+    bool zero = (l > l0) || (u0 == nullptr);
+    const T* src = (l == l0) ? u0 : nullptr;
     const int npre = P.n_pre;
     const int fz = npre < kMaxFuse ? npre : kMaxFuse;  // fused with residual + restriction
     if (l == 0 && upd) {
@@ -2287,19 +2288,19 @@ static int v_cycle_impl(Plan& P, const T* u0, const T* rhs, T* out, T* az, const
     ca.done = sc ? &sc->done : nullptr;
     launch_coarse<T, KIND>(ca, s);
   }
-  // Up leg over the tiled levels lc-1 .. 0.  cur = array holding level l+1's
+  // Up leg over the tiled levels lc-1 .. l0.  cur = array holding level l+1's
   // corrected iterate; each up kernel reads L.u and writes a different array.
   T* cur = (lc < nl) ? H.lv[lc].u : nullptr;
-  for (int l = lc - 1; l >= 0; --l) {
+  for (int l = lc - 1; l >= l0; --l) {
     const LevelDev<T>& L = H.lv[l];
     LevelDev<T> C = H.lv[l + 1];
     C.u = cur;
-    const T* b = (l == 0) ? rhs : L.b;
+    const T* b = (l == l0) ? rhs : L.b;
     const int npost = P.n_post;
     const int fz = npost < kMaxFuse ? npost : kMaxFuse;
     const int tail = npost - fz;
     const bool dk = az && l == 0 && tail == 0;
-    T* dst = (l == 0 && tail == 0) ? out : L.t;
+    T* dst = (l == l0 && tail == 0) ? out : L.t;
     if (dk) {
       if (fz == 3) launch_tile<T, KIND, 3, kUpCG, false>(P, L, &C, L.u, b, dst, w, sc, s, az);
       else if (fz == 2) launch_tile<T, KIND, 2, kUpCG, false>(P, L, &C, L.u, b, dst, w, sc, s, az);
@@ -2313,7 +2314,7 @@ static int v_cycle_impl(Plan& P, const T* u0, const T* rhs, T* out, T* az, const
       else launch_tile<T, KIND, 0, kUp, false>(P, L, &C, L.u, b, dst, nullptr, sc, s);
     }
     if (tail > 0) {
-      dst = smooth_chunks<T, KIND>(P, L, dst, false, b, tail, L.t2, L.t3, l == 0 ? out : nullptr,
+      dst = smooth_chunks<T, KIND>(P, L, dst, false, b, tail, L.t2, L.t3, l == l0 ? out : nullptr,
                                    sc, s);
     }
     cur = dst;
@@ -2323,10 +2324,13 @@ static int v_cycle_impl(Plan& P, const T* u0, const T* rhs, T* out, T* az, const
 }
 
 template <typename T>
-static int v_cycle(Plan& P, const T* u0, const T* rhs, T* out, PcgScalars* sc, hipStream_t s) {
+static int v_cycle(Plan& P, const T* u0, const T* rhs, T* out, PcgScalars* sc, hipStream_t s,
+                   int l0 = 0) {
   return P.kind == kDirichlet
-             ? v_cycle_impl<T, kDirichlet>(P, u0, rhs, out, nullptr, nullptr, sc, s, nullptr)
-             : v_cycle_impl<T, kNoflux>(P, u0, rhs, out, nullptr, nullptr, sc, s, nullptr);
+             ? v_cycle_impl<T, kDirichlet>(P, u0, rhs, out, nullptr, nullptr, sc, s, nullptr,
+                                           nullptr, l0)
+             : v_cycle_impl<T, kNoflux>(P, u0, rhs, out, nullptr, nullptr, sc, s, nullptr,
+                                        nullptr, l0);
 }
 
 static dim3 grid2(int ny, int nx) { return dim3(cdiv(nx, 64), cdiv(ny, 4)); }
@@ -3160,7 +3164,17 @@ int fvx_mg_plan_info(void* plan, int* n_levels, int* shapes, int* coarse_level) 
     FVX_CHECK_ARG(P->has##HN, "plan was not built for this precision");                          \
     FVX_CHECK_ARG(out != rhs && (u == nullptr || out != u), "out must not alias inputs");        \
     hipStream_t s = (hipStream_t)stream;                                                         \
-    return v_cycle<T>(*P, u, rhs, out, nullptr, s);                                                                             \
+    return v_cycle<T>(*P, u, rhs, out, nullptr, s);                                              \
+  }                                                                                              \
+  int fvx_mg_v_cycle_level_##SUF(void* plan, int level, const T* u, const T* rhs, T* out,       \
+                                 void* stream) {                                                 \
+    using namespace fvx;                                                                         \
+    FVX_CHECK_ARG(plan && rhs && out, "null argument");                                          \
+    Plan* P = (Plan*)plan;                                                                       \
+    FVX_CHECK_ARG(P->has##HN, "plan was not built for this precision");                          \
+    FVX_CHECK_ARG(level >= 0 && level < P->nl, "level %d outside 0..%d", level, P->nl - 1);      \
+    FVX_CHECK_ARG(out != rhs && (u == nullptr || out != u), "out must not alias inputs");        \
+    return v_cycle<T>(*P, u, rhs, out, nullptr, (hipStream_t)stream, level);                     \
   }                                                                                              \
   int fvx_mg_pcg_solve_##SUF(void* plan, const T* rhs, int64_t ld_rhs, T* x, int64_t ld_x,       \
                              double rtol, double atol, int max_iters, int use_x0,                \

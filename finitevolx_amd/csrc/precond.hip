@@ -1,6 +1,6 @@
 // Preconditioner choice for the mixed-precision PCG Helmholtz solve: the
 // multigrid V-cycle (the reference's make_multigrid_preconditioner,
-// solvers/preconditioners.py:86-134), the learned U-Net (as the coarse solve
+// solvers/preconditioners.py:207-244), the learned U-Net (as the coarse solve
 // of a two-grid cycle), or the diagonal (Jacobi, z = r / diag(A) on wet
 // cells).  The diagonal option is the
 // baseline a learned preconditioner has to beat; the V-cycle is the one it

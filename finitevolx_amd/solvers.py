@@ -58,6 +58,7 @@ for _s in ("f32", "f64"):
         f"fvx_mg_restrict_{_s}": [_vp, _i, _vp, _vp, _vp],
         f"fvx_mg_prolongate_{_s}": [_vp, _i, _vp, _vp, _vp],
         f"fvx_mg_v_cycle_{_s}": [_vp, _vp, _vp, _vp, _vp],
+        f"fvx_mg_v_cycle_level_{_s}": [_vp, _i, _vp, _vp, _vp, _vp],
         f"fvx_mg_pcg_solve_{_s}": [_vp, _vp, _i64, _vp, _i64, _d, _d, _i, _i,
                                   C.POINTER(_CgInfoC), _vp],
     })
@@ -181,14 +182,21 @@ class MultigridSolver:
         return t
 
     def v_cycle(self, u, rhs, level_idx: int = 0):
-        """One V-cycle from ``u`` (``multigrid.py:808-883``); level 0 only."""
-        if level_idx != 0:
-            raise NotImplementedError("v_cycle runs from the finest level")
-        rhs = self._field(rhs)
-        u = None if u is None else self._field(u)
+        """One V-cycle from ``u`` (``None`` = zero) on level ``level_idx``
+        down to the coarsest (``multigrid.py:808-883``); ``u`` and ``rhs``
+        have that level's shape."""
+        if not 0 <= level_idx < self.n_levels:
+            raise ValueError(f"level_idx must be in [0, {self.n_levels - 1}], got {level_idx}")
+        shape = self.levels[level_idx].shape
+        rhs = self._field(rhs, shape)
+        u = None if u is None else self._field(u, shape)
         out = torch.empty_like(rhs)
-        _lib.call(f"fvx_mg_v_cycle_{self._suffix(rhs)}", self._h, _lib.ptr(u), rhs.data_ptr(),
-                  out.data_ptr(), _lib.stream())
+        if level_idx == 0:
+            _lib.call(f"fvx_mg_v_cycle_{self._suffix(rhs)}", self._h, _lib.ptr(u),
+                      rhs.data_ptr(), out.data_ptr(), _lib.stream())
+        else:
+            _lib.call(f"fvx_mg_v_cycle_level_{self._suffix(rhs)}", self._h, int(level_idx),
+                      _lib.ptr(u), rhs.data_ptr(), out.data_ptr(), _lib.stream())
         return out
 
     def _run_vcycles(self, rhs):

@@ -1,4 +1,6 @@
-"""Explicit SSP-RK3 stepping (``timestepping/explicit_rk.py:74-121``).
+"""Explicit Runge-Kutta stepping (``timestepping/explicit_rk.py``):
+``euler_step`` (:22-40), ``heun_step`` (:43-71), ``rk3_ssp_step`` (:74-121)
+and ``rk4_step`` (:124-168).
 
 The stage combinations run as one fused HIP kernel per leaf; ``state`` may
 be a tensor or a (nested) tuple / list / dict of tensors, like the
@@ -49,3 +51,27 @@ def rk3_ssp_step(state, rhs_fn, dt: float):
     k3 = rhs_fn(y2)
     return _tree_map(lambda y, a, f: lincomb(1.0 / 3.0, y, 2.0 / 3.0, a, 2.0 / 3.0 * dt, f),
                      state, y2, k3)
+
+
+def euler_step(state, rhs_fn, dt: float):
+    """Forward Euler: ``y + dt f(y)``."""
+    k1 = rhs_fn(state)
+    return _tree_map(lambda y, f: lincomb(1.0, y, dt, f), state, k1)
+
+
+def heun_step(state, rhs_fn, dt: float):
+    """Heun (RK2): ``y* = y + dt k1``, ``y + dt/2 (k1 + k2)`` with ``k2 = f(y*)``."""
+    k1 = rhs_fn(state)
+    ys = _tree_map(lambda y, f: lincomb(1.0, y, dt, f), state, k1)
+    k2 = rhs_fn(ys)
+    return _tree_map(lambda y, f1, f2: lincomb(1.0, y, 0.5 * dt, f1, 0.5 * dt, f2), state, k1, k2)
+
+
+def rk4_step(state, rhs_fn, dt: float):
+    """Classical RK4: ``y + dt/6 (k1 + 2 k2 + 2 k3 + k4)``."""
+    k1 = rhs_fn(state)
+    k2 = rhs_fn(_tree_map(lambda y, f: lincomb(1.0, y, 0.5 * dt, f), state, k1))
+    k3 = rhs_fn(_tree_map(lambda y, f: lincomb(1.0, y, 0.5 * dt, f), state, k2))
+    k4 = rhs_fn(_tree_map(lambda y, f: lincomb(1.0, y, dt, f), state, k3))
+    part = _tree_map(lambda y, a, b: lincomb(1.0, y, dt / 6.0, a, dt / 3.0, b), state, k1, k2)
+    return _tree_map(lambda y, a, b: lincomb(1.0, y, dt / 3.0, a, dt / 6.0, b), part, k3, k4)

@@ -106,6 +106,17 @@ int fvx_advection_rk3_step_noalloc_f64(int method, int limiter, double* h, const
                                        int nx, int ng, double dx, double dy, double dt,
                                        void* stream);
 
+/* ---- Reconstruction2D face fluxes (advection/reconstruction.py:572-1307) ---
+ * <method>_x (dir 0, east faces) / <method>_y (dir 1, north faces): every face
+ * 1..n-2 along the direction on the interior of the other, zero ring; method /
+ * limiter codes as above (FVX_ADV_WENOZ3 included).  tier (nullable): the
+ * direction's stencil tiers for the *_masked variants (weno5, wenoz5, TVD;
+ * also weno3), selected by the upwind cell (flux.py:100-253).                */
+int fvx_face_flux_f32(int method, int limiter, int dir, const float* h, const float* vel,
+                      float* out, const uint8_t* tier, int ny, int nx, void* stream);
+int fvx_face_flux_f64(int method, int limiter, int dir, const double* h, const double* vel,
+                      double* out, const uint8_t* tier, int ny, int nx, void* stream);
+
 /* ---- Diffusion2D (diffusion/diffusion.py:120-329) --------------------------
  * kappa: device field or NULL (then kappa0 is used everywhere).             */
 int fvx_diffusion2d_f32(const float* h, const float* kappa, double kappa0, float* out, int ny,
@@ -193,6 +204,12 @@ int fvx_mg_prolongate_f64(void* plan, int level, const double* vc, double* out, 
 int fvx_mg_v_cycle_f32(void* plan, const float* u, const float* rhs, float* out, void* stream);
 int fvx_mg_v_cycle_f64(void* plan, const double* u, const double* rhs, double* out,
                        void* stream);
+/* MultigridSolver.v_cycle(u, rhs, level_idx) (multigrid.py:808-883) from a
+ * coarser level: u (NULL = 0), rhs and out have that level's shape.          */
+int fvx_mg_v_cycle_level_f32(void* plan, int level, const float* u, const float* rhs,
+                             float* out, void* stream);
+int fvx_mg_v_cycle_level_f64(void* plan, int level, const double* u, const double* rhs,
+                             double* out, void* stream);
 
 typedef struct fvx_cg_info {
   int iterations;       /* inner PCG iterations (last refinement sweep for mixed) */
@@ -221,7 +238,7 @@ int fvx_mg_mixed_solve(void* plan, const double* rhs, int64_t ld_rhs, double* x,
                        fvx_cg_info* info, void* stream);
 /* The same mixed solve with the inner preconditioner chosen by ``precond``:
  * 0 multigrid V-cycle (make_multigrid_preconditioner,
- * solvers/preconditioners.py:86-134), 1 learned U-Net ``net`` (from
+ * solvers/preconditioners.py:207-244), 1 learned U-Net ``net`` (from
  * fvx_unet_create; no reference counterpart), 2 diagonal (Jacobi).
  * out_scale <= 0 selects 1 / diag(A).  The learned preconditioner is a
  * two-grid cycle whose coarse solve is the network on multigrid level 1:

@@ -955,6 +955,27 @@ def dirichlet_helmholtz_direct(rhs, mask, dx, dy, lambda_):
 # ---------------------------------------------------------------------------
 
 
+def euler_step(state, rhs_fn, dt):
+    """``timestepping/explicit_rk.py:22-40``."""
+    return state + dt * rhs_fn(state)
+
+
+def heun_step(state, rhs_fn, dt):
+    """``timestepping/explicit_rk.py:43-71`` (Heun / RK2)."""
+    k1 = rhs_fn(state)
+    k2 = rhs_fn(state + dt * k1)
+    return state + 0.5 * dt * (k1 + k2)
+
+
+def rk4_step(state, rhs_fn, dt):
+    """``timestepping/explicit_rk.py:124-168`` (classical RK4)."""
+    k1 = rhs_fn(state)
+    k2 = rhs_fn(state + 0.5 * dt * k1)
+    k3 = rhs_fn(state + 0.5 * dt * k2)
+    k4 = rhs_fn(state + dt * k3)
+    return state + dt / 6.0 * (k1 + 2.0 * k2 + 2.0 * k3 + k4)
+
+
 def rk3_ssp_step(state, rhs_fn, dt):
     """``timestepping/explicit_rk.py:74-121`` (Shu-Osher SSP-RK3)."""
     k1 = rhs_fn(state)
@@ -1085,3 +1106,67 @@ def qg_step(q, p, F, wet, solver=None):
     def rhs(qs):
         return qg_rhs(qs, qg_invert(qs, p, wet, solver), p, F, wet)
     return rk3_ssp_step(q, rhs, p.dt)
+
+
+# ---------------------------------------------------------------------------
+# The reference's 1.5-layer QG example (scripts/qg_1p5_layer.py): upwind1
+# advection, geostrophic velocities through X points, DST Helmholtz
+# inversion (here the exact sparse solve of the same Dirichlet operator),
+# Heun stepping.
+# ---------------------------------------------------------------------------
+
+
+class QGScriptParams:
+    """``QuasiGeostrophicConfig`` defaults (``scripts/qg_1p5_layer.py:280-300``)."""
+
+    def __init__(self, nx=64, ny=64, Lx=5.12e6, Ly=5.12e6, beta=1.754e-11, rossby_radius=4.0e4,
+                 drag=5.0e-8, viscosity=5.0e4, wind_curl_forcing=2.0e-12, dt=4000.0):
+        self.nx, self.ny, self.Lx, self.Ly = nx, ny, Lx, Ly
+        self.dx, self.dy = Lx / nx, Ly / ny
+        self.beta, self.drag, self.viscosity, self.dt = beta, drag, viscosity, dt
+        self.lambda_ = 1.0 / rossby_radius**2
+        self.F0 = wind_curl_forcing
+
+    def fields(self):
+        """q0 and wind curl with zero ghost rings (``:332-371``)."""
+        x = (np.arange(self.nx) + 0.5) * self.dx
+        y = (np.arange(self.ny) + 0.5) * self.dy
+        X, Y = np.meshgrid(x, y)
+        q0 = np.pad(5.0e-9 * np.sin(2 * np.pi * X / self.Lx) * np.sin(np.pi * Y / self.Ly), 1)
+        wind = np.pad(-self.F0 * np.sin(2 * np.pi * Y / self.Ly), 1)
+        return q0, wind
+
+
+def qg_script_invert(q, p):
+    """``invert_streamfunction`` (``:432-452``)."""
+    psi = np.zeros_like(q)
+    psi[1:-1, 1:-1] = dirichlet_helmholtz_direct(q[1:-1, 1:-1], np.ones((p.ny, p.nx)), p.dx,
+                                                 p.dy, p.lambda_)
+    return psi
+
+
+def qg_script_velocity(psi, p):
+    """``geostrophic_velocity_from_streamfunction`` (``:115-145``)."""
+    px = T_to_X(psi)
+    return -diff_y_X_to_U(px, p.dy), diff_x_X_to_V(px, p.dx)
+
+
+def qg_script_tendency(q, p, wind):
+    """``tendency`` (``:454-478``): (q_rhs, psi, u, v)."""
+    psi = qg_script_invert(q, p)
+    u, v = qg_script_velocity(psi, p)
+    q_rhs = advection_2d(q, u, v, p.dx, p.dy, "upwind1")
+    q_rhs[1:-1, 1:-1] += -p.beta * V_to_T(v)[1:-1, 1:-1]
+    q_rhs = q_rhs + wind - p.drag * laplacian(psi, p.dx, p.dy) + \
+        p.viscosity * laplacian(q, p.dx, p.dy)
+    return q_rhs, psi, u, v
+
+
+def qg_script_step(q, p, wind):
+    """``step`` (``:484-497``): Heun with the wall condition before each
+    tendency and after the step; returns (q, psi, u, v)."""
+    bc = lambda f: np.pad(f[1:-1, 1:-1], 1)  # noqa: E731  (pad_interior, zero ring)
+    qn = bc(heun_step(q, lambda s: qg_script_tendency(bc(s), p, wind)[0], p.dt))
+    psi = qg_script_invert(qn, p)
+    u, v = qg_script_velocity(psi, p)
+    return qn, psi, u, v

@@ -82,3 +82,71 @@ def test_rk3_step_rejects_bad_arguments():
         adv.rk3_step(h, h, h, 0.1, "weno5", ng=5)
     with pytest.raises(ValueError):
         adv.rk3_step(h, h, h, 0.1, "bogus")
+
+
+RECON = ["naive", "upwind1", "upwind2", "upwind3", "weno3", "wenoz3", "weno5", "wenoz5"]
+
+
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-12), (torch.float32, 1e-5)])
+def test_reconstruction2d_face_fluxes(dtype, tol):
+    """Every Reconstruction2D.<method>_x / _y (and tvd_x / tvd_y with each
+    limiter) on random data with flows of both signs, faces 1..n-2 including
+    the wall fallbacks, against the oracle's face fluxes."""
+    need_gpu()
+    ny, nx = 23, 37
+    rng = np.random.default_rng(13)
+    h = rng.standard_normal((ny, nx))
+    u, v = rng.standard_normal((2, ny, nx))
+    rec = fx.Reconstruction2D(fx.CartesianGrid2D.from_interior(nx - 2, ny - 2, 1.0, 1.0))
+    t = lambda a: torch.tensor(a, dtype=dtype)  # noqa: E731
+    for m in RECON:
+        assert_close_scaled(to_np(getattr(rec, f"{m}_x")(t(h), t(u))), fvx.face_flux_x(h, u, m),
+                            tol, f"{m}_x")
+        assert_close_scaled(to_np(getattr(rec, f"{m}_y")(t(h), t(v))), fvx.face_flux_y(h, v, m),
+                            tol, f"{m}_y")
+    for lim in ("minmod", "van_leer", "superbee", "mc"):
+        assert_close_scaled(to_np(rec.tvd_x(t(h), t(u), lim)), fvx.face_flux_x(h, u, "tvd", lim),
+                            tol, f"tvd_x {lim}")
+        assert_close_scaled(to_np(rec.tvd_y(t(h), t(v), lim)), fvx.face_flux_y(h, v, "tvd", lim),
+                            tol, f"tvd_y {lim}")
+
+
+def test_reconstruction2d_masked_face_fluxes():
+    """The *_masked variants (reconstruction.py:1104-1307) against the
+    oracle's tiered upwind_flux on a ragged mask."""
+    need_gpu()
+    n = 40
+    rng = np.random.default_rng(14)
+    h = rng.standard_normal((n, n))
+    u, v = rng.standard_normal((2, n, n))
+    wet = random_mask(n, n, 15)
+    om = fvx.Mask2D(wet)
+    pm = fx.Mask2D(wet)
+    rec = fx.Reconstruction2D(fx.CartesianGrid2D.from_interior(n - 2, n - 2, 1.0, 1.0))
+    t = torch.tensor
+    for name, meth, lim in (("weno5", "weno5", None), ("wenoz5", "wenoz5", None),
+                            ("tvd", "van_leer", "van_leer")):
+        funcs = fvx._tier_funcs(meth)
+        sizes = tuple(sorted(funcs))
+        for dim, axis, vel in ((1, "x", u), (0, "y", v)):
+            hier = fvx.narrow_mask_hierarchy(om.get_adaptive_masks(axis, (2, 4, 6)), sizes)
+            ref = fvx.upwind_flux(h, vel, dim, funcs, hier)
+            f = getattr(rec, f"{name}_{axis}_masked")
+            got = f(t(h), t(vel), pm, lim) if lim else f(t(h), t(vel), pm)
+            assert_close_scaled(to_np(got), ref, 1e-12, f"{name}_{axis}_masked")
+
+
+@pytest.mark.parametrize("name", ["euler_step", "heun_step", "rk4_step", "rk3_ssp_step"])
+def test_explicit_rk_steps_match_oracle(name):
+    """explicit_rk.py steppers on a pytree of device arrays against the oracle
+    (the oracle pins each scheme's stability polynomial and order)."""
+    need_gpu()
+    rng = np.random.default_rng(16)
+    a = rng.standard_normal((20, 30))
+    b = rng.standard_normal((7, 9))
+    f = lambda y: {"a": -0.7 * y["a"] + 0.1 * y["a"] ** 2, "b": 0.3 * y["b"]}  # noqa: E731
+    got = getattr(fx, name)({"a": torch.tensor(a), "b": torch.tensor(b)}, f, 0.05)
+    ref_a = getattr(fvx, name)(a, lambda y: -0.7 * y + 0.1 * y**2, 0.05)
+    ref_b = getattr(fvx, name)(b, lambda y: 0.3 * y, 0.05)
+    assert_close_scaled(to_np(got["a"]), ref_a, 1e-14, f"{name} a")
+    assert_close_scaled(to_np(got["b"]), ref_b, 1e-14, f"{name} b")

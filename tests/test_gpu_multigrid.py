@@ -273,3 +273,28 @@ def test_kernel_bench_down_step_mode():
     _lib.call("fvx_mg_kernel_bench", masked._h, 3, 32, 0, _lib.stream())
     with pytest.raises(ValueError):
         _lib.call("fvx_mg_kernel_bench", masked._h, 3, 32, 1, _lib.stream())
+
+
+@pytest.mark.parametrize("ny,nx,kind,masked,varcoef", [CASES[3], CASES[4], CASES[5]])
+def test_v_cycle_from_coarser_levels(ny, nx, kind, masked, varcoef):
+    """MultigridSolver.v_cycle(u, rhs, level_idx) from every level below the
+    finest (multigrid.py:808-883) against the oracle's recursion from that
+    level: FP64 1e-10, FP32 1e-4."""
+    need_gpu()
+    m, coeff, dx, dy, lam = _setup(ny, nx, kind, masked, varcoef, seed=6)
+    kw = dict(n_pre=2, n_post=2, n_coarse=30, omega=0.9)
+    levels = fvx.build_multigrid(m, dx, dy, lam, coeff, kind=kind)
+    mg = S.build_multigrid_solver(m, dx, dy, lam, coeff, kind=kind, **kw)
+    mg32 = S.build_multigrid_solver(m, dx, dy, lam, coeff, kind=kind, precision="float32", **kw)
+    rng = np.random.default_rng(7)
+    for li in range(1, len(levels)):
+        shape = levels[li].mask.shape
+        b = rng.standard_normal(shape) * levels[li].mask
+        u0 = rng.standard_normal(shape)
+        ref = fvx.v_cycle(levels, u0, b, li, **kw)
+        assert_close_scaled(to_np(mg.v_cycle(u0, b, level_idx=li)), ref, 1e-10, f"level {li}")
+        ref0 = fvx.v_cycle(levels, np.zeros(shape), b, li, **kw)
+        got = mg32.v_cycle(None, torch.tensor(b, dtype=torch.float32), level_idx=li)
+        assert_close_scaled(to_np(got), ref0, 1e-4, f"level {li} f32")
+    with pytest.raises(ValueError):
+        mg.v_cycle(None, np.zeros((ny, nx)), level_idx=len(levels))

@@ -142,3 +142,27 @@ def test_config2_basin_mixed_within_pv_gate():
     l2 = np.linalg.norm(out["mixed"] - ref) / np.linalg.norm(ref)
     print(f"config-2 PV error after {steps} steps: max {err:.3e}, L2 {l2:.3e}")
     assert err <= 1e-6 and l2 <= 1e-6, (err, l2)
+
+
+@pytest.mark.parametrize("nx,ny", [(64, 64), (48, 80)])
+def test_qg_1p5_layer_script_steps_match_oracle(nx, ny):
+    """The reference example's composition (upwind1, X-point velocities, DST
+    inversion, heun_step; scripts/qg_1p5_layer.py:420-497): 4 steps in FP64
+    against the oracle's composition with exact Helmholtz solves."""
+    need_gpu()
+    from finitevolx_amd.qg_1p5_layer import QG1p5LayerModel, QuasiGeostrophicConfig
+
+    cfg = QuasiGeostrophicConfig(nx=nx, ny=ny, Ly=5.12e6 * ny / nx)
+    model = QG1p5LayerModel(cfg)
+    p = fvx.QGScriptParams(nx=nx, ny=ny, Ly=5.12e6 * ny / nx)
+    q_ref, wind = p.fields()
+    assert_close_scaled(to_np(model.q0), q_ref, 1e-15, "q0")
+    assert_close_scaled(to_np(model.wind_curl), wind, 1e-15, "wind curl")
+    q = model.q0.clone()
+    for n in range(4):
+        q, psi, u, v = model.step(q)
+        q_ref, psi_r, u_r, v_r = fvx.qg_script_step(q_ref, p, wind)
+        for got, ref, name in ((q, q_ref, "q"), (psi, psi_r, "psi"), (u, u_r, "u"), (v, v_r, "v")):
+            assert_close_scaled(to_np(got), ref, 1e-11, f"step {n} {name}")
+    out = model.run(steps=6, snapshot_interval=3)
+    assert out["q"].shape == (2, ny + 2, nx + 2) and list(out["time"]) == [3 * cfg.dt, 6 * cfg.dt]

@@ -609,3 +609,45 @@ def test_vertical_mode_planted_mistake_is_caught():
         return A
 
     assert _coupling_failures(wrong)
+
+
+# ---------------------------------------------------------------------------
+# euler_step / heun_step / rk4_step (timestepping/explicit_rk.py:22-168)
+# ---------------------------------------------------------------------------
+
+RK_POLY = {  # stability polynomial R(z) of each scheme on y' = lambda y
+    "euler_step": lambda z: 1 + z,
+    "heun_step": lambda z: 1 + z + z * z / 2,
+    "rk4_step": lambda z: 1 + z + z * z / 2 + z**3 / 6 + z**4 / 24,
+}
+RK_ORDER = {"euler_step": 1, "heun_step": 2, "rk4_step": 4}
+
+
+def _rk_failures(name, step):
+    fails = []
+    for lam, dt in ((-1.0, 0.1), (-3.0, 0.5), (2.0, 0.01)):
+        got = step(np.array([1.0]), lambda y: lam * y, dt)[0]
+        if abs(got - RK_POLY[name](lam * dt)) > 1e-14 * max(1.0, abs(got)):
+            fails.append(f"R(z) lam={lam}")
+    errs = []
+    for n in (8, 16, 32):
+        y = np.array([1.0])
+        for _ in range(n):
+            y = step(y, lambda v: -v * v, 1.0 / n)
+        errs.append(abs(y[0] - 0.5))
+    if np.log2(errs[-2] / errs[-1]) < RK_ORDER[name] - 0.15:
+        fails.append("order")
+    return fails
+
+
+@pytest.mark.parametrize("name", sorted(RK_POLY))
+def test_explicit_rk_pins(name):
+    assert _rk_failures(name, getattr(fvx, name)) == []
+
+
+def test_heun_planted_mistake_is_caught():
+    def wrong(state, rhs_fn, dt):  # corrector evaluated at the predictor's start
+        k1 = rhs_fn(state)
+        return state + 0.5 * dt * (k1 + rhs_fn(state + 0.5 * dt * k1))
+
+    assert _rk_failures("heun_step", wrong)
