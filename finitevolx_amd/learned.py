@@ -222,19 +222,26 @@ def generate_pairs(mg, rhs_fields, vcycles=(1, 2, 3), sweeps: int = 2, rtol=1e-1
     return rs, es, float(np.median(ratios))
 
 
-def crop_batch(rs, es, crop: int, B: int, out_scale: float, gen: torch.Generator):
+def crop_batch(rs, es, crop: int, B: int, out_scale: float, gen: torch.Generator,
+               min_signal: float = 0.25, tries: int = 16):
     """Random ``crop x crop`` windows, normalised per crop: x = r / rms(r),
-    target = e / (rms(r) * out_scale)."""
+    target = e / (rms(r) * out_scale).  Windows whose residual rms is below
+    ``min_signal`` times the whole field's (mostly land, or a quiet corner:
+    their normalised targets are outliers that destabilise training) are
+    redrawn, up to ``tries`` times."""
     xb, eb = [], []
     n = len(rs)
     for _ in range(B):
-        k = int(torch.randint(n, (1,), generator=gen))
-        H, W = rs[k].shape
-        j0 = int(torch.randint(H - crop + 1, (1,), generator=gen))
-        i0 = int(torch.randint(W - crop + 1, (1,), generator=gen))
-        r = rs[k][j0:j0 + crop, i0:i0 + crop]
-        e = es[k][j0:j0 + crop, i0:i0 + crop]
-        s = torch.sqrt(torch.mean(r * r)).clamp_min(1e-30)
+        for _t in range(tries):
+            k = int(torch.randint(n, (1,), generator=gen))
+            H, W = rs[k].shape
+            j0 = int(torch.randint(H - crop + 1, (1,), generator=gen))
+            i0 = int(torch.randint(W - crop + 1, (1,), generator=gen))
+            r = rs[k][j0:j0 + crop, i0:i0 + crop]
+            e = es[k][j0:j0 + crop, i0:i0 + crop]
+            s = torch.sqrt(torch.mean(r * r)).clamp_min(1e-30)
+            if float(s) >= min_signal * float(torch.sqrt(torch.mean(rs[k] * rs[k]))):
+                break
         xb.append(r / s)
         eb.append(e / (s * out_scale))
     return torch.stack(xb).contiguous(), torch.stack(eb).contiguous()
@@ -257,14 +264,15 @@ def train(net: UNet, rs, es, out_scale: float, steps: int = 600, crop: int = 128
 
 
 def fit_preconditioner(mask, dx: float, dy: float, lambda_: float, rhs_fields, *,
-                       kind: str = "dirichlet", n_pre: int = 2, n_post: int = 2,
+                       coeff=None, kind: str = "dirichlet", n_pre: int = 2, n_post: int = 2,
                        n_coarse: int = 20, omega: float = 0.8, sweeps: int = 2,
                        channels=(16, 32, 48, 64), steps: int = 600, crop: int = 128,
                        batch: int = 8, lr: float = 2e-3, lr_final: float = 2e-4, seed: int = 0,
                        vcycles=(1, 2, 3)):
     """Train a U-Net preconditioner for one Helmholtz operator on the GPU.
 
-    Builds the FP64 multigrid solver of the operator, generates coarse
+    Builds the FP64 multigrid solver of the operator (``coeff``: the
+    variable coefficient of the no-flux operator, or None), generates coarse
     (residual, error) pairs from it on ``rhs_fields`` and trains the network
     (sized for multigrid level 1) with this build's kernels.  Returns
     ``(net, out_scale, loss_history)``; pass the network, ``out_scale`` and
@@ -274,8 +282,8 @@ def fit_preconditioner(mask, dx: float, dy: float, lambda_: float, rhs_fields, *
     from finitevolx_amd.solvers import build_multigrid_solver
 
     m = np.asarray(mask, dtype=np.float64)
-    mg = build_multigrid_solver(m, dx, dy, lambda_, kind=kind, n_pre=n_pre, n_post=n_post,
-                                n_coarse=n_coarse, omega=omega)
+    mg = build_multigrid_solver(m, dx, dy, lambda_, coeff, kind=kind, n_pre=n_pre,
+                                n_post=n_post, n_coarse=n_coarse, omega=omega)
     rs, es, out_scale = generate_pairs(mg, [np.asarray(f) * m for f in rhs_fields],
                                        vcycles=vcycles, sweeps=sweeps)
     del mg
