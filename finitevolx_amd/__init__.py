@@ -9,7 +9,12 @@ fallback.
 
 from finitevolx_amd.advection import Advection2D, Reconstruction2D
 from finitevolx_amd.boundary import enforce_periodic, pad_interior
-from finitevolx_amd.diffusion import BiharmonicDiffusion2D, Diffusion2D, diffusion_2d
+from finitevolx_amd.diffusion import (
+    BiharmonicDiffusion2D,
+    Diffusion2D,
+    MomentumAdvection2D,
+    diffusion_2d,
+)
 from finitevolx_amd.grid import ArakawaCGrid2D, CartesianGrid2D
 from finitevolx_amd.mask import Mask2D, StencilCapability2D
 from finitevolx_amd.operators import (
@@ -49,7 +54,7 @@ from finitevolx_amd.vertical import (
 
 __all__ = [
     "Advection2D", "Reconstruction2D", "ArakawaCGrid2D", "BiharmonicDiffusion2D", "CGInfo", "CartesianGrid2D",
-    "Difference2D", "Diffusion2D", "Divergence2D", "Interpolation2D", "Mask2D",
+    "Difference2D", "Diffusion2D", "MomentumAdvection2D", "Divergence2D", "Interpolation2D", "Mask2D",
     "MultigridLevel", "MultigridSolver", "QGModel", "StencilCapability2D", "arakawa_jacobian",
     "build_coupling_matrix", "build_multigrid_solver", "decompose_vertical_modes",
     "diffusion_2d", "divergence_2d", "enforce_periodic", "interior", "layer_to_mode",

@@ -83,3 +83,40 @@ class BiharmonicDiffusion2D:
                   float(self.grid.dx), float(self.grid.dy), float(kappa), _lib.ptr(mh),
                   _lib.stream())
         return out
+
+
+_lib.extra_signatures({
+    f"fvx_momentum_advection2d_{s}": [_lib._vp, _lib._vp, _lib._vp, _lib._vp, _lib._i, _lib._vp,
+                                      _lib._vp, _lib._vp, _lib._vp, _lib._i, _lib._i, _lib._d,
+                                      _lib._d, _lib._vp]
+    for s in ("f32", "f64")
+})
+_SCHEMES = {"energy": 0, "enstrophy": 1, "al": 2}
+
+
+class MomentumAdvection2D:
+    """Vector-invariant momentum advection (``diffusion/momentum.py:37-248``):
+    ``(du, dv) = (+(zeta v)_u - dK/dx, -(zeta u)_v - dK/dy)`` with Sadourny's
+    energy or enstrophy vorticity flux or the Arakawa-Lamb blend, as one HIP
+    kernel (every intermediate stays in registers)."""
+
+    def __init__(self, grid: CartesianGrid2D, mask: Mask2D | None = None):
+        self.grid = grid
+        self.mask = mask
+
+    def __call__(self, u, v, scheme: str = "energy"):
+        if scheme not in _SCHEMES:
+            raise ValueError(f"Unknown scheme: {scheme!r}.  Choose 'energy', 'enstrophy', or 'al'.")
+        u = _lib.as_field(u)
+        v = _lib.as_field(v, u.dtype)
+        if u.dim() != 2 or u.shape != v.shape:
+            raise ValueError("u and v must share one 2-D shape")
+        ny, nx = u.shape
+        m = [None] * 4
+        if self.mask is not None:
+            m = [self.mask.device(k) for k in ("h", "u", "v", "xy_corner_strict")]
+        du, dv = torch.empty_like(u), torch.empty_like(u)
+        _lib.call(f"fvx_momentum_advection2d_{_lib.suffix(u)}", u.data_ptr(), v.data_ptr(),
+                  du.data_ptr(), dv.data_ptr(), _SCHEMES[scheme], *[_lib.ptr(t) for t in m],
+                  ny, nx, float(self.grid.dx), float(self.grid.dy), _lib.stream())
+        return du, dv

@@ -138,6 +138,20 @@ int fvx_biharmonic2d_f32(const float* h, float* out, int ny, int nx, double dx, 
 int fvx_biharmonic2d_f64(const double* h, double* out, int ny, int nx, double dx, double dy,
                          double kappa, const uint8_t* mask_h, void* stream);
 
+/* ---- MomentumAdvection2D.__call__ (diffusion/momentum.py:37-248) ------------
+ * (du, dv) = (+(zeta v)_u - dK/dx, -(zeta u)_v - dK/dy) on [2:-2, 2:-2], zero
+ * elsewhere; scheme 0 energy (Sadourny E), 1 enstrophy (Sadourny Z), 2
+ * Arakawa-Lamb (1/3 E + 2/3 Z).  Masks: all four (h, u, v, xy_corner_strict
+ * stagger masks) or all NULL.                                               */
+int fvx_momentum_advection2d_f32(const float* u, const float* v, float* du, float* dv, int scheme,
+                                 const uint8_t* mask_h, const uint8_t* mask_u,
+                                 const uint8_t* mask_v, const uint8_t* mask_xy, int ny, int nx,
+                                 double dx, double dy, void* stream);
+int fvx_momentum_advection2d_f64(const double* u, const double* v, double* du, double* dv,
+                                 int scheme, const uint8_t* mask_h, const uint8_t* mask_u,
+                                 const uint8_t* mask_v, const uint8_t* mask_xy, int ny, int nx,
+                                 double dx, double dy, void* stream);
+
 /* ---- arakawa_jacobian (operators/jacobian.py:22-131) -----------------------
  * f, g: (batch, ny, nx); out: (batch, ny-2, nx-2).                          */
 int fvx_arakawa_jacobian_f32(const float* f, const float* g, float* out, int batch, int ny,

@@ -278,6 +278,96 @@ def V_to_T(v, mask=None):
     return _apply_mask(out, None if mask is None else mask.h)
 
 
+def _avg(h, a, b, c=None, d=None):
+    out = 0.5 * (h[a] + h[b]) if c is None else 0.25 * (h[a] + h[b] + h[c] + h[d])
+    return out
+
+
+_C = (slice(1, -1), slice(1, -1))
+_E = (slice(1, -1), slice(2, None))
+_W = (slice(1, -1), slice(None, -2))
+_N = (slice(2, None), slice(1, -1))
+_S = (slice(None, -2), slice(1, -1))
+_NW = (slice(2, None), slice(None, -2))
+_SE = (slice(None, -2), slice(2, None))
+
+
+def _interp(out, h, stagger, mask):
+    o = interior(out, h.shape)
+    return o if mask is None else o * getattr(mask, stagger)
+
+
+def U_to_T(u, mask=None):
+    """``operators/interpolation.py:196-204`` (avg_x_bwd)."""
+    return _interp(_avg(u, _C, _W), u, "h", mask)
+
+
+def X_to_U(q, mask=None):
+    """``operators/interpolation.py:172-180`` (avg_y_bwd)."""
+    return _interp(_avg(q, _C, _S), q, "u", mask)
+
+
+def X_to_V(q, mask=None):
+    """``operators/interpolation.py:182-190`` (avg_x_bwd)."""
+    return _interp(_avg(q, _C, _W), q, "v", mask)
+
+
+def U_to_X(u, mask=None):
+    """``operators/interpolation.py:231-239`` (avg_y_fwd)."""
+    return _interp(_avg(u, _C, _N), u, "xy_corner_strict", mask)
+
+
+def V_to_X(v, mask=None):
+    """``operators/interpolation.py:241-249`` (avg_x_fwd)."""
+    return _interp(_avg(v, _C, _E), v, "xy_corner_strict", mask)
+
+
+def U_to_V(u, mask=None):
+    """``operators/interpolation.py:255-264`` (avg_xbwd_yfwd)."""
+    return _interp(_avg(u, _C, _N, _W, _NW), u, "v", mask)
+
+
+def V_to_U(v, mask=None):
+    """``operators/interpolation.py:266-275`` (avg_xfwd_ybwd)."""
+    return _interp(_avg(v, _C, _S, _E, _SE), v, "u", mask)
+
+
+def momentum_advection_2d(u, v, dx, dy, scheme="energy", mask=None):
+    """``MomentumAdvection2D.__call__`` (``diffusion/momentum.py:97-248``):
+    (du, dv) = (+(zeta v)_u - dK/dx, -(zeta u)_v - dK/dy) on [2:-2, 2:-2]."""
+    u = np.asarray(u, np.float64)
+    v = np.asarray(v, np.float64)
+    zeta = curl(u, v, dx, dy, mask)
+    uT, vT = U_to_T(u, mask), V_to_T(v, mask)
+    K = interior(0.5 * (uT[1:-1, 1:-1] ** 2 + vT[1:-1, 1:-1] ** 2), u.shape)
+    dKdx, dKdy = diff_x_T_to_U(K, dx, mask), diff_y_T_to_V(K, dy, mask)
+
+    def energy():
+        return (interior((X_to_U(zeta, mask) * V_to_U(v, mask))[1:-1, 1:-1], u.shape),
+                interior((X_to_V(zeta, mask) * U_to_V(u, mask))[1:-1, 1:-1], u.shape))
+
+    def enstrophy():
+        zvq = interior((zeta * V_to_X(v, mask))[1:-1, 1:-1], u.shape)
+        zuq = interior((zeta * U_to_X(u, mask))[1:-1, 1:-1], u.shape)
+        return X_to_U(zvq, mask), X_to_V(zuq, mask)
+
+    if scheme == "energy":
+        zv_u, zu_v = energy()
+    elif scheme == "enstrophy":
+        zv_u, zu_v = enstrophy()
+    elif scheme == "al":
+        (ea, eb), (sa, sb) = energy(), enstrophy()
+        zv_u = interior(ea[1:-1, 1:-1] / 3.0 + (2.0 / 3.0) * sa[1:-1, 1:-1], u.shape)
+        zu_v = interior(eb[1:-1, 1:-1] / 3.0 + (2.0 / 3.0) * sb[1:-1, 1:-1], u.shape)
+    else:
+        raise ValueError(f"Unknown scheme: {scheme!r}.  Choose 'energy', 'enstrophy', or 'al'.")
+    du = interior(zv_u[2:-2, 2:-2] - dKdx[2:-2, 2:-2], u.shape, ghost=2)
+    dv = interior(-zu_v[2:-2, 2:-2] - dKdy[2:-2, 2:-2], u.shape, ghost=2)
+    if mask is not None:
+        du, dv = du * mask.u, dv * mask.v
+    return du, dv
+
+
 # ---------------------------------------------------------------------------
 # WENO point kernels and TVD limiters
 # ---------------------------------------------------------------------------

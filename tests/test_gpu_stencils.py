@@ -180,3 +180,37 @@ def test_periodic_and_rk3():
     out = fx.rk3_ssp_step(y, lambda s: -0.5 * s, 0.1)
     ref = fvx.rk3_ssp_step(a, lambda s: -0.5 * s, 0.1)
     assert_close_scaled(to_np(out), ref, 1e-14, "rk3")
+
+
+@pytest.mark.parametrize("variant", ["unmasked", "masked"])
+def test_momentum_advection_goldens(canon, variant):
+    """MomentumAdvection2D (energy scheme) against the reference's goldens."""
+    need_gpu()
+    from tests.conftest import load_golden
+
+    g = load_golden("MomentumAdvection2D", "__call__", variant)
+    m = fx.Mask2D(canon["hmask"]) if variant == "masked" else None
+    grid = fx.CartesianGrid2D(Nx=18, Ny=18, Lx=1.0, Ly=1.0, dx=canon["dx"], dy=canon["dy"])
+    du, dv = fx.MomentumAdvection2D(grid, m)(canon["u"], canon["v"])
+    assert_close_scaled(to_np(du), g["out0"], 1e-12, "du")
+    assert_close_scaled(to_np(dv), g["out1"], 1e-12, "dv")
+
+
+@pytest.mark.parametrize("scheme", ["energy", "enstrophy", "al"])
+@pytest.mark.parametrize("masked", [False, True])
+def test_momentum_advection_schemes(scheme, masked):
+    need_gpu()
+    ny, nx = 45, 70
+    rng = np.random.default_rng(19)
+    u, v = rng.standard_normal((2, ny, nx))
+    wet = random_mask(ny, nx, 20) if masked else None
+    om = fvx.Mask2D(wet) if masked else None
+    ref = fvx.momentum_advection_2d(u, v, 0.3, 0.2, scheme, om)
+    grid = fx.CartesianGrid2D(Nx=nx, Ny=ny, Lx=0.3 * (nx - 2), Ly=0.2 * (ny - 2), dx=0.3, dy=0.2)
+    madv = fx.MomentumAdvection2D(grid, fx.Mask2D(wet) if masked else None)
+    for dtype, tol in ((torch.float64, 1e-12), (torch.float32, 1e-5)):
+        du, dv = madv(torch.tensor(u, dtype=dtype), torch.tensor(v, dtype=dtype), scheme)
+        assert_close_scaled(to_np(du), ref[0], tol, f"du {scheme} {dtype}")
+        assert_close_scaled(to_np(dv), ref[1], tol, f"dv {scheme} {dtype}")
+    with pytest.raises(ValueError):
+        madv(torch.tensor(u), torch.tensor(v), "bogus")

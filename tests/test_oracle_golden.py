@@ -73,6 +73,13 @@ def test_diffusion2d(canon, variant):
     _check(fvx.biharmonic_2d(h, 1e-3, dx, dy, m), "BiharmonicDiffusion2D", "__call__", variant)
 
 
+@pytest.mark.parametrize("variant", ["unmasked", "masked"])
+def test_momentum_advection2d(canon, variant):
+    m = fvx.Mask2D(canon["hmask"]) if variant == "masked" else None
+    _check(fvx.momentum_advection_2d(canon["u"], canon["v"], canon["dx"], canon["dy"], "energy", m),
+           "MomentumAdvection2D", "__call__", variant)
+
+
 def test_planted_mistake_is_caught(canon):
     """A plausible slip (forward instead of backward x-difference) fails the golden."""
     h, dx = canon["u"], canon["dx"]

@@ -651,3 +651,59 @@ def test_heun_planted_mistake_is_caught():
         return state + 0.5 * dt * (k1 + rhs_fn(state + 0.5 * dt * k1))
 
     assert _rk_failures("heun_step", wrong)
+
+
+# ---------------------------------------------------------------------------
+# MomentumAdvection2D (diffusion/momentum.py; energy scheme also pinned by the
+# reference goldens in test_oracle_golden.py)
+# ---------------------------------------------------------------------------
+
+
+def _momentum_failures(madv):
+    fails = []
+    n = 14
+    dx, dy = 0.3, 0.4
+    for scheme in ("energy", "enstrophy", "al"):
+        # uniform flow and zero flow: zero tendency (tests/test_momentum.py:50-63, 142-150)
+        du, dv = madv(np.full((n, n), 0.7), np.full((n, n), -0.2), dx, dy, scheme)
+        if not (np.allclose(du, 0, atol=1e-13) and np.allclose(dv, 0, atol=1e-13)):
+            fails.append(f"uniform {scheme}")
+    rng = np.random.default_rng(17)
+    u, v = rng.standard_normal((2, n, n))
+    e = madv(u, v, dx, dy, "energy")
+    z = madv(u, v, dx, dy, "enstrophy")
+    a = madv(u, v, dx, dy, "al")
+    # Arakawa-Lamb = 1/3 energy + 2/3 enstrophy (tests/test_momentum.py:93-106)
+    for k in (0, 1):
+        if not np.allclose(a[k], e[k] / 3 + 2 * z[k] / 3, rtol=1e-12, atol=1e-12):
+            fails.append("al blend")
+    # solid-body rotation u = -y, v = x (tests/test_momentum.py:108-140):
+    # zeta = 2 everywhere, and with K = (u_T^2 + v_T^2)/2 the energy-scheme
+    # tendencies reduce to du = 2 x - x = x at U points and dv = y at V points
+    y = np.arange(n, dtype=float) * dy
+    x = np.arange(n, dtype=float) * dx
+    ur = np.broadcast_to(-y[:, None], (n, n)).copy()
+    vr = np.broadcast_to(x[None, :], (n, n)).copy()
+    du, dv = madv(ur, vr, dx, dy, "energy")
+    if not np.allclose(du[2:-2, 2:-2], np.broadcast_to(x[None, 2:-2] + 0.5 * dx, (n - 4, n - 4)),
+                       rtol=1e-12, atol=1e-12):
+        fails.append("solid-body du")
+    if not np.allclose(dv[2:-2, 2:-2], np.broadcast_to(y[2:-2, None] + 0.5 * dy, (n - 4, n - 4)),
+                       rtol=1e-12, atol=1e-12):
+        fails.append("solid-body dv")
+    return fails
+
+
+def test_momentum_pins():
+    assert _momentum_failures(fvx.momentum_advection_2d) == []
+
+
+def test_momentum_planted_mistake_is_caught():
+    def wrong_k(u, v, dx, dy, scheme="energy", mask=None):  # K without the 1/2
+        du, dv = fvx.momentum_advection_2d(u, v, dx, dy, scheme, mask)
+        uT, vT = fvx.U_to_T(u), fvx.V_to_T(v)
+        K = fvx.interior(0.5 * (uT[1:-1, 1:-1] ** 2 + vT[1:-1, 1:-1] ** 2), u.shape)
+        extra = fvx.diff_x_T_to_U(K, dx)
+        return du - fvx.interior(extra[2:-2, 2:-2], u.shape, ghost=2), dv
+
+    assert _momentum_failures(wrong_k)
