@@ -67,8 +67,11 @@ __device__ __forceinline__ int lin_tid() {
   return threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
 }
 
+// Sums of NV values over the block into partials: block ``bid`` of ``nblk``
+// stores value k at partials[k * nblk + bid].
 template <int NV, int kThreads, typename A>
-__device__ __forceinline__ void block_partials(const A (&v)[NV], double* partials) {
+__device__ __forceinline__ void block_partials(const A (&v)[NV], double* partials, int64_t nblk,
+                                               int64_t bid) {
   static_assert(NV <= 64, "one summing thread per value");
   __shared__ double red[kThreads / 64][NV];
   const int tid = lin_tid();
@@ -83,13 +86,16 @@ __device__ __forceinline__ void block_partials(const A (&v)[NV], double* partial
     // Value-major: value k of block b at partials[k * nblocks + b], so the
     // finishing kernel's reads are coalesced.  One thread per value sums the
     // waves in order.
-    const int64_t nblk = (int64_t)gridDim.x * gridDim.y;
-    const int64_t bid = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
     double s = 0.0;
 #pragma unroll
     for (int w = 0; w < kThreads / 64; ++w) s += red[w][tid];
     partials[tid * nblk + bid] = s;
   }
+}
+template <int NV, int kThreads, typename A>
+__device__ __forceinline__ void block_partials(const A (&v)[NV], double* partials) {
+  block_partials<NV, kThreads>(v, partials, (int64_t)gridDim.x * gridDim.y,
+                               (int64_t)blockIdx.y * gridDim.x + blockIdx.x);
 }
 
 // Reduction sites (which scalars the finishing kernel updates).
@@ -127,9 +133,9 @@ enum CgDot : int { kZR, kZQ, kZA, kRR, kRA, kRQ, kAA, kAQ, kQQ, kNumCgDots };
 constexpr int kFinishThreads = 1024;
 
 template <int NV, int SITE>
-__global__ __launch_bounds__(kFinishThreads) void reduce_finish_kernel(
-    const double* __restrict__ partials, int nblk, PcgScalars* sc, double rtol2, double atol2,
-    double inner_rtol = 0.0) {
+__device__ __forceinline__ void reduce_finish_body(const double* __restrict__ partials,
+                                                   int nblk, PcgScalars* sc, double rtol2,
+                                                   double atol2, double inner_rtol) {
   // kRedRefineReset: the inner solve's scalars sit just before the outer ones.
   constexpr bool kReset = SITE == kRedRefineReset;
   // The same guards as the producing kernels, read together with the
@@ -239,6 +245,12 @@ __global__ __launch_bounds__(kFinishThreads) void reduce_finish_kernel(
     sc->out_done = out[1] <= sc->out_tol2 ? 1 : 0;
     if constexpr (kReset) inner_reset(sc - 1, sc, inner_rtol);
   }
+}
+template <int NV, int SITE>
+__global__ __launch_bounds__(kFinishThreads) void reduce_finish_kernel(
+    const double* __restrict__ partials, int nblk, PcgScalars* sc, double rtol2, double atol2,
+    double inner_rtol = 0.0) {
+  reduce_finish_body<NV, SITE>(partials, nblk, sc, rtol2, atol2, inner_rtol);
 }
 
 // ---------------------------------------------------------------------------
@@ -392,7 +404,7 @@ __device__ __forceinline__ T prolong_at(const LevelDev<T>& C, int j, int i) {
 // just "inside the level", computed on the fly, which saves LDS (and raises
 // occupancy) for all-wet levels.
 template <typename T, int KIND, int NS, int MODE, bool ZERO, bool MK>
-__global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
+__device__ __forceinline__ void mg_tile_body(const TileArgs<T>& a, int bx) {
   // The convergence guard is checked after the halo loads are issued, so its
   // round trip overlaps theirs.
   const bool skip = a.sc && a.sc->done;
@@ -406,7 +418,7 @@ __global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
   __shared__ T sI[KIND == kNoflux ? N : 1];
 
   const LevelDev<T>& L = a.L;
-  const int t = xcd_swizzle(blockIdx.x, a.ntiles);
+  const int t = xcd_swizzle(bx, a.ntiles);
   const int i0 = (t % a.tiles_x) * kTx, j0 = (t / a.tiles_x) * kTy;
   const int tid = threadIdx.x;
 
@@ -645,7 +657,7 @@ __global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
         d[kQQ] += qd * qd;
       }
     }
-    block_partials<kNumCgDots, kTileThreads>(d, a.partials);
+    block_partials<kNumCgDots, kTileThreads>(d, a.partials, a.ntiles, bx);
   } else {
     // Write the tile interior.
 #pragma unroll
@@ -655,6 +667,10 @@ __global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
       if (gj < L.ny && gi < L.nx) a.u_out[(int64_t)gj * L.nx + gi] = sU[cur][y * W + x];
     }
   }
+}
+template <typename T, int KIND, int NS, int MODE, bool ZERO, bool MK>
+__global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
+  mg_tile_body<T, KIND, NS, MODE, ZERO, MK>(a, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1232,9 +1248,9 @@ struct CoarseLds {
 };
 
 template <typename T, int KIND>
-__global__ __launch_bounds__(kCoarseThreads) void mg_coarse_lds_kernel(CoarseArgs<T> a) {
+__device__ __forceinline__ void mg_coarse_lds_body(const CoarseArgs<T>& a,
+                                                   unsigned char* coarse_smem) {
   const bool skip = a.done && *a.done;
-  extern __shared__ __align__(16) unsigned char coarse_smem[];
   CoarseLds<T, KIND> S{reinterpret_cast<T*>(coarse_smem)};
   T* sm = S.sm;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
@@ -1347,6 +1363,11 @@ __global__ __launch_bounds__(kCoarseThreads) void mg_coarse_lds_kernel(CoarseArg
     for (int j = ty; j < G.ny; j += 32)
       for (int i = tx; i < G.nx; i += 32) G.u[j * G.nx + i] = sm[ou[a.lc] + j * G.nx + i];
   }
+}
+template <typename T, int KIND>
+__global__ __launch_bounds__(kCoarseThreads) void mg_coarse_lds_kernel(CoarseArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char coarse_smem[];
+  mg_coarse_lds_body<T, KIND>(a, coarse_smem);
 }
 
 // Dynamic LDS of the coarse kernel for levels lc .. nl-1 (u, t, b each).
@@ -1561,17 +1582,17 @@ __global__ __launch_bounds__(256) void pcg_az_dots_kernel(LevelDev<T> L, const T
 // in place, pointwise.  Each wave updates a 64-column strip of kDirRows rows
 // (grid_dir).  Runs unless done == 1 (done == 2: this is the last update).
 template <typename T>
-__global__ __launch_bounds__(256) void pcg_cg_update_kernel(int ny, int nx, T* __restrict__ x,
+__device__ __forceinline__ void pcg_cg_update_body(int ny, int nx, T* __restrict__ x,
                                                             int64_t ld_x, const T* __restrict__ z,
                                                             const T* __restrict__ az,
                                                             T* __restrict__ p, T* __restrict__ q,
-                                                            T* __restrict__ r, PcgScalars* sc) {
+                                                            T* __restrict__ r, PcgScalars* sc, int bx, int by) {
   // The guard first: budgeted iterations after convergence then cost no
   // traffic (this kernel is bandwidth-bound, so the flag's round trip hides
   // behind the other blocks' loads).
   if (sc->done == 1) return;
-  const int i = blockIdx.x * 64 + threadIdx.x;
-  const int j0 = (blockIdx.y * 4 + threadIdx.y) * kDirRows;
+  const int i = bx * 64 + threadIdx.x;
+  const int j0 = (by * 4 + threadIdx.y) * kDirRows;
   const double beta64 = sc->beta;
   const T alpha = (T)sc->alpha, beta = (T)beta64;
   T xv[kDirRows], zv[kDirRows], av[kDirRows], pv[kDirRows], qv[kDirRows], rv[kDirRows];
@@ -1602,6 +1623,14 @@ __global__ __launch_bounds__(256) void pcg_cg_update_kernel(int ny, int nx, T* _
       r[k] = rv[t] - alpha * qn;
     }
   }
+}
+template <typename T>
+__global__ __launch_bounds__(256) void pcg_cg_update_kernel(int ny, int nx, T* __restrict__ x,
+                                                            int64_t ld_x, const T* __restrict__ z,
+                                                            const T* __restrict__ az,
+                                                            T* __restrict__ p, T* __restrict__ q,
+                                                            T* __restrict__ r, PcgScalars* sc) {
+  pcg_cg_update_body<T>(ny, nx, x, ld_x, z, az, p, q, r, sc, blockIdx.x, blockIdx.y);
 }
 
 // The same PCG step for FP32 levels without a land mask under the Dirichlet
@@ -1676,14 +1705,14 @@ constexpr int kResRows = 8;
 static inline dim3 grid_res(int ny, int nx) { return dim3(cdiv(nx, 64), cdiv(ny, 4 * kResRows)); }
 
 template <int KIND, typename TB>
-__global__ __launch_bounds__(256) void refine_residual_kernel(
+__device__ __forceinline__ void refine_residual_body(
     LevelDev<double> L, const TB* __restrict__ b, int64_t ld_b, const double* __restrict__ x,
     int64_t ld_x, float* __restrict__ r32, float* __restrict__ e32, double rtol2, double atol2,
-    double* partials, PcgScalars* sc) {
+    double* partials, PcgScalars* sc, int bx, int by, int gx, int gy) {
   const int done = sc->out_done;  // checked once the first rows are in flight
   const int lane = threadIdx.x;
-  const int i = blockIdx.x * 64 + lane;
-  const int j0 = (blockIdx.y * 4 + threadIdx.y) * kResRows;  // uniform within a wave
+  const int i = bx * 64 + lane;
+  const int j0 = (by * 4 + threadIdx.y) * kResRows;  // uniform within a wave
   const int nx = L.nx, ny = L.ny;
   const bool col = i < nx;
   // x at (row j, column ii), masked for the Dirichlet operator (the reference
@@ -1740,17 +1769,24 @@ __global__ __launch_bounds__(256) void refine_residual_kernel(
     cys = cyn;
   }
   double v[2] = {bb, rr};
-  block_partials<2, 256>(v, partials);
+  block_partials<2, 256>(v, partials, (int64_t)gx * gy, (int64_t)by * gx + bx);
+}
+template <int KIND, typename TB>
+__global__ __launch_bounds__(256) void refine_residual_kernel(
+    LevelDev<double> L, const TB* __restrict__ b, int64_t ld_b, const double* __restrict__ x,
+    int64_t ld_x, float* __restrict__ r32, float* __restrict__ e32, double rtol2, double atol2,
+    double* partials, PcgScalars* sc) {
+  refine_residual_body<KIND, TB>(L, b, ld_b, x, ld_x, r32, e32, rtol2, atol2, partials, sc,
+                                 blockIdx.x, blockIdx.y, gridDim.x, gridDim.y);
 }
 
-__global__ __launch_bounds__(256) void refine_update_kernel(int ny, int nx, double* __restrict__ x,
-                                                            int64_t ld_x,
-                                                            const float* __restrict__ e,
-                                                            PcgScalars* sc) {
+__device__ __forceinline__ void refine_update_body(int ny, int nx, double* __restrict__ x,
+                                                   int64_t ld_x, const float* __restrict__ e,
+                                                   PcgScalars* sc, int bx, int by) {
   // x += e over kDirRows rows per wave (grid_dir), loads first.
   if (sc->out_done) return;
-  const int i = blockIdx.x * 64 + threadIdx.x;
-  const int j0 = (blockIdx.y * 4 + threadIdx.y) * kDirRows;
+  const int i = bx * 64 + threadIdx.x;
+  const int j0 = (by * 4 + threadIdx.y) * kDirRows;
   double xv[kDirRows];
   float ev[kDirRows];
 #pragma unroll
@@ -1766,6 +1802,12 @@ __global__ __launch_bounds__(256) void refine_update_kernel(int ny, int nx, doub
     if (i < nx && j < ny) x[(int64_t)j * ld_x + i] = xv[t] + (double)ev[t];
   }
   if (i == 0 && j0 == 0) sc->outer += 1;
+}
+__global__ __launch_bounds__(256) void refine_update_kernel(int ny, int nx, double* __restrict__ x,
+                                                            int64_t ld_x,
+                                                            const float* __restrict__ e,
+                                                            PcgScalars* sc) {
+  refine_update_body(ny, nx, x, ld_x, e, sc, blockIdx.x, blockIdx.y);
 }
 
 // The same update when the inner solve's last PCG step is still pending (its
