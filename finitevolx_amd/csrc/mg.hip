@@ -24,6 +24,7 @@
 #include <cmath>
 #include <cstring>
 #include <type_traits>
+#include <list>
 #include <vector>
 
 #include "fvx_common.hpp"
@@ -3277,6 +3278,560 @@ int fvx_mg_kernel_bench(void* plan, int which, int bits, int reps, void* stream)
                                  : kernel_bench_impl<float, kNoflux>(*P, which, reps, s);
   return P->kind == kDirichlet ? kernel_bench_impl<double, kDirichlet>(*P, which, reps, s)
                                : kernel_bench_impl<double, kNoflux>(*P, which, reps, s);
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Work-list batches of independent mixed-precision solves.
+//
+// An ensemble of QG members (uneven grid sizes, some with land masks, each
+// its own Ld) or the vertical modes of a multi-layer model (one grid, one
+// lambda per mode) hold one plan per problem.  Solving them one after the
+// other would cost every member the full chain of dependent launches of its
+// own solve, most of them latency-bound small levels.  A batch runs each
+// operator of the mixed solve -- refinement residual, every V-cycle level's
+// down and up kernel, the coarse solve, the PCG reductions and updates --
+// as ONE launch over a work-list: entry e of a list is one member's argument
+// block and the range of launch blocks that serve it; a block finds its entry
+// by binary search over the ranges and runs the ordinary kernel body on the
+// entry's tile.  Members whose hierarchy is shallower simply have no entry in
+// the deeper positions.  Each member keeps its own device scalars, partials
+// and convergence flags, so a converged member's blocks return at once while
+// the others iterate.  Levels run on the LDS-tile kernels (masked and
+// unmasked entries in separate lists, since the mask decides the kernel's
+// LDS layout).
+// ===========================================================================
+namespace fvx {
+
+// Entry owning launch block b: the last e with first[e] <= b.
+__device__ __forceinline__ int wl_entry(const int* __restrict__ first, int n, int b) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (first[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+struct FinishArgs {
+  const double* partials;
+  int nblk;
+  PcgScalars* sc;
+};
+struct UpdArgs {
+  int ny, nx, gx;
+  float *x, *z, *az, *p, *q, *r;
+  int64_t ld_x;
+  PcgScalars* sc;
+};
+struct ResArgs {
+  LevelDev<double> L;
+  const double* b;
+  int64_t ld_b;
+  const double* x;
+  int64_t ld_x;
+  float *r32, *e32;
+  double* partials;
+  PcgScalars* sc;
+  int gx, gy;
+};
+struct RefUpdArgs {
+  int ny, nx, gx;
+  double* x;
+  int64_t ld_x;
+  const float* e;
+  PcgScalars* sc;
+};
+
+template <typename T, int KIND, int NS, int MODE, bool ZERO, bool MK>
+__global__ __launch_bounds__(kTileThreads) void mg_tile_batch_kernel(
+    const TileArgs<T>* __restrict__ args, const int* __restrict__ first, int n) {
+  const int e = wl_entry(first, n, blockIdx.x);
+  mg_tile_body<T, KIND, NS, MODE, ZERO, MK>(args[e], blockIdx.x - first[e]);
+}
+
+template <typename T, int KIND>
+__global__ __launch_bounds__(kCoarseThreads) void mg_coarse_batch_kernel(
+    const CoarseArgs<T>* __restrict__ args) {
+  extern __shared__ __align__(16) unsigned char coarse_smem[];
+  mg_coarse_lds_body<T, KIND>(args[blockIdx.x], coarse_smem);
+}
+
+template <int NV, int SITE>
+__global__ __launch_bounds__(kFinishThreads) void finish_batch_kernel(
+    const FinishArgs* __restrict__ args, double rtol2, double atol2, double inner_rtol) {
+  const FinishArgs& a = args[blockIdx.x];
+  reduce_finish_body<NV, SITE>(a.partials, a.nblk, a.sc, rtol2, atol2, inner_rtol);
+}
+
+__global__ __launch_bounds__(256) void pcg_update_batch_kernel(const UpdArgs* __restrict__ args,
+                                                               const int* __restrict__ first,
+                                                               int n) {
+  const int e = wl_entry(first, n, blockIdx.x);
+  const UpdArgs& a = args[e];
+  const int lid = blockIdx.x - first[e];
+  pcg_cg_update_body<float>(a.ny, a.nx, a.x, a.ld_x, a.z, a.az, a.p, a.q, a.r, a.sc,
+                            lid % a.gx, lid / a.gx);
+}
+
+template <int KIND>
+__global__ __launch_bounds__(256) void refine_residual_batch_kernel(
+    const ResArgs* __restrict__ args, const int* __restrict__ first, int n, double rtol2,
+    double atol2) {
+  const int e = wl_entry(first, n, blockIdx.x);
+  const ResArgs& a = args[e];
+  const int lid = blockIdx.x - first[e];
+  refine_residual_body<KIND, double>(a.L, a.b, a.ld_b, a.x, a.ld_x, a.r32, a.e32, rtol2, atol2,
+                                     a.partials, a.sc, lid % a.gx, lid / a.gx, a.gx, a.gy);
+}
+
+__global__ __launch_bounds__(256) void refine_update_batch_kernel(
+    const RefUpdArgs* __restrict__ args, const int* __restrict__ first, int n) {
+  const int e = wl_entry(first, n, blockIdx.x);
+  const RefUpdArgs& a = args[e];
+  const int lid = blockIdx.x - first[e];
+  refine_update_body(a.ny, a.nx, a.x, a.ld_x, a.e, a.sc, lid % a.gx, lid / a.gx);
+}
+
+// Clears the scalar pairs of one slot of every member (block e: member e).
+__global__ void scalars_reset_batch_kernel(PcgScalars* const* __restrict__ scs) {
+  PcgScalars* sc = scs[blockIdx.x];
+  const int k = threadIdx.x;
+  if (k < 2) {
+    PcgScalars& x = sc[k];
+    x.rz = x.pq = x.rr = x.bb = x.tol2 = x.alpha = x.beta = x.zq = x.out_rr = x.out_tol2 = 0.0;
+    x.iter = x.done = x.first = x.out_done = x.outer = x.total = x.max_inner = 0;
+    x.it_outer[0] = x.it_outer[1] = x.it_outer[2] = x.it_outer[3] = 0;
+  }
+}
+
+// Status gather: the (inner, outer) scalar pair of one slot of every member.
+__global__ void scalars_gather_kernel(PcgScalars* const* __restrict__ scs, PcgScalars* out) {
+  const int e = blockIdx.x, k = threadIdx.x;
+  if (k < 2) out[2 * e + k] = scs[e][k];
+}
+
+// One work-list on the device: entry arguments and block ranges.
+template <typename A>
+struct DevList {
+  A* args = nullptr;
+  int* first = nullptr;
+  int n = 0, blocks = 0;
+};
+
+template <typename A>
+static int upload_list(const std::vector<A>& args, const std::vector<int>& nblk, DevList<A>& d,
+                       std::vector<void*>& allocs) {
+  d.n = (int)args.size();
+  d.blocks = 0;
+  if (d.n == 0) return kOk;
+  std::vector<int> first(d.n + 1, 0);
+  for (int e = 0; e < d.n; ++e) first[e + 1] = first[e] + nblk[e];
+  d.blocks = first[d.n];
+  FVX_HIP(hipMalloc(&d.args, sizeof(A) * d.n));
+  allocs.push_back(d.args);
+  FVX_HIP(hipMemcpy(d.args, args.data(), sizeof(A) * d.n, hipMemcpyHostToDevice));
+  FVX_HIP(hipMalloc(&d.first, sizeof(int) * (d.n + 1)));
+  allocs.push_back(d.first);
+  FVX_HIP(hipMemcpy(d.first, first.data(), sizeof(int) * (d.n + 1), hipMemcpyHostToDevice));
+  return kOk;
+}
+
+// The V-cycle and PCG work-lists of one scalar slot.
+struct BatchSlot {
+  bool built = false;
+  // [MK][level] down and up lists (level 0's up list is the kUpCG one)
+  std::vector<DevList<TileArgs<float>>> down[2], up[2];
+  DevList<CoarseArgs<float>> coarse;
+  size_t coarse_lds = 0;
+  DevList<FinishArgs> fin_cg, fin_res;
+  DevList<UpdArgs> upd;
+  DevList<RefUpdArgs> refupd;
+  PcgScalars** scs_in = nullptr;  // device array of each member's inner scalars
+  PcgScalars* status = nullptr;   // gathered pairs (device)
+  PcgScalars* hstatus = nullptr;  // pinned host mirror
+  // refinement residual and update lists by (rhs, x) array set (a list:
+  // entries keep their addresses)
+  struct ResEntry {
+    std::vector<const void*> key;
+    DevList<ResArgs> res;
+    DevList<RefUpdArgs> ref;
+  };
+  std::list<ResEntry> res;
+};
+
+struct MgBatch {
+  std::vector<Plan*> P;
+  int kind = kDirichlet, n_pre = 0, n_post = 0, lc_max = 0;
+  BatchSlot slot[kSlots];
+  std::vector<void*> allocs;
+};
+
+static int batch_build_slot(MgBatch& B, int s) {
+  BatchSlot& S = B.slot[s];
+  if (S.built) return kOk;
+  const int nm = (int)B.P.size();
+  for (int mk = 0; mk < 2; ++mk) {
+    S.down[mk].resize(B.lc_max);
+    S.up[mk].resize(B.lc_max);
+  }
+  for (int l = 0; l < B.lc_max; ++l) {
+    std::vector<TileArgs<float>> dn[2], upl[2];
+    std::vector<int> dnb[2], upb[2];
+    for (int m = 0; m < nm; ++m) {
+      Plan& Pm = *B.P[m];
+      if (l >= Pm.lc) continue;
+      Hier<float>& H = Pm.h32;
+      PcgScalars* sc = &Pm.sc[2 * s];
+      const LevelDev<float>& L = H.lv[l];
+      const int mk = L.mask ? 1 : 0;
+      TileArgs<float> a;
+      std::memset(&a, 0, sizeof(a));
+      a.L = L;
+      a.C = H.lv[l + 1];
+      a.omega = (float)Pm.omega;
+      a.partials = Pm.partials;
+      a.sc = sc;
+      a.tiles_x = cdiv(L.nx, kTx);
+      a.ntiles = a.tiles_x * cdiv(L.ny, kTy);
+      // down: zero guess, b = (level 0: the inner residual r; else the
+      // restricted residual), iterate into L.u, restriction into C.b
+      TileArgs<float> d = a;
+      d.b = (l == 0) ? H.r : L.b;
+      d.u_out = L.u;
+      dn[mk].push_back(d);
+      dnb[mk].push_back(d.ntiles);
+      // up: coarse correction from level l+1's corrected iterate (the coarse
+      // kernel's output at lc, else that level's up output L.t), post-sweeps
+      TileArgs<float> u = a;
+      u.C.u = (l + 1 == Pm.lc) ? H.lv[l + 1].u : H.lv[l + 1].t;
+      u.u_in = L.u;
+      u.b = (l == 0) ? H.r : L.b;
+      u.u_out = (l == 0) ? H.z : L.t;
+      if (l == 0) {
+        u.w = H.q;
+        u.az_out = H.az;
+      }
+      upl[mk].push_back(u);
+      upb[mk].push_back(u.ntiles);
+    }
+    for (int mk = 0; mk < 2; ++mk) {
+      if (int rc = upload_list(dn[mk], dnb[mk], S.down[mk][l], B.allocs)) return rc;
+      if (int rc = upload_list(upl[mk], upb[mk], S.up[mk][l], B.allocs)) return rc;
+    }
+  }
+  std::vector<CoarseArgs<float>> ca;
+  std::vector<int> cb;
+  std::vector<FinishArgs> fc, fr;
+  std::vector<int> ones;
+  std::vector<UpdArgs> ua;
+  std::vector<int> ub;
+  std::vector<RefUpdArgs> ra;
+  std::vector<int> rb;
+  std::vector<PcgScalars*> scs;
+  for (int m = 0; m < nm; ++m) {
+    Plan& Pm = *B.P[m];
+    Hier<float>& H = Pm.h32;
+    PcgScalars* sc = &Pm.sc[2 * s];
+    scs.push_back(sc);
+    if (Pm.lc < Pm.nl) {
+      CoarseArgs<float> c;
+      std::memset(&c, 0, sizeof(c));
+      for (int l = 0; l < Pm.nl; ++l) c.lv[l] = H.lv[l];
+      c.lc = Pm.lc;
+      c.nl = Pm.nl;
+      c.n_pre = Pm.n_pre;
+      c.n_post = Pm.n_post;
+      c.n_coarse = Pm.n_coarse;
+      c.nonzero_init = 0;
+      c.omega = (float)Pm.omega;
+      c.done = &sc->done;
+      S.coarse_lds = std::max(S.coarse_lds, coarse_lds_bytes(c));
+      ca.push_back(c);
+      cb.push_back(1);
+    }
+    const LevelDev<float>& L0 = H.lv[0];
+    fc.push_back(FinishArgs{Pm.partials, L0.ny > 0 ? cdiv(L0.nx, kTx) * cdiv(L0.ny, kTy) : 0, sc});
+    const dim3 gr = grid_res(Pm.h64.lv[0].ny, Pm.h64.lv[0].nx);
+    fr.push_back(FinishArgs{Pm.partials, (int)(gr.x * gr.y), sc + 1});
+    ones.push_back(1);
+    const dim3 gd = grid_dir(L0.ny, L0.nx);
+    ua.push_back(UpdArgs{L0.ny, L0.nx, (int)gd.x, H.x, H.z, H.az, H.p, H.q, H.r, L0.nx, sc});
+    ub.push_back((int)(gd.x * gd.y));
+  }
+  FVX_CHECK_ARG(S.coarse_lds <= kCoarseLdsMax, "a member's coarse levels exceed the LDS");
+  if (int rc = upload_list(ca, cb, S.coarse, B.allocs)) return rc;
+  if (int rc = upload_list(fc, ones, S.fin_cg, B.allocs)) return rc;
+  if (int rc = upload_list(fr, ones, S.fin_res, B.allocs)) return rc;
+  if (int rc = upload_list(ua, ub, S.upd, B.allocs)) return rc;
+  FVX_HIP(hipMalloc(&S.scs_in, sizeof(PcgScalars*) * nm));
+  B.allocs.push_back(S.scs_in);
+  FVX_HIP(hipMemcpy(S.scs_in, scs.data(), sizeof(PcgScalars*) * nm, hipMemcpyHostToDevice));
+  FVX_HIP(hipMalloc(&S.status, sizeof(PcgScalars) * 2 * nm));
+  B.allocs.push_back(S.status);
+  FVX_HIP(hipHostMalloc((void**)&S.hstatus, sizeof(PcgScalars) * 2 * nm, 0));
+  S.built = true;
+  return kOk;
+}
+
+// The refinement residual and update lists for one set of (rhs, x) arrays.
+static int batch_res_list(MgBatch& B, int s, const double* const* rhs, const int64_t* ld_rhs,
+                          double* const* x, const int64_t* ld_x, const DevList<ResArgs>** res,
+                          const DevList<RefUpdArgs>** ref) {
+  BatchSlot& S = B.slot[s];
+  const int nm = (int)B.P.size();
+  std::vector<const void*> key;
+  for (int m = 0; m < nm; ++m) {
+    key.push_back(rhs[m]);
+    key.push_back(x[m]);
+    key.push_back((const void*)(intptr_t)ld_rhs[m]);
+    key.push_back((const void*)(intptr_t)ld_x[m]);
+  }
+  for (auto& e : S.res)
+    if (e.key == key) {
+      *res = &e.res;
+      *ref = &e.ref;
+      return kOk;
+    }
+  std::vector<ResArgs> ra;
+  std::vector<RefUpdArgs> ua;
+  std::vector<int> rb, ub;
+  for (int m = 0; m < nm; ++m) {
+    Plan& Pm = *B.P[m];
+    const LevelDev<double>& L64 = Pm.h64.lv[0];
+    const dim3 g = grid_res(L64.ny, L64.nx);
+    ResArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.L = L64;
+    a.b = rhs[m];
+    a.ld_b = ld_rhs[m];
+    a.x = x[m];
+    a.ld_x = ld_x[m];
+    a.r32 = Pm.h32.r;
+    a.e32 = Pm.h32.x;
+    a.partials = Pm.partials;
+    a.sc = &Pm.sc[2 * s + 1];
+    a.gx = (int)g.x;
+    a.gy = (int)g.y;
+    ra.push_back(a);
+    rb.push_back((int)(g.x * g.y));
+    const LevelDev<float>& L0 = Pm.h32.lv[0];
+    const dim3 gd = grid_dir(L0.ny, L0.nx);
+    ua.push_back(RefUpdArgs{L0.ny, L0.nx, (int)gd.x, x[m], ld_x[m], Pm.h32.x, &Pm.sc[2 * s + 1]});
+    ub.push_back((int)(gd.x * gd.y));
+  }
+  S.res.emplace_back();
+  BatchSlot::ResEntry& e = S.res.back();
+  e.key = key;
+  if (int rc = upload_list(ra, rb, e.res, B.allocs)) return rc;
+  if (int rc = upload_list(ua, ub, e.ref, B.allocs)) return rc;
+  *res = &e.res;
+  *ref = &e.ref;
+  return kOk;
+}
+
+template <int KIND, int NPRE, int NPOST>
+static void batch_vcycle(MgBatch& B, BatchSlot& S, hipStream_t s) {
+  for (int l = 0; l < B.lc_max; ++l) {
+    const DevList<TileArgs<float>>& d0 = S.down[0][l];
+    const DevList<TileArgs<float>>& d1 = S.down[1][l];
+    if (d0.n)
+      hipLaunchKernelGGL((mg_tile_batch_kernel<float, KIND, NPRE, kDown, true, false>),
+                         dim3(d0.blocks), dim3(kTileThreads), 0, s, d0.args, d0.first, d0.n);
+    if (d1.n)
+      hipLaunchKernelGGL((mg_tile_batch_kernel<float, KIND, NPRE, kDown, true, true>),
+                         dim3(d1.blocks), dim3(kTileThreads), 0, s, d1.args, d1.first, d1.n);
+  }
+  if (S.coarse.n)
+    hipLaunchKernelGGL((mg_coarse_batch_kernel<float, KIND>), dim3(S.coarse.n),
+                       dim3(kCoarseThreads), S.coarse_lds, s, S.coarse.args);
+  for (int l = B.lc_max - 1; l >= 0; --l) {
+    const DevList<TileArgs<float>>& u0 = S.up[0][l];
+    const DevList<TileArgs<float>>& u1 = S.up[1][l];
+    if (l > 0) {
+      if (u0.n)
+        hipLaunchKernelGGL((mg_tile_batch_kernel<float, KIND, NPOST, kUp, false, false>),
+                           dim3(u0.blocks), dim3(kTileThreads), 0, s, u0.args, u0.first, u0.n);
+      if (u1.n)
+        hipLaunchKernelGGL((mg_tile_batch_kernel<float, KIND, NPOST, kUp, false, true>),
+                           dim3(u1.blocks), dim3(kTileThreads), 0, s, u1.args, u1.first, u1.n);
+    } else {
+      if (u0.n)
+        hipLaunchKernelGGL((mg_tile_batch_kernel<float, KIND, NPOST, kUpCG, false, false>),
+                           dim3(u0.blocks), dim3(kTileThreads), 0, s, u0.args, u0.first, u0.n);
+      if (u1.n)
+        hipLaunchKernelGGL((mg_tile_batch_kernel<float, KIND, NPOST, kUpCG, false, true>),
+                           dim3(u1.blocks), dim3(kTileThreads), 0, s, u1.args, u1.first, u1.n);
+    }
+  }
+}
+
+template <int KIND, int NPRE, int NPOST>
+static int batch_mixed_impl(MgBatch& B, int s, const DevList<ResArgs>& res,
+                            const DevList<RefUpdArgs>& ref, double rtol, double atol,
+                            int n_outer, const int* n_inner, double inner_rtol, hipStream_t st) {
+  BatchSlot& S = B.slot[s];
+  const int nm = (int)B.P.size();
+  const double r2 = rtol * rtol, a2 = atol * atol;
+  hipLaunchKernelGGL(scalars_reset_batch_kernel, dim3(nm), dim3(64), 0, st, S.scs_in);
+  auto residual = [&](bool reset) {
+    hipLaunchKernelGGL((refine_residual_batch_kernel<KIND>), dim3(res.blocks), dim3(64, 4), 0, st,
+                       res.args, res.first, res.n, r2, a2);
+    if (reset)
+      hipLaunchKernelGGL((finish_batch_kernel<2, kRedRefineReset>), dim3(nm),
+                         dim3(kFinishThreads), 0, st, S.fin_res.args, r2, a2, inner_rtol);
+    else
+      hipLaunchKernelGGL((finish_batch_kernel<2, kRedRefine>), dim3(nm), dim3(kFinishThreads), 0,
+                         st, S.fin_res.args, r2, a2, 0.0);
+  };
+  for (int o = 0; o < n_outer; ++o) {
+    residual(true);
+    for (int k = 0; k < n_inner[o]; ++k) {
+      batch_vcycle<KIND, NPRE, NPOST>(B, S, st);
+      hipLaunchKernelGGL((finish_batch_kernel<kNumCgDots, kRedCG>), dim3(nm),
+                         dim3(kFinishThreads), 0, st, S.fin_cg.args, 0.0, 0.0, 0.0);
+      hipLaunchKernelGGL(pcg_update_batch_kernel, dim3(S.upd.blocks), dim3(64, 4), 0, st,
+                         S.upd.args, S.upd.first, S.upd.n);
+    }
+    hipLaunchKernelGGL(refine_update_batch_kernel, dim3(ref.blocks), dim3(64, 4), 0, st, ref.args,
+                       ref.first, ref.n);
+  }
+  residual(false);
+  FVX_LAUNCH_CHECK();
+  return kOk;
+}
+
+template <int KIND>
+static int batch_mixed_dispatch(MgBatch& B, int s, const DevList<ResArgs>& res,
+                                const DevList<RefUpdArgs>& ref, double rtol, double atol,
+                                int n_outer, const int* n_inner, double inner_rtol,
+                                hipStream_t st) {
+#define FVX_BM(A, C)                                                                            \
+  if (B.n_pre == A && B.n_post == C)                                                            \
+    return batch_mixed_impl<KIND, A, C>(B, s, res, ref, rtol, atol, n_outer, n_inner, inner_rtol, \
+                                        st);
+  FVX_BM(1, 1) FVX_BM(2, 2) FVX_BM(3, 3) FVX_BM(1, 3) FVX_BM(2, 3) FVX_BM(3, 2)
+#undef FVX_BM
+  set_error("batched solves support V(1,1), V(2,2), V(3,3), V(1,3), V(2,3), V(3,2)");
+  return kInvalidArgument;
+}
+
+}  // namespace fvx
+
+extern "C" {
+
+// A batch over existing mixed-precision plans (fvx_mg_plan_create with
+// precision 3264): same operator kind and pre/post sweep counts, each plan's
+// finest level tiled (more than 64 x 64 cells).  The plans must outlive the
+// batch and must not run other solves on the batch's slots concurrently.
+int fvx_mg_batch_create(void* const* plans, int n, void** batch_out) {
+  using namespace fvx;
+  FVX_CHECK_ARG(plans && n > 0 && batch_out, "null argument or empty batch");
+  MgBatch* B = new MgBatch();
+  for (int m = 0; m < n; ++m) {
+    Plan* Pm = (Plan*)plans[m];
+    if (!Pm || !Pm->has32 || !Pm->has64 || Pm->lc < 1) {
+      delete B;
+      set_error("member %d: needs a mixed-precision plan whose finest level is tiled", m);
+      return kInvalidArgument;
+    }
+    if (m > 0 && (Pm->kind != B->kind || Pm->n_pre != B->n_pre || Pm->n_post != B->n_post)) {
+      delete B;
+      set_error("member %d: every member needs the same operator kind and sweep counts", m);
+      return kInvalidArgument;
+    }
+    if (m == 0) {
+      B->kind = Pm->kind;
+      B->n_pre = Pm->n_pre;
+      B->n_post = Pm->n_post;
+    }
+    B->lc_max = std::max(B->lc_max, Pm->lc);
+    B->P.push_back(Pm);
+  }
+  coarse_lds_attributes();
+  (void)hipFuncSetAttribute((const void*)mg_coarse_batch_kernel<float, kDirichlet>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)kCoarseLdsMax);
+  (void)hipFuncSetAttribute((const void*)mg_coarse_batch_kernel<float, kNoflux>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)kCoarseLdsMax);
+  *batch_out = B;
+  return kOk;
+}
+
+int fvx_mg_batch_destroy(void* batch) {
+  using namespace fvx;
+  if (!batch) return kOk;
+  MgBatch* B = (MgBatch*)batch;
+  (void)hipDeviceSynchronize();
+  for (void* p : B->allocs) (void)hipFree(p);
+  for (int s = 0; s < kSlots; ++s)
+    if (B->slot[s].hstatus) (void)hipHostFree(B->slot[s].hstatus);
+  delete B;
+  return kOk;
+}
+
+// Enqueue (no host synchronisation) the mixed-precision solve of every
+// member on scalar slot ``slot``: rhs[m] / x[m] are member m's FP64 pitched
+// right-hand side and warm start / solution; n_outer refinement steps with
+// n_inner[o] inner PCG iterations each, device-guarded per member.  The first
+// call with a given (slot, arrays) set builds its work-lists (synchronous
+// uploads): make it before any graph capture that replays the call.
+int fvx_mg_batch_mixed_enqueue(void* batch, const double* const* rhs, const int64_t* ld_rhs,
+                               double* const* x, const int64_t* ld_x, double rtol, double atol,
+                               int n_outer, const int* n_inner, double inner_rtol, int slot,
+                               void* stream) {
+  using namespace fvx;
+  FVX_CHECK_ARG(batch && rhs && ld_rhs && x && ld_x && n_inner, "null argument");
+  FVX_CHECK_ARG(slot >= 0 && slot < kSlots, "slot out of range");
+  FVX_CHECK_ARG(n_outer >= 1 && n_outer <= 8, "n_outer must be in [1, 8]");
+  MgBatch& B = *(MgBatch*)batch;
+  if (int rc = batch_build_slot(B, slot)) return rc;
+  const DevList<ResArgs>* res = nullptr;
+  const DevList<RefUpdArgs>* ref = nullptr;
+  if (int rc = batch_res_list(B, slot, rhs, ld_rhs, x, ld_x, &res, &ref)) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  return B.kind == kDirichlet
+             ? batch_mixed_dispatch<kDirichlet>(B, slot, *res, *ref, rtol, atol, n_outer,
+                                                n_inner, inner_rtol, s)
+             : batch_mixed_dispatch<kNoflux>(B, slot, *res, *ref, rtol, atol, n_outer, n_inner,
+                                             inner_rtol, s);
+}
+
+// Per-member result of the last solve on ``slot`` (synchronises the stream):
+// info[m] = {inner iterations, refinement steps, converged (FP64 residual),
+// -, ||b - A x||, ||b||}; counts[m * 4 + o] = inner iterations of refinement
+// step o (o < 4; the last step's when it was the last budgeted one).
+int fvx_mg_batch_status(void* batch, int slot, int n_outer_budget, fvx_cg_info* info,
+                        int* counts, void* stream) {
+  using namespace fvx;
+  FVX_CHECK_ARG(batch && info, "null argument");
+  FVX_CHECK_ARG(slot >= 0 && slot < kSlots, "slot out of range");
+  MgBatch& B = *(MgBatch*)batch;
+  BatchSlot& S = B.slot[slot];
+  FVX_CHECK_ARG(S.built, "no solve has been enqueued on this slot");
+  const int nm = (int)B.P.size();
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(scalars_gather_kernel, dim3(nm), dim3(64), 0, s, S.scs_in, S.status);
+  FVX_HIP(hipMemcpyAsync(S.hstatus, S.status, sizeof(PcgScalars) * 2 * nm,
+                         hipMemcpyDeviceToHost, s));
+  FVX_HIP(hipStreamSynchronize(s));
+  for (int m = 0; m < nm; ++m) {
+    const PcgScalars& in = S.hstatus[2 * m];
+    const PcgScalars& out = S.hstatus[2 * m + 1];
+    info[m].iterations = in.total;
+    info[m].outer = out.outer;
+    info[m].converged = out.out_done ? 1 : 0;
+    info[m].pad = 0;
+    info[m].residual_norm = std::sqrt(out.out_rr);
+    info[m].rhs_norm = std::sqrt(out.bb);
+    if (counts)
+      for (int o = 0; o < 4; ++o)
+        counts[m * 4 + o] = (o >= out.outer) ? 0
+                            : (o == n_outer_budget - 1) ? in.iter
+                                                        : in.it_outer[o];
+  }
+  return kOk;
 }
 
 }  // extern "C"

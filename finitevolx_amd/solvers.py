@@ -650,3 +650,85 @@ def pv_inversion(pv, dx, dy, lambda_, bc="dst", method="spectral", mask=None,
     out = torch.stack([torch.stack([one(f[k], float(lam[k])) for k in range(nl)])
                        for f in flat])
     return out.reshape(pv.shape)
+
+
+# ---------------------------------------------------------------------------
+# Work-list batches of independent mixed-precision solves
+# ---------------------------------------------------------------------------
+
+_lib.extra_signatures({
+    "fvx_mg_batch_create": [C.POINTER(_vp), _i, C.POINTER(_vp)],
+    "fvx_mg_batch_destroy": [_vp],
+    "fvx_mg_batch_mixed_enqueue": [_vp, C.POINTER(_vp), C.POINTER(_i64), C.POINTER(_vp),
+                                   C.POINTER(_i64), _d, _d, _i, C.POINTER(_i), _d, _i, _vp],
+    "fvx_mg_batch_status": [_vp, _i, _i, C.POINTER(_CgInfoC), C.POINTER(_i), _vp],
+})
+
+
+class MultigridBatch:
+    """Independent mixed-precision multigrid-PCG solves of several
+    ``MultigridSolver`` plans (``precision="mixed"``; any sizes, masks and
+    lambdas, the same operator kind and sweep counts) run together: each
+    operator of the solve is one work-list launch over all members, and each
+    member stops on its own FP64-certified convergence.  Batches the
+    reference's per-layer ``vmap`` solves (``solvers/elliptic.py:532-541``)
+    and ensembles of independent models."""
+
+    def __init__(self, solvers):
+        self.solvers = list(solvers)
+        if not self.solvers:
+            raise ValueError("an empty batch")
+        for s in self.solvers:
+            if s.precision != "mixed":
+                raise ValueError("batched solves need precision='mixed' plans")
+        arr = (_vp * len(self.solvers))(*[s._h for s in self.solvers])
+        h = _vp()
+        _lib.call("fvx_mg_batch_create", arr, len(self.solvers), C.byref(h))
+        self._h = h
+        self.n = len(self.solvers)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            try:
+                _lib.call("fvx_mg_batch_destroy", h)
+            except Exception:
+                pass
+            self._h = None
+
+    def enqueue(self, rhs, x, rtol=1e-8, atol=0.0, n_outer=2, n_inner=(8, 8),
+                inner_rtol=1e-4, slot=0):
+        """Enqueue the solves ``A_m x_m = rhs_m`` (FP64 (ny, nx) views with unit
+        column stride; ``x`` is the warm start, updated in place) on ``slot``
+        with ``n_outer`` refinement steps of ``n_inner[o]`` inner iterations
+        (device-guarded: converged members skip the rest).  No host sync."""
+        if len(rhs) != self.n or len(x) != self.n:
+            raise ValueError(f"need {self.n} right-hand sides and solutions")
+        for s, b, xx in zip(self.solvers, rhs, x):
+            for t in (b, xx):
+                if tuple(t.shape) != s.shape or t.dtype != torch.float64 or t.stride(1) != 1:
+                    raise ValueError("rhs / x must be float64 (ny, nx) views of each member")
+        ni = (_i * len(n_inner))(*[int(k) for k in n_inner])
+        if len(n_inner) < n_outer:
+            raise ValueError("n_inner needs one entry per refinement step")
+        _lib.call("fvx_mg_batch_mixed_enqueue", self._h,
+                  (_vp * self.n)(*[b.data_ptr() for b in rhs]),
+                  (_i64 * self.n)(*[b.stride(0) for b in rhs]),
+                  (_vp * self.n)(*[xx.data_ptr() for xx in x]),
+                  (_i64 * self.n)(*[xx.stride(0) for xx in x]), float(rtol), float(atol),
+                  int(n_outer), ni, float(inner_rtol), int(slot), _lib.stream())
+
+    def status(self, slot=0, n_outer=2):
+        """Per member: ``(CGInfo, refinement steps, inner counts per step)``
+        of the last solve on ``slot`` (synchronises)."""
+        infos = (_CgInfoC * self.n)()
+        counts = (_i * (4 * self.n))()
+        _lib.call("fvx_mg_batch_status", self._h, int(slot), int(n_outer), infos, counts,
+                  _lib.stream())
+        out = []
+        for m in range(self.n):
+            ci = infos[m]
+            out.append((CGInfo(iterations=int(ci.iterations), residual_norm=float(ci.residual_norm),
+                               converged=bool(ci.converged)), int(ci.outer),
+                        [counts[4 * m + o] for o in range(4)]))
+        return out
