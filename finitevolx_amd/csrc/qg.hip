@@ -179,16 +179,18 @@ __device__ __forceinline__ double load_state(const double* a, int n, unsigned co
   return a[(int64_t)row_off + col];
 }
 
+// Body over block ``bx`` of ``nblk`` (four tiles per block); with MASKED a
+// null mask means all wet.
 template <typename TS, typename TC, typename TP, bool MASKED>
-__global__ __launch_bounds__(256) void qg_stage_wave_kernel(
+__device__ __forceinline__ void qg_stage_wave_body(
     const TS* __restrict__ qs, const TP* __restrict__ psi, const TS* __restrict__ qn,
-    TS* __restrict__ out, const uint8_t* __restrict__ mask, int Ny, int Nx, QGStageParams p,
-    int tiles_x, int ntiles) {
+    TS* __restrict__ out, const uint8_t* __restrict__ mask, int Ny, int Nx, const QGStageParams& p,
+    int tiles_x, int ntiles, int bx, int nblk) {
   using T = TC;
   constexpr int R = kQWRows;
   const int lane = threadIdx.x & 63;
   const int tile =
-      __builtin_amdgcn_readfirstlane(xcd_swizzle(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6));
+      __builtin_amdgcn_readfirstlane(xcd_swizzle(bx, nblk) * 4 + (threadIdx.x >> 6));
   const bool active = tile < ntiles;
   const int tyi = active ? tile / tiles_x : 0;
   const int i0 = (active ? tile - tyi * tiles_x : 0) * 60;
@@ -281,12 +283,54 @@ __global__ __launch_bounds__(256) void qg_stage_wave_kernel(
       TS res = TS(0);
       if (ci && gj >= 1 && gj <= Ny - 2) {
         TS r = (TS)rhs;
-        if constexpr (MASKED) r = mask[g] ? r : TS(0);
+        if constexpr (MASKED) r = (!mask || mask[g]) ? r : TS(0);
         res = (a != TS(0) ? a * qn[g] : TS(0)) + b * qc[t] + cdt * r;
       }
       out[g] = res;
     }
   }
+}
+
+template <typename TS, typename TC, typename TP, bool MASKED>
+__global__ __launch_bounds__(256) void qg_stage_wave_kernel(
+    const TS* __restrict__ qs, const TP* __restrict__ psi, const TS* __restrict__ qn,
+    TS* __restrict__ out, const uint8_t* __restrict__ mask, int Ny, int Nx, QGStageParams p,
+    int tiles_x, int ntiles) {
+  qg_stage_wave_body<TS, TC, TP, MASKED>(qs, psi, qn, out, mask, Ny, Nx, p, tiles_x, ntiles,
+                                         blockIdx.x, gridDim.x);
+}
+
+// Work-list launch of the stage over many fields (ensemble members, layers):
+// entry e owns blocks first[e] .. first[e+1]-1.
+struct StageEntry {
+  const double* qs;
+  const double* psi;
+  const double* qn;
+  double* out;
+  const uint8_t* mask;
+  int Ny, Nx, tiles_x, ntiles;
+  QGStageParams p;
+};
+
+__device__ __forceinline__ int stage_entry(const int* __restrict__ first, int n, int b) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (first[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <typename TC>
+__global__ __launch_bounds__(256) void qg_stage_batch_kernel(const StageEntry* __restrict__ list,
+                                                             const int* __restrict__ first,
+                                                             int n) {
+  const int e = stage_entry(first, n, blockIdx.x);
+  const StageEntry& a = list[e];
+  qg_stage_wave_body<double, TC, double, true>(a.qs, a.psi, a.qn, a.out, a.mask, a.Ny, a.Nx, a.p,
+                                               a.tiles_x, a.ntiles, blockIdx.x - first[e],
+                                               first[e + 1] - first[e]);
 }
 
 // tend32: compute the tendency of an FP64 state in FP32 (FP32 states always
@@ -1092,6 +1136,104 @@ int fvx_qg_step_graph(void* plan, void* q, double* psi, int nsteps, fvx_qg_info*
   if (Q->state == 64)
     return qg_run_graph<double>(*Q, (double*)q, psi, nsteps, (QGInfoC*)info, s);
   return qg_run_graph<float>(*Q, (float*)q, psi, nsteps, (QGInfoC*)info, s);
+}
+
+typedef struct fvx_qg_stage_desc {
+  const double* qs;
+  const double* psi;
+  const double* qn;
+  double* out;
+  const uint8_t* mask;  // full-array wet mask or NULL
+  int Ny, Nx;
+  double dx, dy, beta, Ly, F0, nu4, a, b, cdt;
+} fvx_qg_stage_desc;
+
+struct StageBatchHandle {
+  fvx::StageEntry* list = nullptr;
+  int* first = nullptr;
+  int n = 0, blocks = 0, tend32 = 0;
+};
+
+// A fixed work-list of QG stage evaluations (FP64 states): one launch runs
+// every entry (ensemble members, layers of a multi-layer model).
+int fvx_qg_stage_batch_create(int n, const fvx_qg_stage_desc* d, int tend32, void** out) {
+  using namespace fvx;
+  FVX_CHECK_ARG(n > 0 && d && out, "null argument or empty list");
+  std::vector<StageEntry> e(n);
+  std::vector<int> first(n + 1, 0);
+  for (int k = 0; k < n; ++k) {
+    const fvx_qg_stage_desc& x = d[k];
+    FVX_CHECK_ARG(x.qs && x.psi && x.qn && x.out, "entry %d: null field pointer", k);
+    FVX_CHECK_ARG(x.out != x.qs && x.out != x.psi, "entry %d: out aliases q_s or psi", k);
+    FVX_CHECK_ARG(x.Ny >= 3 && x.Nx >= 3 && (int64_t)x.Ny * x.Nx < (int64_t(1) << 29),
+                  "entry %d: bad shape %d x %d", k, x.Ny, x.Nx);
+    StageEntry& a = e[k];
+    a.qs = x.qs;
+    a.psi = x.psi;
+    a.qn = x.qn;
+    a.out = x.out;
+    a.mask = x.mask;
+    a.Ny = x.Ny;
+    a.Nx = x.Nx;
+    a.tiles_x = cdiv(x.Nx, 60);
+    a.ntiles = a.tiles_x * cdiv(x.Ny, kQWRows);
+    a.p.idx = 1.0 / x.dx;
+    a.p.idy = 1.0 / x.dy;
+    a.p.inv12 = 1.0 / (12.0 * x.dx * x.dy);
+    a.p.beta = x.beta;
+    a.p.dy = x.dy;
+    a.p.y0 = 0.5 * x.Ly;
+    a.p.F0 = x.F0;
+    a.p.kF = 2.0 * M_PI / x.Ly;
+    a.p.nu4 = x.nu4;
+    a.p.a = x.a;
+    a.p.b = x.b;
+    a.p.cdt = x.cdt;
+    first[k + 1] = first[k] + cdiv(a.ntiles, 4);
+  }
+  StageBatchHandle* H = new StageBatchHandle();
+  H->n = n;
+  H->blocks = first[n];
+  H->tend32 = tend32 ? 1 : 0;
+  hipError_t err = hipMalloc(&H->list, sizeof(StageEntry) * n);
+  if (err == hipSuccess) err = hipMalloc(&H->first, sizeof(int) * (n + 1));
+  if (err == hipSuccess)
+    err = hipMemcpy(H->list, e.data(), sizeof(StageEntry) * n, hipMemcpyHostToDevice);
+  if (err == hipSuccess)
+    err = hipMemcpy(H->first, first.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice);
+  if (err != hipSuccess) {
+    (void)hipFree(H->list);
+    (void)hipFree(H->first);
+    delete H;
+    set_error("HIP error %s", hipGetErrorString(err));
+    return kHipError;
+  }
+  *out = H;
+  return kOk;
+}
+
+int fvx_qg_stage_batch_launch(void* h, void* stream) {
+  using namespace fvx;
+  FVX_CHECK_ARG(h, "null handle");
+  StageBatchHandle* H = (StageBatchHandle*)h;
+  if (H->tend32)
+    hipLaunchKernelGGL(qg_stage_batch_kernel<float>, dim3(H->blocks), dim3(256), 0,
+                       (hipStream_t)stream, H->list, H->first, H->n);
+  else
+    hipLaunchKernelGGL(qg_stage_batch_kernel<double>, dim3(H->blocks), dim3(256), 0,
+                       (hipStream_t)stream, H->list, H->first, H->n);
+  FVX_LAUNCH_CHECK();
+  return kOk;
+}
+
+int fvx_qg_stage_batch_destroy(void* h) {
+  if (!h) return 0;
+  StageBatchHandle* H = (StageBatchHandle*)h;
+  (void)hipDeviceSynchronize();
+  (void)hipFree(H->list);
+  (void)hipFree(H->first);
+  delete H;
+  return 0;
 }
 
 #define FVX_QG_STAGE(SUF, TS, T32)                                                                \

@@ -28,15 +28,21 @@ def _tree_map(fn, *trees):
     return fn(*trees)
 
 
-def lincomb(a, x, b=0.0, y=None, c=0.0, z=None):
-    """``a*x + b*y + c*z`` on device tensors in one kernel."""
+def lincomb(a, x, b=0.0, y=None, c=0.0, z=None, out=None):
+    """``a*x + b*y + c*z`` on device tensors in one kernel (into ``out`` when
+    given: a contiguous tensor of x's shape and type that aliases none of the
+    inputs)."""
     x = _lib.as_field(x)
     y = None if y is None else _lib.as_field(y, x.dtype)
     z = None if z is None else _lib.as_field(z, x.dtype)
     for t in (y, z):
         if t is not None and t.shape != x.shape:
             raise ValueError("shape mismatch in lincomb")
-    out = torch.empty_like(x)
+    if out is None:
+        out = torch.empty_like(x)
+    elif (out.shape != x.shape or out.dtype != x.dtype or not out.is_contiguous()
+          or any(t is not None and t.data_ptr() == out.data_ptr() for t in (x, y, z))):
+        raise ValueError("out must be a contiguous tensor like x that aliases no input")
     _lib.call(f"fvx_lincomb3_{_lib.suffix(x)}", x.numel(), float(a), x.data_ptr(), float(b),
               _lib.ptr(y), float(c), _lib.ptr(z), out.data_ptr(), _lib.stream())
     return out

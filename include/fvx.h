@@ -278,6 +278,27 @@ int fvx_mg_precond_solve(void* plan, int precond, void* net, double out_scale, i
  * 32/64 picks the hierarchy; reps = 0 prepares and must come first.        */
 int fvx_mg_kernel_bench(void* plan, int which, int bits, int reps, void* stream);
 
+/* ---- Work-list batches of independent solves (no reference counterpart:
+ * batches the reference's per-layer vmap solves, solvers/elliptic.py:532-541,
+ * and ensembles of models).  Members are mixed-precision plans (precision
+ * 3264) with one operator kind and sweep count, each finest level tiled
+ * (> 64 x 64 cells); every operator of the solve is ONE launch over all
+ * members, each member guarded by its own convergence flags.               */
+int fvx_mg_batch_create(void* const* plans, int n, void** batch_out);
+int fvx_mg_batch_destroy(void* batch);
+/* rhs[m], x[m]: member m's FP64 pitched right-hand side and warm start /
+ * solution; n_outer refinement steps of n_inner[o] inner PCG iterations on
+ * scalar slot ``slot``; enqueue only (the first call with a given array set
+ * uploads its work-lists synchronously: make it before a graph capture). */
+int fvx_mg_batch_mixed_enqueue(void* batch, const double* const* rhs, const int64_t* ld_rhs,
+                               double* const* x, const int64_t* ld_x, double rtol, double atol,
+                               int n_outer, const int* n_inner, double inner_rtol, int slot,
+                               void* stream);
+/* Per-member outcome of the last solve on ``slot`` (synchronises); counts
+ * (nullable, 4 per member): inner iterations of each refinement step.      */
+int fvx_mg_batch_status(void* batch, int slot, int n_outer_budget, fvx_cg_info* info,
+                        int* counts, void* stream);
+
 /* ---- Spectral rectangular Helmholtz solvers: solve_helmholtz_{dst,dct,fft}
  * (solvers/spectral.py:14-128, re-exported from spectraldiffx v0.0.10;
  * used by scripts/qg_1p5_layer.py:443).  bc 0 = DST-I (Dirichlet just
@@ -329,6 +350,21 @@ int fvx_qg_budgets(void* plan, int* out);
  * redone with a wider budget.  Budgets are calibrated by the first call. */
 int fvx_qg_step_graph(void* plan, void* q, double* psi, int nsteps, fvx_qg_info* info,
                       void* stream);
+/* A fixed work-list of QG stage evaluations over many FP64 fields (ensemble
+ * members, layers): one launch evaluates every entry; tend32 computes the
+ * tendencies in FP32 (no reference counterpart).                          */
+typedef struct fvx_qg_stage_desc {
+  const double* qs;
+  const double* psi;
+  const double* qn;
+  double* out;
+  const uint8_t* mask; /* full-array wet mask or NULL */
+  int Ny, Nx;
+  double dx, dy, beta, Ly, F0, nu4, a, b, cdt;
+} fvx_qg_stage_desc;
+int fvx_qg_stage_batch_create(int n, const fvx_qg_stage_desc* descs, int tend32, void** out);
+int fvx_qg_stage_batch_launch(void* handle, void* stream);
+int fvx_qg_stage_batch_destroy(void* handle);
 /* One fused tendency + stage combination out = a q_n + b q_s + cdt rhs(q_s, psi). */
 int fvx_qg_stage_f32(const float* qs, const double* psi, const float* qn, float* out,
                      const uint8_t* mask, int Ny, int Nx, double dx, double dy, double beta,
