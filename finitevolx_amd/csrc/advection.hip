@@ -185,12 +185,20 @@ __device__ __forceinline__ T face_flux(const T* s, int st, int fi, int n, T vel,
 
 constexpr int kAdvTx = 64, kAdvTy = 8, kAdvHalo = 3, kAdvThreads = kAdvTx * kAdvTy;
 
+// Optional fused Runge-Kutta stage: with hn set the kernel writes
+// a hn + b h + cdt L(h) in place of L(h) (out may alias hn, not h).
+template <typename T>
+struct StageCoef {
+  const T* hn = nullptr;
+  double a = 0.0, b = 0.0, cdt = 0.0;
+};
+
 template <typename T, int M, int L, bool MASKED>
 __global__ __launch_bounds__(kAdvThreads) void advection_kernel(
     const T* __restrict__ h, const T* __restrict__ u, const T* __restrict__ v,
-    T* __restrict__ out, const uint8_t* __restrict__ tier_x,
-    const uint8_t* __restrict__ tier_y, const uint8_t* __restrict__ mh, int ny, int nx,
-    T idx, T idy, int tiles_x, int ntiles) {
+    T* out, const uint8_t* __restrict__ tier_x, const uint8_t* __restrict__ tier_y,
+    const uint8_t* __restrict__ mh, int ny, int nx, T idx, T idy, int tiles_x, int ntiles,
+    const T* hn, T ca, T cb, T ccdt) {
   constexpr int SW = kAdvTx + 2 * kAdvHalo;
   __shared__ T sh[kAdvTy + 2 * kAdvHalo][SW];
   __shared__ T sfe[kAdvTy][kAdvTx + 1];
@@ -244,6 +252,7 @@ __global__ __launch_bounds__(kAdvThreads) void advection_kernel(
       r = -((sfe[ty][tx + 1] - sfe[ty][tx]) * idx + (sfn[ty + 1][tx] - sfn[ty][tx]) * idy);
     const int64_t c = (int64_t)gj * nx + gi;
     if (mh) r = r * (T)(mh[c] ? 1 : 0);
+    if (hn) r = ca * hn[c] + cb * sh[ty + kAdvHalo][tx + kAdvHalo] + ccdt * r;
     out[c] = r;
   }
 }
@@ -251,18 +260,20 @@ __global__ __launch_bounds__(kAdvThreads) void advection_kernel(
 template <typename T, int M, int L, bool MASKED>
 static void launch_adv(const T* h, const T* u, const T* v, T* out, const uint8_t* tx_,
                        const uint8_t* ty_, const uint8_t* mh, int ny, int nx, double dx,
-                       double dy, hipStream_t s) {
+                       double dy, const StageCoef<T>& sc, hipStream_t s) {
   const int tiles_x = cdiv(nx, kAdvTx), tiles_y = cdiv(ny, kAdvTy);
   hipLaunchKernelGGL((advection_kernel<T, M, L, MASKED>), dim3(tiles_x * tiles_y),
                      dim3(kAdvThreads), 0, s, h, u, v, out, tx_, ty_, mh, ny, nx,
-                     (T)(1.0 / dx), (T)(1.0 / dy), tiles_x, tiles_x * tiles_y);
+                     (T)(1.0 / dx), (T)(1.0 / dy), tiles_x, tiles_x * tiles_y, sc.hn, (T)sc.a,
+                     (T)sc.b, (T)sc.cdt);
 }
 
 template <typename T, bool MASKED>
 static int dispatch_adv(int method, int lim, const T* h, const T* u, const T* v, T* out,
                         const uint8_t* tx_, const uint8_t* ty_, const uint8_t* mh, int ny,
-                        int nx, double dx, double dy, hipStream_t s) {
-#define FVX_ADV(M, L) launch_adv<T, M, L, MASKED>(h, u, v, out, tx_, ty_, mh, ny, nx, dx, dy, s)
+                        int nx, double dx, double dy, const StageCoef<T>& sc, hipStream_t s) {
+#define FVX_ADV(M, L) \
+  launch_adv<T, M, L, MASKED>(h, u, v, out, tx_, ty_, mh, ny, nx, dx, dy, sc, s)
   if constexpr (MASKED) {
     switch (method) {
       case kWeno3: FVX_ADV(kWeno3, 0); break;
@@ -311,13 +322,17 @@ static int dispatch_adv(int method, int lim, const T* h, const T* u, const T* v,
 template <typename T>
 int advection(int method, int lim, const T* h, const T* u, const T* v, T* out,
               const uint8_t* tier_x, const uint8_t* tier_y, const uint8_t* mh, int ny, int nx,
-              double dx, double dy, hipStream_t s) {
+              double dx, double dy, hipStream_t s, const StageCoef<T>& sc = StageCoef<T>()) {
   FVX_CHECK_ARG(h && u && v && out, "null field pointer");
   FVX_CHECK_ARG(ny >= 5 && nx >= 5, "array too small (%d x %d); need >= 5 x 5", ny, nx);
   FVX_CHECK_ARG(dx > 0 && dy > 0, "grid spacing must be positive");
   FVX_CHECK_ARG((tier_x == nullptr) == (tier_y == nullptr), "tier_x and tier_y go together");
-  if (tier_x) return dispatch_adv<T, true>(method, lim, h, u, v, out, tier_x, tier_y, mh, ny, nx, dx, dy, s);
-  return dispatch_adv<T, false>(method, lim, h, u, v, out, nullptr, nullptr, mh, ny, nx, dx, dy, s);
+  FVX_CHECK_ARG(out != h, "out must not alias h (neighbouring tiles still read it)");
+  if (tier_x)
+    return dispatch_adv<T, true>(method, lim, h, u, v, out, tier_x, tier_y, mh, ny, nx, dx, dy,
+                                 sc, s);
+  return dispatch_adv<T, false>(method, lim, h, u, v, out, nullptr, nullptr, mh, ny, nx, dx, dy,
+                                sc, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -618,6 +633,25 @@ int fvx_face_flux_f32(int method, int limiter, int dir, const float* h, const fl
   return fvx::face_flux_call<float>(method, limiter, dir, h, vel, out, tier, ny, nx,
                                     (hipStream_t)stream);
 }
+
+#define FVX_ADV_STAGE(SUF, T)                                                                    \
+  int fvx_advection_stage_##SUF(int method, int limiter, const T* hs, const T* u, const T* v,    \
+                                const T* hn, T* out, const uint8_t* tier_x,                     \
+                                const uint8_t* tier_y, const uint8_t* mask_h, int ny, int nx,   \
+                                double dx, double dy, double a, double b, double cdt,           \
+                                void* stream) {                                                 \
+    FVX_CHECK_ARG(hn, "null hn");                                                               \
+    fvx::StageCoef<T> sc;                                                                        \
+    sc.hn = hn;                                                                                  \
+    sc.a = a;                                                                                    \
+    sc.b = b;                                                                                    \
+    sc.cdt = cdt;                                                                                \
+    return fvx::advection<T>(method, limiter, hs, u, v, out, tier_x, tier_y, mask_h, ny, nx, dx, \
+                             dy, (hipStream_t)stream, sc);                                       \
+  }
+FVX_ADV_STAGE(f32, float)
+FVX_ADV_STAGE(f64, double)
+#undef FVX_ADV_STAGE
 
 int fvx_advection2d_f64(int method, int limiter, const double* h, const double* u,
                         const double* v, double* out, const uint8_t* tier_x,

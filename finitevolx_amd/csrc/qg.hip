@@ -908,6 +908,62 @@ static int qg_run_graph(QGPlan& Q, TS* q, double* psi, int nsteps, QGInfoC* info
   return kOk;
 }
 
+// ---------------------------------------------------------------------------
+// Geostrophic face velocities of a T-point streamfunction (the composition
+// of scripts/qg_1p5_layer.py:115-145): psi is averaged to X points
+// (Interpolation2D.T_to_X, interior only, times mask.xy_corner_strict), then
+// u = -diff_y_X_to_U(psi_X) (times mask.u) and v = diff_x_X_to_V(psi_X)
+// (times mask.v), both on the interior with a zero ring.  One thread per
+// cell computes the three corner values it needs from nine psi loads.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void geostrophic_faces_kernel(
+    const double* __restrict__ psi, const uint8_t* __restrict__ mxy,
+    const uint8_t* __restrict__ mu, const uint8_t* __restrict__ mv, T* __restrict__ u,
+    T* __restrict__ v, int ny, int nx, double dx, double dy) {
+  const int64_t n = (int64_t)ny * nx;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n; c += stride) {
+    const int j = (int)(c / nx), i = (int)(c - (int64_t)j * nx);
+    auto X = [&](int jj, int ii) -> double {
+      if (jj < 1 || jj > ny - 2 || ii < 1 || ii > nx - 2) return 0.0;
+      const int64_t k = (int64_t)jj * nx + ii;
+      double x = 0.25 * (psi[k] + psi[k + 1] + psi[k + nx] + psi[k + nx + 1]);
+      if (mxy) x = x * (double)(mxy[k] ? 1 : 0);
+      return x;
+    };
+    double uu = 0.0, vv = 0.0;
+    if (j >= 1 && j <= ny - 2 && i >= 1 && i <= nx - 2) {
+      const double xc = X(j, i);
+      uu = -((xc - X(j - 1, i)) / dy);
+      vv = (xc - X(j, i - 1)) / dx;
+      if (mu) uu = uu * (double)(mu[c] ? 1 : 0);
+      if (mv) vv = vv * (double)(mv[c] ? 1 : 0);
+    }
+    u[c] = (T)uu;
+    v[c] = (T)vv;
+  }
+}
+
+template <typename T>
+static int geostrophic_faces(const double* psi, const uint8_t* mxy, const uint8_t* mu,
+                             const uint8_t* mv, T* u, T* v, int ny, int nx, double dx, double dy,
+                             hipStream_t s) {
+  FVX_CHECK_ARG(psi && u && v, "null field pointer");
+  FVX_CHECK_ARG((void*)u != (void*)psi && (void*)v != (void*)psi && (void*)u != (void*)v,
+                "u, v and psi must be distinct arrays");
+  FVX_CHECK_ARG(ny >= 3 && nx >= 3, "array too small (%d x %d)", ny, nx);
+  FVX_CHECK_ARG(dx > 0 && dy > 0, "grid spacing must be positive");
+  FVX_CHECK_ARG((mxy == nullptr) == (mu == nullptr) && (mu == nullptr) == (mv == nullptr),
+                "mask_xy, mask_u and mask_v go together");
+  const int64_t n = (int64_t)ny * nx;
+  const int blocks = (int)std::min<int64_t>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(geostrophic_faces_kernel<T>, dim3(blocks), dim3(256), 0, s, psi, mxy, mu,
+                     mv, u, v, ny, nx, dx, dy);
+  FVX_LAUNCH_CHECK();
+  return kOk;
+}
+
 }  // namespace fvx
 
 extern "C" {
@@ -1262,4 +1318,17 @@ FVX_QG_STAGE(f32, float, true)
 // FP64 state, tendency computed in FP32, combination in FP64.
 FVX_QG_STAGE(f64t32, double, true)
 #undef FVX_QG_STAGE
+
+int fvx_geostrophic_faces_f64(const double* psi, const uint8_t* mask_xy, const uint8_t* mask_u,
+                              const uint8_t* mask_v, double* u, double* v, int ny, int nx,
+                              double dx, double dy, void* stream) {
+  return fvx::geostrophic_faces<double>(psi, mask_xy, mask_u, mask_v, u, v, ny, nx, dx, dy,
+                                        (hipStream_t)stream);
+}
+int fvx_geostrophic_faces_f32(const double* psi, const uint8_t* mask_xy, const uint8_t* mask_u,
+                              const uint8_t* mask_v, float* u, float* v, int ny, int nx,
+                              double dx, double dy, void* stream) {
+  return fvx::geostrophic_faces<float>(psi, mask_xy, mask_u, mask_v, u, v, ny, nx, dx, dy,
+                                       (hipStream_t)stream);
+}
 }

@@ -58,20 +58,195 @@ class Member:
     mask: np.ndarray | None = None
 
 
-def hierarchy_depth(ny, nx, dx, Ld):
+def hierarchy_depth(ny, nx, dx, Ld, max_coarse_cells=64 * 64):
     """Coarsen until the cell size reaches 2 Ld (the QG model's rule), within
-    the halving rule of multigrid.py:50-76."""
-    ratio = 2.0 * Ld / dx
-    n = max(2, 1 + int(np.ceil(np.log2(max(ratio, 1.0)))))
+    the halving rule of multigrid.py:50-76, and at least until the coarsest
+    level has at most ``max_coarse_cells`` cells (a batched solve runs its
+    coarse levels in one workgroup's LDS)."""
     deepest, hy, hx = 1, ny, nx
+    fits = 1 if ny * nx <= max_coarse_cells else None
     while hy % 2 == 0 and hx % 2 == 0 and hy // 2 >= 8 and hx // 2 >= 8:
         hy //= 2
         hx //= 2
         deepest += 1
+        if fits is None and hy * hx <= max_coarse_cells:
+            fits = deepest
+    ratio = 2.0 * Ld / dx
+    if not np.isfinite(ratio):  # an infinite radius (Poisson): the deepest hierarchy
+        return deepest
+    n = max(2, 1 + int(np.ceil(np.log2(max(ratio, 1.0)))), fits or deepest)
     return min(n, deepest)
 
 
-class QGEnsemble:
+class BudgetedSteps:
+    """Graph-replayed rk3_ssp_step steps whose three stages each end in a
+    batch of mixed-precision solves (``self.batch``, slot = stage) with a
+    per-stage budget (refinement steps, inner iterations per step).
+
+    The first three steps run eagerly; the third (warm) step's counts become
+    the budgets.  Then each parity of the extrapolation buffers has a
+    captured graph; after ``win_len`` converged steps each stage's budget
+    comes down to the largest counts used in that window; a stage that
+    misses widens, the step is redone from the saved state and the window
+    doubles.  Subclasses define ``_enqueue_step(e, fresh)`` (which saves the
+    state first), ``_restore()``, ``_build_lists()`` and ``_state()`` (the
+    tensors a trial step may change)."""
+
+    def _init_budgets(self):
+        self.parity = 0
+        self.fresh = True
+        self.budgets = [(3, (12, 8, 8)) for _ in range(3)]
+        self.graphs = {}
+        self.steps_done = 0
+        self.retries = 0
+        self.recaptures = 0
+        self.win = None
+        self.win_steps = 0
+        self.win_len = 4
+        self._lists_ready = False
+
+    def _status(self):
+        """Per stage slot: (every member converged, refinement steps used,
+        inner iterations used per step (largest over members), total inner
+        iterations)."""
+        out = []
+        for k in range(3):
+            no, _ = self.budgets[k]
+            st = self.batch.status(slot=k, n_outer=no)
+            ok = all(s[0].converged for s in st)
+            outer = max(max(s[1] for s in st), 1)
+            inner = tuple(max(max(s[2][o] for s in st), 1) for o in range(min(outer, 4)))
+            out.append((ok, outer, inner, sum(s[0].iterations for s in st)))
+        return out
+
+    def _capture(self):
+        self.graphs = {}
+        for e in (0, 1):
+            g = torch.cuda.CUDAGraph()
+            s = torch.cuda.Stream()
+            with torch.cuda.stream(s):
+                with torch.cuda.graph(g, stream=s):
+                    self._enqueue_step(e)
+            self.graphs[e] = g
+        self.recaptures += 1
+
+    def step(self, nsteps: int = 1, graph: bool = True):
+        """Advance by ``nsteps`` rk3_ssp_step steps.  Returns a dict of
+        totals (solves, inner iterations, redone steps, converged)."""
+        info = dict(solves=0, iterations=0, retries=0, converged=True)
+        done = 0
+        while done < nsteps:
+            calib = self.steps_done < 3 or not graph
+            if calib:
+                self._enqueue_step(self.parity, fresh=self.fresh)
+            else:
+                if not self.graphs:
+                    if not self._lists_ready:
+                        self._warm_lists()
+                        self._lists_ready = True
+                    self._capture()
+                self.graphs[self.parity].replay()
+            st = self._status()
+            ok = all(s[0] for s in st)
+            if not ok:
+                # widen the missed stages' budgets and redo the step
+                for k, s in enumerate(st):
+                    if not s[0]:
+                        no, ni = self.budgets[k]
+                        ni = tuple(v + max(1, v // 4) for v in ni)
+                        self.budgets[k] = (min(no + 1, 8), ni + (ni[-1],) * (9 - len(ni)))
+                if not calib:
+                    self.win_len = min(64, 2 * self.win_len)
+                    self.win = None
+                    self.win_steps = 0
+                    self.graphs = {}
+                self._restore()
+                self.retries += 1
+                info["retries"] += 1
+                if self.retries > 50:
+                    raise RuntimeError("batched solves keep missing their budgets")
+                continue
+            for s in st:
+                info["solves"] += self.batch.n
+                info["iterations"] += s[3]
+            if calib and self.steps_done == 2:
+                # budgets from the third (warm) step's counts
+                self.budgets = [(s[1], s[2]) for s in st]
+            elif not calib:
+                self._tighten(st)
+            self.parity ^= 1
+            self.fresh = False
+            self.steps_done += 1
+            done += 1
+        return info
+
+    def _warm_lists(self):
+        # build every work-list and solver slot list eagerly, outside capture
+        self._build_lists()
+        saved = [t.clone() for t in self._state()]
+        for e in (0, 1):
+            self._enqueue_step(e)
+        torch.cuda.synchronize()
+        for t, sv in zip(self._state(), saved):
+            t.copy_(sv)
+
+    def _tighten(self, st):
+        """After win_len converged graph steps, each stage's budgets come down
+        to the largest counts its members used in that window."""
+        used = [(s[1], s[2]) for s in st]
+        if self.win is None:
+            self.win = used
+        else:
+            merged = []
+            for (ao, ai), (bo, bi) in zip(self.win, used):
+                n = max(ao, bo)
+                pad = lambda t: tuple(t) + (0,) * (n - len(t))  # noqa: E731
+                merged.append((n, tuple(max(x, y) for x, y in zip(pad(ai), pad(bi)))))
+            self.win = merged
+        self.win_steps += 1
+        if self.win_steps < self.win_len:
+            return
+        new = []
+        for (no, ni), (wo, wi) in zip(self.budgets, self.win):
+            o = min(no, max(1, wo))
+            wi = tuple(wi) + (1,) * max(0, o - len(wi))
+            new.append((o, tuple(min(ni[j], max(1, wi[j])) for j in range(o))))
+        self.win = None
+        self.win_steps = 0
+        if new != self.budgets:
+            self.budgets = new
+            self.graphs = {}
+
+
+def extrapolated_guess(ps, k, e, fresh):
+    """Warm start of stage k's streamfunction (qg.hip's stage_guess: psi is
+    taken linear in time over the last step's stages) over whole buffers.
+    ``ps`` holds four buffers: the previous and next step-start solutions
+    (which of ps[0] / ps[3] is which alternates with the parity ``e``) and
+    the stage-1 and stage-2 solutions."""
+    P, N = (ps[0], ps[3]) if e == 0 else (ps[3], ps[0])
+    B1, B2 = ps[1], ps[2]
+    if k == 0:
+        if fresh:
+            lincomb(1.0, B2, out=N)
+        else:
+            lincomb(-1 / 3, P, 2 / 3, B1, 2 / 3, B2, out=N)
+    elif k == 1:
+        if fresh:
+            lincomb(1.0, N, out=B1)
+        else:
+            lincomb(2.0, N, -1.0, P, out=B1)
+    else:
+        lincomb(0.5, N, 0.5, B1, out=B2)
+
+
+def stage_psi_buffer(ps, k, e):
+    """The buffer stage k of parity e solves into."""
+    N = ps[3] if e == 0 else ps[0]
+    return (N, ps[1], ps[2])[k]
+
+
+class QGEnsemble(BudgetedSteps):
     """Batched mixed-precision QG steps (FP64 PV state, FP32 tendencies and
     inner solves, FP64 refinement) of independent members."""
 
@@ -108,17 +283,7 @@ class QGEnsemble:
             self.view(self.q, k)[1:-1, 1:-1] = qi.to(dev)
         self.batch = MultigridBatch(self.solvers)
         self._stage = {}
-        self.parity = 0
-        self.fresh = True
-        self.budgets = [(3, (12, 8, 8)) for _ in range(3)]
-        self.graphs = {}
-        self.steps_done = 0
-        self.retries = 0
-        self.recaptures = 0
-        self.win = None
-        self.win_steps = 0
-        self.win_len = 4
-        self._lists_ready = False
+        self._init_budgets()
 
     # -- buffers ---------------------------------------------------------
     def view(self, buf, k):
@@ -158,26 +323,7 @@ class QGEnsemble:
         return self._stage[key]
 
     def _psi_buffer(self, k, e):
-        P, N = (self.ps[0], self.ps[3]) if e == 0 else (self.ps[3], self.ps[0])
-        return (N, self.ps[1], self.ps[2])[k]
-
-    def _guess(self, k, e, fresh):
-        """Warm start of stage k (the extrapolation of qg.hip's stage_guess)
-        over every member at once."""
-        P, N = (self.ps[0], self.ps[3]) if e == 0 else (self.ps[3], self.ps[0])
-        B1, B2 = self.ps[1], self.ps[2]
-        if k == 0:
-            if fresh:
-                lincomb(1.0, B2, out=N)
-            else:
-                lincomb(-1 / 3, P, 2 / 3, B1, 2 / 3, B2, out=N)
-        elif k == 1:
-            if fresh:
-                lincomb(1.0, N, out=B1)
-            else:
-                lincomb(2.0, N, -1.0, P, out=B1)
-        else:
-            lincomb(0.5, N, 0.5, B1, out=B2)
+        return stage_psi_buffer(self.ps, k, e)
 
     def _solve(self, k, e):
         qs = (self.q, self.q1, self.q2)[k]
@@ -191,125 +337,20 @@ class QGEnsemble:
     def _enqueue_step(self, e, fresh=False):
         self.backup.copy_(self.q)
         for k in range(3):
-            self._guess(k, e, fresh)
+            extrapolated_guess(self.ps, k, e, fresh)
             self._solve(k, e)
             _lib.call("fvx_qg_stage_batch_launch", self._stage_list(k, e), _lib.stream())
 
-    def _status(self):
-        """Per stage slot: (every member converged, refinement steps used,
-        inner iterations used per step (largest over members), total inner
-        iterations)."""
-        out = []
-        for k in range(3):
-            no, _ = self.budgets[k]
-            st = self.batch.status(slot=k, n_outer=no)
-            ok = all(s[0].converged for s in st)
-            outer = max(max(s[1] for s in st), 1)
-            inner = tuple(max(max(s[2][o] for s in st), 1) for o in range(min(outer, 4)))
-            out.append((ok, outer, inner, sum(s[0].iterations for s in st)))
-        return out
+    def _restore(self):
+        self.q.copy_(self.backup)
 
-    def _capture(self):
-        self.graphs = {}
-        for e in (0, 1):
-            # the first call of each list set builds its work-lists outside the capture
-            g = torch.cuda.CUDAGraph()
-            s = torch.cuda.Stream()
-            with torch.cuda.stream(s):
-                with torch.cuda.graph(g, stream=s):
-                    self._enqueue_step(e)
-            self.graphs[e] = g
-        self.recaptures += 1
+    def _state(self):
+        return [self.q] + list(self.ps)
 
-    def step(self, nsteps: int = 1, graph: bool = True):
-        """Advance every member by ``nsteps`` rk3_ssp_step steps.  Returns a
-        dict of totals (solves, inner iterations, redone steps, converged)."""
-        info = dict(solves=0, iterations=0, retries=0, converged=True)
-        done = 0
-        while done < nsteps:
-            calib = self.steps_done < 3 or not graph
-            if calib:
-                self._enqueue_step(self.parity, fresh=self.fresh)
-            else:
-                if not self.graphs:
-                    if not self._lists_ready:
-                        self._warm_lists()
-                        self._lists_ready = True
-                    self._capture()
-                self.graphs[self.parity].replay()
-            st = self._status()
-            ok = all(s[0] for s in st)
-            if not ok:
-                # widen the missed stages' budgets and redo the step
-                for k, s in enumerate(st):
-                    if not s[0]:
-                        no, ni = self.budgets[k]
-                        ni = tuple(v + max(1, v // 4) for v in ni)
-                        self.budgets[k] = (min(no + 1, 8), ni + (ni[-1],) * (9 - len(ni)))
-                if not calib:
-                    self.win_len = min(64, 2 * self.win_len)
-                    self.win = None
-                    self.win_steps = 0
-                    self.graphs = {}
-                self.q.copy_(self.backup)
-                self.retries += 1
-                info["retries"] += 1
-                if self.retries > 50:
-                    raise RuntimeError("ensemble solves keep missing their budgets")
-                continue
-            for s in st:
-                info["solves"] += self.n
-                info["iterations"] += s[3]
-            if calib and self.steps_done == 2:
-                # budgets from the third (warm) step's counts
-                self.budgets = [(s[1], s[2]) for s in st]
-            elif not calib:
-                self._tighten(st)
-            self.parity ^= 1
-            self.fresh = False
-            self.steps_done += 1
-            done += 1
-        return info
-
-    def _warm_lists(self):
-        # build every (stage, parity) work-list and solver slot list eagerly
+    def _build_lists(self):
         for e in (0, 1):
             for k in range(3):
                 self._stage_list(k, e)
-        saved = self.q.clone(), [p.clone() for p in self.ps]
-        for e in (0, 1):
-            self._enqueue_step(e)
-        torch.cuda.synchronize()
-        self.q.copy_(saved[0])
-        for p, sv in zip(self.ps, saved[1]):
-            p.copy_(sv)
-
-    def _tighten(self, st):
-        """After win_len converged graph steps, each stage's budgets come down
-        to the largest counts its members used in that window."""
-        used = [(s[1], s[2]) for s in st]
-        if self.win is None:
-            self.win = used
-        else:
-            merged = []
-            for (ao, ai), (bo, bi) in zip(self.win, used):
-                n = max(ao, bo)
-                pad = lambda t: tuple(t) + (0,) * (n - len(t))  # noqa: E731
-                merged.append((n, tuple(max(x, y) for x, y in zip(pad(ai), pad(bi)))))
-            self.win = merged
-        self.win_steps += 1
-        if self.win_steps < self.win_len:
-            return
-        new = []
-        for (no, ni), (wo, wi) in zip(self.budgets, self.win):
-            o = min(no, max(1, wo))
-            wi = tuple(wi) + (1,) * max(0, o - len(wi))
-            new.append((o, tuple(min(ni[j], max(1, wi[j])) for j in range(o))))
-        self.win = None
-        self.win_steps = 0
-        if new != self.budgets:
-            self.budgets = new
-            self.graphs = {}
 
 
 def config4_members(n_members: int = 96, seed: int = 0, dx: float = 5000.0,

@@ -106,6 +106,22 @@ int fvx_advection_rk3_step_noalloc_f64(int method, int limiter, double* h, const
                                        int nx, int ng, double dx, double dy, double dt,
                                        void* stream);
 
+/* ---- One Runge-Kutta stage of a tracer with an Advection2D tendency ---------
+ * out = a hn + b hs + cdt L(hs), L the Advection2D.__call__ tendency
+ * (advection.py:329-420; method / limiter / tiers / mask as fvx_advection2d_*)
+ * fused with the stage combination of rk3_ssp_step
+ * (timestepping/explicit_rk.py:74-121), the form the reference evaluates as
+ * a tendency call plus array arithmetic.  out may alias hn, not hs.          */
+int fvx_advection_stage_f32(int method, int limiter, const float* hs, const float* u,
+                            const float* v, const float* hn, float* out, const uint8_t* tier_x,
+                            const uint8_t* tier_y, const uint8_t* mask_h, int ny, int nx,
+                            double dx, double dy, double a, double b, double cdt, void* stream);
+int fvx_advection_stage_f64(int method, int limiter, const double* hs, const double* u,
+                            const double* v, const double* hn, double* out,
+                            const uint8_t* tier_x, const uint8_t* tier_y, const uint8_t* mask_h,
+                            int ny, int nx, double dx, double dy, double a, double b,
+                            double cdt, void* stream);
+
 /* ---- Reconstruction2D face fluxes (advection/reconstruction.py:572-1307) ---
  * <method>_x (dir 0, east faces) / <method>_y (dir 1, north faces): every face
  * 1..n-2 along the direction on the interior of the other, zero ring; method /
@@ -365,6 +381,19 @@ typedef struct fvx_qg_stage_desc {
 int fvx_qg_stage_batch_create(int n, const fvx_qg_stage_desc* descs, int tend32, void** out);
 int fvx_qg_stage_batch_launch(void* handle, void* stream);
 int fvx_qg_stage_batch_destroy(void* handle);
+/* ---- Geostrophic face velocities of a T-point streamfunction ---------------
+ * geostrophic_velocity_from_streamfunction (scripts/qg_1p5_layer.py:115-145):
+ * psi averaged to X points (Interpolation2D.T_to_X, interpolation.py:158-168),
+ * u = -diff_y_X_to_U, v = diff_x_X_to_V (difference.py:280-320), interior
+ * only with a zero ring; with masks (all three or none) the masked operators'
+ * xy_corner_strict / u / v multiply each result.  psi is FP64; u, v the
+ * suffix's type (a tracer's).                                               */
+int fvx_geostrophic_faces_f32(const double* psi, const uint8_t* mask_xy, const uint8_t* mask_u,
+                              const uint8_t* mask_v, float* u, float* v, int ny, int nx,
+                              double dx, double dy, void* stream);
+int fvx_geostrophic_faces_f64(const double* psi, const uint8_t* mask_xy, const uint8_t* mask_u,
+                              const uint8_t* mask_v, double* u, double* v, int ny, int nx,
+                              double dx, double dy, void* stream);
 /* One fused tendency + stage combination out = a q_n + b q_s + cdt rhs(q_s, psi). */
 int fvx_qg_stage_f32(const float* qs, const double* psi, const float* qn, float* out,
                      const uint8_t* mask, int Ny, int Nx, double dx, double dy, double beta,

@@ -46,6 +46,13 @@ mistake; ``tests/test_oracle_pins.py`` plants one per family):
 * vertical modes: closed forms of reference ``tests/test_vertical_modes.py``.
 * QG step (build-defined composition, no single reference function):
   pinned through its parts and against a direct-solve composition.
+* multi-layer QG step (composition of the vertical-mode API): the inverted
+  streamfunction satisfies the textbook layer equations
+  lap psi - f0^2 A psi = q when every eigenvalue is positive (equal layer
+  thicknesses); one layer reduces to the 1.5-layer step with
+  Ld^2 = H g / f0^2; the tracer velocity is exactly uniform for a linear
+  streamfunction and discretely non-divergent; forcing enters the top layer
+  only (``test_oracle_pins.py``).
 """
 
 from __future__ import annotations
@@ -1260,3 +1267,90 @@ def qg_script_step(q, p, wind):
     psi = qg_script_invert(qn, p)
     u, v = qg_script_velocity(psi, p)
     return qn, psi, u, v
+
+
+# ---------------------------------------------------------------------------
+# Multi-layer QG with a passive tracer (build-defined composition of the
+# reference's vertical-mode API, vertical/vertical_modes.py:50-263, with the
+# 1.5-layer step above and Advection2D; no single reference function)
+# ---------------------------------------------------------------------------
+
+
+class MultiLayerParams(QGParams):
+    """Layer stack and modes of the multi-layer QG step.
+
+    Layer l obeys dq_l/dt = -J(psi_l, q_l + beta (y - y0)) + F delta_{l0}
+    - nu4 D(D(q_l)), with the wind forcing on the top layer only.  The
+    streamfunction comes from the vertical modes: ``radii, Cl2m, Cm2l =
+    decompose_vertical_modes(build_coupling_matrix(H, g'), f0)``; mode k
+    solves (lap - 1/radii_k^2) psi_k = (Cl2m q)_k (psi = 0 on land and the
+    walls; an infinite radius gives a Poisson solve) and psi = Cm2l psi_modes.
+    The tracer c is advected by the top layer's geostrophic face velocities
+    (``scripts/qg_1p5_layer.py:115-145``) with masked Advection2D.
+    """
+
+    def __init__(self, nx, ny, Lx, Ly, H, g_prime, f0=9.375e-5, beta=1.754e-11, nu4=1e9,
+                 F0=2e-12, dt=900.0, tracer_method="weno5"):
+        super().__init__(nx, ny, Lx, Ly, f0=f0, beta=beta, Ld=1.0, nu4=nu4, F0=F0, dt=dt)
+        self.H = np.asarray(H, np.float64)
+        self.g_prime = np.asarray(g_prime, np.float64)
+        A = build_coupling_matrix(self.H, self.g_prime)
+        self.radii, self.Cl2m, self.Cm2l = decompose_vertical_modes(A, f0)
+        self.mode_lambda = np.where(np.isfinite(self.radii), 1.0 / self.radii**2, 0.0)
+        self.tracer_method = tracer_method
+
+    @property
+    def nl(self):
+        return self.H.size
+
+
+def multilayer_invert(q, p, wet):
+    """Layer streamfunctions (nl, ny+2, nx+2) from layer PV through the
+    vertical modes, one direct Dirichlet solve per mode."""
+    qm = layer_to_mode(q, p.Cl2m)
+    pm = np.zeros_like(q)
+    for k in range(p.nl):
+        pm[k, 1:-1, 1:-1] = dirichlet_helmholtz_direct(qm[k, 1:-1, 1:-1] * wet, wet, p.dx, p.dy,
+                                                       p.mode_lambda[k]) * wet
+    return mode_to_layer(pm, p.Cm2l)
+
+
+def multilayer_rhs(q, psi, p, F, wet):
+    """Per-layer QG tendency; forcing on layer 0 (the top) only."""
+    zero = np.zeros_like(F)
+    return np.stack([qg_rhs(q[l], psi[l], p, F if l == 0 else zero, wet) for l in range(p.nl)])
+
+
+def tracer_velocity(psi_top, p, mask=None):
+    """Top-layer face velocities (``scripts/qg_1p5_layer.py:115-145``, with
+    the masked operators when a mask is given)."""
+    px = T_to_X(psi_top, mask)
+    return -diff_y_X_to_U(px, p.dy, mask), diff_x_X_to_V(px, p.dx, mask)
+
+
+def multilayer_step(q, c, p, F, wet, mask=None):
+    """One ``rk3_ssp_step`` of the layer PV ``q`` (nl, ny+2, nx+2) and the
+    tracer ``c`` (ny+2, nx+2) together; returns (q, c, psi of the last
+    stage)."""
+    last = {}
+
+    def rhs(state):
+        qs, cs = state
+        psi = multilayer_invert(qs, p, wet)
+        u, v = tracer_velocity(psi[0], p, mask)
+        last["psi"] = psi
+        return (multilayer_rhs(qs, psi, p, F, wet),
+                advection_2d(cs, u, v, p.dx, p.dy, p.tracer_method, mask))
+
+    def comb(a, x, b, y, cdt, k):
+        # a x + b (y + cdt k) over the (q, c) pair
+        return tuple(a * xi + b * (yi + cdt * ki) for xi, yi, ki in zip(x, y, k))
+
+    s0 = (q, c)
+    k1 = rhs(s0)
+    y1 = comb(0.0, s0, 1.0, s0, p.dt, k1)
+    k2 = rhs(y1)
+    y2 = comb(0.75, s0, 0.25, y1, p.dt, k2)
+    k3 = rhs(y2)
+    qn, cn = comb(1.0 / 3.0, s0, 2.0 / 3.0, y2, p.dt, k3)
+    return qn, cn, last["psi"]

@@ -707,3 +707,106 @@ def test_momentum_planted_mistake_is_caught():
         return du - fvx.interior(extra[2:-2, 2:-2], u.shape, ghost=2), dv
 
     assert _momentum_failures(wrong_k)
+
+
+# ---------------------------------------------------------------------------
+# Multi-layer QG step (composition of vertical/vertical_modes.py:50-263 with
+# the 1.5-layer step and Advection2D)
+# ---------------------------------------------------------------------------
+
+
+def _basin(ny, nx):
+    wet = np.ones((ny, nx))
+    wet[3:5, 4:7] = 0.0  # an island
+    wet[0, :3] = 0.0     # a notch in the coast
+    return wet
+
+
+def _multilayer_failures(invert, rhs, velocity):
+    fails = []
+    f0 = 9.375e-5
+    ny, nx = 10, 12
+    L = 1.2e6
+    wet = _basin(ny, nx)
+    # equal thicknesses: A symmetric positive definite, so the modal solves
+    # must reproduce the textbook layer equations lap psi_l - f0^2 (A psi)_l = q_l
+    p = fvx.MultiLayerParams(nx, ny, L, L * ny / nx, [500.0, 500.0, 500.0], [9.81, 0.02, 0.02],
+                             f0=f0)
+    rng = np.random.default_rng(23)
+    q = np.zeros((3, ny + 2, nx + 2))
+    q[:, 1:-1, 1:-1] = rng.standard_normal((3, ny, nx)) * 1e-6 * wet
+    psi = invert(q, p, wet)
+    A = fvx.build_coupling_matrix(p.H, p.g_prime)
+    lap, idx = fvx.dirichlet_helmholtz_matrix(wet, p.dx, p.dy, 0.0)
+    m = wet > 0
+    for l in range(3):
+        lhs = lap @ psi[l, 1:-1, 1:-1][m] - f0**2 * sum(A[l, k] * psi[k, 1:-1, 1:-1][m]
+                                                         for k in range(3))
+        if not np.allclose(lhs, q[l, 1:-1, 1:-1][m], rtol=1e-9, atol=1e-9 * np.abs(q).max()):
+            fails.append(f"layer equation {l}")
+    if np.any(psi[:, 1:-1, 1:-1][:, ~m] != 0) or np.any(psi[:, 0] != 0):
+        fails.append("psi on land")
+    # forcing enters the top layer only
+    F = p.forcing()
+    r = rhs(np.zeros_like(q), np.zeros_like(q), p, F, wet)
+    if not (np.allclose(r[0, 1:-1, 1:-1], F[1:-1, 1:-1] * wet, rtol=1e-14, atol=0)
+            and np.all(r[1:] == 0)):
+        fails.append("forcing layer")
+    # tracer velocity: uniform for a linear streamfunction psi = -U0 y,
+    # discretely non-divergent for any streamfunction
+    U0 = 0.7
+    yc = p.y_centres()
+    lin = np.broadcast_to((-U0 * yc)[:, None], (ny + 2, nx + 2)).copy()
+    u, v = velocity(lin, p)
+    if not (np.allclose(u[2:-1, 1:-1], U0, rtol=1e-12) and np.allclose(v[1:-1, 2:-1], 0.0,
+                                                                        atol=1e-15)):
+        fails.append("uniform flow")
+    u, v = velocity(rng.standard_normal((ny + 2, nx + 2)), p)
+    div = (u[2:-2, 2:-2] - u[2:-2, 1:-3]) / p.dx + (v[2:-2, 2:-2] - v[1:-3, 2:-2]) / p.dy
+    if not np.allclose(div, 0.0, atol=1e-12 / p.dx):
+        fails.append("divergence")
+    return fails
+
+
+def test_multilayer_pins():
+    assert _multilayer_failures(fvx.multilayer_invert, fvx.multilayer_rhs,
+                                fvx.tracer_velocity) == []
+
+
+def test_multilayer_reduces_to_one_and_a_half_layers():
+    ny = nx = 12
+    L = 1.2e6
+    H, g = 800.0, 0.02
+    p = fvx.MultiLayerParams(nx, ny, L, L, [H], [g], nu4=1e9, F0=2e-12, dt=1200.0)
+    p1 = fvx.QGParams(nx, ny, L, L, Ld=np.sqrt(H * g) / p.f0, nu4=1e9, F0=2e-12, dt=1200.0)
+    wet = _basin(ny, nx)
+    q = np.zeros((ny + 2, nx + 2))
+    q[1:-1, 1:-1] = np.random.default_rng(5).standard_normal((ny, nx)) * 1e-6 * wet
+    F = p.forcing()
+    got, c, _ = fvx.multilayer_step(q[None], np.zeros_like(q), p, F, wet, fvx.Mask2D(np.pad(wet, 1)))
+    want = fvx.qg_step(q, p1, F, wet)
+    np.testing.assert_allclose(got[0], want, rtol=1e-10, atol=1e-10 * np.abs(want).max())
+    assert np.all(c == 0)
+
+
+def test_multilayer_planted_mistake_is_caught():
+    def swapped(q, p, wet):  # the two mode transforms exchanged
+        class P:
+            pass
+        w = P()
+        w.__dict__.update(p.__dict__)
+        w.Cl2m, w.Cm2l = p.Cm2l, p.Cl2m
+        w.nl = p.nl
+        return fvx.multilayer_invert(q, w, wet)
+
+    def bottom_forced(q, psi, p, F, wet):
+        return fvx.multilayer_rhs(q, psi, p, F, wet)[::-1].copy()
+
+    def flipped_v(psi, p, mask=None):
+        u, v = fvx.tracer_velocity(psi, p, mask)
+        return u, -v
+
+    assert "layer equation 0" in _multilayer_failures(swapped, fvx.multilayer_rhs,
+                                                      fvx.tracer_velocity)
+    assert _multilayer_failures(fvx.multilayer_invert, bottom_forced, fvx.tracer_velocity)
+    assert _multilayer_failures(fvx.multilayer_invert, fvx.multilayer_rhs, flipped_v)
