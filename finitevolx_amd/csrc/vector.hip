@@ -5,37 +5,47 @@
 
 namespace fvx {
 
-// out = a*x + b*y + c*z   (y or z may be null -> term dropped)
+// out = a*x + b*y + c*z + d*w   (y, z or w may be null -> term dropped).
+// out may be one of the inputs: each element is read before it is written.
 template <typename T>
-__global__ __launch_bounds__(256) void lincomb3_kernel(int64_t n, T a, const T* __restrict__ x,
-                                                       T b, const T* __restrict__ y, T c,
-                                                       const T* __restrict__ z,
-                                                       T* __restrict__ out) {
+__global__ __launch_bounds__(256) void lincomb4_kernel(int64_t n, T a, const T* x, T b,
+                                                       const T* y, T c, const T* z, T d,
+                                                       const T* w, T* out) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += stride) {
     T r = a * x[k];
     if (y) r += b * y[k];
     if (z) r += c * z[k];
+    if (w) r += d * w[k];
     out[k] = r;
   }
 }
 
 template <typename T>
-int lincomb3(int64_t n, double a, const T* x, double b, const T* y, double c, const T* z,
-             T* out, hipStream_t s) {
+int lincomb4(int64_t n, double a, const T* x, double b, const T* y, double c, const T* z,
+             double d, const T* w, T* out, hipStream_t s) {
   FVX_CHECK_ARG(x && out, "null pointer");
   FVX_CHECK_ARG(n >= 0, "negative length");
   if (n == 0) return kOk;
   const int blocks = (int)std::min<int64_t>((n + 255) / 256, 8192);
-  hipLaunchKernelGGL(lincomb3_kernel<T>, dim3(blocks), dim3(256), 0, s, n, (T)a, x, (T)b, y,
-                     (T)c, z, out);
+  hipLaunchKernelGGL(lincomb4_kernel<T>, dim3(blocks), dim3(256), 0, s, n, (T)a, x, (T)b, y,
+                     (T)c, z, (T)d, w, out);
   FVX_LAUNCH_CHECK();
   return kOk;
+}
+template <typename T>
+int lincomb3(int64_t n, double a, const T* x, double b, const T* y, double c, const T* z,
+             T* out, hipStream_t s) {
+  return lincomb4<T>(n, a, x, b, y, c, z, 0.0, nullptr, out, s);
 }
 template int lincomb3<float>(int64_t, double, const float*, double, const float*, double,
                              const float*, float*, hipStream_t);
 template int lincomb3<double>(int64_t, double, const double*, double, const double*, double,
                               const double*, double*, hipStream_t);
+template int lincomb4<float>(int64_t, double, const float*, double, const float*, double,
+                             const float*, double, const float*, float*, hipStream_t);
+template int lincomb4<double>(int64_t, double, const double*, double, const double*, double,
+                              const double*, double, const double*, double*, hipStream_t);
 
 // out[k, p] = sum_l C[k, l] in[l, p] for nl <= 8 layers (layer_to_mode /
 // mode_to_layer, vertical/vertical_modes.py:193-263): one pass, each
@@ -100,5 +110,13 @@ int fvx_lincomb3_f64(int64_t n, double a, const double* x, double b, const doubl
 int fvx_lincomb3_f32(int64_t n, double a, const float* x, double b, const float* y, double c,
                      const float* z, float* out, void* stream) {
   return fvx::lincomb3<float>(n, a, x, b, y, c, z, out, (hipStream_t)stream);
+}
+int fvx_lincomb4_f64(int64_t n, double a, const double* x, double b, const double* y, double c,
+                     const double* z, double d, const double* w, double* out, void* stream) {
+  return fvx::lincomb4<double>(n, a, x, b, y, c, z, d, w, out, (hipStream_t)stream);
+}
+int fvx_lincomb4_f32(int64_t n, double a, const float* x, double b, const float* y, double c,
+                     const float* z, double d, const float* w, float* out, void* stream) {
+  return fvx::lincomb4<float>(n, a, x, b, y, c, z, d, w, out, (hipStream_t)stream);
 }
 }

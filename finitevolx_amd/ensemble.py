@@ -219,8 +219,9 @@ class BudgetedSteps:
 
 
 def extrapolated_guess(ps, k, e, fresh):
-    """Warm start of stage k's streamfunction (qg.hip's stage_guess: psi is
-    taken linear in time over the last step's stages) over whole buffers.
+    """Warm start of stage k's streamfunction over whole buffers (the
+    extrapolation of qg.hip's stage_guess: psi of dt times the tendency,
+    known exactly from the previous step's stages, taken linear in time).
     ``ps`` holds four buffers: the previous and next step-start solutions
     (which of ps[0] / ps[3] is which alternates with the parity ``e``) and
     the stage-1 and stage-2 solutions."""
@@ -230,14 +231,17 @@ def extrapolated_guess(ps, k, e, fresh):
         if fresh:
             lincomb(1.0, B2, out=N)
         else:
-            lincomb(-1 / 3, P, 2 / 3, B1, 2 / 3, B2, out=N)
+            lincomb(2.0, B2, -1.0, P, out=N)
     elif k == 1:
         if fresh:
             lincomb(1.0, N, out=B1)
         else:
-            lincomb(2.0, N, -1.0, P, out=B1)
+            lincomb(4.0, N, -2.0, B2, -1.0, B1, out=B1)
     else:
-        lincomb(0.5, N, 0.5, B1, out=B2)
+        if fresh:
+            lincomb(1.0, B1, out=B2)
+        else:
+            lincomb(1.0, B1, -0.75, N, 0.25, P, out=B2, d=0.5, w=B2)
 
 
 def stage_psi_buffer(ps, k, e):

@@ -14,8 +14,11 @@ import torch
 from finitevolx_amd import _lib
 
 _lib.extra_signatures({
-    f"fvx_lincomb3_{s}": [_lib.C.c_int64, _lib._d, _lib._vp, _lib._d, _lib._vp, _lib._d,
-                          _lib._vp, _lib._vp, _lib._vp] for s in ("f32", "f64")
+    **{f"fvx_lincomb3_{s}": [_lib.C.c_int64, _lib._d, _lib._vp, _lib._d, _lib._vp, _lib._d,
+                             _lib._vp, _lib._vp, _lib._vp] for s in ("f32", "f64")},
+    **{f"fvx_lincomb4_{s}": [_lib.C.c_int64, _lib._d, _lib._vp, _lib._d, _lib._vp, _lib._d,
+                             _lib._vp, _lib._d, _lib._vp, _lib._vp, _lib._vp]
+       for s in ("f32", "f64")},
 })
 
 
@@ -28,23 +31,28 @@ def _tree_map(fn, *trees):
     return fn(*trees)
 
 
-def lincomb(a, x, b=0.0, y=None, c=0.0, z=None, out=None):
-    """``a*x + b*y + c*z`` on device tensors in one kernel (into ``out`` when
-    given: a contiguous tensor of x's shape and type that aliases none of the
-    inputs)."""
+def lincomb(a, x, b=0.0, y=None, c=0.0, z=None, out=None, d=0.0, w=None):
+    """``a*x + b*y + c*z + d*w`` on device tensors in one kernel (into ``out``
+    when given: a contiguous tensor of x's shape and type, which may be one of
+    the inputs -- each element is read before it is written)."""
     x = _lib.as_field(x)
     y = None if y is None else _lib.as_field(y, x.dtype)
     z = None if z is None else _lib.as_field(z, x.dtype)
-    for t in (y, z):
+    w = None if w is None else _lib.as_field(w, x.dtype)
+    for t in (y, z, w):
         if t is not None and t.shape != x.shape:
             raise ValueError("shape mismatch in lincomb")
     if out is None:
         out = torch.empty_like(x)
-    elif (out.shape != x.shape or out.dtype != x.dtype or not out.is_contiguous()
-          or any(t is not None and t.data_ptr() == out.data_ptr() for t in (x, y, z))):
-        raise ValueError("out must be a contiguous tensor like x that aliases no input")
-    _lib.call(f"fvx_lincomb3_{_lib.suffix(x)}", x.numel(), float(a), x.data_ptr(), float(b),
-              _lib.ptr(y), float(c), _lib.ptr(z), out.data_ptr(), _lib.stream())
+    elif out.shape != x.shape or out.dtype != x.dtype or not out.is_contiguous():
+        raise ValueError("out must be a contiguous tensor like x")
+    if w is None:
+        _lib.call(f"fvx_lincomb3_{_lib.suffix(x)}", x.numel(), float(a), x.data_ptr(), float(b),
+                  _lib.ptr(y), float(c), _lib.ptr(z), out.data_ptr(), _lib.stream())
+    else:
+        _lib.call(f"fvx_lincomb4_{_lib.suffix(x)}", x.numel(), float(a), x.data_ptr(), float(b),
+                  _lib.ptr(y), float(c), _lib.ptr(z), float(d), w.data_ptr(), out.data_ptr(),
+                  _lib.stream())
     return out
 
 

@@ -186,11 +186,16 @@ int fvx_enforce_periodic_f32(float* field, int ny, int nx, int ng, void* stream)
 int fvx_enforce_periodic_f64(double* field, int ny, int nx, int ng, void* stream);
 
 /* ---- rk3_ssp_step stage combination (timestepping/explicit_rk.py:104-121) --
- * out = a*x + b*y + c*z over n elements (y, z nullable).                    */
+ * out = a*x + b*y + c*z (+ d*w) over n elements (y, z, w nullable); out may
+ * be one of the inputs (each element is read before it is written).         */
 int fvx_lincomb3_f32(int64_t n, double a, const float* x, double b, const float* y, double c,
                      const float* z, float* out, void* stream);
 int fvx_lincomb3_f64(int64_t n, double a, const double* x, double b, const double* y, double c,
                      const double* z, double* out, void* stream);
+int fvx_lincomb4_f32(int64_t n, double a, const float* x, double b, const float* y, double c,
+                     const float* z, double d, const float* w, float* out, void* stream);
+int fvx_lincomb4_f64(int64_t n, double a, const double* x, double b, const double* y, double c,
+                     const double* z, double d, const double* w, double* out, void* stream);
 
 /* ---- layer_to_mode / mode_to_layer (vertical/vertical_modes.py:193-263) ---
  * out[k, :] = sum_l C[k, l] in[l, :]; C is a HOST row-major nl x nl matrix

@@ -182,6 +182,24 @@ def test_periodic_and_rk3():
     assert_close_scaled(to_np(out), ref, 1e-14, "rk3")
 
 
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_lincomb_four_terms_in_place(dtype):
+    from finitevolx_amd.timestepping import lincomb
+
+    need_gpu()
+    rng = np.random.default_rng(8)
+    x, y, z, w = (torch.tensor(v, dtype=dtype, device="cuda") for v in rng.standard_normal((4, 37, 41)))
+    x0 = to_np(x)
+    want = x0 - 0.75 * to_np(y) + 0.25 * to_np(z) + 0.5 * to_np(w)
+    tol = 1e-15 if dtype == torch.float64 else 1e-6
+    assert_close_scaled(to_np(lincomb(1.0, x, -0.75, y, 0.25, z, d=0.5, w=w)), want, tol, "lincomb4")
+    # in place into the fourth and into the first input
+    lincomb(1.0, x, -0.75, y, 0.25, z, out=w, d=0.5, w=w)
+    assert_close_scaled(to_np(w), want, tol, "lincomb4 in place")
+    lincomb(2.0, x, -1.0, y, out=x)
+    assert_close_scaled(to_np(x), 2.0 * x0 - to_np(y), tol, "lincomb3 in place")
+
+
 @pytest.mark.parametrize("variant", ["unmasked", "masked"])
 def test_momentum_advection_goldens(canon, variant):
     """MomentumAdvection2D (energy scheme) against the reference's goldens."""
