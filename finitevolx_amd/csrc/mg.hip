@@ -2317,8 +2317,14 @@ static int v_cycle_impl(Plan& P, const T* u0, const T* rhs, T* out, T* az, const
       else launch_tile<T, KIND, 0, kDown, false>(P, L, &C, src, b, L.u, nullptr, sc, s);
     }
   }
-  // Coarse levels lc .. nl-1 in one workgroup (zero guess at level lc).
-  if (lc < nl) {
+  // Coarse levels lc .. nl-1 in one workgroup (zero guess at level lc); a
+  // bottom level too large for it gets n_coarse damped-Jacobi sweeps from
+  // zero on the tiled kernels instead.
+  if (lc == nl - 1 && (int64_t)H.lv[lc].ny * H.lv[lc].nx > kCoarseCells) {
+    const LevelDev<T>& B = H.lv[lc];
+    smooth_chunks<T, KIND>(P, B, nullptr, true, B.b, std::max(P.n_coarse, 1), B.t, B.t2, B.u, sc,
+                           s);
+  } else if (lc < nl) {
     CoarseArgs<T> ca;
     std::memset(&ca, 0, sizeof(ca));
     for (int l = 0; l < nl; ++l) ca.lv[l] = H.lv[l];

@@ -59,10 +59,11 @@ class Member:
 
 
 def hierarchy_depth(ny, nx, dx, Ld, max_coarse_cells=64 * 64):
-    """Coarsen until the cell size reaches 2 Ld (the QG model's rule), within
-    the halving rule of multigrid.py:50-76, and at least until the coarsest
-    level has at most ``max_coarse_cells`` cells (a batched solve runs its
-    coarse levels in one workgroup's LDS)."""
+    """Coarsen until the cell size reaches 2 Ld, within the halving rule of
+    multigrid.py:50-76, and at least until the coarsest level has at most
+    ``max_coarse_cells`` cells: a batched solve runs its coarse levels in one
+    workgroup's LDS (the single-model QG plan stops at Ld / 2 and smooths a
+    large bottom level on the tiled kernels instead)."""
     deepest, hy, hx = 1, ny, nx
     fits = 1 if ny * nx <= max_coarse_cells else None
     while hy % 2 == 0 and hx % 2 == 0 and hy // 2 >= 8 and hx // 2 >= 8:
