@@ -1917,6 +1917,7 @@ struct Plan {
   Hier<double> h64;
   Hier<float> h32;
   double* partials = nullptr;
+  size_t npart = 0;  // capacity of partials in doubles
   PcgScalars* sc = nullptr;   // kSlots pairs: [2k] inner, [2k+1] outer
   PcgScalars* hsc = nullptr;  // pinned host mirror
   int last_iters = 4, last_outer = 2, last_inner = 4;
@@ -2206,7 +2207,7 @@ static void finish2(const Plan& P, int ny, int nx, PcgScalars* sc, hipStream_t s
                      P.partials, nblk, sc, rtol2, atol2);
 }
 
-// nu Jacobi sweeps on level L, up to 2 per kernel.  Ping-pongs between the
+// nu Jacobi sweeps on level L, up to 3 per kernel.  Ping-pongs between the
 // scratch arrays s1 / s2 (which must differ from u_in and final_out); the last
 // chunk writes final_out when given.  Returns the array holding the result.
 template <typename T, int KIND>
@@ -2215,12 +2216,15 @@ static T* smooth_chunks(const Plan& P, const LevelDev<T>& L, const T* u_in, bool
                         hipStream_t s) {
   const T* src = u_in;
   T* dst = nullptr;
-  const int K = (nu + 1) / 2;
+  const int K = (nu + 2) / 3;
   for (int c = 0; c < K; ++c) {
-    const int ns = (c == K - 1 && (nu & 1)) ? 1 : 2;
+    const int ns = std::min(3, nu - 3 * c);
     dst = (c == K - 1 && final_out) ? final_out : ((c & 1) ? s2 : s1);
     const bool z = zero && c == 0;
-    if (ns == 2) {
+    if (ns == 3) {
+      if (z) launch_tile<T, KIND, 3, kSmooth, true>(P, L, nullptr, src, b, dst, nullptr, sc, s);
+      else launch_tile<T, KIND, 3, kSmooth, false>(P, L, nullptr, src, b, dst, nullptr, sc, s);
+    } else if (ns == 2) {
       if (z) launch_tile<T, KIND, 2, kSmooth, true>(P, L, nullptr, src, b, dst, nullptr, sc, s);
       else launch_tile<T, KIND, 2, kSmooth, false>(P, L, nullptr, src, b, dst, nullptr, sc, s);
     } else {
@@ -2600,10 +2604,26 @@ static void launch_refine_update(Plan& P, double* x, int64_t ld_x, bool deferred
                        L32.nx, x, ld_x, H.x, out);
 }
 
+double* plan_partials(Plan& P, size_t* capacity) {
+  if (capacity) *capacity = P.npart;
+  return P.partials;
+}
+
+void certify_finish(Plan& P, int nblk, double rtol, double atol, int slot, bool fp64_true,
+                    hipStream_t s) {
+  PcgScalars* out = &P.sc[2 * slot + 1];
+  if (fp64_true)
+    hipLaunchKernelGGL((reduce_finish_kernel<2, kRedTrue>), dim3(1), dim3(kFinishThreads), 0, s,
+                       P.partials, nblk, out, rtol * rtol, atol * atol, 0.0);
+  else
+    hipLaunchKernelGGL((reduce_finish_kernel<2, kRedRefine>), dim3(1), dim3(kFinishThreads), 0,
+                       s, P.partials, nblk, out, rtol * rtol, atol * atol, 0.0);
+}
+
 template <typename TB>
 int mixed_enqueue(Plan& P, const TB* rhs, int64_t ld_rhs, double* x, int64_t ld_x, double rtol,
                   double atol, int n_outer, const int* n_inner, double inner_rtol, int slot,
-                  hipStream_t s, PrecondFn pf, void* ctx) {
+                  hipStream_t s, PrecondFn pf, void* ctx, bool defer_final) {
   Hier<float>& H = P.h32;
   const auto& L64 = P.h64.lv[0];
   const auto& L32 = H.lv[0];
@@ -2626,7 +2646,7 @@ int mixed_enqueue(Plan& P, const TB* rhs, int64_t ld_rhs, double* x, int64_t ld_
     if (rc) return rc;
     launch_refine_update(P, x, ld_x, deferred, in, out, s);
   }
-  residual(false);
+  if (!defer_final) residual(false);
   FVX_LAUNCH_CHECK();
   return kOk;
 }
@@ -2977,10 +2997,11 @@ template int pcg_enqueue<float>(Plan&, const float*, int64_t, float*, int64_t, d
 template int pcg_enqueue<double>(Plan&, const double*, int64_t, double*, int64_t, double, double,
                                  bool, int, int, hipStream_t);
 template int mixed_enqueue<float>(Plan&, const float*, int64_t, double*, int64_t, double, double,
-                                  int, const int*, double, int, hipStream_t, PrecondFn, void*);
+                                  int, const int*, double, int, hipStream_t, PrecondFn, void*,
+                                  bool);
 template int mixed_enqueue<double>(Plan&, const double*, int64_t, double*, int64_t, double,
                                    double, int, const int*, double, int, hipStream_t, PrecondFn,
-                                   void*);
+                                   void*, bool);
 template int certify_enqueue<double>(Plan&, const double*, int64_t, const double*, int64_t,
                                      double, double, int, hipStream_t);
 template int certify_enqueue<float>(Plan&, const float*, int64_t, const float*, int64_t, double,
@@ -3077,6 +3098,7 @@ int fvx_mg_plan_create(const double* mask, const double* coeff, int ny, int nx, 
     const size_t ncg = (size_t)kNumCgDots * std::max<size_t>(
         (size_t)cdiv(nx, kTx) * cdiv(ny, kTy), (size_t)cdiv(nx, 256) * cdiv(ny, kStencilRows));
     const size_t npart = std::max(n2, ncg) + 64;
+    P->npart = npart;
     hipError_t e = hipMalloc(&P->partials, npart * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&P->sc, 2 * kSlots * sizeof(PcgScalars));
     if (e == hipSuccess) e = hipMemset(P->sc, 0, 2 * kSlots * sizeof(PcgScalars));

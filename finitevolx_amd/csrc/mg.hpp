@@ -54,10 +54,22 @@ template <typename T>
 int pcg_solve(Plan& P, const T* rhs, int64_t ld_rhs, T* x, int64_t ld_x, double rtol,
               double atol, int max_iters, bool use_x0, CgInfoC* info, int slot, hipStream_t s,
               double cert_rtol = -1.0);
+// defer_final: leave the closing FP64 residual to the caller, which writes
+// its ||b||^2 and ||b - A x||^2 block partials into plan_partials() and calls
+// certify_finish (the QG step does this inside its stage kernel).
 template <typename TB>
 int mixed_enqueue(Plan& P, const TB* rhs, int64_t ld_rhs, double* x, int64_t ld_x, double rtol,
                   double atol, int n_outer, const int* n_inner, double inner_rtol, int slot,
-                  hipStream_t s, PrecondFn pf = nullptr, void* ctx = nullptr);
+                  hipStream_t s, PrecondFn pf = nullptr, void* ctx = nullptr,
+                  bool defer_final = false);
+// The plan's reduction scratch and its capacity in doubles.
+double* plan_partials(Plan& P, size_t* capacity);
+// Finishing kernel of a certification residual whose block partials
+// (value-major: ||b||^2 of block k at [k], ||r||^2 at [nblk + k]) a caller has
+// written: the mixed solve's closing residual (fp64_true = false) or the
+// FP64 PCG's true residual (fp64_true = true) of slot ``slot``.
+void certify_finish(Plan& P, int nblk, double rtol, double atol, int slot, bool fp64_true,
+                    hipStream_t s);
 // Synchronous mixed solve.  With ``pf`` the inner FP32 solves use it; an
 // inner solve that hits ``max_inner`` without reaching its tolerance makes the
 // remaining sweeps fall back to the multigrid V-cycle (counted in

@@ -182,13 +182,40 @@ __device__ __forceinline__ double load_state(const double* a, int n, unsigned co
   return a[(int64_t)row_off + col];
 }
 
+// Optional certification residual of the Helmholtz solve that produced psi,
+// fused into the stage that consumes it: on every interior cell the FP64
+// r = (q_s m - (lap(psi m) - lam psi m) m) m of the solve's operator (the
+// refinement residual of mg.hip; psi is the solve's solution, q_s its
+// right-hand side), and block partials of ||q_s m||^2 and ||r||^2
+// (value-major, as the solver's finishing kernel reads them).
+struct StageCert {
+  double* partials;  // 2 x nblk doubles
+  double lam, idx2, idy2;
+};
+
+// Block sums of (bb, rr) of a 256-thread block into partials[k * nblk + bid].
+__device__ __forceinline__ void stage_partials(double bb, double rr, double* partials, int nblk,
+                                               int bid) {
+  __shared__ double red[4][2];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const double s0 = wave_sum_last(bb), s1 = wave_sum_last(rr);
+  if (lane == 63) {
+    red[wid][0] = s0;
+    red[wid][1] = s1;
+  }
+  __syncthreads();
+  if (tid < 2) partials[(int64_t)tid * nblk + bid] = ((red[0][tid] + red[1][tid]) + red[2][tid]) +
+                                                     red[3][tid];
+}
+
 // Body over block ``bx`` of ``nblk`` (four tiles per block); with MASKED a
-// null mask means all wet.
-template <typename TS, typename TC, typename TP, bool MASKED>
+// null mask means all wet.  With CERT the block also forms the certification
+// residual partials of ``cert`` (block index bx of nblk).
+template <typename TS, typename TC, typename TP, bool MASKED, bool CERT = false>
 __device__ __forceinline__ void qg_stage_wave_body(
     const TS* __restrict__ qs, const TP* __restrict__ psi, const TS* __restrict__ qn,
     TS* __restrict__ out, const uint8_t* __restrict__ mask, int Ny, int Nx, const QGStageParams& p,
-    int tiles_x, int ntiles, int bx, int nblk) {
+    int tiles_x, int ntiles, int bx, int nblk, const StageCert& cert = StageCert()) {
   using T = TC;
   constexpr int R = kQWRows;
   const int lane = threadIdx.x & 63;
@@ -214,11 +241,23 @@ __device__ __forceinline__ void qg_stage_wave_body(
     q[t] = (T)v;
     if (t >= 2 && t < R + 2) qc[t - 2] = v;
   }
+  // With CERT: psi * mask in FP64 on the interior, zero elsewhere (the
+  // solve's masked, zero-padded operand), and the mask of each row.
+  double xm[R + 2];
+  double md[R + 2];
 #pragma unroll
   for (int t = 0; t < R + 2; ++t) {
     const int gj = j0 - 1 + t;
-    const TP v = psi[(int64_t)min(max(gj, 0), Ny - 1) * Nx + gic];
+    const int64_t g = (int64_t)min(max(gj, 0), Ny - 1) * Nx + gic;
+    const TP v = psi[g];
     f[t] = (colin && gj >= 0 && gj < Ny) ? (T)v : T(0);
+    if constexpr (CERT) {
+      const bool in = colin && gj >= 1 && gj <= Ny - 2 && gi >= 1 && gi <= Nx - 2;
+      double m = 1.0;
+      if constexpr (MASKED) m = (!mask || mask[g]) ? 1.0 : 0.0;
+      md[t] = m;
+      xm[t] = in ? (double)v * m : 0.0;
+    }
   }
   // beta (y - y0) at cell centres y_j = (j - 1/2) dy for rows j0-1 .. j0+R and
   // F = -F0 sin(kF y) for rows j0 .. j0+R-1: one row per lane, then read back
@@ -255,8 +294,22 @@ __device__ __forceinline__ void qg_stage_wave_body(
   const T nu4 = (T)p.nu4, inv12 = (T)p.inv12;
   const TS a = (TS)p.a, b = (TS)p.b, cdt = (TS)p.cdt;
   const bool lout = colin && lane >= 2 && lane < 62;
+  double bb = 0.0, rr = 0.0;
 #pragma unroll
   for (int t = 0; t < R; ++t) {
+    if constexpr (CERT) {
+      const int y = t + 1, gj = j0 + t;
+      const double c = xm[y];
+      const double e = lane_e(c), w = lane_w(c);
+      if (lout && ci && gj >= 1 && gj <= Ny - 2) {
+        const double lap = (e + w - 2.0 * c) * cert.idx2 + (xm[y + 1] + xm[y - 1] - 2.0 * c) * cert.idy2;
+        const double m = md[y];
+        const double bv = (double)qc[t] * m;
+        const double rv = (bv - (lap - cert.lam * c) * m) * m;
+        bb += bv * bv;
+        rr += rv * rv;
+      }
+    }
     const int y = t + 1;  // f, l1, qE, qW, by index of this row
     const int qy = t + 2;  // q index of this row
     const int gj = j0 + t;
@@ -292,15 +345,16 @@ __device__ __forceinline__ void qg_stage_wave_body(
       out[g] = res;
     }
   }
+  if constexpr (CERT) stage_partials(bb, rr, cert.partials, nblk, bx);
 }
 
-template <typename TS, typename TC, typename TP, bool MASKED>
+template <typename TS, typename TC, typename TP, bool MASKED, bool CERT = false>
 __global__ __launch_bounds__(256) void qg_stage_wave_kernel(
     const TS* __restrict__ qs, const TP* __restrict__ psi, const TS* __restrict__ qn,
     TS* __restrict__ out, const uint8_t* __restrict__ mask, int Ny, int Nx, QGStageParams p,
-    int tiles_x, int ntiles) {
-  qg_stage_wave_body<TS, TC, TP, MASKED>(qs, psi, qn, out, mask, Ny, Nx, p, tiles_x, ntiles,
-                                         blockIdx.x, gridDim.x);
+    int tiles_x, int ntiles, StageCert cert) {
+  qg_stage_wave_body<TS, TC, TP, MASKED, CERT>(qs, psi, qn, out, mask, Ny, Nx, p, tiles_x, ntiles,
+                                               blockIdx.x, gridDim.x, cert);
 }
 
 // Work-list launch of the stage over many fields (ensemble members, layers):
@@ -336,31 +390,43 @@ __global__ __launch_bounds__(256) void qg_stage_batch_kernel(const StageEntry* _
                                                first[e + 1] - first[e]);
 }
 
+// Blocks of the wavefront-tile stage kernel on an Ny x Nx array.
+static int stage_blocks(int Ny, int Nx) { return cdiv(cdiv(Nx, 60) * cdiv(Ny, kQWRows), 4); }
+
+template <typename TS, typename TC, typename TP, bool MASKED>
+static void launch_stage_wave(const TS* qs, const TP* psi, const TS* qn, TS* out,
+                              const uint8_t* mask, int Ny, int Nx, const QGStageParams& p,
+                              const StageCert* cert, hipStream_t s) {
+  const int wtx = cdiv(Nx, 60), ntiles = wtx * cdiv(Ny, kQWRows);
+  const dim3 grid(stage_blocks(Ny, Nx)), block(256);
+  if (cert)
+    hipLaunchKernelGGL((qg_stage_wave_kernel<TS, TC, TP, MASKED, true>), grid, block, 0, s, qs,
+                       psi, qn, out, mask, Ny, Nx, p, wtx, ntiles, *cert);
+  else
+    hipLaunchKernelGGL((qg_stage_wave_kernel<TS, TC, TP, MASKED, false>), grid, block, 0, s, qs,
+                       psi, qn, out, mask, Ny, Nx, p, wtx, ntiles, StageCert());
+}
+
 // tend32: compute the tendency of an FP64 state in FP32 (FP32 states always
-// use FP32).
+// use FP32).  cert (FP64 states only): also form the certification residual
+// partials of the solve that produced psi (stage_blocks(Ny, Nx) blocks).
 template <typename TS, typename TP>
 int qg_stage(const TS* qs, const TP* psi, const TS* qn, TS* out, const uint8_t* mask, int Ny,
-             int Nx, const QGStageParams& p, hipStream_t s, bool tend32 = false) {
+             int Nx, const QGStageParams& p, hipStream_t s, bool tend32 = false,
+             const StageCert* cert = nullptr) {
   FVX_CHECK_ARG(qs && psi && qn && out, "null field pointer");
   FVX_CHECK_ARG(Ny >= 3 && Nx >= 3, "array too small");
   FVX_CHECK_ARG(out != qs && out != (const TS*)psi, "out must not alias q_s or psi");
+  FVX_CHECK_ARG(!cert || (std::is_same<TS, double>::value &&
+                          (int64_t)Ny * Nx < (int64_t(1) << 29)),
+                "the fused certification residual needs an FP64 state");
   if ((int64_t)Ny * Nx < (int64_t(1) << 29)) {
-    const int wtx = cdiv(Nx, 60), ntiles = wtx * cdiv(Ny, kQWRows);
-    const dim3 grid(cdiv(ntiles, 4)), block(256);
     if (std::is_same<TS, float>::value || tend32) {
-      if (mask)
-        hipLaunchKernelGGL((qg_stage_wave_kernel<TS, float, TP, true>), grid, block, 0, s, qs,
-                           psi, qn, out, mask, Ny, Nx, p, wtx, ntiles);
-      else
-        hipLaunchKernelGGL((qg_stage_wave_kernel<TS, float, TP, false>), grid, block, 0, s, qs,
-                           psi, qn, out, mask, Ny, Nx, p, wtx, ntiles);
+      if (mask) launch_stage_wave<TS, float, TP, true>(qs, psi, qn, out, mask, Ny, Nx, p, cert, s);
+      else launch_stage_wave<TS, float, TP, false>(qs, psi, qn, out, mask, Ny, Nx, p, cert, s);
     } else if constexpr (std::is_same<TS, double>::value) {
-      if (mask)
-        hipLaunchKernelGGL((qg_stage_wave_kernel<TS, double, TP, true>), grid, block, 0, s, qs,
-                           psi, qn, out, mask, Ny, Nx, p, wtx, ntiles);
-      else
-        hipLaunchKernelGGL((qg_stage_wave_kernel<TS, double, TP, false>), grid, block, 0, s, qs,
-                           psi, qn, out, mask, Ny, Nx, p, wtx, ntiles);
+      if (mask) launch_stage_wave<TS, double, TP, true>(qs, psi, qn, out, mask, Ny, Nx, p, cert, s);
+      else launch_stage_wave<TS, double, TP, false>(qs, psi, qn, out, mask, Ny, Nx, p, cert, s);
     }
     FVX_LAUNCH_CHECK();
     return kOk;
@@ -606,15 +672,28 @@ static int qg_step_enqueue(QGPlan& Q, const TS* q_in, TS* q_out, int e, hipStrea
     int rc = kOk;
     double* psi = stage_guess(Q, k, e, s, &rc);
     if (rc) return rc;
+    // FP64 states: the solve's closing FP64 residual is formed inside the
+    // stage kernel that reads the same q_s and psi (one pass less per stage);
+    // its finishing kernel then certifies the solve as before.  The stage's
+    // output is only kept when the host finds every solve certified.
+    const bool fuse = std::is_same<TS, double>::value;
+    StageCert cert;
+    if (fuse) {
+      size_t cap = 0;
+      cert.partials = plan_partials(*Q.mg, &cap);
+      cert.lam = Q.lam;
+      cert.idx2 = 1.0 / (Q.dx * Q.dx);
+      cert.idy2 = 1.0 / (Q.dy * Q.dy);
+      FVX_CHECK_ARG(cap >= (size_t)2 * stage_blocks(Q.Ny, Q.Nx), "reduction scratch too small");
+    }
     if (Q.solve == kSolveMixed) {
       rc = mixed_enqueue<TS>(*Q.mg, qs + off, Nx, psi + off, Nx, Q.rtol, 0.0, Q.kb[k].outer,
                              Q.kb[k].inner, Q.inner_rtol, k, s, Q.net ? &net_precond : nullptr,
-                             Q.net ? &Q : nullptr);
+                             Q.net ? &Q : nullptr, fuse);
     } else {
       if constexpr (std::is_same<TS, double>::value) {
         rc = pcg_enqueue<double>(*Q.mg, qs + off, Nx, psi + off, Nx, 0.5 * Q.rtol, 0.0, true,
                                  Q.kb[k].iters, k, s);
-        if (!rc) rc = certify_enqueue<double>(*Q.mg, qs + off, Nx, psi + off, Nx, Q.rtol, 0.0, k, s);
       } else {
         set_error("FP64 MG-PCG needs an FP64 state");
         return kInvalidArgument;
@@ -622,8 +701,11 @@ static int qg_step_enqueue(QGPlan& Q, const TS* q_in, TS* q_out, int e, hipStrea
     }
     if (rc) return rc;
     rc = qg_stage<TS, double>(qs, psi, q_in, out, Q.mask, Q.Ny, Q.Nx,
-                              stage_params(Q, coef[k][0], coef[k][1], coef[k][2]), s, Q.tend32);
+                              stage_params(Q, coef[k][0], coef[k][1], coef[k][2]), s, Q.tend32,
+                              fuse ? &cert : nullptr);
     if (rc) return rc;
+    if (fuse)
+      certify_finish(*Q.mg, stage_blocks(Q.Ny, Q.Nx), Q.rtol, 0.0, k, Q.solve != kSolveMixed, s);
   }
   return kOk;
 }
@@ -1332,6 +1414,46 @@ FVX_QG_STAGE(f32, float, true)
 // FP64 state, tendency computed in FP32, combination in FP64.
 FVX_QG_STAGE(f64t32, double, true)
 #undef FVX_QG_STAGE
+
+// The same stage with the fused certification residual of the Helmholtz
+// solve (lap - lam) psi = q_s on the interior: block partials of ||q_s m||^2
+// (partials[0 .. nblk-1]) and ||q_s m - A psi||^2 (partials[nblk .. 2nblk-1]).
+#define FVX_QG_STAGE_CERT(SUF, T32)                                                               \
+  int fvx_qg_stage_cert_##SUF(const double* qs, const double* psi, const double* qn, double* out, \
+                              const uint8_t* mask, int Ny, int Nx, double dx, double dy,         \
+                              double beta, double Ly, double F0, double nu4, double a, double b, \
+                              double cdt, double lam, double* partials, int64_t partials_cap,    \
+                              int* nblk, void* stream) {                                         \
+    using namespace fvx;                                                                          \
+    FVX_CHECK_ARG(partials && nblk, "null partials or nblk");                                     \
+    FVX_CHECK_ARG(Ny >= 3 && Nx >= 3 && dx > 0 && dy > 0, "bad grid");                           \
+    const int nb = stage_blocks(Ny, Nx);                                                          \
+    FVX_CHECK_ARG(partials_cap >= 2 * (int64_t)nb, "partials needs %d doubles", 2 * nb);         \
+    QGStageParams p;                                                                              \
+    p.idx = 1.0 / dx;                                                                             \
+    p.idy = 1.0 / dy;                                                                             \
+    p.inv12 = 1.0 / (12.0 * dx * dy);                                                             \
+    p.beta = beta;                                                                                \
+    p.dy = dy;                                                                                    \
+    p.y0 = 0.5 * Ly;                                                                              \
+    p.F0 = F0;                                                                                    \
+    p.kF = 2.0 * M_PI / Ly;                                                                       \
+    p.nu4 = nu4;                                                                                  \
+    p.a = a;                                                                                      \
+    p.b = b;                                                                                      \
+    p.cdt = cdt;                                                                                  \
+    StageCert c;                                                                                  \
+    c.partials = partials;                                                                        \
+    c.lam = lam;                                                                                  \
+    c.idx2 = 1.0 / (dx * dx);                                                                     \
+    c.idy2 = 1.0 / (dy * dy);                                                                     \
+    *nblk = nb;                                                                                   \
+    return qg_stage<double, double>(qs, psi, qn, out, mask, Ny, Nx, p, (hipStream_t)stream, T32,  \
+                                    &c);                                                          \
+  }
+FVX_QG_STAGE_CERT(f64, false)
+FVX_QG_STAGE_CERT(f64t32, true)
+#undef FVX_QG_STAGE_CERT
 
 int fvx_geostrophic_faces_f64(const double* psi, const uint8_t* mask_xy, const uint8_t* mask_u,
                               const uint8_t* mask_v, double* u, double* v, int ny, int nx,

@@ -71,7 +71,7 @@ class QGModel:
     def __init__(self, nx: int, ny: int, Lx: float, Ly: float, *, beta: float = 1.754e-11,
                  Ld: float = 4e4, nu4: float = 1e9, F0: float = 2e-12, dt: float = 1200.0,
                  mask=None, precision: str = "float64", n_pre: int = 3, n_post: int = 3,
-                 n_coarse: int = 4, omega: float = 0.85, rtol: float = 1e-8,
+                 n_coarse: int = 3, omega: float = 0.85, rtol: float = 1e-8,
                  inner_rtol: float = 1e-4, max_iters: int = 100, max_outer: int = 10):
         if precision not in MODES:
             raise ValueError(f"precision must be one of {list(MODES)}")

@@ -408,6 +408,25 @@ int fvx_qg_stage_f64(const double* qs, const double* psi, const double* qn, doub
                      const uint8_t* mask, int Ny, int Nx, double dx, double dy, double beta,
                      double Ly, double F0, double nu4, double a, double b, double cdt,
                      void* stream);
+/* FP64 state with the tendency computed in FP32 (the mixed-precision step). */
+int fvx_qg_stage_f64t32(const double* qs, const double* psi, const double* qn, double* out,
+                        const uint8_t* mask, int Ny, int Nx, double dx, double dy, double beta,
+                        double Ly, double F0, double nu4, double a, double b, double cdt,
+                        void* stream);
+/* The FP64-state stage with the certification residual of the Helmholtz solve
+ * (lap - lam) psi = q_s (interior, masked Dirichlet operator) fused in: block
+ * partials of ||q_s m||^2 into partials[0 .. *nblk-1] and of ||r||^2 into
+ * partials[*nblk .. 2 *nblk - 1] (device array of partials_cap doubles). */
+int fvx_qg_stage_cert_f64(const double* qs, const double* psi, const double* qn, double* out,
+                          const uint8_t* mask, int Ny, int Nx, double dx, double dy, double beta,
+                          double Ly, double F0, double nu4, double a, double b, double cdt,
+                          double lam, double* partials, int64_t partials_cap, int* nblk,
+                          void* stream);
+int fvx_qg_stage_cert_f64t32(const double* qs, const double* psi, const double* qn, double* out,
+                             const uint8_t* mask, int Ny, int Nx, double dx, double dy,
+                             double beta, double Ly, double F0, double nu4, double a, double b,
+                             double cdt, double lam, double* partials, int64_t partials_cap,
+                             int* nblk, void* stream);
 
 /* ---- Learned U-Net preconditioner (no reference counterpart) -------------
  * Bias-free U-Net: nlev levels of two 3x3 convs + ReLU (channels[l], each a

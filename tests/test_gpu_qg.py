@@ -56,6 +56,52 @@ def test_stage_kernel_matches_oracle(masked, shape):
 
 
 @pytest.mark.parametrize("masked", [False, True])
+@pytest.mark.parametrize("suffix", ["f64", "f64t32"])
+def test_stage_with_certification_residual(masked, suffix):
+    """The stage kernel's fused certification residual: block partials of
+    ||q_s||^2 and ||q_s - (lap - lambda) psi||^2 (masked Dirichlet operator of
+    the solve) sum to the oracle's norms; the stage output is unchanged."""
+    import ctypes as C
+
+    from finitevolx_amd import _lib
+
+    need_gpu()
+    ny, nx = 97, 130
+    p = _params(nx, ny)
+    wet = _basin_mask(ny, nx) if masked else np.ones((ny, nx), bool)
+    rng = np.random.default_rng(4)
+    qs = np.zeros((ny + 2, nx + 2))
+    qs[1:-1, 1:-1] = rng.standard_normal((ny, nx)) * 1e-5 * wet
+    psi = np.zeros_like(qs)
+    psi[1:-1, 1:-1] = rng.standard_normal((ny, nx)) * 1e4 * wet
+    b = qs[1:-1, 1:-1] * wet
+    A, _ = fvx.dirichlet_helmholtz_matrix(wet, p.dx, p.dy, p.lambda_)
+    m = wet > 0
+    r = b[m] - A @ psi[1:-1, 1:-1][m]
+    full = np.zeros((ny + 2, nx + 2), np.uint8)
+    full[1:-1, 1:-1] = wet
+    dev = torch.device("cuda")
+    t_qs, t_psi = torch.tensor(qs, device=dev), torch.tensor(psi, device=dev)
+    out = torch.zeros_like(t_qs)
+    mask = torch.tensor(full, device=dev) if masked else None
+    parts = torch.zeros(4096, dtype=torch.float64, device=dev)
+    nblk = C.c_int(0)
+    _lib.extra_signatures({f"fvx_qg_stage_cert_{suffix}": [_lib._vp] * 5 + [_lib._i, _lib._i]
+                           + [_lib._d] * 10 + [_lib._vp, C.c_int64, C.POINTER(C.c_int), _lib._vp]})
+    _lib.call(f"fvx_qg_stage_cert_{suffix}", t_qs.data_ptr(), t_psi.data_ptr(), t_qs.data_ptr(),
+              out.data_ptr(), _lib.ptr(mask), ny + 2, nx + 2, p.dx, p.dy, p.beta, p.Ly, p.F0,
+              p.nu4, 0.75, 0.25, 0.25 * p.dt, p.lambda_, parts.data_ptr(), parts.numel(),
+              C.byref(nblk), _lib.stream())
+    pv = parts.cpu().numpy()
+    n = nblk.value
+    assert 0 < n <= 2048
+    np.testing.assert_allclose(pv[:n].sum(), (b * b).sum(), rtol=1e-12)
+    np.testing.assert_allclose(pv[n:2 * n].sum(), (r * r).sum(), rtol=1e-10)
+    ref = 0.75 * qs + 0.25 * qs + 0.25 * p.dt * fvx.qg_rhs(qs, psi, p, p.forcing(), wet)
+    assert_close_scaled(to_np(out), ref, 1e-12 if suffix == "f64" else 2e-5, "stage output")
+
+
+@pytest.mark.parametrize("masked", [False, True])
 def test_fp64_steps_match_oracle(masked):
     need_gpu()
     ny, nx = 64, 96
