@@ -466,7 +466,7 @@ struct QGPlan {
   // before (four arrays B0..B3, rotating with period two): stage k of a step
   // solves into its own array, warm-started from an extrapolation of the
   // previous stages (stage_guess).
-  double* ps[4] = {nullptr, nullptr, nullptr, nullptr};
+  double* ps[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   int parity = 0;          // 0: B0 holds the last step's stage-0 psi, 1: B3
   bool fresh = true;       // no step taken since the history was reset
   const double* last_psi = nullptr;  // caller's psi written by the last call
@@ -549,26 +549,83 @@ static QGStageParams stage_params(const QGPlan& Q, double a, double b, double c)
 // The first step after a reset has no P: stage 0 starts from the last psi,
 // stage 1 from N, stage 2 from B1.  warm == 0 starts every stage from the
 // previous stage's solution.  Returns the stage's streamfunction array.
-static double* stage_guess(QGPlan& Q, int k, int e, hipStream_t s, int* rc) {
-  const int64_t n = (int64_t)Q.Ny * Q.Nx;
-  double* P = Q.ps[e == 0 ? 0 : 3];
-  double* N = Q.ps[e == 0 ? 3 : 0];
-  double* B1 = Q.ps[1];
-  double* B2 = Q.ps[2];
+// The six history arrays in the roles of a step of parity e: the stage-0
+// solutions of the previous and this step (P, N), and the stage-1 / stage-2
+// solutions of the previous step (B1o, B2o) and of this one (B1n, B2n); the
+// roles alternate with the parity, so no warm start overwrites an array it
+// reads.
+struct StageBufs {
+  double *P, *N, *B1o, *B1n, *B2o, *B2n;
+};
+static StageBufs stage_bufs(const QGPlan& Q, int e) {
+  StageBufs b;
+  b.P = Q.ps[e == 0 ? 0 : 3];
+  b.N = Q.ps[e == 0 ? 3 : 0];
+  b.B1o = Q.ps[e == 0 ? 1 : 4];
+  b.B1n = Q.ps[e == 0 ? 4 : 1];
+  b.B2o = Q.ps[e == 0 ? 2 : 5];
+  b.B2n = Q.ps[e == 0 ? 5 : 2];
+  return b;
+}
+
+struct GuessTerms {
+  const double* h[4];
+  double c[4];
+  int n;
+};
+
+// Terms of stage k's warm start (x = sum g.c[t] g.h[t]) and the array the
+// stage solves into.
+static double* stage_guess_terms(const QGPlan& Q, int k, int e, GuessTerms* g) {
+  const StageBufs b = stage_bufs(Q, e);
   const bool extrap = Q.warm == 1 && !Q.fresh;
+  auto set = [&](int t, double c, const double* h) {
+    g->c[t] = c;
+    g->h[t] = h;
+  };
+  for (int t = 0; t < 4; ++t) set(t, 0.0, nullptr);
   if (k == 0) {
-    *rc = extrap ? lincomb3<double>(n, 2.0, B2, -1.0, P, 0.0, nullptr, N, s)
-                 : lincomb3<double>(n, 1.0, B2, 0.0, nullptr, 0.0, nullptr, N, s);
-    return N;
+    if (extrap) {
+      set(0, 2.0, b.B2o);
+      set(1, -1.0, b.P);
+      g->n = 2;
+    } else {
+      set(0, 1.0, b.B2o);
+      g->n = 1;
+    }
+    return b.N;
   }
   if (k == 1) {
-    *rc = extrap ? lincomb3<double>(n, 4.0, N, -2.0, B2, -1.0, B1, B1, s)
-                 : lincomb3<double>(n, 1.0, N, 0.0, nullptr, 0.0, nullptr, B1, s);
-    return B1;
+    if (extrap) {
+      set(0, 4.0, b.N);
+      set(1, -2.0, b.B2o);
+      set(2, -1.0, b.B1o);
+      g->n = 3;
+    } else {
+      set(0, 1.0, b.N);
+      g->n = 1;
+    }
+    return b.B1n;
   }
-  *rc = extrap ? lincomb4<double>(n, 1.0, B1, -0.75, N, 0.25, P, 0.5, B2, B2, s)
-               : lincomb3<double>(n, 1.0, B1, 0.0, nullptr, 0.0, nullptr, B2, s);
-  return B2;
+  if (extrap) {
+    set(0, 1.0, b.B1n);
+    set(1, -0.75, b.N);
+    set(2, 0.25, b.P);
+    set(3, 0.5, b.B2o);
+    g->n = 4;
+  } else {
+    set(0, 1.0, b.B1n);
+    g->n = 1;
+  }
+  return b.B2n;
+}
+
+static double* stage_guess(QGPlan& Q, int k, int e, hipStream_t s, int* rc) {
+  const int64_t n = (int64_t)Q.Ny * Q.Nx;
+  GuessTerms g;
+  double* x = stage_guess_terms(Q, k, e, &g);
+  *rc = lincomb4<double>(n, g.c[0], g.h[0], g.c[1], g.h[1], g.c[2], g.h[2], g.c[3], g.h[3], x, s);
+  return x;
 }
 
 // Take the caller's psi as the warm start unless it is the array the last
@@ -576,14 +633,15 @@ static double* stage_guess(QGPlan& Q, int k, int e, hipStream_t s, int* rc) {
 static int sync_history(QGPlan& Q, const double* psi, hipStream_t s) {
   if (psi == Q.last_psi && !Q.fresh) return kOk;
   const size_t bytes = (size_t)Q.Ny * Q.Nx * sizeof(double);
-  for (int b = 0; b < 4; ++b)
+  for (int b = 0; b < 6; ++b)
     FVX_HIP(hipMemcpyAsync(Q.ps[b], psi, bytes, hipMemcpyDeviceToDevice, s));
   Q.fresh = true;
   return kOk;
 }
 
+// The last stage's streamfunction: B2n of the last step (B2o of the next).
 static int return_psi(QGPlan& Q, double* psi, hipStream_t s) {
-  FVX_HIP(hipMemcpyAsync(psi, Q.ps[2], (size_t)Q.Ny * Q.Nx * sizeof(double),
+  FVX_HIP(hipMemcpyAsync(psi, stage_bufs(Q, Q.parity).B2o, (size_t)Q.Ny * Q.Nx * sizeof(double),
                          hipMemcpyDeviceToDevice, s));
   Q.last_psi = psi;
   return kOk;
@@ -1150,7 +1208,7 @@ int fvx_qg_plan_create(int ny, int nx, double Lx, double Ly, double beta, double
   if (e == hipSuccess) e = hipMalloc(&Q->q2, N * es);
   if (e == hipSuccess) e = hipMalloc(&Q->qa, N * es);
   if (e == hipSuccess) e = hipMalloc(&Q->qb, N * es);
-  for (int b = 0; b < 4 && e == hipSuccess; ++b) {
+  for (int b = 0; b < 6 && e == hipSuccess; ++b) {
     e = hipMalloc(&Q->ps[b], N * sizeof(double));
     if (e == hipSuccess) e = hipMemset(Q->ps[b], 0, N * sizeof(double));
   }
@@ -1188,7 +1246,7 @@ int fvx_qg_plan_destroy(void* plan) {
   (void)hipFree(Q->q2);
   (void)hipFree(Q->qa);
   (void)hipFree(Q->qb);
-  for (int b = 0; b < 4; ++b) (void)hipFree(Q->ps[b]);
+  for (int b = 0; b < 6; ++b) (void)hipFree(Q->ps[b]);
   (void)hipFree(Q->mask);
   delete Q;
   return kOk;

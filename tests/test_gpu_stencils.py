@@ -198,6 +198,10 @@ def test_lincomb_four_terms_in_place(dtype):
     assert_close_scaled(to_np(w), want, tol, "lincomb4 in place")
     lincomb(2.0, x, -1.0, y, out=x)
     assert_close_scaled(to_np(x), 2.0 * x0 - to_np(y), tol, "lincomb3 in place")
+    # views one element in: not 16-byte aligned (the scalar kernel)
+    xs, ys = x.reshape(-1)[1:], y.reshape(-1)[1:]
+    assert_close_scaled(to_np(lincomb(0.5, xs, 3.0, ys)), 0.5 * to_np(xs) + 3.0 * to_np(ys), tol,
+                        "lincomb unaligned")
 
 
 @pytest.mark.parametrize("variant", ["unmasked", "masked"])
