@@ -79,6 +79,13 @@ def hierarchy_depth(ny, nx, dx, Ld, max_coarse_cells=64 * 64):
     return min(n, deepest)
 
 
+def _budget(outer, inner, margin):
+    """(refinement steps, inner iterations per step) from observed counts
+    (reported for the first four steps; later steps repeat the fourth)."""
+    ni = tuple(v + margin for v in inner)
+    return outer, ni + (ni[-1],) * max(0, outer - len(ni))
+
+
 class BudgetedSteps:
     """Graph-replayed rk3_ssp_step steps whose three stages each end in a
     batch of mixed-precision solves (``self.batch``, slot = stage) with a
@@ -87,11 +94,15 @@ class BudgetedSteps:
     The first three steps run eagerly; the third (warm) step's counts become
     the budgets.  Then each parity of the extrapolation buffers has a
     captured graph; after ``win_len`` converged steps each stage's budget
-    comes down to the largest counts used in that window; a stage that
-    misses widens, the step is redone from the saved state and the window
-    doubles.  Subclasses define ``_enqueue_step(e, fresh)`` (which saves the
+    comes down to the largest counts used in that window.  When a stage
+    misses, the step is redone eagerly from the saved state with a generous
+    budget for that stage, and its observed counts plus one iteration per
+    refinement step become the new budgets (the window doubles, up to 16
+    steps).  Subclasses define ``_enqueue_step(e, fresh)`` (which saves the
     state first), ``_restore()``, ``_build_lists()`` and ``_state()`` (the
     tensors a trial step may change)."""
+
+    WIDE = (6, (48, 32, 16, 16, 16, 16))
 
     def _init_budgets(self):
         self.parity = 0
@@ -105,6 +116,7 @@ class BudgetedSteps:
         self.win_steps = 0
         self.win_len = 4
         self._lists_ready = False
+        self._recal = False
 
     def _status(self):
         """Per stage slot: (every member converged, refinement steps used,
@@ -137,7 +149,7 @@ class BudgetedSteps:
         info = dict(solves=0, iterations=0, retries=0, converged=True)
         done = 0
         while done < nsteps:
-            calib = self.steps_done < 3 or not graph
+            calib = self.steps_done < 3 or not graph or self._recal
             if calib:
                 self._enqueue_step(self.parity, fresh=self.fresh)
             else:
@@ -150,17 +162,17 @@ class BudgetedSteps:
             st = self._status()
             ok = all(s[0] for s in st)
             if not ok:
-                # widen the missed stages' budgets and redo the step
+                # redo the step eagerly with a generous budget for the missed
+                # stages; its counts then set the budgets
                 for k, s in enumerate(st):
                     if not s[0]:
-                        no, ni = self.budgets[k]
-                        ni = tuple(v + max(1, v // 4) for v in ni)
-                        self.budgets[k] = (min(no + 1, 8), ni + (ni[-1],) * (9 - len(ni)))
+                        self.budgets[k] = self.WIDE
                 if not calib:
-                    self.win_len = min(64, 2 * self.win_len)
-                    self.win = None
-                    self.win_steps = 0
-                    self.graphs = {}
+                    self.win_len = min(16, 2 * self.win_len)
+                self.win = None
+                self.win_steps = 0
+                self.graphs = {}
+                self._recal = True
                 self._restore()
                 self.retries += 1
                 info["retries"] += 1
@@ -170,9 +182,13 @@ class BudgetedSteps:
             for s in st:
                 info["solves"] += self.batch.n
                 info["iterations"] += s[3]
-            if calib and self.steps_done == 2:
+            if self._recal:
+                # observed counts with one iteration of margin per refinement step
+                self.budgets = [_budget(s[1], s[2], 1) for s in st]
+                self._recal = False
+            elif calib and self.steps_done == 2:
                 # budgets from the third (warm) step's counts
-                self.budgets = [(s[1], s[2]) for s in st]
+                self.budgets = [_budget(s[1], s[2], 0) for s in st]
             elif not calib:
                 self._tighten(st)
             self.parity ^= 1
