@@ -1703,19 +1703,22 @@ __global__ __launch_bounds__(256) void pcg_cg_update_az_kernel(
 // east/west neighbours come from lane shuffles, a strip's two edge columns
 // from memory.  Grid: grid_res (cdiv(nx, 64) x cdiv(ny, 4 kResRows)).
 constexpr int kResRows = 8;
-static inline dim3 grid_res(int ny, int nx) { return dim3(cdiv(nx, 64), cdiv(ny, 4 * kResRows)); }
+// Dirichlet operators use 62 output columns per wave (the two edge lanes only
+// load the neighbouring columns).
+static inline dim3 grid_res(int ny, int nx, int kind) {
+  return dim3(cdiv(nx, kind == kDirichlet ? 62 : 64), cdiv(ny, 4 * kResRows));
+}
 
 template <int KIND, typename TB>
 __device__ __forceinline__ void refine_residual_body(
     LevelDev<double> L, const TB* __restrict__ b, int64_t ld_b, const double* __restrict__ x,
     int64_t ld_x, float* __restrict__ r32, float* __restrict__ e32, double rtol2, double atol2,
     double* partials, PcgScalars* sc, int bx, int by, int gx, int gy) {
-  const int done = sc->out_done;  // checked once the first rows are in flight
   const int lane = threadIdx.x;
-  const int i = bx * 64 + lane;
+  const int i = KIND == kDirichlet ? bx * 62 - 1 + lane : bx * 64 + lane;
   const int j0 = (by * 4 + threadIdx.y) * kResRows;  // uniform within a wave
   const int nx = L.nx, ny = L.ny;
-  const bool col = i < nx;
+  const bool col = i >= 0 && i < nx;
   // x at (row j, column ii), masked for the Dirichlet operator (the reference
   // applies the stencil to u * mask, multigrid.py:383); zero outside.
   auto Xat = [&](int j, int ii) -> double {
@@ -1726,6 +1729,52 @@ __device__ __forceinline__ void refine_residual_body(
   auto CY = [&](int j) -> double {  // north-face coefficient of row j
     return (KIND == kNoflux && j >= 0 && j < ny && col) ? L.cy[(int64_t)j * nx + i] : 0.0;
   };
+  if constexpr (KIND == kDirichlet) {
+    // Lanes 1..62 own a column each, lanes 0 and 63 load the neighbouring
+    // columns; every load of the strip is issued before any arithmetic (the
+    // rows are independent), so each wave keeps ~30 loads in flight instead
+    // of walking its rows one latency at a time.
+    constexpr int R = kResRows;
+    const bool own = col && lane >= 1 && lane <= 62;
+    double X[R + 2], B[R];
+    float M[R];
+#pragma unroll
+    for (int t = 0; t < R + 2; ++t) X[t] = col ? Xat(j0 - 1 + t, i) : 0.0;
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+      const int j = j0 + t;
+      const bool in = own && j < ny;
+      const int ic = min(max(i, 0), nx - 1), jc = min(j, ny - 1);
+      M[t] = in ? (float)mval<double>(L.mask, (int64_t)jc * nx + ic) : 0.0f;
+      B[t] = in ? (double)b[(int64_t)jc * ld_b + ic] : 0.0;
+    }
+    // A certified solve has not moved x since the residual that certified it
+    // (its updates are skipped), so the finishing kernel keeps that result.
+    if (sc->out_done) return;
+    double bb = 0.0, rr = 0.0;
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+      const int j = j0 + t;
+      const double c = X[t + 1];
+      const double w = dpp_move<0x138>(c);
+      const double e = dpp_move<0x130>(c);
+      const double lap = (e + w - 2.0 * c) * L.idx2 + (X[t + 2] + X[t] - 2.0 * c) * L.idy2;
+      const double m = (double)M[t];
+      const double bv = B[t] * m;
+      const double rv = (bv - (lap - L.lam * c) * m) * m;
+      if (own && j < ny) {
+        const int64_t k = (int64_t)j * nx + i;
+        r32[k] = (float)rv;
+        e32[k] = 0.0f;
+      }
+      bb += bv * bv;
+      rr += rv * rv;
+    }
+    double v[2] = {bb, rr};
+    block_partials<2, 256>(v, partials, (int64_t)gx * gy, (int64_t)by * gx + bx);
+    return;
+  }
+  const int done = sc->out_done;  // checked once the first rows are in flight
   double xs = col ? Xat(j0 - 1, i) : 0.0, xc = col ? Xat(j0, i) : 0.0;
   double cys = CY(j0 - 1);
   // A certified solve has not moved x since the residual that certified it
@@ -2571,7 +2620,7 @@ static void launch_refine_residual(Plan& P, const TB* rhs, int64_t ld_rhs, const
                                    double inner_rtol = 0.0) {
   Hier<float>& H = P.h32;
   const auto& L64 = P.h64.lv[0];
-  const dim3 g = grid_res(L64.ny, L64.nx);
+  const dim3 g = grid_res(L64.ny, L64.nx, P.kind);
   if (P.kind == kDirichlet)
     hipLaunchKernelGGL((refine_residual_kernel<kDirichlet, TB>), g, dim3(64, 4), 0, s, L64, rhs,
                        ld_rhs, x, ld_x, H.r, H.x, rtol * rtol, atol * atol, P.partials, out);
@@ -3582,7 +3631,7 @@ static int batch_build_slot(MgBatch& B, int s) {
     }
     const LevelDev<float>& L0 = H.lv[0];
     fc.push_back(FinishArgs{Pm.partials, L0.ny > 0 ? cdiv(L0.nx, kTx) * cdiv(L0.ny, kTy) : 0, sc});
-    const dim3 gr = grid_res(Pm.h64.lv[0].ny, Pm.h64.lv[0].nx);
+    const dim3 gr = grid_res(Pm.h64.lv[0].ny, Pm.h64.lv[0].nx, Pm.kind);
     fr.push_back(FinishArgs{Pm.partials, (int)(gr.x * gr.y), sc + 1});
     ones.push_back(1);
     const dim3 gd = grid_dir(L0.ny, L0.nx);
@@ -3629,7 +3678,7 @@ static int batch_res_list(MgBatch& B, int s, const double* const* rhs, const int
   for (int m = 0; m < nm; ++m) {
     Plan& Pm = *B.P[m];
     const LevelDev<double>& L64 = Pm.h64.lv[0];
-    const dim3 g = grid_res(L64.ny, L64.nx);
+    const dim3 g = grid_res(L64.ny, L64.nx, Pm.kind);
     ResArgs a;
     std::memset(&a, 0, sizeof(a));
     a.L = L64;
