@@ -1696,6 +1696,12 @@ __global__ __launch_bounds__(256) void pcg_cg_update_az_kernel(
   }
 }
 
+__device__ __forceinline__ void scalars_clear(PcgScalars& x) {
+  x.rz = x.pq = x.rr = x.bb = x.tol2 = x.alpha = x.beta = x.zq = x.out_rr = x.out_tol2 = 0.0;
+  x.iter = x.done = x.first = x.out_done = x.outer = x.total = x.max_inner = 0;
+  x.it_outer[0] = x.it_outer[1] = x.it_outer[2] = x.it_outer[3] = 0;
+}
+
 // Mixed precision: r32 = (b64 - A64 x64) m in FP64, ||r||^2 and ||b||^2 in
 // FP64; the inner FP32 solve starts from e = 0 so e is cleared here too.
 // Each wave owns a 64-column strip of kResRows rows and slides a three-row
@@ -1713,7 +1719,13 @@ template <int KIND, typename TB>
 __device__ __forceinline__ void refine_residual_body(
     LevelDev<double> L, const TB* __restrict__ b, int64_t ld_b, const double* __restrict__ x,
     int64_t ld_x, float* __restrict__ r32, float* __restrict__ e32, double rtol2, double atol2,
-    double* partials, PcgScalars* sc, int bx, int by, int gx, int gy) {
+    double* partials, PcgScalars* sc, int bx, int by, int gx, int gy, bool first = false) {
+  // first: the residual that starts a new solve; one thread clears the
+  // slot's inner and outer scalars (sc - 1, sc) and nobody reads them here.
+  if (first && bx == 0 && by == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
+    scalars_clear(sc[-1]);
+    scalars_clear(sc[0]);
+  }
   const int lane = threadIdx.x;
   const int i = KIND == kDirichlet ? bx * 62 - 1 + lane : bx * 64 + lane;
   const int j0 = (by * 4 + threadIdx.y) * kResRows;  // uniform within a wave
@@ -1750,7 +1762,7 @@ __device__ __forceinline__ void refine_residual_body(
     }
     // A certified solve has not moved x since the residual that certified it
     // (its updates are skipped), so the finishing kernel keeps that result.
-    if (sc->out_done) return;
+    if (!first && sc->out_done) return;
     double bb = 0.0, rr = 0.0;
 #pragma unroll
     for (int t = 0; t < R; ++t) {
@@ -1774,7 +1786,7 @@ __device__ __forceinline__ void refine_residual_body(
     block_partials<2, 256>(v, partials, (int64_t)gx * gy, (int64_t)by * gx + bx);
     return;
   }
-  const int done = sc->out_done;  // checked once the first rows are in flight
+  const int done = first ? 0 : sc->out_done;  // checked once the first rows are in flight
   double xs = col ? Xat(j0 - 1, i) : 0.0, xc = col ? Xat(j0, i) : 0.0;
   double cys = CY(j0 - 1);
   // A certified solve has not moved x since the residual that certified it
@@ -1825,9 +1837,9 @@ template <int KIND, typename TB>
 __global__ __launch_bounds__(256) void refine_residual_kernel(
     LevelDev<double> L, const TB* __restrict__ b, int64_t ld_b, const double* __restrict__ x,
     int64_t ld_x, float* __restrict__ r32, float* __restrict__ e32, double rtol2, double atol2,
-    double* partials, PcgScalars* sc) {
+    double* partials, PcgScalars* sc, int first) {
   refine_residual_body<KIND, TB>(L, b, ld_b, x, ld_x, r32, e32, rtol2, atol2, partials, sc,
-                                 blockIdx.x, blockIdx.y, gridDim.x, gridDim.y);
+                                 blockIdx.x, blockIdx.y, gridDim.x, gridDim.y, first != 0);
 }
 
 __device__ __forceinline__ void refine_update_body(int ny, int nx, double* __restrict__ x,
@@ -1933,12 +1945,7 @@ __global__ __launch_bounds__(256) void true_residual_kernel(LevelDev<T> L, const
 // a memset).
 __global__ void scalars_reset_kernel(PcgScalars* sc, int n) {
   const int k = threadIdx.x;
-  if (k < n) {
-    PcgScalars& x = sc[k];
-    x.rz = x.pq = x.rr = x.bb = x.tol2 = x.alpha = x.beta = x.zq = x.out_rr = x.out_tol2 = 0.0;
-    x.iter = x.done = x.first = x.out_done = x.outer = x.total = x.max_inner = 0;
-    x.it_outer[0] = x.it_outer[1] = x.it_outer[2] = x.it_outer[3] = 0;
-  }
+  if (k < n) scalars_clear(sc[k]);
 }
 
 // ---------------------------------------------------------------------------
@@ -2614,19 +2621,22 @@ int pcg_solve(Plan& P, const T* rhs, int64_t ld_rhs, T* x, int64_t ld_x, double 
 // FP64 refinement residual of x into the FP32 hierarchy's r (and e = 0), with
 // its finishing reduction into the outer scalars ``out``.
 template <typename TB>
+// first: the residual starts a new solve and also clears its scalars.
 static void launch_refine_residual(Plan& P, const TB* rhs, int64_t ld_rhs, const double* x,
                                    int64_t ld_x, double rtol, double atol, PcgScalars* out,
                                    hipStream_t s, bool reset_inner = false,
-                                   double inner_rtol = 0.0) {
+                                   double inner_rtol = 0.0, bool first = false) {
   Hier<float>& H = P.h32;
   const auto& L64 = P.h64.lv[0];
   const dim3 g = grid_res(L64.ny, L64.nx, P.kind);
   if (P.kind == kDirichlet)
     hipLaunchKernelGGL((refine_residual_kernel<kDirichlet, TB>), g, dim3(64, 4), 0, s, L64, rhs,
-                       ld_rhs, x, ld_x, H.r, H.x, rtol * rtol, atol * atol, P.partials, out);
+                       ld_rhs, x, ld_x, H.r, H.x, rtol * rtol, atol * atol, P.partials, out,
+                       (int)first);
   else
     hipLaunchKernelGGL((refine_residual_kernel<kNoflux, TB>), g, dim3(64, 4), 0, s, L64, rhs,
-                       ld_rhs, x, ld_x, H.r, H.x, rtol * rtol, atol * atol, P.partials, out);
+                       ld_rhs, x, ld_x, H.r, H.x, rtol * rtol, atol * atol, P.partials, out,
+                       (int)first);
   // with reset_inner, the same kernel resets the inner solve (out - 1) on
   // this residual
   if (reset_inner)
@@ -2678,14 +2688,14 @@ int mixed_enqueue(Plan& P, const TB* rhs, int64_t ld_rhs, double* x, int64_t ld_
   const auto& L32 = H.lv[0];
   PcgScalars* in = &P.sc[2 * slot];
   PcgScalars* out = &P.sc[2 * slot + 1];
-  hipLaunchKernelGGL(scalars_reset_kernel, dim3(1), dim3(64), 0, s, in, 2);
-  // out == in + 1: the residual's finishing kernel also resets the inner solve
-  auto residual = [&](bool reset_inner) {
+  // out == in + 1: the residual's finishing kernel also resets the inner
+  // solve; the first residual clears both scalar blocks of the slot
+  auto residual = [&](bool reset_inner, bool first) {
     launch_refine_residual<TB>(P, rhs, ld_rhs, x, ld_x, rtol, atol, out, s, reset_inner,
-                               inner_rtol);
+                               inner_rtol, first);
   };
   for (int o = 0; o < n_outer; ++o) {
-    residual(true);
+    residual(true, o == 0);
     bool deferred = false;
     int rc = (P.kind == kDirichlet)
                  ? pcg_iters<float, kDirichlet>(P, H, H.x, L32.nx, n_inner[o], in, s, pf, ctx,
@@ -2695,7 +2705,8 @@ int mixed_enqueue(Plan& P, const TB* rhs, int64_t ld_rhs, double* x, int64_t ld_
     if (rc) return rc;
     launch_refine_update(P, x, ld_x, deferred, in, out, s);
   }
-  if (!defer_final) residual(false);
+  if (n_outer == 0) hipLaunchKernelGGL(scalars_reset_kernel, dim3(1), dim3(64), 0, s, in, 2);
+  if (!defer_final) residual(false, false);
   FVX_LAUNCH_CHECK();
   return kOk;
 }

@@ -109,6 +109,49 @@ def test_qg_model_with_learned_preconditioner(trained):
     mod.set_preconditioner(None)
 
 
+def test_qg_model_learned_graph_replay_and_fallback(trained):
+    """The learned preconditioner under graph replay (steps 4-7 replay the
+    captured step graphs) matches the host-driven steps; with an inner
+    iteration cap it cannot meet, the graph path falls back to multigrid-PCG
+    (counted in fallbacks) and every solve still certifies."""
+    from finitevolx_amd.qg import QGModel
+
+    net, untrained, _, out_scale = trained
+    m, dx, dy, lam = _operator()
+    Lx = 1.0
+    kw = dict(beta=2.0, Ld=LD_CELLS * dx, nu4=1e-9, F0=0.5, dt=1e-3, mask=m, rtol=1e-8)
+    q0 = spectral_field(N, N, 3.0, 11) * m * 10.0
+    ref = QGModel(N, N, Lx, Lx, precision="float64", **kw, max_iters=400)
+    qr, _, _ = ref.step(ref.new_state(q0), nsteps=7, graph=False)
+    host = QGModel(N, N, Lx, Lx, precision="mixed", **kw, max_iters=400)
+    host.set_preconditioner(net, out_scale, SWEEPS)
+    qh, _, ih = host.step(host.new_state(q0), nsteps=7, graph=False)
+    rep = QGModel(N, N, Lx, Lx, precision="mixed", **kw, max_iters=400)
+    rep.set_preconditioner(net, out_scale, SWEEPS)
+    qg, _, ig = rep.step(rep.new_state(q0), nsteps=7, graph=True)
+    print(f"[learned] graph replay: {ig}, fallbacks {rep.fallbacks}", flush=True)
+    assert ih["converged"] and ig["converged"] and ig["solves"] == 21
+    assert ig["max_rel_residual"] <= 1e-8 * 1.0001
+    assert rep.fallbacks == 0
+    assert_close_scaled(to_np(qg), to_np(qh), 1e-6, "learned graph vs host-driven")
+    assert_close_scaled(to_np(qg), to_np(qr), 1e-5, "learned graph vs fp64")
+    # caps the untrained network cannot meet (4 inner iterations, 10 refinement
+    # steps; multigrid meets them from a warm start): the graph path falls
+    # back.  The three eager calibration steps run under the same caps from a
+    # cold start and need not certify (the synchronous path reports them).
+    cap = QGModel(N, N, Lx, Lx, precision="mixed", **kw, max_iters=4, max_outer=10)
+    cap.set_preconditioner(untrained, out_scale, SWEEPS)
+    qc, pc, ic = cap.step(cap.new_state(q0), nsteps=3, graph=True)  # the eager steps
+    f_eager = cap.fallbacks
+    qc, _, ic2 = cap.step(qc, pc, nsteps=4, graph=True)  # graph replays only
+    print(f"[learned] capped: {ic} then {ic2}, fallbacks {f_eager} -> {cap.fallbacks}",
+          flush=True)
+    assert ic2["converged"] and ic2["solves"] == 12, ic2
+    assert ic2["max_rel_residual"] <= 1e-8 * 1.0001, ic2
+    assert cap.fallbacks > f_eager  # the graph path dropped the network
+    assert_close_scaled(to_np(qc), to_np(qr), 1e-4, "after fallback vs fp64")
+
+
 def test_config3_masked_variable_coefficient_2048():
     """The 2048^2 masked variable-coefficient Helmholtz configuration: basin
     mask (k^-4 field above its 30th percentile, largest wet component, seed
