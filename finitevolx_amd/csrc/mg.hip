@@ -397,6 +397,9 @@ __device__ __forceinline__ T prolong_at(const LevelDev<T>& C, int j, int i) {
   const T vc = vm(J, I, mc), vx = vm(J, I + si, mx), vy = vm(J + sj, I, my),
           vd = vm(J + sj, I + si, md);
   const T w = T(9) * vc + T(3) * vx + T(3) * vy + vd;
+  // Dirichlet hierarchy: land and the space beyond the walls hold psi = 0, so
+  // the weights are not renormalised (that would extrapolate to the wall).
+  if constexpr (KIND == kDirichlet) return w * T(0.0625);
   const T d = T(9) * mc + T(3) * mx + T(3) * my + md;
   return w / (d > T(1) ? d : T(1));
 }
@@ -537,7 +540,7 @@ __device__ __forceinline__ void mg_tile_body(const TileArgs<T>& a, int bx) {
       }
       const T dd = d > T(1) ? d : T(1);
       T e;
-      if (!MK && inner_coarse) e = w * T(0.0625);  // exact: w / 16
+      if (KIND == kDirichlet || (!MK && inner_coarse)) e = w * T(0.0625);  // exact: w / 16
       else if constexpr (std::is_same<T, float>::value) e = w * __builtin_amdgcn_rcpf(dd);
       else e = w / dd;
       const T m = rM[it];
@@ -844,12 +847,11 @@ __global__ __launch_bounds__(256) void mg_wave_tile_kernel(TileArgs<float> a) {
       for (int r = 0; r < R; ++r) {
         const int gj = j0 - H + r;
         const int J = gj >> 1, Jn = J + ((gj & 1) ? 1 : -1);
-        const int rf = 3 * (J >= 0 && J < C.ny) + (Jn >= 0 && Jn < C.ny);
-        rowf[r] = !row_in(r) ? T(0)
-                  : rf == 4  ? T(0.25)
-                  : rf == 3  ? T(1) / T(3)
-                  : rf == 1  ? T(1)
-                             : T(0);
+        // Dirichlet weights are not renormalised at the walls (the coarse
+        // values beyond them are zero): 1/4 on every row inside the level.
+        (void)J;
+        (void)Jn;
+        rowf[r] = row_in(r) ? T(0.25) : T(0);
       }
     }
   }
@@ -921,9 +923,7 @@ __global__ __launch_bounds__(256) void mg_wave_tile_kernel(TileArgs<float> a) {
   // 16 wherever all four coarse cells exist; where the row or column factor
   // is zero, so is every weighted coarse value.
   if constexpr (kUpMode) {
-    const LevelDev<T>& C = a.C;
-    const T cf = T(3) * T(I >= 0 && I < C.nx) + T(In >= 0 && In < C.nx);
-    const T lanef = colin ? __builtin_amdgcn_rcpf(cf > T(1) ? cf : T(1)) : T(0);
+    const T lanef = colin ? T(0.25) : T(0);  // as rowf: no renormalisation at the walls
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int p = ((r - H) >> 1) - RH + 1;
@@ -1352,7 +1352,7 @@ __device__ __forceinline__ void mg_coarse_lds_body(const CoarseArgs<T>& a,
                 vd = vm(J + sj, I + si, md);
         const T w = T(9) * vc + T(3) * vx + T(3) * vy + vd;
         const T d = T(9) * mc + T(3) * mx + T(3) * my + md;
-        const T e = w / (d > T(1) ? d : T(1));
+        const T e = KIND == kDirichlet ? w * T(0.0625) : w / (d > T(1) ? d : T(1));
         sm[ou[l] + k] = (sm[ou[l] + k] + e * m) * m;
       }
     __syncthreads();

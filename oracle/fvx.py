@@ -891,8 +891,27 @@ def prolongate(vc, mc_, mf):
     return out * mf
 
 
+def prolongate_dirichlet(vc, mc_, mf):
+    """The Dirichlet hierarchy's prolongation (no reference counterpart):
+    the 9/3/3/1 weights of ``prolongate`` over 16 everywhere, land and the
+    space beyond the walls contributing zero (psi = 0 there), instead of
+    renormalising by the wet weights, which would extrapolate the coarse
+    correction to the wall."""
+    nyc, nxc = vc.shape
+    vp = np.pad(vc * mc_, 1)
+    out = np.zeros((2 * nyc, 2 * nxc))
+    for dj in (0, 1):
+        for di in (0, 1):
+            sj, si = 2 * dj - 1, 2 * di - 1
+            w = (9 * vp[1:-1, 1:-1] + 3 * vp[1:-1, 1 + si:nxc + 1 + si]
+                 + 3 * vp[1 + sj:nyc + 1 + sj, 1:-1] + vp[1 + sj:nyc + 1 + sj, 1 + si:nxc + 1 + si])
+            out[dj::2, di::2] = w / 16.0
+    return out * mf
+
+
 def v_cycle(levels, u, rhs, idx=0, n_pre=6, n_post=6, n_coarse=50, omega=0.95):
-    """``MultigridSolver.v_cycle`` (``solvers/multigrid.py:808-883``)."""
+    """``MultigridSolver.v_cycle`` (``solvers/multigrid.py:808-883``); the
+    Dirichlet hierarchy prolongates with ``prolongate_dirichlet``."""
     lev = levels[idx]
     if idx == len(levels) - 1:
         return weighted_jacobi(u, rhs, lev, omega, n_coarse)
@@ -901,7 +920,8 @@ def v_cycle(levels, u, rhs, idx=0, n_pre=6, n_post=6, n_coarse=50, omega=0.95):
     cl = levels[idx + 1]
     rc = restrict(r, lev.mask, cl.mask)
     ec = v_cycle(levels, np.zeros_like(rc), rc, idx + 1, n_pre, n_post, n_coarse, omega)
-    u = (u + prolongate(ec, cl.mask, lev.mask)) * lev.mask
+    P = prolongate_dirichlet if lev.kind == "dirichlet" else prolongate
+    u = (u + P(ec, cl.mask, lev.mask)) * lev.mask
     return weighted_jacobi(u, rhs, lev, omega, n_post)
 
 

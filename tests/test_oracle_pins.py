@@ -810,3 +810,50 @@ def test_multilayer_planted_mistake_is_caught():
                                                       fvx.tracer_velocity)
     assert _multilayer_failures(fvx.multilayer_invert, bottom_forced, fvx.tracer_velocity)
     assert _multilayer_failures(fvx.multilayer_invert, fvx.multilayer_rhs, flipped_v)
+
+
+# ---------------------------------------------------------------------------
+# Prolongation of the Dirichlet multigrid hierarchy (no reference counterpart)
+# ---------------------------------------------------------------------------
+
+
+def _prolong_dirichlet_failures(prolong):
+    fails = []
+    nc = 8
+    mc = np.ones((nc, nc))
+    mf = np.ones((2 * nc, 2 * nc))
+    # coarse centres X = (I + 1/2) 2h, fine centres x = (i + 1/2) h (h = 1):
+    # the 9/3/3/1 weights are bilinear, so a linear field is reproduced on
+    # every fine cell whose four coarse neighbours exist
+    X = (np.arange(nc) + 0.5) * 2.0
+    x = np.arange(2 * nc) + 0.5
+    g = 0.3 * X[None, :] - 1.7 * X[:, None] + 2.0
+    f = prolong(g, mc, mf)
+    want = 0.3 * x[None, :] - 1.7 * x[:, None] + 2.0
+    if not np.allclose(f[1:-1, 1:-1], want[1:-1, 1:-1], rtol=0, atol=1e-12):
+        fails.append("bilinear interior")
+    # at a wall the coarse value beyond it is zero (psi = 0): the first fine
+    # cell gets 3/4 of its coarse neighbour's value in 1-D
+    c = np.ones((nc, nc))
+    e = prolong(c, mc, mf)
+    if not (np.isclose(e[5, 0], 0.75) and np.isclose(e[0, 0], 0.75 * 0.75)
+            and np.isclose(e[5, 5], 1.0)):
+        fails.append("walls")
+    # land: a dry coarse cell contributes zero, the fine mask is applied
+    mc2 = mc.copy()
+    mc2[3, 3] = 0.0
+    mf2 = mf.copy()
+    mf2[6, 6] = 0.0
+    e2 = prolong(c, mc2, mf2)
+    if not (e2[6, 6] == 0.0 and np.isclose(e2[6, 7], 7 / 16) and np.isclose(e2[5, 5], 15 / 16)):
+        fails.append("land")
+    return fails
+
+
+def test_prolongate_dirichlet_pins():
+    assert _prolong_dirichlet_failures(fvx.prolongate_dirichlet) == []
+
+
+def test_prolongate_dirichlet_planted_mistake_is_caught():
+    # the reference's mask-renormalised weights extrapolate to the wall
+    assert "walls" in _prolong_dirichlet_failures(fvx.prolongate)
