@@ -623,7 +623,9 @@ __device__ __forceinline__ void mg_tile_body(const TileArgs<T>& a, int bx) {
       const T s4 = R[q00] * m00 + R[q10] * m10 + R[q01] * m01 + R[q11] * m11;
       const T cnt = m00 + m10 + m01 + m11;
       const int64_t g = (int64_t)J * C.nx + I;
-      C.b[g] = (s4 / (cnt > T(1) ? cnt : T(1))) * mval<T>(C.mask, g);
+      // Dirichlet: dry children count as zeros (psi on land), not renormalised
+      const T rs = KIND == kDirichlet ? s4 * T(0.25) : s4 / (cnt > T(1) ? cnt : T(1));
+      C.b[g] = rs * mval<T>(C.mask, g);
     }
   }
 
@@ -964,8 +966,8 @@ __global__ __launch_bounds__(256) void mg_wave_tile_kernel(TileArgs<float> a) {
 #pragma unroll
   for (int k = 0; k < kNumCgDots; ++k) d[k] = T(0);
   if constexpr (kDownMode) {
-    // r = (b - A u) * m on the interior rows, 2x2 mask-weighted restriction
-    // (multigrid.py:464-507): rows summed in registers, the column pair
+    // r = (b - A u) * m on the interior rows, 2x2 restriction (sum / 4):
+    // rows summed in registers, the column pair
     // (gi even, gi + 1) through one lane shift.
     const LevelDev<T>& C = a.C;
     const bool even = lin && ((gi & 1) == 0);
@@ -976,14 +978,12 @@ __global__ __launch_bounds__(256) void mg_wave_tile_kernel(TileArgs<float> a) {
       const T m1 = (colin && row_in(r0 + 1)) ? T(1) : T(0);
       const T res0 = (b[r0] - Au(r0)) * m0, res1 = (b[r0 + 1] - Au(r0 + 1)) * m1;
       const T s2 = res0 + res1;
-      const T c2 = m0 + m1;
       const T s4 = s2 + lane_east(s2);
-      const T cnt = c2 + lane_east(c2);
       const int J = (j0 >> 1) + k2;
       if (even && J < C.ny && I < C.nx) {
-        // the coarse level of an all-wet level is all wet (mask 1)
-        buf_store(buf_rsrc(C.b, C.ny * C.nx), (unsigned)I, J * C.nx,
-                  s4 / (cnt > T(1) ? cnt : T(1)));
+        // the coarse level of an all-wet level is all wet (mask 1); children
+        // beyond the level count as zeros (the Dirichlet restriction)
+        buf_store(buf_rsrc(C.b, C.ny * C.nx), (unsigned)I, J * C.nx, s4 * T(0.25));
       }
     }
   }
@@ -1150,7 +1150,8 @@ __global__ __launch_bounds__(kCoarseThreads) void mg_coarse_kernel(CoarseArgs<T>
         s4 += r * m;
         cnt += m;
       }
-      C.b[k] = (s4 / (cnt > T(1) ? cnt : T(1))) * mval<T>(C.mask, k);
+      const T rs = KIND == kDirichlet ? s4 * T(0.25) : s4 / (cnt > T(1) ? cnt : T(1));
+      C.b[k] = rs * mval<T>(C.mask, k);
     }
     __syncthreads();
   }
@@ -1311,7 +1312,8 @@ __device__ __forceinline__ void mg_coarse_lds_body(const CoarseArgs<T>& a,
                      sm[ot[l] + f + L.nx + 1] * m11;
         const T cnt = m00 + m01 + m10 + m11;
         const int kc = J * C.nx + I;
-        sm[ob[l + 1] + kc] = (s4 / (cnt > T(1) ? cnt : T(1))) * S.m_at(C, kc);
+        const T rs = KIND == kDirichlet ? s4 * T(0.25) : s4 / (cnt > T(1) ? cnt : T(1));
+        sm[ob[l + 1] + kc] = rs * S.m_at(C, kc);
       }
     __syncthreads();
   }
@@ -1445,7 +1447,7 @@ __global__ __launch_bounds__(256) void mask_copy_kernel(LevelDev<T> L, const T* 
 template <typename T>
 __global__ __launch_bounds__(256) void restrict_kernel(LevelDev<T> L, LevelDev<T> C,
                                                        const T* __restrict__ v,
-                                                       T* __restrict__ out) {
+                                                       T* __restrict__ out, int dirichlet) {
   const int I = blockIdx.x * 64 + threadIdx.x, J = blockIdx.y * 4 + threadIdx.y;
   if (I >= C.nx || J >= C.ny) return;
   T s4 = T(0), cnt = T(0);
@@ -1458,7 +1460,8 @@ __global__ __launch_bounds__(256) void restrict_kernel(LevelDev<T> L, LevelDev<T
     cnt += m;
   }
   const int64_t g = (int64_t)J * C.nx + I;
-  out[g] = (s4 / (cnt > T(1) ? cnt : T(1))) * mval<T>(C.mask, g);
+  const T rs = dirichlet ? s4 * T(0.25) : s4 / (cnt > T(1) ? cnt : T(1));
+  out[g] = rs * mval<T>(C.mask, g);
 }
 
 template <typename T, int KIND>
@@ -2895,7 +2898,8 @@ static int learned_two_grid_impl(Plan& P, CoarseNetFn fn, void* ctx, int sweeps,
                        (int64_t)L.nx);
   }
   // coarse correction c = P net(R res), scaled by the energy-minimising alpha
-  hipLaunchKernelGGL(restrict_kernel<float>, grid2(C.ny, C.nx), dim3(64, 4), 0, s, L, C, res, rc);
+  hipLaunchKernelGGL(restrict_kernel<float>, grid2(C.ny, C.nx), dim3(64, 4), 0, s, L, C, res, rc,
+                     (int)(P.kind == kDirichlet));
   FVX_LAUNCH_CHECK();
   int rc_ = fn(ctx, rc, zc, C.ny, C.nx, C.mask, guard, s);
   if (rc_) return rc_;
@@ -3266,7 +3270,7 @@ int fvx_mg_plan_info(void* plan, int* n_levels, int* shapes, int* coarse_level) 
     const auto& L = hier<T>(*P).lv[level];                                                       \
     const auto& C = hier<T>(*P).lv[level + 1];                                                   \
     hipLaunchKernelGGL(restrict_kernel<T>, grid2(C.ny, C.nx), dim3(64, 4), 0,                    \
-                       (hipStream_t)stream, L, C, v, out);                                       \
+                       (hipStream_t)stream, L, C, v, out, (int)(P->kind == kDirichlet));         \
     FVX_LAUNCH_CHECK();                                                                          \
     return kOk;                                                                                  \
   }                                                                                              \

@@ -891,6 +891,14 @@ def prolongate(vc, mc_, mf):
     return out * mf
 
 
+def restrict_dirichlet(v, mf, mc_):
+    """The Dirichlet hierarchy's restriction (no reference counterpart): the
+    2x2 sum of the wet children over 4 (dry children hold zero, as psi on
+    land), not renormalised by the number of wet children."""
+    vm = v * mf
+    return 0.25 * (vm[::2, ::2] + vm[1::2, ::2] + vm[::2, 1::2] + vm[1::2, 1::2]) * mc_
+
+
 def prolongate_dirichlet(vc, mc_, mf):
     """The Dirichlet hierarchy's prolongation (no reference counterpart):
     the 9/3/3/1 weights of ``prolongate`` over 16 everywhere, land and the
@@ -911,14 +919,16 @@ def prolongate_dirichlet(vc, mc_, mf):
 
 def v_cycle(levels, u, rhs, idx=0, n_pre=6, n_post=6, n_coarse=50, omega=0.95):
     """``MultigridSolver.v_cycle`` (``solvers/multigrid.py:808-883``); the
-    Dirichlet hierarchy prolongates with ``prolongate_dirichlet``."""
+    Dirichlet hierarchy transfers with ``restrict_dirichlet`` and
+    ``prolongate_dirichlet``."""
     lev = levels[idx]
     if idx == len(levels) - 1:
         return weighted_jacobi(u, rhs, lev, omega, n_coarse)
     u = weighted_jacobi(u, rhs, lev, omega, n_pre)
     r = (rhs - apply_operator(u, lev)) * lev.mask
     cl = levels[idx + 1]
-    rc = restrict(r, lev.mask, cl.mask)
+    Rs = restrict_dirichlet if lev.kind == "dirichlet" else restrict
+    rc = Rs(r, lev.mask, cl.mask)
     ec = v_cycle(levels, np.zeros_like(rc), rc, idx + 1, n_pre, n_post, n_coarse, omega)
     P = prolongate_dirichlet if lev.kind == "dirichlet" else prolongate
     u = (u + P(ec, cl.mask, lev.mask)) * lev.mask

@@ -857,3 +857,41 @@ def test_prolongate_dirichlet_pins():
 def test_prolongate_dirichlet_planted_mistake_is_caught():
     # the reference's mask-renormalised weights extrapolate to the wall
     assert "walls" in _prolong_dirichlet_failures(fvx.prolongate)
+
+
+def _restrict_dirichlet_failures(restrict):
+    fails = []
+    rng = np.random.default_rng(5)
+    nc = 8
+    mc = np.ones((nc, nc))
+    mf = np.ones((2 * nc, 2 * nc))
+    # wet everywhere: the mean of the 2x2 children, so a constant is kept and
+    # 4 x the coarse sum equals the fine sum (the residual's integral)
+    if not np.allclose(restrict(np.full((2 * nc, 2 * nc), 3.0), mf, mc), 3.0, rtol=0, atol=1e-14):
+        fails.append("constant")
+    v = rng.standard_normal((2 * nc, 2 * nc))
+    if not np.isclose(4.0 * restrict(v, mf, mc).sum(), v.sum(), rtol=1e-12):
+        fails.append("integral")
+    if not np.isclose(restrict(v, mf, mc)[2, 5], v[4:6, 10:12].mean()):
+        fails.append("children")
+    # land: a dry child counts as zero (psi on land), not renormalised away;
+    # a dry coarse cell receives zero
+    mf2 = mf.copy()
+    mf2[6, 6] = 0.0
+    mc2 = mc.copy()
+    mc2[0, 7] = 0.0
+    r = restrict(np.ones((2 * nc, 2 * nc)), mf2, mc2)
+    if not np.isclose(r[3, 3], 0.75):
+        fails.append("land")
+    if r[0, 7] != 0.0:
+        fails.append("coarse mask")
+    return fails
+
+
+def test_restrict_dirichlet_pins():
+    assert _restrict_dirichlet_failures(fvx.restrict_dirichlet) == []
+
+
+def test_restrict_dirichlet_planted_mistake_is_caught():
+    # the reference's wet-count renormalisation keeps 1 where a child is dry
+    assert _restrict_dirichlet_failures(fvx.restrict) == ["land"]
