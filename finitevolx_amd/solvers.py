@@ -210,6 +210,10 @@ class MultigridSolver:
         """``n_cycles`` V-cycles from zero (``multigrid.py:692-733``)."""
         return self._run_vcycles(rhs)
 
+    # The reference's three solve entry points differ only in how autodiff
+    # sees the V-cycles (multigrid.py:692-806); their forward results are
+    # the same n_cycles V-cycles from zero.  Gradients through the solve are
+    # not provided.
     solve_unrolled = __call__
     solve_onestep = __call__
 
@@ -414,8 +418,13 @@ def solve_cg(matvec, rhs, x0=None, preconditioner=None, rtol: float = 1e-6, atol
     return _generic_pcg(matvec, rhs, x0, preconditioner, rtol, atol, max_steps)
 
 
-def _generic_pcg(matvec, rhs, x0, precond, rtol, atol, max_steps):
-    """Flexible PCG for arbitrary device callables (vector algebra in torch)."""
+def _generic_pcg(matvec, rhs, x0, precond, rtol, atol, max_steps, check_every=8):
+    """Flexible PCG for arbitrary device callables (vector algebra in torch).
+
+    The stopping test runs on the device every iteration; the host reads it
+    every ``check_every`` iterations.  Iterations after the test first holds
+    leave ``x`` unchanged, so the result and the iteration count are those
+    of a host test after every iteration."""
     b = _lib.as_field(rhs)
     x = torch.zeros_like(b) if x0 is None else _lib.as_field(x0, b.dtype).clone()
     M = precond if precond is not None else (lambda r: r)
@@ -428,23 +437,26 @@ def _generic_pcg(matvec, rhs, x0, precond, rtol, atol, max_steps):
     z = _lib.as_field(M(r), b.dtype)
     p = z.clone()
     rz = torch.sum(r * z)
+    live = torch.ones((), dtype=torch.bool, device=b.device)
+    its = torch.zeros((), dtype=torch.int64, device=b.device)
+    zero = torch.zeros((), dtype=b.dtype, device=b.device)
     for k in range(1, max_steps + 1):
         q = matvec(p)
-        alpha = rz / torch.sum(p * q)
+        alpha = torch.where(live, rz / torch.sum(p * q), zero)
         x = x + alpha * p
         r_old = r
         r = r - alpha * q
-        rn = float(torch.linalg.vector_norm(r))
-        if rn <= tol:
-            res = float(torch.linalg.vector_norm(b - matvec(x)))
-            return x, CGInfo(k, res, True)
+        its += live
+        live = live & (torch.linalg.vector_norm(r) > tol)
+        if (k % check_every == 0 or k == max_steps) and not bool(live):
+            break
         z = _lib.as_field(M(r), b.dtype)
         rz_new = torch.sum(r * z)
-        beta = torch.sum(z * (r - r_old)) / rz
-        rz = rz_new
+        beta = torch.where(live, torch.sum(z * (r - r_old)) / rz, zero)
+        rz = torch.where(live, rz_new, rz)
         p = z + beta * p
     res = float(torch.linalg.vector_norm(b - matvec(x)))
-    return x, CGInfo(max_steps, res, False)
+    return x, CGInfo(int(its), res, not bool(live))
 
 
 # ---------------------------------------------------------------------------

@@ -196,6 +196,28 @@ def test_pcg_converges_to_direct_solution(ny, nx, kind, masked, varcoef):
     assert r <= 1.01e-11 * np.linalg.norm(b) * 1.5
 
 
+def test_solve_cg_with_callables_polls_every_few_iterations():
+    """solve_cg on plain callables (torch vector algebra): the host reads
+    the device-side stopping test every 8 iterations, with the same iterate
+    and iteration count as a test after every iteration, and the solution
+    of the masked Dirichlet Helmholtz problem."""
+    need_gpu()
+    ny, nx = 40, 56
+    m = random_mask(ny, nx, 6).astype(np.float64)
+    q = np.random.default_rng(7).standard_normal((ny, nx)) * m
+    mt = torch.tensor(m, device="cuda")
+    A = lambda x: S.masked_laplacian(x, mt, 0.1, 0.2, 3.0)  # noqa: E731
+    Mj = lambda r: r / (-2 / 0.01 - 2 / 0.04 - 3.0)  # noqa: E731
+    x8, i8 = fx.solve_cg(A, q, preconditioner=Mj, rtol=1e-10, atol=0.0, max_steps=400)
+    x1, i1 = S._generic_pcg(A, q, None, Mj, 1e-10, 0.0, 400, check_every=1)
+    assert i8.converged and i1.converged
+    assert i8.iterations == i1.iterations, (i8, i1)
+    assert torch.equal(x8, x1)
+    assert_close_scaled(to_np(x8), fvx.dirichlet_helmholtz_direct(q, m, 0.1, 0.2, 3.0), 1e-8, "cg")
+    xs, is_ = fx.solve_cg(A, q, preconditioner=Mj, rtol=1e-10, atol=0.0, max_steps=5)
+    assert not is_.converged and is_.iterations == 5
+
+
 @pytest.mark.parametrize("kind", ["dirichlet", "noflux"])
 def test_mixed_precision_certifies_fp64_residual(kind):
     need_gpu()

@@ -165,6 +165,35 @@ def test_graph_replay_matches_host_driven_steps(precision, masked):
     assert ic["converged"] and ic["solves"] == 6
 
 
+@pytest.mark.parametrize("precision", ["float64", "mixed"])
+def test_graph_retry_after_a_jump_in_q(precision):
+    """A jump in q between calls leaves the extrapolated warm starts poor,
+    so the first replayed steps miss their budgets: each is redone from its
+    untouched input with wider budgets (recaptured graphs), and the run
+    matches host-driven steps from the same state."""
+    need_gpu()
+    ny, nx = 256, 192
+    p = _params(nx, ny)
+    q0 = inputs.qg_initial_pv(ny, nx, seed=4)
+    tol = 1e-12 if precision == "float64" else 1e-9
+    m1 = QGModel(nx, ny, p.Lx, p.Ly, precision=precision, rtol=tol)
+    m2 = QGModel(nx, ny, p.Lx, p.Ly, precision=precision, rtol=tol)
+    qa, pa, _ = m1.step(m1.new_state(q0), nsteps=8, graph=False)
+    qb, pb, ib = m2.step(m2.new_state(q0), nsteps=8, graph=True)
+    assert ib["converged"]
+    jump = m1.new_state(inputs.qg_initial_pv(ny, nx, seed=9))
+    qa += jump
+    qb += jump
+    qa, _, ia = m1.step(qa, pa, nsteps=5, graph=False)
+    qb, _, ib = m2.step(qb, pb, nsteps=5, graph=True)
+    print(f"[retry] {precision}: {ib}", flush=True)
+    assert ib["retries"] >= 1, ib
+    assert ia["converged"] and ib["converged"] and ib["solves"] == 15, ib
+    assert ib["max_rel_residual"] <= tol
+    err = np.max(np.abs(to_np(qa) - to_np(qb))) / np.max(np.abs(to_np(qa)))
+    assert err <= (1e-10 if precision == "float64" else 1e-6), err
+
+
 def test_config2_basin_mixed_within_pv_gate():
     """The 1024^2 closed-basin configuration: f0 = 9.375e-5, beta = 1.754e-11,
     Ld = 4e4 m, nu4 = 1e9, dt = 1200 s, k^-3 PV (rms 1e-5, seed 2) with the
