@@ -680,14 +680,23 @@ __global__ __launch_bounds__(kTileThreads) void mg_tile_kernel(TileArgs<T> a) {
 }
 
 // ---------------------------------------------------------------------------
-// Wave-tile smoother kernel for FP32 levels without a land mask and with the
-// Dirichlet operator (constant coefficients): the same modes and arithmetic
-// as mg_tile_kernel, laid out for the wavefront.  One wavefront owns a tile
-// of 64 columns x (ROWS + 2H) rows including its halo, each lane one
-// column of u and b in registers.  East/west neighbours come from DPP wave
-// shifts, north/south from the neighbouring registers, so the sweeps use no
-// LDS, no barriers and no index arithmetic.  The tile interior is
-// (64 - 2H) x ROWS; four independent tiles per 256-thread block.
+// Wave-tile smoother kernel for FP32 levels with the Dirichlet operator
+// (constant coefficients): the same modes and arithmetic as mg_tile_kernel,
+// laid out for the wavefront.  One wavefront owns a tile of 64 columns x
+// (ROWS + 2H) rows including its halo, each lane one column of u and b in
+// registers.  East/west neighbours come from DPP wave shifts, north/south
+// from the neighbouring registers, so the sweeps use no LDS, no barriers and
+// no index arithmetic.  The tile interior is (64 - 2H) x ROWS; four
+// independent tiles per 256-thread block.
+//
+// MK (land mask): each lane keeps one bit per row of its column (wet and
+// inside the level).  Every iterate of a Dirichlet hierarchy is zero on land
+// (updates and corrections are multiplied by the mask, coarse right-hand
+// sides too), so u * mask is u: the stencils need no masked neighbours, and
+// the mask enters as a select on each updated value, on A z and on the
+// residual, and as the coarse mask on the restriction.  (A plain smoothing
+// call with an arbitrary iterate keeps its land values in the reference; it
+// stays on the LDS kernel.)
 // ---------------------------------------------------------------------------
 
 template <int NS, int MODE>
@@ -715,8 +724,8 @@ constexpr int wt_pcg_rows() {
   return MODE == kUpCG ? 8 : 8;
 }
 
-template <int NS, int MODE, bool ZERO, int ROWS>
-__global__ __launch_bounds__(256) void mg_wave_tile_kernel(TileArgs<float> a) {
+template <int NS, int MODE, bool ZERO, int ROWS, bool MK>
+__device__ __forceinline__ void mg_wave_tile_body(const TileArgs<float>& a, int bx, int nblk) {
   using T = float;
   constexpr int H = wt_halo<NS, MODE>();
   constexpr int HL = wt_lane_halo<NS, MODE>();
@@ -732,8 +741,7 @@ __global__ __launch_bounds__(256) void mg_wave_tile_kernel(TileArgs<float> a) {
   const int lane = threadIdx.x & 63;
   // Tile of this wavefront, made scalar so that every per-tile quantity and
   // branch below is uniform (lane moves are only valid with all lanes on).
-  const int tile =
-      __builtin_amdgcn_readfirstlane(xcd_swizzle(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6));
+  const int tile = __builtin_amdgcn_readfirstlane(bx * 4 + (threadIdx.x >> 6));
   const bool active = tile < a.ntiles;
   const int tyi = active ? tile / a.tiles_x : 0;
   const int i0 = (active ? tile - tyi * a.tiles_x : 0) * (64 - 2 * HL);
@@ -778,6 +786,22 @@ __global__ __launch_bounds__(256) void mg_wave_tile_kernel(TileArgs<float> a) {
       u[r] = colin ? uv : T(0);  // u * mask (multigrid.py:383)
     }
   }
+  // MK: bit r = row r of this lane's column is wet and inside the level
+  // (rows outside the level are cleared with the edge rows below).
+  uint32_t mb = 0;
+  if constexpr (MK) {
+    if (L.mask) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int gjc = min(max(j0 - H + r, 0), L.ny - 1);
+        mb |= (L.mask[gjc * L.nx + (int)gic] ? 1u : 0u) << r;
+      }
+    } else {
+      mb = (R >= 32) ? ~0u : ((1u << R) - 1u);
+    }
+    if (!colin) mb = 0;
+  }
+  auto wet_r = [&](int r) -> bool { return MK ? ((mb >> r) & 1u) != 0 : colin; };
   const bool first = (MODE == kUpCG) ? a.sc->first : true;
   // (Each lane loads both coarse columns: taking the neighbour column from
   // the lane beside it, possible for odd H, held more registers than the
@@ -826,6 +850,7 @@ __global__ __launch_bounds__(256) void mg_wave_tile_kernel(TileArgs<float> a) {
         u[r] = T(0);
         b[r] = T(0);
         if constexpr (kUpd) qv[r] = T(0);
+        if constexpr (MK) mb &= ~(1u << r);
       }
     }
     if constexpr (kUpd) {
@@ -857,6 +882,11 @@ __global__ __launch_bounds__(256) void mg_wave_tile_kernel(TileArgs<float> a) {
       }
     }
   }
+  if constexpr (MK && !ZERO) {
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      if (!wet_r(r)) u[r] = T(0);  // u * mask
+  }
   // Rows outside the level back to zero after a sweep (edge tiles only).
   auto zero_outer_rows = [&]() {
     if (edge) {
@@ -879,7 +909,8 @@ __global__ __launch_bounds__(256) void mg_wave_tile_kernel(TileArgs<float> a) {
     for (int r = 0; r < R; ++r) {
       const T c = zr[r + 1];
       const T e = lane_east(c), w = lane_west(c);
-      const T av = (e + w - T(2) * c) * idx2 + (zr[r + 2] + zr[r] - T(2) * c) * idy2 - lam * c;
+      T av = (e + w - T(2) * c) * idx2 + (zr[r + 2] + zr[r] - T(2) * c) * idy2 - lam * c;
+      if constexpr (MK) av = wet_r(r) ? av : T(0);  // A z = (lap z - lam z) m
       const T qn = restart ? av : av + beta * qv[r];
       qv[r] = qn;
       b[r] = colin ? b[r] - alpha * qn : T(0);  // zero outside the level
@@ -931,7 +962,9 @@ __global__ __launch_bounds__(256) void mg_wave_tile_kernel(TileArgs<float> a) {
       const int p = ((r - H) >> 1) - RH + 1;
       const int pn = p + (((r + H) & 1) ? 1 : -1);
       const T w = T(9) * cI[p] + T(3) * cN[p] + T(3) * cI[pn] + cN[pn];
-      u[r] = u[r] + w * (rowf[r] * lanef);
+      const T un = u[r] + w * (rowf[r] * lanef);
+      if constexpr (MK) u[r] = wet_r(r) ? un : T(0);  // (u + P e) * mask
+      else u[r] = un;
     }
   }
   // Damped-Jacobi sweeps (multigrid.py:408-461); the valid region shrinks
@@ -956,7 +989,8 @@ __global__ __launch_bounds__(256) void mg_wave_tile_kernel(TileArgs<float> a) {
         un = k0 * c + kb * b[r] + kx * (e + w) + ky * (u[r + 1] + prev);
       }
       prev = c;
-      u[r] = un;
+      if constexpr (MK) u[r] = wet_r(r) ? un : T(0);
+      else u[r] = un;
     }
     zero_outer_rows();
   }
@@ -974,16 +1008,19 @@ __global__ __launch_bounds__(256) void mg_wave_tile_kernel(TileArgs<float> a) {
 #pragma unroll
     for (int k2 = 0; k2 < WTR / 2; ++k2) {
       const int r0 = H + 2 * k2;
-      const T m0 = (colin && row_in(r0)) ? T(1) : T(0);
-      const T m1 = (colin && row_in(r0 + 1)) ? T(1) : T(0);
-      const T res0 = (b[r0] - Au(r0)) * m0, res1 = (b[r0 + 1] - Au(r0 + 1)) * m1;
+      const bool w0 = MK ? wet_r(r0) : (colin && row_in(r0));
+      const bool w1 = MK ? wet_r(r0 + 1) : (colin && row_in(r0 + 1));
+      const T res0 = w0 ? b[r0] - Au(r0) : T(0), res1 = w1 ? b[r0 + 1] - Au(r0 + 1) : T(0);
       const T s2 = res0 + res1;
       const T s4 = s2 + lane_east(s2);
       const int J = (j0 >> 1) + k2;
       if (even && J < C.ny && I < C.nx) {
-        // the coarse level of an all-wet level is all wet (mask 1); children
-        // beyond the level count as zeros (the Dirichlet restriction)
-        buf_store(buf_rsrc(C.b, C.ny * C.nx), (unsigned)I, J * C.nx, s4 * T(0.25));
+        // children beyond the level or on land count as zeros (the Dirichlet
+        // restriction); the coarse level of an all-wet level is all wet
+        T v = s4 * T(0.25);
+        if constexpr (MK)
+          if (C.mask && !C.mask[J * C.nx + I]) v = T(0);
+        buf_store(buf_rsrc(C.b, C.ny * C.nx), (unsigned)I, J * C.nx, v);
       }
     }
   }
@@ -1010,7 +1047,7 @@ __global__ __launch_bounds__(256) void mg_wave_tile_kernel(TileArgs<float> a) {
     for (int t = 0; t < WTR; ++t) {
       const int r = H + t;
       const T azv = Au(r);
-      const T az = (lin && (!edge || row_in(r))) ? azv : T(0);
+      const T az = (lin && (MK ? wet_r(r) : (!edge || row_in(r)))) ? azv : T(0);
       const T z = u[r], rd = lin ? b[r] : T(0), qd = rq[t];
       d[kZR] += z * rd;
       d[kZQ] += z * qd;
@@ -1022,13 +1059,23 @@ __global__ __launch_bounds__(256) void mg_wave_tile_kernel(TileArgs<float> a) {
       d[kAQ] += az * qd;
       d[kQQ] += qd * qd;
     }
-    // z only: the PCG update kernel forms A z again from it (a.az_out is
-    // not used on this path).
+    // z; a = A z too when az_out is given (batched solves), else the PCG
+    // update kernel forms A z again from z.
     if (lin) {
 #pragma unroll
       for (int t = 0; t < WTR; ++t) {
         const int gj = j0 + t;
         if (gj < L.ny) buf_store(buf_rsrc(a.u_out, nL), (unsigned)gi, gj * L.nx, u[H + t]);
+      }
+      if (a.az_out) {
+#pragma unroll
+        for (int t = 0; t < WTR; ++t) {
+          const int gj = j0 + t;
+          const int r = H + t;
+          const T azv = Au(r);
+          if (gj < L.ny)
+            buf_store(buf_rsrc(a.az_out, nL), (unsigned)gi, gj * L.nx, wet_r(r) ? azv : T(0));
+        }
       }
     }
   } else {
@@ -1040,7 +1087,11 @@ __global__ __launch_bounds__(256) void mg_wave_tile_kernel(TileArgs<float> a) {
       }
     }
   }
-  if constexpr (MODE == kUpCG) block_partials<kNumCgDots, 256>(d, a.partials);
+  if constexpr (MODE == kUpCG) block_partials<kNumCgDots, 256>(d, a.partials, nblk, bx);
+}
+template <int NS, int MODE, bool ZERO, int ROWS, bool MK = false>
+__global__ __launch_bounds__(256) void mg_wave_tile_kernel(TileArgs<float> a) {
+  mg_wave_tile_body<NS, MODE, ZERO, ROWS, MK>(a, xcd_swizzle(blockIdx.x, gridDim.x), gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1686,9 +1737,10 @@ __global__ __launch_bounds__(256) void pcg_cg_update_az_kernel(
     if (lane == 0) w = zw[t];
     if (lane == 63) e = ze[t];
     const T lap = (e + w - T(2) * c) * L.idx2 + (zr[t + 2] + zr[t] - T(2) * c) * L.idy2;
-    const T av = lap - L.lam * c;  // mask 1 inside the level
+    T av = lap - L.lam * c;  // z is zero on land (z m = z)
     if (col && j < ny) {
       const int64_t k = (int64_t)j * nx + i;
+      if (L.mask && !L.mask[k]) av = T(0);  // (lap z - lam z) m
       const T pn = restart ? c : c + beta * pv[t];
       const T qn = restart ? av : av + beta * qv[t];
       p[k] = pn;
@@ -2170,8 +2222,9 @@ Hier<float>& hier<float>(Plan& P) { return P.h32; }
 template <typename T, int KIND>
 static bool wave_tiled(const LevelDev<T>& L, const LevelDev<T>* C) {
   // (its buffer accesses address each array in 32-bit byte offsets)
+  (void)C;
   if constexpr (std::is_same<T, float>::value && KIND == kDirichlet)
-    return !L.mask && (!C || !C->mask) && (int64_t)L.ny * L.nx < (int64_t(1) << 29);
+    return (int64_t)L.ny * L.nx < (int64_t(1) << 29);
   return false;
 }
 
@@ -2193,8 +2246,10 @@ static void launch_tile(const Plan& P, const LevelDev<T>& L, const LevelDev<T>* 
   a.partials = P.partials;
   a.sc = sc;
   if constexpr (std::is_same<T, float>::value && KIND == kDirichlet) {
-    if (wave_tiled<T, KIND>(L, C)) {
-      // all-wet FP32 level: one wavefront per tile (mg_wave_tile_kernel)
+    // (a plain smoothing call on a masked level keeps the iterate's land
+    // values, as the reference does: the LDS kernel)
+    if (wave_tiled<T, KIND>(L, C) && !(MODE == kSmooth && !ZERO && L.mask)) {
+      // FP32 Dirichlet level: one wavefront per tile (mg_wave_tile_kernel)
       a.az_out = nullptr;  // recomputed by pcg_cg_update_az_kernel
       if (upd) a.upd = *upd;
       constexpr int HL = wt_lane_halo<NS, MODE>();
@@ -2204,16 +2259,21 @@ static void launch_tile(const Plan& P, const LevelDev<T>& L, const LevelDev<T>* 
       constexpr int kLongRows = kMayShort ? kTileRows : wt_pcg_rows<MODE>();
       a.ntiles = a.tiles_x * cdiv(L.ny, short_tiles ? kShortTileRows : kLongRows);
       const int nblk = cdiv(a.ntiles, 4);
+      const bool mk = L.mask || (C && C->mask);
+      auto go = [&](auto rows, auto masked) {
+        hipLaunchKernelGGL((mg_wave_tile_kernel<NS, MODE, ZERO, decltype(rows)::value,
+                                                decltype(masked)::value>),
+                           dim3(nblk), dim3(256), 0, s, a);
+      };
+      using Masked = std::true_type;
+      using Wet = std::false_type;
+      using LongRows = std::integral_constant<int, kLongRows>;
       if constexpr (kMayShort) {
-        if (short_tiles)
-          hipLaunchKernelGGL((mg_wave_tile_kernel<NS, MODE, ZERO, kShortTileRows>), dim3(nblk),
-                             dim3(256), 0, s, a);
-        else
-          hipLaunchKernelGGL((mg_wave_tile_kernel<NS, MODE, ZERO, kTileRows>), dim3(nblk),
-                             dim3(256), 0, s, a);
+        using ShortRows = std::integral_constant<int, kShortTileRows>;
+        if (short_tiles) mk ? go(ShortRows(), Masked()) : go(ShortRows(), Wet());
+        else mk ? go(LongRows(), Masked()) : go(LongRows(), Wet());
       } else {
-        hipLaunchKernelGGL((mg_wave_tile_kernel<NS, MODE, ZERO, kLongRows>), dim3(nblk),
-                           dim3(256), 0, s, a);
+        mk ? go(LongRows(), Masked()) : go(LongRows(), Wet());
       }
       if constexpr (MODE == kUpCG)
         if (finish)
@@ -2970,7 +3030,7 @@ static int kernel_bench_impl(Plan& P, int which, int reps, hipStream_t s) {
   if constexpr (std::is_same<T, float>::value && KIND == kDirichlet)
     step_fusable = wave_tiled<T, KIND>(L, &H.lv[1]) && P.n_pre <= kMaxFuse && H.r2 && H.q2;
   if (which == 3 && !step_fusable) {
-    set_error("the down kernel with the PCG step needs an all-wet FP32 Dirichlet level 0");
+    set_error("the down kernel with the PCG step needs an FP32 Dirichlet level 0");
     return kInvalidArgument;
   }
   for (int k = 0; k < reps; ++k) {

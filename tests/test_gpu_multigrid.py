@@ -277,8 +277,9 @@ def test_large_bottom_level_uses_coarse_kernel():
 
 def test_kernel_bench_down_step_mode():
     """fvx_mg_kernel_bench mode 3 (the level-0 down kernel with the previous
-    PCG step) runs on an all-wet FP32 Dirichlet plan and is refused, with the
-    library's invalid-argument error, on a plan with a land mask."""
+    PCG step) runs on FP32 Dirichlet plans, all-wet or with a land mask, and
+    is refused, with the library's invalid-argument error, on a no-flux
+    plan."""
     need_gpu()
     from finitevolx_amd import _lib
 
@@ -294,9 +295,14 @@ def test_kernel_bench_down_step_mode():
     m[:, :9] = 0.0
     masked = S.build_multigrid_solver(m, 1.0 / nx, 1.0 / ny, 10.0, kind="dirichlet",
                                       precision="mixed", **kw)
-    _lib.call("fvx_mg_kernel_bench", masked._h, 3, 32, 0, _lib.stream())
+    for reps in (0, 3):
+        _lib.call("fvx_mg_kernel_bench", masked._h, 3, 32, reps, _lib.stream())
+    torch.cuda.synchronize()
+    noflux = S.build_multigrid_solver(m, 1.0 / nx, 1.0 / ny, 10.0, kind="noflux",
+                                      precision="mixed", **kw)
+    _lib.call("fvx_mg_kernel_bench", noflux._h, 3, 32, 0, _lib.stream())
     with pytest.raises(ValueError):
-        _lib.call("fvx_mg_kernel_bench", masked._h, 3, 32, 1, _lib.stream())
+        _lib.call("fvx_mg_kernel_bench", noflux._h, 3, 32, 1, _lib.stream())
 
 
 @pytest.mark.parametrize("ny,nx,kind,masked,varcoef", [CASES[3], CASES[4], CASES[5]])
