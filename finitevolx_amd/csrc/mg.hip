@@ -1010,7 +1010,9 @@ __device__ __forceinline__ void mg_wave_tile_body(const TileArgs<float>& a, int 
       const int r0 = H + 2 * k2;
       const bool w0 = MK ? wet_r(r0) : (colin && row_in(r0));
       const bool w1 = MK ? wet_r(r0 + 1) : (colin && row_in(r0 + 1));
-      const T res0 = w0 ? b[r0] - Au(r0) : T(0), res1 = w1 ? b[r0 + 1] - Au(r0 + 1) : T(0);
+      // (A u with every lane active: lane moves read the neighbouring lanes)
+      const T a0 = Au(r0), a1 = Au(r0 + 1);
+      const T res0 = w0 ? b[r0] - a0 : T(0), res1 = w1 ? b[r0 + 1] - a1 : T(0);
       const T s2 = res0 + res1;
       const T s4 = s2 + lane_east(s2);
       const int J = (j0 >> 1) + k2;
@@ -1042,12 +1044,15 @@ __device__ __forceinline__ void mg_wave_tile_body(const TileArgs<float>& a, int 
         if (!row_in(H + t)) rq[t] = T(0);
     }
     // Every product below has a factor that is zero outside the tile
-    // interior (rd, qd, or az): z itself is only stored.
+    // interior (rd, qd, or az): z itself is only stored.  (A z is formed
+    // with every lane active: lane moves read the neighbouring lanes.)
+    T azs[WTR];
 #pragma unroll
     for (int t = 0; t < WTR; ++t) {
       const int r = H + t;
       const T azv = Au(r);
       const T az = (lin && (MK ? wet_r(r) : (!edge || row_in(r)))) ? azv : T(0);
+      azs[t] = az;
       const T z = u[r], rd = lin ? b[r] : T(0), qd = rq[t];
       d[kZR] += z * rd;
       d[kZQ] += z * qd;
@@ -1071,10 +1076,7 @@ __device__ __forceinline__ void mg_wave_tile_body(const TileArgs<float>& a, int 
 #pragma unroll
         for (int t = 0; t < WTR; ++t) {
           const int gj = j0 + t;
-          const int r = H + t;
-          const T azv = Au(r);
-          if (gj < L.ny)
-            buf_store(buf_rsrc(a.az_out, nL), (unsigned)gi, gj * L.nx, wet_r(r) ? azv : T(0));
+          if (gj < L.ny) buf_store(buf_rsrc(a.az_out, nL), (unsigned)gi, gj * L.nx, azs[t]);
         }
       }
     }
@@ -3504,6 +3506,18 @@ __global__ __launch_bounds__(kTileThreads) void mg_tile_batch_kernel(
   mg_tile_body<T, KIND, NS, MODE, ZERO, MK>(args[e], blockIdx.x - first[e]);
 }
 
+// Work-list form of mg_wave_tile_kernel (FP32 Dirichlet levels of batched
+// solves, with or without a land mask): the block's member by binary search
+// of the block offsets, after the XCD-aware renumbering of the whole grid.
+template <int NS, int MODE, bool ZERO>
+__global__ __launch_bounds__(256) void mg_wave_tile_batch_kernel(
+    const TileArgs<float>* __restrict__ args, const int* __restrict__ first, int n) {
+  const int b = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int e = wl_entry(first, n, b);
+  mg_wave_tile_body<NS, MODE, ZERO, kTileRows, true>(args[e], b - first[e],
+                                                      first[e + 1] - first[e]);
+}
+
 template <typename T, int KIND>
 __global__ __launch_bounds__(kCoarseThreads) void mg_coarse_batch_kernel(
     const CoarseArgs<T>* __restrict__ args) {
@@ -3594,8 +3608,10 @@ static int upload_list(const std::vector<A>& args, const std::vector<int>& nblk,
 // The V-cycle and PCG work-lists of one scalar slot.
 struct BatchSlot {
   bool built = false;
-  // [MK][level] down and up lists (level 0's up list is the kUpCG one)
-  std::vector<DevList<TileArgs<float>>> down[2], up[2];
+  // [MK][level] down and up lists of the LDS tile kernel, [level] lists of
+  // the wave-tile kernel (FP32 Dirichlet levels); level 0's up lists are the
+  // kUpCG ones
+  std::vector<DevList<TileArgs<float>>> down[2], up[2], wdown, wup;
   DevList<CoarseArgs<float>> coarse;
   size_t coarse_lds = 0;
   DevList<FinishArgs> fin_cg, fin_res;
@@ -3629,9 +3645,18 @@ static int batch_build_slot(MgBatch& B, int s) {
     S.down[mk].resize(B.lc_max);
     S.up[mk].resize(B.lc_max);
   }
+  S.wdown.resize(B.lc_max);
+  S.wup.resize(B.lc_max);
+  // Wave tiles: 64 - 2 HL interior columns by kTileRows rows, four per block
+  // (HL: the lane halo of the mode, wt_lane_halo).
+  auto wave_tiles = [](TileArgs<float>& t, int hl) {
+    t.tiles_x = cdiv(t.L.nx, 64 - 2 * hl);
+    t.ntiles = t.tiles_x * cdiv(t.L.ny, kTileRows);
+    return cdiv(t.ntiles, 4);
+  };
   for (int l = 0; l < B.lc_max; ++l) {
-    std::vector<TileArgs<float>> dn[2], upl[2];
-    std::vector<int> dnb[2], upb[2];
+    std::vector<TileArgs<float>> dn[2], upl[2], wdn, wupl;
+    std::vector<int> dnb[2], upb[2], wdnb, wupb;
     for (int m = 0; m < nm; ++m) {
       Plan& Pm = *B.P[m];
       if (l >= Pm.lc) continue;
@@ -3650,11 +3675,17 @@ static int batch_build_slot(MgBatch& B, int s) {
       a.ntiles = a.tiles_x * cdiv(L.ny, kTy);
       // down: zero guess, b = (level 0: the inner residual r; else the
       // restricted residual), iterate into L.u, restriction into C.b
+      const bool wave = B.kind == kDirichlet && wave_tiled<float, kDirichlet>(L, nullptr);
       TileArgs<float> d = a;
       d.b = (l == 0) ? H.r : L.b;
       d.u_out = L.u;
-      dn[mk].push_back(d);
-      dnb[mk].push_back(d.ntiles);
+      if (wave) {
+        wdnb.push_back(wave_tiles(d, B.n_pre + 1));
+        wdn.push_back(d);
+      } else {
+        dn[mk].push_back(d);
+        dnb[mk].push_back(d.ntiles);
+      }
       // up: coarse correction from level l+1's corrected iterate (the coarse
       // kernel's output at lc, else that level's up output L.t), post-sweeps
       TileArgs<float> u = a;
@@ -3666,13 +3697,20 @@ static int batch_build_slot(MgBatch& B, int s) {
         u.w = H.q;
         u.az_out = H.az;
       }
-      upl[mk].push_back(u);
-      upb[mk].push_back(u.ntiles);
+      if (wave) {
+        wupb.push_back(wave_tiles(u, B.n_post + (l == 0 ? 1 : 0)));
+        wupl.push_back(u);
+      } else {
+        upl[mk].push_back(u);
+        upb[mk].push_back(u.ntiles);
+      }
     }
     for (int mk = 0; mk < 2; ++mk) {
       if (int rc = upload_list(dn[mk], dnb[mk], S.down[mk][l], B.allocs)) return rc;
       if (int rc = upload_list(upl[mk], upb[mk], S.up[mk][l], B.allocs)) return rc;
     }
+    if (int rc = upload_list(wdn, wdnb, S.wdown[l], B.allocs)) return rc;
+    if (int rc = upload_list(wupl, wupb, S.wup[l], B.allocs)) return rc;
   }
   std::vector<CoarseArgs<float>> ca;
   std::vector<int> cb;
@@ -3705,7 +3743,16 @@ static int batch_build_slot(MgBatch& B, int s) {
       cb.push_back(1);
     }
     const LevelDev<float>& L0 = H.lv[0];
-    fc.push_back(FinishArgs{Pm.partials, L0.ny > 0 ? cdiv(L0.nx, kTx) * cdiv(L0.ny, kTy) : 0, sc});
+    // partials of the level-0 kUpCG kernel: one per LDS tile, or one per
+    // block of four wave tiles
+    int ncg = cdiv(L0.nx, kTx) * cdiv(L0.ny, kTy);
+    if (B.kind == kDirichlet && wave_tiled<float, kDirichlet>(L0, nullptr)) {
+      TileArgs<float> t;
+      std::memset(&t, 0, sizeof(t));
+      t.L = L0;
+      ncg = wave_tiles(t, B.n_post + 1);
+    }
+    fc.push_back(FinishArgs{Pm.partials, ncg, sc});
     const dim3 gr = grid_res(Pm.h64.lv[0].ny, Pm.h64.lv[0].nx, Pm.kind);
     fr.push_back(FinishArgs{Pm.partials, (int)(gr.x * gr.y), sc + 1});
     ones.push_back(1);
@@ -3795,6 +3842,12 @@ static void batch_vcycle(MgBatch& B, BatchSlot& S, hipStream_t s) {
     if (d1.n)
       hipLaunchKernelGGL((mg_tile_batch_kernel<float, KIND, NPRE, kDown, true, true>),
                          dim3(d1.blocks), dim3(kTileThreads), 0, s, d1.args, d1.first, d1.n);
+    if constexpr (KIND == kDirichlet) {
+      const DevList<TileArgs<float>>& w = S.wdown[l];
+      if (w.n)
+        hipLaunchKernelGGL((mg_wave_tile_batch_kernel<NPRE, kDown, true>), dim3(w.blocks),
+                           dim3(256), 0, s, w.args, w.first, w.n);
+    }
   }
   if (S.coarse.n)
     hipLaunchKernelGGL((mg_coarse_batch_kernel<float, KIND>), dim3(S.coarse.n),
@@ -3809,6 +3862,12 @@ static void batch_vcycle(MgBatch& B, BatchSlot& S, hipStream_t s) {
       if (u1.n)
         hipLaunchKernelGGL((mg_tile_batch_kernel<float, KIND, NPOST, kUp, false, true>),
                            dim3(u1.blocks), dim3(kTileThreads), 0, s, u1.args, u1.first, u1.n);
+      if constexpr (KIND == kDirichlet) {
+        const DevList<TileArgs<float>>& w = S.wup[l];
+        if (w.n)
+          hipLaunchKernelGGL((mg_wave_tile_batch_kernel<NPOST, kUp, false>), dim3(w.blocks),
+                             dim3(256), 0, s, w.args, w.first, w.n);
+      }
     } else {
       if (u0.n)
         hipLaunchKernelGGL((mg_tile_batch_kernel<float, KIND, NPOST, kUpCG, false, false>),
@@ -3816,6 +3875,12 @@ static void batch_vcycle(MgBatch& B, BatchSlot& S, hipStream_t s) {
       if (u1.n)
         hipLaunchKernelGGL((mg_tile_batch_kernel<float, KIND, NPOST, kUpCG, false, true>),
                            dim3(u1.blocks), dim3(kTileThreads), 0, s, u1.args, u1.first, u1.n);
+      if constexpr (KIND == kDirichlet) {
+        const DevList<TileArgs<float>>& w = S.wup[0];
+        if (w.n)
+          hipLaunchKernelGGL((mg_wave_tile_batch_kernel<NPOST, kUpCG, false>), dim3(w.blocks),
+                             dim3(256), 0, s, w.args, w.first, w.n);
+      }
     }
   }
 }
