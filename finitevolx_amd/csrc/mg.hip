@@ -3503,19 +3503,26 @@ template <typename T, int KIND, int NS, int MODE, bool ZERO, bool MK>
 __global__ __launch_bounds__(kTileThreads) void mg_tile_batch_kernel(
     const TileArgs<T>* __restrict__ args, const int* __restrict__ first, int n) {
   const int e = wl_entry(first, n, blockIdx.x);
+  if (args[e].sc && args[e].sc->done) return;  // finished member: no loads
   mg_tile_body<T, KIND, NS, MODE, ZERO, MK>(args[e], blockIdx.x - first[e]);
 }
 
 // Work-list form of mg_wave_tile_kernel (FP32 Dirichlet levels of batched
 // solves, with or without a land mask): the block's member by binary search
 // of the block offsets, after the XCD-aware renumbering of the whole grid.
+// (A persistent grid walking the list, stepping over finished members, ran
+// at half the speed: each wave's tiles then run one after the other.)
 template <int NS, int MODE, bool ZERO>
 __global__ __launch_bounds__(256) void mg_wave_tile_batch_kernel(
     const TileArgs<float>* __restrict__ args, const int* __restrict__ first, int n) {
   const int b = xcd_swizzle(blockIdx.x, gridDim.x);
   const int e = wl_entry(first, n, b);
-  mg_wave_tile_body<NS, MODE, ZERO, kTileRows, true>(args[e], b - first[e],
-                                                      first[e + 1] - first[e]);
+  // A member whose solve has stopped loads nothing (the single-solve kernel
+  // checks its flag after issuing the loads, to hide the flag's latency; in
+  // a batch most blocks may belong to finished members).
+  const TileArgs<float>& a = args[e];
+  if (a.sc && a.sc->done) return;
+  mg_wave_tile_body<NS, MODE, ZERO, kTileRows, true>(a, b - first[e], first[e + 1] - first[e]);
 }
 
 template <typename T, int KIND>
