@@ -1805,16 +1805,32 @@ __device__ __forceinline__ void refine_residual_body(
     // of walking its rows one latency at a time.
     constexpr int R = kResRows;
     const bool own = col && lane >= 1 && lane <= 62;
+    const int ic = min(max(i, 0), nx - 1);
+    // bit t: the mask of row j0 - 1 + t in this lane's column, one byte load
+    // per row for both the masked x and the residual's mask
+    uint32_t mb = ~0u;
+    if (L.mask) {
+      mb = 0;
+#pragma unroll
+      for (int t = 0; t < R + 2; ++t) {
+        const int jc = min(max(j0 - 1 + t, 0), ny - 1);
+        mb |= (L.mask[(int64_t)jc * nx + ic] ? 1u : 0u) << t;
+      }
+    }
     double X[R + 2], B[R];
     float M[R];
 #pragma unroll
-    for (int t = 0; t < R + 2; ++t) X[t] = col ? Xat(j0 - 1 + t, i) : 0.0;
+    for (int t = 0; t < R + 2; ++t) {
+      const int j = j0 - 1 + t;
+      const double v = (col && j >= 0 && j < ny) ? x[(int64_t)j * ld_x + i] : 0.0;
+      X[t] = ((mb >> t) & 1u) ? v : 0.0;  // x * mask
+    }
 #pragma unroll
     for (int t = 0; t < R; ++t) {
       const int j = j0 + t;
       const bool in = own && j < ny;
-      const int ic = min(max(i, 0), nx - 1), jc = min(j, ny - 1);
-      M[t] = in ? (float)mval<double>(L.mask, (int64_t)jc * nx + ic) : 0.0f;
+      const int jc = min(j, ny - 1);
+      M[t] = (in && ((mb >> (t + 1)) & 1u)) ? 1.0f : 0.0f;
       B[t] = in ? (double)b[(int64_t)jc * ld_b + ic] : 0.0;
     }
     // A certified solve has not moved x since the residual that certified it

@@ -241,6 +241,20 @@ __device__ __forceinline__ void qg_stage_wave_body(
     q[t] = (T)v;
     if (t >= 2 && t < R + 2) qc[t - 2] = v;
   }
+  // MASKED: bit t = the mask of row j0 - 1 + t in this lane's column (one
+  // byte load per row, read once for the certification and the output).
+  uint32_t mbits = ~0u;
+  if constexpr (MASKED) {
+    if (mask) {
+      mbits = 0;
+#pragma unroll
+      for (int t = 0; t < R + 2; ++t) {
+        const int gjc = min(max(j0 - 1 + t, 0), Ny - 1);
+        mbits |= (mask[gjc * Nx + (int)gic] ? 1u : 0u) << t;
+      }
+    }
+  }
+  auto wet_t = [&](int t) -> bool { return MASKED ? ((mbits >> t) & 1u) != 0 : true; };
   // With CERT: psi * mask in FP64 on the interior, zero elsewhere (the
   // solve's masked, zero-padded operand), and the mask of each row.
   double xm[R + 2];
@@ -253,8 +267,7 @@ __device__ __forceinline__ void qg_stage_wave_body(
     f[t] = (colin && gj >= 0 && gj < Ny) ? (T)v : T(0);
     if constexpr (CERT) {
       const bool in = colin && gj >= 1 && gj <= Ny - 2 && gi >= 1 && gi <= Nx - 2;
-      double m = 1.0;
-      if constexpr (MASKED) m = (!mask || mask[g]) ? 1.0 : 0.0;
+      const double m = wet_t(t) ? 1.0 : 0.0;
       md[t] = m;
       xm[t] = in ? (double)v * m : 0.0;
     }
@@ -339,7 +352,7 @@ __device__ __forceinline__ void qg_stage_wave_body(
       TS res = TS(0);
       if (ci && gj >= 1 && gj <= Ny - 2) {
         TS r = (TS)rhs;
-        if constexpr (MASKED) r = (!mask || mask[g]) ? r : TS(0);
+        if constexpr (MASKED) r = wet_t(t + 1) ? r : TS(0);
         res = (a != TS(0) ? a * qn[g] : TS(0)) + b * qc[t] + cdt * r;
       }
       out[g] = res;

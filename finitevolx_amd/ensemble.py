@@ -17,6 +17,8 @@ batches independent problems with ``vmap`` (``solvers/elliptic.py:532-541``,
 
 from __future__ import annotations
 
+import gc
+
 import ctypes as C
 import time
 from dataclasses import dataclass
@@ -133,14 +135,23 @@ class BudgetedSteps:
         return out
 
     def _capture(self):
+        # Unreachable plans freed by the garbage collector during a capture
+        # would call hipFree, which ends the capture with an error: the
+        # collector stays off while capturing.
         self.graphs = {}
-        for e in (0, 1):
-            g = torch.cuda.CUDAGraph()
-            s = torch.cuda.Stream()
-            with torch.cuda.stream(s):
-                with torch.cuda.graph(g, stream=s):
-                    self._enqueue_step(e)
-            self.graphs[e] = g
+        was_enabled = gc.isenabled()
+        gc.disable()
+        try:
+            for e in (0, 1):
+                g = torch.cuda.CUDAGraph()
+                s = torch.cuda.Stream()
+                with torch.cuda.stream(s):
+                    with torch.cuda.graph(g, stream=s):
+                        self._enqueue_step(e)
+                self.graphs[e] = g
+        finally:
+            if was_enabled:
+                gc.enable()
         self.recaptures += 1
 
     def step(self, nsteps: int = 1, graph: bool = True):
