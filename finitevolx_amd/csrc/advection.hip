@@ -3,12 +3,13 @@
 // exposes: naive, upwind1/2/3, weno3, wenoz3, weno5, wenoz5 and the four TVD
 // limiters, unmasked or with per-face stencil tiers (advection/flux.py:100-253).
 //
-// One pass per tile: the tracer tile plus a 3-cell halo is staged in LDS from
-// coalesced rows; every face flux of the tile is reconstructed exactly once
-// (one upwind-selected evaluation per face, the reference evaluates both
-// branches) and kept in LDS; the flux divergence, the mask and the zero
-// 2-cell output ring are applied in the same kernel.  HBM traffic per cell:
-// read h, u, v and write the tendency (+ 2 tier bytes + 1 mask byte masked).
+// One pass per wavefront tile (advection_kernel): the tracer column of each
+// lane plus a 3-cell halo sits in registers, loaded from coalesced rows;
+// every face flux of the tile is reconstructed exactly once (one
+// upwind-selected, branch-free evaluation per face, the reference evaluates
+// both branches); the flux divergence, the mask and the zero 2-cell output
+// ring are applied in the same kernel.  HBM traffic per cell: read h, u, v
+// and write the tendency (+ 2 tier bytes + 1 mask byte masked).
 
 #include <algorithm>
 #include <cmath>
@@ -32,6 +33,45 @@ enum AdvMethod : int {
 enum Limiter : int { kMinmod = 0, kVanLeer = 1, kSuperbee = 2, kMC = 3 };
 
 // --- point reconstructions (advection/weno.py) -----------------------------
+//
+// The nonlinear weights are the reference's; only how they are divided out
+// differs, because the divisions dominate the instruction count of these
+// VALU-bound kernels.  FP32: each division is a product with the hardware
+// reciprocal (v_rcp_f32, 1 ulp), inside the FP32 stencils' 1e-5 bound.  FP64:
+// the normalised combination sum_k w_k q_k / sum_k w_k with w_k = d_k / t_k is
+// rewritten over the common denominator prod_k t_k, so one IEEE division
+// remains (the t_k are >= eps^2 and products of two or three of them stay in
+// FP64's range); rounding differs from the reference's by a few ulps.
+
+template <typename T>
+__device__ __forceinline__ T rcp(T x) {
+  if constexpr (sizeof(T) == 4) return __builtin_amdgcn_rcpf(x);
+  else return T(1) / x;
+}
+
+// (c1 q1 / t1 + c2 q2 / t2) / (c1 / t1 + c2 / t2)
+template <typename T>
+__device__ __forceinline__ T blend2(T c1, T t1, T q1, T c2, T t2, T q2) {
+  if constexpr (sizeof(T) == 4) {
+    const T w1 = c1 * rcp(t1), w2 = c2 * rcp(t2);
+    return (w1 * q1 + w2 * q2) * rcp(w1 + w2);
+  } else {
+    const T a1 = c1 * t2, a2 = c2 * t1;
+    return (a1 * q1 + a2 * q2) / (a1 + a2);
+  }
+}
+
+// (c1 q1 / t1 + c2 q2 / t2 + c3 q3 / t3) / (c1 / t1 + c2 / t2 + c3 / t3)
+template <typename T>
+__device__ __forceinline__ T blend3(T c1, T t1, T q1, T c2, T t2, T q2, T c3, T t3, T q3) {
+  if constexpr (sizeof(T) == 4) {
+    const T w1 = c1 * rcp(t1), w2 = c2 * rcp(t2), w3 = c3 * rcp(t3);
+    return (w1 * q1 + w2 * q2 + w3 * q3) * rcp(w1 + w2 + w3);
+  } else {
+    const T a1 = c1 * (t2 * t3), a2 = c2 * (t1 * t3), a3 = c3 * (t1 * t2);
+    return (a1 * q1 + a2 * q2 + a3 * q3) / (a1 + a2 + a3);
+  }
+}
 
 template <typename T>
 __device__ __forceinline__ T weno3_left(T qm, T q0, T qp) {  // weno.py:297-320
@@ -40,8 +80,8 @@ __device__ __forceinline__ T weno3_left(T qm, T q0, T qp) {  // weno.py:297-320
   const T qi2 = T(0.5) * (q0 + qp);
   const T b1 = (q0 - qm) * (q0 - qm), b2 = (qp - q0) * (qp - q0);
   const T t1 = b1 + eps, t2 = b2 + eps;
-  const T w1 = T(1.0 / 3.0) / (t1 * t1), w2 = T(2.0 / 3.0) / (t2 * t2);
-  return (w1 * qi1 + w2 * qi2) / (w1 + w2);
+  // w1 = (1/3) / t1^2, w2 = (2/3) / t2^2
+  return blend2(T(1.0 / 3.0), t1 * t1, qi1, T(2.0 / 3.0), t2 * t2, qi2);
 }
 
 template <typename T>
@@ -51,10 +91,10 @@ __device__ __forceinline__ T weno3_right(T q0, T qp, T qpp) {  // weno.py:413-43
   const T qi2 = T(1.5) * qp - T(0.5) * qpp;
   const T b1 = (qp - q0) * (qp - q0), b2 = (qpp - qp) * (qpp - qp);
   const T t1 = b1 + eps, t2 = b2 + eps;
-  const T w1 = T(2.0 / 3.0) / (t1 * t1), w2 = T(1.0 / 3.0) / (t2 * t2);
-  return (w1 * qi1 + w2 * qi2) / (w1 + w2);
+  return blend2(T(2.0 / 3.0), t1 * t1, qi1, T(1.0 / 3.0), t2 * t2, qi2);
 }
 
+// WENO-Z weights d_k (1 + tau / (b_k + eps)) = d_k (b_k + eps + tau) / (b_k + eps).
 template <typename T>
 __device__ __forceinline__ T wenoz3_left(T qm, T q0, T qp) {  // weno.py:323-347
   const T eps = T(1e-14);
@@ -62,9 +102,14 @@ __device__ __forceinline__ T wenoz3_left(T qm, T q0, T qp) {  // weno.py:323-347
   const T qi2 = T(0.5) * (q0 + qp);
   const T b1 = (q0 - qm) * (q0 - qm), b2 = (qp - q0) * (qp - q0);
   const T tau = fabs(b2 - b1);
-  const T w1 = T(1.0 / 3.0) * (T(1) + tau / (b1 + eps));
-  const T w2 = T(2.0 / 3.0) * (T(1) + tau / (b2 + eps));
-  return (w1 * qi1 + w2 * qi2) / (w1 + w2);
+  if constexpr (sizeof(T) == 4) {
+    const T w1 = T(1.0 / 3.0) * (T(1) + tau * rcp(b1 + eps));
+    const T w2 = T(2.0 / 3.0) * (T(1) + tau * rcp(b2 + eps));
+    return (w1 * qi1 + w2 * qi2) * rcp(w1 + w2);
+  } else {
+    const T t1 = b1 + eps, t2 = b2 + eps;
+    return blend2(T(1.0 / 3.0) * (t1 + tau), t1, qi1, T(2.0 / 3.0) * (t2 + tau), t2, qi2);
+  }
 }
 
 template <typename T>
@@ -74,9 +119,14 @@ __device__ __forceinline__ T wenoz3_right(T q0, T qp, T qpp) {  // weno.py:438-4
   const T qi2 = T(1.5) * qp - T(0.5) * qpp;
   const T b1 = (qp - q0) * (qp - q0), b2 = (qpp - qp) * (qpp - qp);
   const T tau = fabs(b2 - b1);
-  const T w1 = T(2.0 / 3.0) * (T(1) + tau / (b1 + eps));
-  const T w2 = T(1.0 / 3.0) * (T(1) + tau / (b2 + eps));
-  return (w1 * qi1 + w2 * qi2) / (w1 + w2);
+  if constexpr (sizeof(T) == 4) {
+    const T w1 = T(2.0 / 3.0) * (T(1) + tau * rcp(b1 + eps));
+    const T w2 = T(1.0 / 3.0) * (T(1) + tau * rcp(b2 + eps));
+    return (w1 * qi1 + w2 * qi2) * rcp(w1 + w2);
+  } else {
+    const T t1 = b1 + eps, t2 = b2 + eps;
+    return blend2(T(2.0 / 3.0) * (t1 + tau), t1, qi1, T(1.0 / 3.0) * (t2 + tau), t2, qi2);
+  }
 }
 
 // WENO5 on the upwind-ordered stencil (a, b, c, d, e) = (qmm, qm, q0, qp, qpp)
@@ -95,21 +145,24 @@ __device__ __forceinline__ T weno5_upwind(T a, T b, T c, T d, T e) {
   const T b1 = k1 * (s1 * s1) + k2 * (r1 * r1);
   const T b2 = k1 * (s2 * s2) + k2 * (r2 * r2);
   const T b3 = k1 * (s3 * s3) + k2 * (r3 * r3);
-  T w1, w2, w3;
-  if constexpr (!Z) {  // Jiang-Shu weights, eps 1e-8
+  if constexpr (!Z) {  // Jiang-Shu weights d_k / (b_k + eps)^2, eps 1e-8
     const T eps = T(1e-8);
     const T t1 = b1 + eps, t2 = b2 + eps, t3 = b3 + eps;
-    w1 = T(0.1) / (t1 * t1);
-    w2 = T(0.6) / (t2 * t2);
-    w3 = T(0.3) / (t3 * t3);
+    return blend3(T(0.1), t1 * t1, qi1, T(0.6), t2 * t2, qi2, T(0.3), t3 * t3, qi3);
   } else {  // WENO-Z (Borges et al. 2008), weno.py:379-410, eps 1e-16
     const T eps = T(1e-16);
     const T tau5 = fabs(b1 - b3);
-    w1 = T(0.1) * (T(1) + tau5 / (b1 + eps));
-    w2 = T(0.6) * (T(1) + tau5 / (b2 + eps));
-    w3 = T(0.3) * (T(1) + tau5 / (b3 + eps));
+    if constexpr (sizeof(T) == 4) {
+      const T w1 = T(0.1) * (T(1) + tau5 * rcp(b1 + eps));
+      const T w2 = T(0.6) * (T(1) + tau5 * rcp(b2 + eps));
+      const T w3 = T(0.3) * (T(1) + tau5 * rcp(b3 + eps));
+      return (w1 * qi1 + w2 * qi2 + w3 * qi3) * rcp(w1 + w2 + w3);
+    } else {
+      const T t1 = b1 + eps, t2 = b2 + eps, t3 = b3 + eps;
+      return blend3(T(0.1) * (t1 + tau5), t1, qi1, T(0.6) * (t2 + tau5), t2, qi2,
+                    T(0.3) * (t3 + tau5), t3, qi3);
+    }
   }
-  return (w1 * qi1 + w2 * qi2 + w3 * qi3) / (w1 + w2 + w3);
 }
 
 template <typename T, int L>
@@ -183,7 +236,6 @@ __device__ __forceinline__ T face_flux(const T* s, int st, int fi, int n, T vel,
   }
 }
 
-constexpr int kAdvTx = 64, kAdvTy = 8, kAdvHalo = 3, kAdvThreads = kAdvTx * kAdvTy;
 
 // Optional fused Runge-Kutta stage: with hn set the kernel writes
 // a hn + b h + cdt L(h) in place of L(h) (out may alias hn, not h).
@@ -193,67 +245,148 @@ struct StageCoef {
   double a = 0.0, b = 0.0, cdt = 0.0;
 };
 
+// Face value from the stencil mirrored to the upwind side of the face:
+// m[k + 2] = h[i + k] for flow towards +x (k = -2 .. 2, the face between
+// cells i and i+1) and h[i + 1 - k] for flow towards -x.  Every scheme's
+// negative-flow formula is its positive-flow formula on the mirrored stencil
+// (weno.py:413-491 against :297-410, reconstruction.py:636-1050), so one
+// branch-free evaluation serves both signs.  ``near``: the face next to the
+// array edge on the upwind side (face 1 for positive, n-3 for negative flow),
+// where the 5-point schemes fall back to the 3-point ones.
+template <typename T, int M, int L>
+__device__ __forceinline__ T upwind_value(const T (&m)[5], bool near) {
+  const T hm1 = m[1], h0 = m[2], h1 = m[3];
+  if constexpr (M == kNaive) return T(0.5) * (h0 + h1);
+  if constexpr (M == kUpwind1) return h0;
+  if constexpr (M == kUpwind2) return T(1.5) * h0 - T(0.5) * hm1;
+  if constexpr (M == kUpwind3)
+    return T(-1.0 / 6.0) * hm1 + T(5.0 / 6.0) * h0 + T(1.0 / 3.0) * h1;
+  if constexpr (M == kWeno3) return weno3_left(hm1, h0, h1);
+  if constexpr (M == kWenoZ3) return wenoz3_left(hm1, h0, h1);
+  if constexpr (M == kWeno5 || M == kWenoZ5) {
+    constexpr bool Z = (M == kWenoZ5);
+    if (near) return Z ? wenoz3_left(hm1, h0, h1) : weno3_left(hm1, h0, h1);
+    return weno5_upwind<T, Z>(m[0], hm1, h0, h1, m[4]);
+  }
+  if constexpr (M == kTvd) {
+    const T eps = T(1e-8);
+    const T dfwd = h1 - h0, dbwd = h0 - hm1;
+    return h0 + T(0.5) * limiter<T, L>(dbwd / (dfwd + eps)) * dfwd;
+  }
+  return T(0);
+}
+
+// The masked tier dispatch of face_flux on the mirrored stencil.
 template <typename T, int M, int L, bool MASKED>
-__global__ __launch_bounds__(kAdvThreads) void advection_kernel(
+__device__ __forceinline__ T upwind_face(const T (&m)[5], bool near, int tier) {
+  if constexpr (!MASKED) {
+    return upwind_value<T, M, L>(m, near);
+  } else {
+    constexpr int cap = (M == kWeno5 || M == kWenoZ5) ? 6 : 4;
+    constexpr int M4 = (M == kWenoZ5) ? kWenoZ3 : (M == kTvd ? kTvd : kWeno3);
+    const int t = tier < cap ? tier : cap;
+    if (t == 2) return upwind_value<T, kUpwind1, L>(m, near);
+    if (t == 4) return upwind_value<T, M4, L>(m, near);
+    if (t == 6) return upwind_value<T, M, L>(m, near);
+    return T(0);
+  }
+}
+
+// One wavefront per tile of 58 output columns (lanes 3..60; three halo lanes
+// each side) x kAdvRows rows.  Each lane keeps its column of h on rows
+// j0-3 .. j0+kAdvRows+2 in registers; the x stencils come from DPP lane
+// shifts, the y stencils from neighbouring registers.  Each lane evaluates
+// the east face of its cell (the west face is the east face of the lane
+// beside it) and the north faces of its column, so every face flux is
+// reconstructed once; no LDS, no barriers, no integer division per cell.
+// Grids up to 512^2 take tiles of 2 rows: more wavefronts for the chip, a
+// shorter chain of loads and arithmetic per wavefront.
+constexpr int kAdvRows = 8, kAdvShortRows = 2, kAdvCols = 58;
+constexpr int64_t kAdvShortMaxCells = 514 * 514;
+
+template <typename T>
+__device__ __forceinline__ T adv_e(T v) {
+  return dpp_move<0x130>(v);  // wave_shl 1: value of lane + 1, 0 in lane 63
+}
+template <typename T>
+__device__ __forceinline__ T adv_w(T v) {
+  return dpp_move<0x138>(v);  // wave_shr 1: value of lane - 1, 0 in lane 0
+}
+
+template <typename T, int M, int L, bool MASKED, int R>
+__global__ __launch_bounds__(256) void advection_kernel(
     const T* __restrict__ h, const T* __restrict__ u, const T* __restrict__ v,
     T* out, const uint8_t* __restrict__ tier_x, const uint8_t* __restrict__ tier_y,
     const uint8_t* __restrict__ mh, int ny, int nx, T idx, T idy, int tiles_x, int ntiles,
     const T* hn, T ca, T cb, T ccdt) {
-  constexpr int SW = kAdvTx + 2 * kAdvHalo;
-  __shared__ T sh[kAdvTy + 2 * kAdvHalo][SW];
-  __shared__ T sfe[kAdvTy][kAdvTx + 1];
-  __shared__ T sfn[kAdvTy + 1][kAdvTx];
-  const int t = xcd_swizzle(blockIdx.x, ntiles);
-  const int i0 = (t % tiles_x) * kAdvTx, j0 = (t / tiles_x) * kAdvTy;
-  const int tid = threadIdx.x;
-
-  for (int k = tid; k < (kAdvTy + 2 * kAdvHalo) * SW; k += kAdvThreads) {
-    const int jj = k / SW, ii = k % SW;
-    const int gj = j0 - kAdvHalo + jj, gi = i0 - kAdvHalo + ii;
-    sh[jj][ii] = (gj >= 0 && gj < ny && gi >= 0 && gi < nx) ? h[(int64_t)gj * nx + gi] : T(0);
+  constexpr int HR = R + 6;
+  const int lane = threadIdx.x & 63;
+  const int tile =
+      __builtin_amdgcn_readfirstlane(xcd_swizzle(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6));
+  if (tile >= ntiles) return;  // whole wavefront (no block-level barrier below)
+  const int tyi = tile / tiles_x;
+  const int i0 = (tile - tyi * tiles_x) * kAdvCols, j0 = tyi * R;
+  const int gi = i0 - 3 + lane;
+  const bool colin = gi >= 0 && gi < nx;
+  const int64_t gic = min(max(gi, 0), nx - 1);
+  // h on rows j0-3+r (zero outside the array)
+  T hr[HR];
+#pragma unroll
+  for (int r = 0; r < HR; ++r) {
+    const int gj = j0 - 3 + r;
+    const int64_t gjc = min(max(gj, 0), ny - 1);
+    const T x = h[gjc * nx + gic];
+    hr[r] = (colin && gj >= 0 && gj < ny) ? x : T(0);
   }
-  __syncthreads();
-
-  // East faces fi = i0-1 .. i0+TX-1 on the tile's rows.
-  for (int k = tid; k < kAdvTy * (kAdvTx + 1); k += kAdvThreads) {
-    const int jj = k / (kAdvTx + 1), kk = k % (kAdvTx + 1);
-    const int gj = j0 + jj, fi = i0 - 1 + kk;
-    T f = T(0);
-    if (gj >= 2 && gj <= ny - 3 && fi >= 1 && fi <= nx - 3) {
-      const int64_t c = (int64_t)gj * nx + fi;
-      const T vel = u[c];
-      int tier = 0;
-      if constexpr (MASKED) tier = (vel >= T(0)) ? tier_x[c] : tier_x[c + 1];
-      f = face_flux<T, M, L, MASKED>(&sh[jj + kAdvHalo][kk + kAdvHalo - 1], 1, fi, nx, vel, tier);
+  // North faces fj = j0-1+t (between rows j0-1+t and j0+t), t = 0 .. R.
+  T fn[R + 1];
+#pragma unroll
+  for (int t = 0; t <= R; ++t) {
+    const int r = t + 2, fj = j0 - 1 + t;
+    const int64_t fjc = min(max(fj, 0), ny - 1);
+    const int64_t c = fjc * nx + gic;
+    const T vel = v[c];
+    const bool pos = vel >= T(0);
+    const T m[5] = {pos ? hr[r - 2] : hr[r + 3], pos ? hr[r - 1] : hr[r + 2],
+                    pos ? hr[r] : hr[r + 1], pos ? hr[r + 1] : hr[r],
+                    pos ? hr[r + 2] : hr[r - 1]};
+    int tier = 0;
+    if constexpr (MASKED) tier = pos ? tier_y[c] : tier_y[min(fjc + 1, (int64_t)ny - 1) * nx + gic];
+    const bool valid = colin && gi >= 2 && gi <= nx - 3 && fj >= 1 && fj <= ny - 3;
+    const T f = upwind_face<T, M, L, MASKED>(m, pos ? fj == 1 : fj == ny - 3, tier) * vel;
+    fn[t] = valid ? f : T(0);
+  }
+  const bool lout = colin && lane >= 3 && lane <= 60;
+#pragma unroll
+  for (int t = 0; t < R; ++t) {
+    const int r = t + 3, gj = j0 + t;
+    const int64_t gjc = min(gj, ny - 1);
+    const int64_t c = gjc * nx + gic;
+    // East face fi = gi of this row (between columns gi and gi+1).
+    const T hc = hr[r];
+    const T e1 = adv_e(hc), e2 = adv_e(e1), e3 = adv_e(e2);
+    const T w1 = adv_w(hc), w2 = adv_w(w1);
+    const T vel = u[c];
+    const bool pos = vel >= T(0);
+    const T m[5] = {pos ? w2 : e3, pos ? w1 : e2, pos ? hc : e1, pos ? e1 : hc, pos ? e2 : w1};
+    int tier = 0;
+    if constexpr (MASKED) {
+      const int tx = tier_x[c];
+      const int txe = __builtin_amdgcn_mov_dpp(tx, 0x130, 0xf, 0xf, true);  // lane + 1
+      tier = pos ? tx : txe;
     }
-    sfe[jj][kk] = f;
-  }
-  // North faces fj = j0-1 .. j0+TY-1 on the tile's columns.
-  for (int k = tid; k < (kAdvTy + 1) * kAdvTx; k += kAdvThreads) {
-    const int kk = k / kAdvTx, ii = k % kAdvTx;
-    const int gi = i0 + ii, fj = j0 - 1 + kk;
-    T f = T(0);
-    if (gi >= 2 && gi <= nx - 3 && fj >= 1 && fj <= ny - 3) {
-      const int64_t c = (int64_t)fj * nx + gi;
-      const T vel = v[c];
-      int tier = 0;
-      if constexpr (MASKED) tier = (vel >= T(0)) ? tier_y[c] : tier_y[c + nx];
-      f = face_flux<T, M, L, MASKED>(&sh[kk + kAdvHalo - 1][ii + kAdvHalo], SW, fj, ny, vel, tier);
+    const bool valid = colin && gj >= 2 && gj <= ny - 3 && gi >= 1 && gi <= nx - 3;
+    const T fe0 = upwind_face<T, M, L, MASKED>(m, pos ? gi == 1 : gi == nx - 3, tier) * vel;
+    const T fe = valid ? fe0 : T(0);
+    const T fw = adv_w(fe);
+    if (lout && gj < ny) {
+      T res = T(0);
+      if (gj >= 2 && gj <= ny - 3 && gi >= 2 && gi <= nx - 3)
+        res = -((fe - fw) * idx + (fn[t + 1] - fn[t]) * idy);
+      if (mh) res = res * (T)(mh[c] ? 1 : 0);
+      if (hn) res = ca * hn[c] + cb * hc + ccdt * res;
+      out[c] = res;
     }
-    sfn[kk][ii] = f;
-  }
-  __syncthreads();
-
-  const int tx = tid % kAdvTx, ty = tid / kAdvTx;
-  const int gi = i0 + tx, gj = j0 + ty;
-  if (gi < nx && gj < ny) {
-    T r = T(0);
-    if (gj >= 2 && gj <= ny - 3 && gi >= 2 && gi <= nx - 3)
-      r = -((sfe[ty][tx + 1] - sfe[ty][tx]) * idx + (sfn[ty + 1][tx] - sfn[ty][tx]) * idy);
-    const int64_t c = (int64_t)gj * nx + gi;
-    if (mh) r = r * (T)(mh[c] ? 1 : 0);
-    if (hn) r = ca * hn[c] + cb * sh[ty + kAdvHalo][tx + kAdvHalo] + ccdt * r;
-    out[c] = r;
   }
 }
 
@@ -261,11 +394,18 @@ template <typename T, int M, int L, bool MASKED>
 static void launch_adv(const T* h, const T* u, const T* v, T* out, const uint8_t* tx_,
                        const uint8_t* ty_, const uint8_t* mh, int ny, int nx, double dx,
                        double dy, const StageCoef<T>& sc, hipStream_t s) {
-  const int tiles_x = cdiv(nx, kAdvTx), tiles_y = cdiv(ny, kAdvTy);
-  hipLaunchKernelGGL((advection_kernel<T, M, L, MASKED>), dim3(tiles_x * tiles_y),
-                     dim3(kAdvThreads), 0, s, h, u, v, out, tx_, ty_, mh, ny, nx,
-                     (T)(1.0 / dx), (T)(1.0 / dy), tiles_x, tiles_x * tiles_y, sc.hn, (T)sc.a,
-                     (T)sc.b, (T)sc.cdt);
+  const bool short_tiles = (int64_t)ny * nx <= kAdvShortMaxCells;
+  const int rows = short_tiles ? kAdvShortRows : kAdvRows;
+  const int tiles_x = cdiv(nx, kAdvCols), tiles_y = cdiv(ny, rows);
+  const int ntiles = tiles_x * tiles_y;
+  if (short_tiles)
+    hipLaunchKernelGGL((advection_kernel<T, M, L, MASKED, kAdvShortRows>), dim3(cdiv(ntiles, 4)),
+                       dim3(256), 0, s, h, u, v, out, tx_, ty_, mh, ny, nx, (T)(1.0 / dx),
+                       (T)(1.0 / dy), tiles_x, ntiles, sc.hn, (T)sc.a, (T)sc.b, (T)sc.cdt);
+  else
+    hipLaunchKernelGGL((advection_kernel<T, M, L, MASKED, kAdvRows>), dim3(cdiv(ntiles, 4)),
+                       dim3(256), 0, s, h, u, v, out, tx_, ty_, mh, ny, nx, (T)(1.0 / dx),
+                       (T)(1.0 / dy), tiles_x, ntiles, sc.hn, (T)sc.a, (T)sc.b, (T)sc.cdt);
 }
 
 template <typename T, bool MASKED>
