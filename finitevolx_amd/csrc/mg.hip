@@ -48,6 +48,10 @@ struct LevelDev {
   T* t;                    // scratch (ping-pong for single-workgroup sweeps)
   T* t2;                   // scratch for tiled sweep chains
   T* t3;
+  // Dirichlet coarse levels with the wall correction: each dry or outside
+  // neighbour adds wall_g / h^2 to the diagonal (a ghost value wall_g u_c
+  // instead of 0); 0 = off.
+  T wall_g;
 };
 
 template <typename T>
@@ -724,7 +728,7 @@ constexpr int wt_pcg_rows() {
   return MODE == kUpCG ? 8 : 8;
 }
 
-template <int NS, int MODE, bool ZERO, int ROWS, bool MK>
+template <int NS, int MODE, bool ZERO, int ROWS, bool MK, bool WALL = false>
 __device__ __forceinline__ void mg_wave_tile_body(const TileArgs<float>& a, int bx, int nblk) {
   using T = float;
   constexpr int H = wt_halo<NS, MODE>();
@@ -897,6 +901,35 @@ __device__ __forceinline__ void mg_wave_tile_body(const TileArgs<float>& a, int 
   };
 
   const T om = a.omega, idx2 = L.idx2, idy2 = L.idy2, lam = L.lam, invd = L.inv_d;
+  // WALL: per row of this lane's column, the diagonal correction
+  // wall_g (dry x neighbours / dx^2 + dry y neighbours / dy^2) and the
+  // cell's inverse diagonal (0 on dry or outside cells).  Neighbours outside
+  // the tile's rows or lanes only occur in its halo, whose values the sweeps
+  // discard.
+  T dgr[WALL ? R : 1], ivr[WALL ? R : 1];
+  if constexpr (WALL) {
+    uint32_t wm = mb;
+    if constexpr (!MK) {
+      wm = colin ? ((R >= 32) ? ~0u : ((1u << R) - 1u)) : 0u;
+      if (edge) {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          if (!row_in(r)) wm &= ~(1u << r);
+      }
+    }
+    const uint32_t wb = (uint32_t)__builtin_amdgcn_mov_dpp((int)wm, 0x138, 0xf, 0xf, true);
+    const uint32_t eb = (uint32_t)__builtin_amdgcn_mov_dpp((int)wm, 0x130, 0xf, 0xf, true);
+    const T g = L.wall_g, d0 = -(T(2) * idx2 + T(2) * idy2 + lam);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const T nlx = T(2 - (int)((wb >> r) & 1u) - (int)((eb >> r) & 1u));
+      const T nly = T(2 - (r + 1 < R ? (int)((wm >> (r + 1)) & 1u) : 1) -
+                      (r > 0 ? (int)((wm >> (r - 1)) & 1u) : 1));
+      const T dg = g * (nlx * idx2 + nly * idy2);
+      dgr[r] = dg;
+      ivr[r] = ((wm >> r) & 1u) ? T(1) / (d0 + dg) : T(0);
+    }
+  }
   if constexpr (kUpd) {
     // The previous PCG step (pcg_cg_update_kernel's arithmetic) on every
     // cell of the tile, so that the sweeps below start from r_{k+1}:
@@ -946,6 +979,7 @@ __device__ __forceinline__ void mg_wave_tile_body(const TileArgs<float>& a, int 
     const T c = u[r];
     const T e = lane_east(c), w = lane_west(c);
     const T lap = (e + w - T(2) * c) * idx2 + (u[r + 1] + u[r - 1] - T(2) * c) * idy2;
+    if constexpr (WALL) return lap - lam * c + dgr[r] * c;
     return lap - lam * c;
   };
   // u = (u + P e_c) * mask  (multigrid.py:875-876), _prolongate
@@ -982,7 +1016,17 @@ __device__ __forceinline__ void mg_wave_tile_body(const TileArgs<float>& a, int 
     for (int r = sw; r < R - sw; ++r) {
       const T c = u[r];
       T un;
-      if (ZERO && sw == 1 && !kUpMode) {
+      if constexpr (WALL) {
+        // u + om (b - A u) / d = (1 - om) u + om (b - offdiag(u)) / d with
+        // the cell's own d
+        if (ZERO && sw == 1 && !kUpMode) {
+          un = (om * ivr[r]) * b[r];
+        } else {
+          const T e = lane_east(c), w = lane_west(c);
+          un = (T(1) - om) * c +
+               (om * ivr[r]) * (b[r] - idx2 * (e + w) - idy2 * (u[r + 1] + prev));
+        }
+      } else if (ZERO && sw == 1 && !kUpMode) {
         un = kb * b[r];
       } else {
         const T e = lane_east(c), w = lane_west(c);
@@ -1091,9 +1135,10 @@ __device__ __forceinline__ void mg_wave_tile_body(const TileArgs<float>& a, int 
   }
   if constexpr (MODE == kUpCG) block_partials<kNumCgDots, 256>(d, a.partials, nblk, bx);
 }
-template <int NS, int MODE, bool ZERO, int ROWS, bool MK = false>
+template <int NS, int MODE, bool ZERO, int ROWS, bool MK = false, bool WALL = false>
 __global__ __launch_bounds__(256) void mg_wave_tile_kernel(TileArgs<float> a) {
-  mg_wave_tile_body<NS, MODE, ZERO, ROWS, MK>(a, xcd_swizzle(blockIdx.x, gridDim.x), gridDim.x);
+  mg_wave_tile_body<NS, MODE, ZERO, ROWS, MK, WALL>(a, xcd_swizzle(blockIdx.x, gridDim.x),
+                                                    gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -2278,7 +2323,18 @@ static void launch_tile(const Plan& P, const LevelDev<T>& L, const LevelDev<T>* 
       a.ntiles = a.tiles_x * cdiv(L.ny, short_tiles ? kShortTileRows : kLongRows);
       const int nblk = cdiv(a.ntiles, 4);
       const bool mk = L.mask || (C && C->mask);
+      // the wall correction exists on coarse levels only (no PCG modes)
+      constexpr bool kMayWall = MODE == kSmooth || MODE == kDown || MODE == kUp;
+      const bool wall = kMayWall && L.wall_g != T(0);
       auto go = [&](auto rows, auto masked) {
+        if constexpr (kMayWall) {
+          if (wall) {
+            hipLaunchKernelGGL((mg_wave_tile_kernel<NS, MODE, ZERO, decltype(rows)::value,
+                                                    decltype(masked)::value, true>),
+                               dim3(nblk), dim3(256), 0, s, a);
+            return;
+          }
+        }
         hipLaunchKernelGGL((mg_wave_tile_kernel<NS, MODE, ZERO, decltype(rows)::value,
                                                 decltype(masked)::value>),
                            dim3(nblk), dim3(256), 0, s, a);
@@ -3107,6 +3163,33 @@ const PcgScalars* fetch_slots(Plan& P, int first_slot, int n, hipStream_t s) {
   if (hipStreamSynchronize(s) != hipSuccess) return nullptr;
   return &P.hsc[2 * first_slot];
 }
+int plan_set_wall(Plan& P, bool on) {
+  if (!P.has32 || P.kind != kDirichlet) {
+    set_error("the wall correction needs an FP32 Dirichlet hierarchy");
+    return kInvalidArgument;
+  }
+  if (on) {
+    // Every level that carries it must run on the wave-tile kernels: all
+    // levels tiled down to a bottom level smoothed in tiles, sweeps fused
+    // (at most 3 a side), and the bottom's later sweep chunks (non-zero
+    // start) only on an unmasked bottom level.
+    const Hier<float>& H = P.h32;
+    const LevelDev<float>& B = H.lv[P.nl - 1];
+    const bool tiled = P.nl == 1 || (P.lc == P.nl - 1 && (int64_t)B.ny * B.nx > kCoarseCells);
+    const bool fused = P.n_pre <= kMaxFuse && P.n_post <= kMaxFuse;
+    const bool bottom = P.n_coarse <= 3 || !B.mask;
+    if (!(tiled && fused && bottom)) {
+      set_error("the wall correction needs every level on the tiled kernels (bottom level over "
+                "64x64 cells, at most 3 fused sweeps a side, 3 bottom sweeps on a masked bottom)");
+      return kInvalidArgument;
+    }
+  }
+  for (int l = 0; l < P.nl; ++l) {
+    const double r = std::ldexp(1.0, l);
+    P.h32.lv[l].wall_g = on ? (float)((1.0 - r) / (1.0 + r)) : 0.0f;
+  }
+  return kOk;
+}
 int plan_levels(const Plan& P) { return P.nl; }
 bool plan_has_mixed(const Plan& P) { return P.has32 && P.has64; }
 int plan_nx(const Plan& P) { return P.has64 ? P.h64.lv[0].nx : P.h32.lv[0].nx; }
@@ -3275,6 +3358,14 @@ int fvx_mg_plan_destroy(void* plan) {
   return kOk;
 }
 
+// Coarse-level wall correction of a mixed-precision Dirichlet plan's FP32
+// hierarchy (on = 1) or off (0); see plan_set_wall.
+int fvx_mg_plan_set_wall(void* plan, int on) {
+  using namespace fvx;
+  FVX_CHECK_ARG(plan, "null plan");
+  return plan_set_wall(*(Plan*)plan, on != 0);
+}
+
 // Number of levels, level shapes (2 ints per level) and the coarse-kernel level.
 int fvx_mg_plan_info(void* plan, int* n_levels, int* shapes, int* coarse_level) {
   using namespace fvx;
@@ -3302,6 +3393,8 @@ int fvx_mg_plan_info(void* plan, int* n_levels, int* shapes, int* coarse_level) 
     FVX_CHECK_ARG(P->has##HN, "plan was not built for this precision");                          \
     FVX_CHECK_ARG(level >= 0 && level < P->nl, "level %d out of range", level);                  \
     const auto& L = hier<T>(*P).lv[level];                                                       \
+    FVX_CHECK_ARG(L.wall_g == T(0), "level %d carries the wall correction (solver-internal)",    \
+                  level);                                                                        \
     if (P->kind == kDirichlet)                                                                   \
       hipLaunchKernelGGL((apply_op_kernel<T, kDirichlet>), grid2(L.ny, L.nx), dim3(64, 4), 0,    \
                          (hipStream_t)stream, L, u, out);                                        \
@@ -3321,6 +3414,8 @@ int fvx_mg_plan_info(void* plan, int* n_levels, int* shapes, int* coarse_level) 
     FVX_CHECK_ARG(n_iters >= 0, "negative n_iters");                                             \
     FVX_CHECK_ARG(out != u && out != rhs, "out must not alias inputs");                          \
     const auto& L = hier<T>(*P).lv[level];                                                       \
+    FVX_CHECK_ARG(L.wall_g == T(0), "level %d carries the wall correction (solver-internal)",    \
+                  level);                                                                        \
     hipStream_t s = (hipStream_t)stream;                                                         \
     if (n_iters == 0) {                                                                          \
       hipLaunchKernelGGL(mask_copy_kernel<T>, grid2(L.ny, L.nx), dim3(64, 4), 0, s, L, u, out,   \

@@ -88,6 +88,10 @@ double plan_inv_diag0(const Plan& P);
 // returns the host pointer (2 blocks per slot).
 const PcgScalars* fetch_slots(Plan& P, int first_slot, int n, hipStream_t s);
 int plan_levels(const Plan& P);
+// Coarse-level wall correction of an FP32 Dirichlet hierarchy (level l's
+// dry or outside neighbours take the ghost value g_l u, g_l = (1 - 2^l) /
+// (1 + 2^l)); refused unless every level runs on the wave-tile kernels.
+int plan_set_wall(Plan& P, bool on);
 bool plan_has_mixed(const Plan& P);
 // Learned two-grid preconditioner on the FP32 hierarchy, z = M(r):
 //   z0 = S^sweeps(0; r), res = r - A z0          (damped-Jacobi smoother S)

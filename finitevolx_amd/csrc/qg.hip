@@ -1297,6 +1297,25 @@ int fvx_qg_set_warm_start(void* plan, int mode) {
   return kOk;
 }
 
+// Coarse-level wall correction of the inner FP32 V-cycles of a
+// mixed-precision plan (fvx_mg_plan_set_wall); on = 1 enables it where every
+// level runs on the tiled kernels and returns 1 if it is on, 0 if the plan's
+// hierarchy cannot take it (or on = 0).  Invalidates captured graphs.
+int fvx_qg_set_wall(void* plan, int on, int* enabled) {
+  using namespace fvx;
+  FVX_CHECK_ARG(plan && enabled, "null argument");
+  QGPlan* Q = (QGPlan*)plan;
+  *enabled = 0;
+  if (Q->solve == kSolveMixed) {
+    if (on && plan_set_wall(*Q->mg, true) == kOk) *enabled = 1;
+    else (void)plan_set_wall(*Q->mg, false);
+  }
+  drop_graphs(*Q);
+  Q->calibrated = false;
+  Q->eager_steps = 0;
+  return kOk;
+}
+
 // Use a learned U-Net (fvx_unet_create) as the preconditioner of the inner
 // FP32 solves of a mixed-precision plan (net = NULL restores multigrid);
 // z = out_scale * rms(r) * net(r / rms(r)).  out_scale <= 0 selects the

@@ -40,6 +40,7 @@ _lib.extra_signatures({
     "fvx_qg_fallbacks": [_vp],
     "fvx_qg_budgets": [_vp, C.POINTER(C.c_int)],
     "fvx_qg_set_warm_start": [_vp, _i],
+    "fvx_qg_set_wall": [_vp, _i, C.POINTER(_i)],
     "fvx_qg_stage_f64": [_vp, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _d, _d, _d, _d, _d, _vp],
     "fvx_qg_stage_f32": [_vp, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _d, _d, _d, _d, _d, _vp],
     "fvx_qg_stage_f64t32": [_vp, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _d, _d, _d, _d, _d,
@@ -124,6 +125,17 @@ class QGModel:
         self._net = net  # keep the plan alive while the model uses it
         _lib.call("fvx_qg_set_preconditioner", self._h, None if net is None else net._h,
                   float(out_scale), int(sweeps))
+
+    def set_wall_correction(self, on: bool = True) -> bool:
+        """Coarse-level wall correction of the inner FP32 V-cycles (mixed
+        precision; no reference counterpart): on level l each dry or outside
+        neighbour takes the ghost value g_l u, g_l = (1 - 2^l) / (1 + 2^l), so
+        every level keeps the fine operator's effective wall.  Returns whether
+        it is on (hierarchies with a coarse level below 64 x 64 cells, and the
+        FP64 path, cannot take it).  The FP64 certification is unchanged."""
+        en = _i(0)
+        _lib.call("fvx_qg_set_wall", self._h, int(bool(on)), C.byref(en))
+        return bool(en.value)
 
     def set_warm_start(self, mode: str = "extrapolate"):
         """Warm start of each stage's Helmholtz solve: ``"extrapolate"``

@@ -44,6 +44,7 @@ _sigs = {
     "fvx_mg_plan_create": [_vp, _vp, _i, _i, _d, _d, _d, _i, _i, _i, _i, _d, _i, _i,
                            C.POINTER(_vp)],
     "fvx_mg_plan_destroy": [_vp],
+    "fvx_mg_plan_set_wall": [_vp, _i],
     "fvx_mg_plan_info": [_vp, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i)],
     "fvx_mg_mixed_solve": [_vp, _vp, _i64, _vp, _i64, _d, _d, _i, _i, _d,
                            C.POINTER(_CgInfoC), _vp],
@@ -329,11 +330,14 @@ def prolongate(vc, level: MultigridLevel):
 def build_multigrid_solver(mask, dx: float, dy: float, lambda_: float = 0.0, coeff=None,
                            n_levels: int | None = None, n_pre: int = 6, n_post: int = 6,
                            n_coarse: int = 50, omega: float = 0.95, n_cycles: int = 5, *,
-                           kind: str = "noflux", precision: str = "float64") -> MultigridSolver:
+                           kind: str = "noflux", precision: str = "float64",
+                           wall: bool = False) -> MultigridSolver:
     """Offline hierarchy build (``multigrid.py:905-1062``).
 
     ``mask`` may be a ``Mask2D`` (its ``xy_corner_strict`` is used, as in the
     reference) or any 0/1 array.  Non-divisible grids raise ``ValueError``.
+    ``wall`` (mixed-precision Dirichlet plans whose levels are all tiled; no
+    reference counterpart): the coarse-level wall correction of the V-cycle.
     """
     if kind not in KINDS:
         raise ValueError(f"kind must be one of {list(KINDS)}, got {kind!r}")
@@ -360,8 +364,11 @@ def build_multigrid_solver(mask, dx: float, dy: float, lambda_: float = 0.0, coe
     _lib.call("fvx_mg_plan_create", m.ctypes.data, None if c is None else c.ctypes.data, ny, nx,
               float(dx), float(dy), float(lambda_), int(n_levels or 0), int(n_pre), int(n_post),
               int(n_coarse), float(omega), KINDS[kind], PRECISIONS[precision], C.byref(h))
-    return MultigridSolver(h, ny, nx, dx, dy, lambda_, n_pre, n_post, n_coarse, omega, n_cycles,
-                           kind, precision)
+    solver = MultigridSolver(h, ny, nx, dx, dy, lambda_, n_pre, n_post, n_coarse, omega, n_cycles,
+                             kind, precision)
+    if wall:
+        _lib.call("fvx_mg_plan_set_wall", h, 1)
+    return solver
 
 
 # ---------------------------------------------------------------------------

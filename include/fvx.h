@@ -217,6 +217,14 @@ int fvx_mg_plan_create(const double* mask, const double* coeff, int ny, int nx, 
                        double dy, double lambda_, int n_levels, int n_pre, int n_post,
                        int n_coarse, double omega, int kind, int precision, void** plan_out);
 int fvx_mg_plan_destroy(void* plan);
+/* Coarse-level wall correction of a Dirichlet plan's FP32 hierarchy (on = 1):
+ * on level l each dry or outside neighbour takes the ghost value g_l u with
+ * g_l = (1 - 2^l) / (1 + 2^l), which keeps the fine operator's effective wall
+ * on every level.  No counterpart in the reference (its hierarchy is the
+ * no-flux one, solvers/multigrid.py:905-1062); it changes the V-cycle
+ * preconditioner only.  Refused (-1) unless every level runs on the tiled
+ * kernels; the level operators of such a plan are solver-internal. */
+int fvx_mg_plan_set_wall(void* plan, int on);
 /* Level count, level shapes (2 ints per level) and the first level solved by
  * the single-workgroup coarse kernel. */
 int fvx_mg_plan_info(void* plan, int* n_levels, int* shapes, int* coarse_level);
@@ -350,6 +358,11 @@ int fvx_qg_plan_create(int ny, int nx, double Lx, double Ly, double beta, double
                        double rtol, double inner_rtol, int max_iters, int max_outer,
                        void** plan_out);
 int fvx_qg_plan_destroy(void* plan);
+/* Coarse-level wall correction (fvx_mg_plan_set_wall) of the inner FP32
+ * V-cycles of a mixed-precision plan: *enabled = 1 when on, 0 when the plan's
+ * hierarchy cannot take it or on = 0.  No reference counterpart; the FP64
+ * certification is unchanged.  Invalidates captured graphs. */
+int fvx_qg_set_wall(void* plan, int on, int* enabled);
 /* nsteps rk3_ssp_step steps of q (full (ny+2, nx+2), state precision) in
  * place; psi (FP64 full array) is the warm start and the last solution.    */
 int fvx_qg_step(void* plan, void* q, double* psi, int nsteps, fvx_qg_info* info, void* stream);

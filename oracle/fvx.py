@@ -793,14 +793,36 @@ class Level:
     and is this build's hierarchy for the Dirichlet QG inversion.
     """
 
-    def __init__(self, mask, cx, cy, dx, dy, lambda_, inv_diag, kind):
+    def __init__(self, mask, cx, cy, dx, dy, lambda_, inv_diag, kind, wall_g=0.0):
         self.mask, self.cx, self.cy = mask, cx, cy
         self.dx, self.dy, self.lambda_ = dx, dy, lambda_
         self.inv_diag, self.kind = inv_diag, kind
+        self.wall_g = wall_g
 
 
-def build_multigrid(mask, dx, dy, lambda_=0.0, coeff=None, n_levels=None, kind="noflux"):
-    """``build_multigrid_solver`` (``solvers/multigrid.py:905-1062``)."""
+def wall_counts(m):
+    """Numbers of dry (or outside) x and y neighbours of every cell."""
+    mp = np.pad(m, 1)
+    nlx = 2.0 - mp[1:-1, 2:] - mp[1:-1, :-2]
+    nly = 2.0 - mp[2:, 1:-1] - mp[:-2, 1:-1]
+    return nlx, nly
+
+
+def wall_g(level):
+    """Coarse-level wall correction of the Dirichlet hierarchy (no reference
+    counterpart).  The fine operator's zero ghost puts psi = 0 one fine cell
+    beyond the last wet cell centre; on level l (cells 2^l times larger) the
+    ghost value that extrapolates a linear profile to that same point is
+    g u_0 with g = (1 - 2^l) / (1 + 2^l) (0 on level 0)."""
+    r = 2.0 ** level
+    return (1.0 - r) / (1.0 + r)
+
+
+def build_multigrid(mask, dx, dy, lambda_=0.0, coeff=None, n_levels=None, kind="noflux",
+                    wall=False):
+    """``build_multigrid_solver`` (``solvers/multigrid.py:905-1062``).  ``wall``
+    (Dirichlet hierarchies only, no reference counterpart): coarse levels take
+    the wall correction of ``wall_g`` on cells beside land or the edge."""
     m = np.asarray(mask, np.float64)
     ny, nx = m.shape
     if n_levels is None:
@@ -818,9 +840,12 @@ def build_multigrid(mask, dx, dy, lambda_=0.0, coeff=None, n_levels=None, kind="
             inv = compute_inv_diagonal(cx, cy, cur_m, cdx, cdy, lambda_)
         else:
             cx = cy = None
-            d = -2.0 / cdx**2 - 2.0 / cdy**2 - lambda_
+            g = wall_g(lev) if wall else 0.0
+            nlx, nly = wall_counts(cur_m)
+            d = -2.0 / cdx**2 - 2.0 / cdy**2 - lambda_ + g * (nlx / cdx**2 + nly / cdy**2)
             inv = cur_m / d
-        levels.append(Level(cur_m, cx, cy, cdx, cdy, float(lambda_), inv, kind))
+        levels.append(Level(cur_m, cx, cy, cdx, cdy, float(lambda_), inv, kind,
+                            g if kind != "noflux" else 0.0))
         if lev < n_levels - 1:
             nm = restrict_mask(cur_m)
             cur_c = restrict_coeff(cur_c, cur_m, nm)
@@ -852,6 +877,9 @@ def apply_operator(u, lev):
               (cy * (un - um) - cys * (um - us)) / lev.dy**2
     else:
         lap = (ue + uw - 2.0 * um) / lev.dx**2 + (un + us - 2.0 * um) / lev.dy**2
+        if lev.wall_g:
+            nlx, nly = wall_counts(m)
+            lap = lap + lev.wall_g * (nlx / lev.dx**2 + nly / lev.dy**2) * um
     return (lap - lev.lambda_ * um) * m
 
 
@@ -1029,7 +1057,9 @@ def level_matrix(lev):
         add(idx[:-1, :], idx[1:, :], (cy[:-1, :] * iy2) * m[:-1, :] * m[1:, :])
         add(idx[1:, :], idx[:-1, :], (cys[1:, :] * iy2) * m[1:, :] * m[:-1, :])
     else:
-        diag = (-2.0 * ix2 - 2.0 * iy2 - lev.lambda_) * m * m
+        nlx, nly = wall_counts(m)
+        diag = (-2.0 * ix2 - 2.0 * iy2 - lev.lambda_
+                + lev.wall_g * (nlx * ix2 + nly * iy2)) * m * m
         add(idx, idx, diag)
         add(idx[:, :-1], idx[:, 1:], ix2 * m[:, :-1] * m[:, 1:])
         add(idx[:, 1:], idx[:, :-1], ix2 * m[:, 1:] * m[:, :-1])

@@ -94,6 +94,42 @@ def test_v_cycle_matches_oracle(ny, nx, kind, masked, varcoef, npre, npost):
     assert_close_scaled(to_np(got32), ref0, 1e-4, "V(0,b) f32")
 
 
+@pytest.mark.parametrize("masked", [False, True])
+@pytest.mark.parametrize("ny,nx,nl", [(512, 512, 3), (384, 640, 2)])
+def test_wall_corrected_fp32_v_cycle_matches_oracle(ny, nx, nl, masked):
+    """The coarse-level wall correction (every level tiled, wave-tile
+    kernels, ghost value g_l u beside land and the edge) against the oracle's
+    corrected hierarchy, from zero and from a nonzero initial guess; it is
+    refused where a coarse level would fall to the single-workgroup kernel,
+    and the corrected levels' operators are not exposed."""
+    need_gpu()
+    m = basin_mask(ny, nx, 31).astype(np.float64) if masked else np.ones((ny, nx))
+    dx, dy, lam = 1.0 / nx, 1.0 / ny, 40.0
+    kw = dict(n_pre=3, n_post=3, n_coarse=3, omega=0.85)
+    levels = fvx.build_multigrid(m, dx, dy, lam, n_levels=nl, kind="dirichlet", wall=True)
+    mg = S.build_multigrid_solver(m, dx, dy, lam, n_levels=nl, kind="dirichlet",
+                                  precision="float32", wall=True, **kw)
+    rng = np.random.default_rng(22)
+    b = rng.standard_normal((ny, nx)) * m
+    u0 = rng.standard_normal((ny, nx)) * m
+    for u in (None, u0):
+        ref = fvx.v_cycle(levels, np.zeros_like(b) if u is None else u, b, **kw)
+        got = mg.v_cycle(None if u is None else torch.tensor(u, dtype=torch.float32),
+                         torch.tensor(b, dtype=torch.float32))
+        assert_close_scaled(to_np(got), ref, 1e-4, f"wall V-cycle u0={u is not None}")
+    # without the correction the cycle differs (the test sees the correction)
+    plain = fvx.v_cycle(fvx.build_multigrid(m, dx, dy, lam, n_levels=nl, kind="dirichlet"),
+                        np.zeros_like(b), b, **kw)
+    corr = fvx.v_cycle(levels, np.zeros_like(b), b, **kw)
+    assert np.abs(plain - corr).max() > 1e-3 * np.abs(plain).max()
+    with pytest.raises(ValueError):
+        S.apply_operator(torch.zeros(mg.levels[1].shape, dtype=torch.float32), mg.levels[1])
+    deep = S.build_multigrid_solver(m, dx, dy, lam, kind="dirichlet", precision="float32", **kw)
+    with pytest.raises(ValueError):
+        from finitevolx_amd import _lib
+        _lib.call("fvx_mg_plan_set_wall", deep._h, 1)
+
+
 @pytest.mark.parametrize("ny,nx", [(256, 192), (200, 168), (96, 520)])
 @pytest.mark.parametrize("npre,npost", [(2, 2), (1, 3), (6, 6)])
 def test_all_wet_dirichlet_fp32_v_cycle_matches_oracle(ny, nx, npre, npost):

@@ -895,3 +895,51 @@ def test_restrict_dirichlet_pins():
 def test_restrict_dirichlet_planted_mistake_is_caught():
     # the reference's wet-count renormalisation keeps 1 where a child is dry
     assert _restrict_dirichlet_failures(fvx.restrict) == ["land"]
+
+
+# ---------------------------------------------------------------------------
+# Coarse-level wall correction of the Dirichlet hierarchy (no reference
+# counterpart)
+# ---------------------------------------------------------------------------
+
+
+def _wall_failures(g_of_level):
+    fails = []
+    if g_of_level(0) != 0.0:
+        fails.append("level 0")
+    # The fine operator's zero ghost puts psi = 0 at x = -h/2 (the ghost
+    # centre); a profile linear in x that vanishes there has a zero second
+    # difference on every coarse level once the wall-side ghost is g u_0.
+    h, n = 1.0 / 64, 16
+    for level in (1, 2, 3):
+        H = h * 2**level
+        x = (np.arange(n) + 0.5) * H
+        u = np.tile(x + 0.5 * h, (n, 1))
+        lev = fvx.Level(np.ones((n, n)), None, None, H, H, 0.0, None, "dirichlet",
+                        g_of_level(level))
+        lap = fvx.apply_operator(u, lev)
+        if not np.allclose(lap[1:-1, :-1], 0.0, atol=1e-9 * np.abs(u).max() / H**2):
+            fails.append(f"linear profile level {level}")
+    return fails
+
+
+def test_wall_correction_pins():
+    assert _wall_failures(fvx.wall_g) == []
+    # the hierarchy wires it on coarse Dirichlet levels only, diagonal included
+    m = np.ones((32, 32))
+    levs = fvx.build_multigrid(m, 1.0, 1.0, 0.5, n_levels=3, kind="dirichlet", wall=True)
+    assert [lv.wall_g for lv in levs] == [fvx.wall_g(l) for l in range(3)]
+    for lv in levs:
+        A = fvx.level_matrix(lv)
+        assert np.allclose(A.diagonal().reshape(lv.mask.shape) * lv.inv_diag, 1.0)
+        e = np.zeros(lv.mask.shape)
+        e[0, 3] = 1.0
+        assert np.allclose(fvx.apply_operator(e, lv).ravel(), A @ e.ravel())
+    plain = fvx.build_multigrid(m, 1.0, 1.0, 0.5, n_levels=3, kind="dirichlet")
+    assert all(lv.wall_g == 0.0 for lv in plain)
+
+
+def test_wall_correction_planted_mistakes_are_caught():
+    assert _wall_failures(lambda l: -fvx.wall_g(l))
+    assert _wall_failures(lambda l: (1.0 - 2.0**l) / 2.0**l)
+    assert _wall_failures(lambda l: 0.0)
