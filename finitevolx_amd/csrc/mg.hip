@@ -377,6 +377,8 @@ struct TileArgs {
   const T* w;
   T* az_out;
   T omega;
+  T omw[3];  // wave-tile kernels: the weight of sweep k (= omega unless the
+             // plan smooths with Chebyshev weights)
   double* partials;
   PcgScalars* sc;
   int tiles_x, ntiles;
@@ -1007,10 +1009,12 @@ __device__ __forceinline__ void mg_wave_tile_body(const TileArgs<float>& a, int 
   // + 2 idy2 + lam) u and 1 / diag = invd, one coefficient per term; the
   // coefficients are zero on lanes outside the level.
   const T lk = colin ? T(1) : T(0);
-  const T kb = om * invd * lk, kx = -om * invd * idx2 * lk, ky = -om * invd * idy2 * lk;
-  const T k0 = (T(1) + om * invd * (T(2) * idx2 + T(2) * idy2 + lam)) * lk;
 #pragma unroll
   for (int sw = 1; sw <= NS; ++sw) {
+    // this sweep's weight (uniform: omega, or a Chebyshev weight)
+    const T oms = a.omw[(sw - 1) % 3];
+    const T kb = oms * invd * lk, kx = -oms * invd * idx2 * lk, ky = -oms * invd * idy2 * lk;
+    const T k0 = (T(1) + oms * invd * (T(2) * idx2 + T(2) * idy2 + lam)) * lk;
     T prev = u[sw - 1];
 #pragma unroll
     for (int r = sw; r < R - sw; ++r) {
@@ -1020,11 +1024,11 @@ __device__ __forceinline__ void mg_wave_tile_body(const TileArgs<float>& a, int 
         // u + om (b - A u) / d = (1 - om) u + om (b - offdiag(u)) / d with
         // the cell's own d
         if (ZERO && sw == 1 && !kUpMode) {
-          un = (om * ivr[r]) * b[r];
+          un = (oms * ivr[r]) * b[r];
         } else {
           const T e = lane_east(c), w = lane_west(c);
-          un = (T(1) - om) * c +
-               (om * ivr[r]) * (b[r] - idx2 * (e + w) - idy2 * (u[r + 1] + prev));
+          un = (T(1) - oms) * c +
+               (oms * ivr[r]) * (b[r] - idx2 * (e + w) - idy2 * (u[r + 1] + prev));
         }
       } else if (ZERO && sw == 1 && !kUpMode) {
         un = kb * b[r];
@@ -2092,6 +2096,11 @@ struct Plan {
   int kind, prec, nl, ny, nx;
   int n_pre, n_post, n_coarse;
   double omega, lam, dx, dy;
+  // Per-sweep weights of the wave-tile sweeps: [0] pre-smoothing (down
+  // legs), [1] post-smoothing (up legs), [2] the bottom level; omega unless
+  // Chebyshev smoothing is on (plan_set_chebyshev).
+  double omw[3][3];
+  bool cheb = false;
   int lc;  // first level handled by the single-workgroup coarse kernel
   bool has64 = false, has32 = false;
   Hier<double> h64;
@@ -2301,7 +2310,7 @@ template <typename T, int KIND, int NS, int MODE, bool ZERO>
 static void launch_tile(const Plan& P, const LevelDev<T>& L, const LevelDev<T>* C, const T* u_in,
                         const T* b, T* u_out, const T* w, PcgScalars* sc, hipStream_t s,
                         T* az_out = nullptr, bool finish = true,
-                        const PcgUpdArgs<T>* upd = nullptr) {
+                        const PcgUpdArgs<T>* upd = nullptr, bool uniform = false) {
   TileArgs<T> a;
   std::memset(&a, 0, sizeof(a));
   a.L = L;
@@ -2312,6 +2321,11 @@ static void launch_tile(const Plan& P, const LevelDev<T>& L, const LevelDev<T>* 
   a.w = w;
   a.az_out = az_out;
   a.omega = (T)P.omega;
+  {
+    // (sweep weights by leg: down, up, bottom; ``uniform``: omega throughout)
+    constexpr int role = (MODE == kDown || MODE == kDownUpd) ? 0 : (MODE == kSmooth ? 2 : 1);
+    for (int k = 0; k < 3; ++k) a.omw[k] = uniform ? a.omega : (T)P.omw[role][k];
+  }
   a.partials = P.partials;
   a.sc = sc;
   if constexpr (std::is_same<T, float>::value && KIND == kDirichlet) {
@@ -2409,10 +2423,11 @@ static void finish2(const Plan& P, int ny, int nx, PcgScalars* sc, hipStream_t s
 // nu Jacobi sweeps on level L, up to 3 per kernel.  Ping-pongs between the
 // scratch arrays s1 / s2 (which must differ from u_in and final_out); the last
 // chunk writes final_out when given.  Returns the array holding the result.
+// The sweeps take the plan's bottom-level weights unless ``uniform`` (omega).
 template <typename T, int KIND>
 static T* smooth_chunks(const Plan& P, const LevelDev<T>& L, const T* u_in, bool zero,
                         const T* b, int nu, T* s1, T* s2, T* final_out, PcgScalars* sc,
-                        hipStream_t s) {
+                        hipStream_t s, bool uniform = false) {
   const T* src = u_in;
   T* dst = nullptr;
   const int K = (nu + 2) / 3;
@@ -2421,14 +2436,26 @@ static T* smooth_chunks(const Plan& P, const LevelDev<T>& L, const T* u_in, bool
     dst = (c == K - 1 && final_out) ? final_out : ((c & 1) ? s2 : s1);
     const bool z = zero && c == 0;
     if (ns == 3) {
-      if (z) launch_tile<T, KIND, 3, kSmooth, true>(P, L, nullptr, src, b, dst, nullptr, sc, s);
-      else launch_tile<T, KIND, 3, kSmooth, false>(P, L, nullptr, src, b, dst, nullptr, sc, s);
+      if (z)
+        launch_tile<T, KIND, 3, kSmooth, true>(P, L, nullptr, src, b, dst, nullptr, sc, s, nullptr,
+                                                 true, nullptr, uniform);
+      else
+        launch_tile<T, KIND, 3, kSmooth, false>(P, L, nullptr, src, b, dst, nullptr, sc, s,
+                                                  nullptr, true, nullptr, uniform);
     } else if (ns == 2) {
-      if (z) launch_tile<T, KIND, 2, kSmooth, true>(P, L, nullptr, src, b, dst, nullptr, sc, s);
-      else launch_tile<T, KIND, 2, kSmooth, false>(P, L, nullptr, src, b, dst, nullptr, sc, s);
+      if (z)
+        launch_tile<T, KIND, 2, kSmooth, true>(P, L, nullptr, src, b, dst, nullptr, sc, s, nullptr,
+                                                 true, nullptr, uniform);
+      else
+        launch_tile<T, KIND, 2, kSmooth, false>(P, L, nullptr, src, b, dst, nullptr, sc, s,
+                                                  nullptr, true, nullptr, uniform);
     } else {
-      if (z) launch_tile<T, KIND, 1, kSmooth, true>(P, L, nullptr, src, b, dst, nullptr, sc, s);
-      else launch_tile<T, KIND, 1, kSmooth, false>(P, L, nullptr, src, b, dst, nullptr, sc, s);
+      if (z)
+        launch_tile<T, KIND, 1, kSmooth, true>(P, L, nullptr, src, b, dst, nullptr, sc, s, nullptr,
+                                                 true, nullptr, uniform);
+      else
+        launch_tile<T, KIND, 1, kSmooth, false>(P, L, nullptr, src, b, dst, nullptr, sc, s,
+                                                  nullptr, true, nullptr, uniform);
     }
     src = dst;
   }
@@ -3028,7 +3055,7 @@ static int learned_two_grid_impl(Plan& P, CoarseNetFn fn, void* ctx, int sweeps,
   float* zc = cw + (size_t)C.ny * C.nx;  // level 1 holds a quarter of level 0
   // z0 = S^sweeps(0; r), res = (r - A z0) m
   if (sweeps > 0) {
-    smooth_chunks<float, KIND>(P, L, nullptr, true, r, sweeps, L.t, L.t2, z0, nullptr, s);
+    smooth_chunks<float, KIND>(P, L, nullptr, true, r, sweeps, L.t, L.t2, z0, nullptr, s, true);
     hipLaunchKernelGGL((residual_kernel<float, KIND>), grid2(L.ny, L.nx), dim3(64, 4), 0, s, L,
                        z0, r, res);
   } else {
@@ -3053,7 +3080,7 @@ static int learned_two_grid_impl(Plan& P, CoarseNetFn fn, void* ctx, int sweeps,
                      guard);
   // post-smoothing from z0 into z
   if (sweeps > 0)
-    smooth_chunks<float, KIND>(P, L, z0, false, r, sweeps, L.t, L.t2, z, nullptr, s);
+    smooth_chunks<float, KIND>(P, L, z0, false, r, sweeps, L.t, L.t2, z, nullptr, s, true);
   else
     hipLaunchKernelGGL(mask_copy_kernel<float>, grid2(L.ny, L.nx), dim3(64, 4), 0, s, L, z0, z,
                        (int64_t)L.nx);
@@ -3196,6 +3223,49 @@ int plan_set_wall(Plan& P, bool on) {
   }
   return kOk;
 }
+// Chebyshev weights of an m-sweep polynomial smoother on [lo, 2] (the
+// spectrum of D^-1 A that the smoother has to damp): the reciprocals of the
+// roots of the degree-m Chebyshev polynomial on that interval, ascending.
+static void chebyshev_weights(double lo, int m, double* w) {
+  for (int k = 0; k < m; ++k) {
+    const double root =
+        0.5 * (2.0 + lo) + 0.5 * (2.0 - lo) * std::cos(M_PI * (2 * k + 1) / (2.0 * m));
+    w[k] = 1.0 / root;
+  }
+}
+
+int plan_set_chebyshev(Plan& P, double lo) {
+  if (lo <= 0.0) {
+    for (int r = 0; r < 3; ++r)
+      for (int k = 0; k < 3; ++k) P.omw[r][k] = P.omega;
+    P.cheb = false;
+    return kOk;
+  }
+  if (!P.has32 || P.kind != kDirichlet || lo >= 2.0) {
+    set_error("Chebyshev smoothing needs an FP32 Dirichlet hierarchy and 0 < lo < 2");
+    return kInvalidArgument;
+  }
+  const Hier<float>& H = P.h32;
+  const LevelDev<float>& B = H.lv[P.nl - 1];
+  const bool tiled = P.nl > 1 && P.lc == P.nl - 1 && (int64_t)B.ny * B.nx > kCoarseCells;
+  if (!(tiled && P.n_pre >= 1 && P.n_pre <= 3 && P.n_post >= 1 && P.n_post <= 3 &&
+        P.n_coarse >= 1 && P.n_coarse <= 3)) {
+    set_error("Chebyshev smoothing needs every level on the tiled kernels (bottom level over "
+              "64x64 cells) and 1-3 sweeps before, after and at the bottom");
+    return kInvalidArgument;
+  }
+  // pre-smoothing in ascending weight order, post-smoothing descending (the
+  // same polynomial: the V-cycle stays symmetric), the bottom ascending
+  double w[3];
+  chebyshev_weights(lo, P.n_pre, w);
+  for (int k = 0; k < 3; ++k) P.omw[0][k] = k < P.n_pre ? w[k] : P.omega;
+  chebyshev_weights(lo, P.n_post, w);
+  for (int k = 0; k < 3; ++k) P.omw[1][k] = k < P.n_post ? w[P.n_post - 1 - k] : P.omega;
+  chebyshev_weights(lo, P.n_coarse, w);
+  for (int k = 0; k < 3; ++k) P.omw[2][k] = k < P.n_coarse ? w[k] : P.omega;
+  P.cheb = true;
+  return kOk;
+}
 int plan_levels(const Plan& P) { return P.nl; }
 bool plan_has_mixed(const Plan& P) { return P.has32 && P.has64; }
 int plan_nx(const Plan& P) { return P.has64 ? P.h64.lv[0].nx : P.h32.lv[0].nx; }
@@ -3296,6 +3366,8 @@ int fvx_mg_plan_create(const double* mask, const double* coeff, int ny, int nx, 
   P->n_post = n_post;
   P->n_coarse = n_coarse;
   P->omega = omega;
+  for (int r = 0; r < 3; ++r)
+    for (int k = 0; k < 3; ++k) P->omw[r][k] = omega;
   P->lam = lambda_;
   P->dx = dx;
   P->dy = dy;
@@ -3364,6 +3436,15 @@ int fvx_mg_plan_destroy(void* plan) {
   return kOk;
 }
 
+// Chebyshev weights for the V-cycle's damped-Jacobi sweeps of a Dirichlet
+// plan's FP32 hierarchy (smoothing interval [lo, 2] of D^-1 A); lo <= 0
+// restores the uniform weight omega.
+int fvx_mg_plan_set_chebyshev(void* plan, double lo) {
+  using namespace fvx;
+  FVX_CHECK_ARG(plan, "null plan");
+  return plan_set_chebyshev(*(Plan*)plan, lo);
+}
+
 // Coarse-level wall correction of a mixed-precision Dirichlet plan's FP32
 // hierarchy (on = 1) or off (0); see plan_set_wall.
 int fvx_mg_plan_set_wall(void* plan, int on) {
@@ -3422,6 +3503,7 @@ int fvx_mg_plan_info(void* plan, int* n_levels, int* shapes, int* coarse_level) 
     const auto& L = hier<T>(*P).lv[level];                                                       \
     FVX_CHECK_ARG(L.wall_g == T(0), "level %d carries the wall correction (solver-internal)",    \
                   level);                                                                        \
+    FVX_CHECK_ARG(!P->cheb, "the plan smooths with Chebyshev weights (solver-internal)");        \
     hipStream_t s = (hipStream_t)stream;                                                         \
     if (n_iters == 0) {                                                                          \
       hipLaunchKernelGGL(mask_copy_kernel<T>, grid2(L.ny, L.nx), dim3(64, 4), 0, s, L, u, out,   \
@@ -3816,6 +3898,7 @@ static int batch_build_slot(MgBatch& B, int s) {
       a.L = L;
       a.C = H.lv[l + 1];
       a.omega = (float)Pm.omega;
+      for (int k = 0; k < 3; ++k) a.omw[k] = (float)Pm.omega;  // batches: uniform weights
       a.partials = Pm.partials;
       a.sc = sc;
       a.tiles_x = cdiv(L.nx, kTx);
@@ -3889,6 +3972,7 @@ static int batch_build_slot(MgBatch& B, int s) {
         a.L = L;
         a.C = H.lv[1];
         a.omega = (float)Pm.omega;
+        for (int k = 0; k < 3; ++k) a.omw[k] = (float)Pm.omega;
         a.partials = Pm.partials;
         a.sc = &Pm.sc[2 * s];
         // down leg after the step that takes (rc, qc) to (rn, qn)

@@ -92,6 +92,10 @@ int plan_levels(const Plan& P);
 // dry or outside neighbours take the ghost value g_l u, g_l = (1 - 2^l) /
 // (1 + 2^l)); refused unless every level runs on the wave-tile kernels.
 int plan_set_wall(Plan& P, bool on);
+// Chebyshev weights for the wave-tile V-cycle's sweeps (smoothing interval
+// [lo, 2] of D^-1 A; lo <= 0: the uniform omega); refused unless every level
+// runs on the tiled kernels with 1-3 sweeps per leg and at the bottom.
+int plan_set_chebyshev(Plan& P, double lo);
 bool plan_has_mixed(const Plan& P);
 // Learned two-grid preconditioner on the FP32 hierarchy, z = M(r):
 //   z0 = S^sweeps(0; r), res = r - A z0          (damped-Jacobi smoother S)

@@ -41,6 +41,7 @@ _lib.extra_signatures({
     "fvx_qg_budgets": [_vp, C.POINTER(C.c_int)],
     "fvx_qg_set_warm_start": [_vp, _i],
     "fvx_qg_set_wall": [_vp, _i, C.POINTER(_i)],
+    "fvx_qg_set_chebyshev": [_vp, _d, C.POINTER(_i)],
     "fvx_qg_stage_f64": [_vp, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _d, _d, _d, _d, _d, _vp],
     "fvx_qg_stage_f32": [_vp, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _d, _d, _d, _d, _d, _vp],
     "fvx_qg_stage_f64t32": [_vp, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _d, _d, _d, _d, _d,
@@ -135,6 +136,20 @@ class QGModel:
         FP64 path, cannot take it).  The FP64 certification is unchanged."""
         en = _i(0)
         _lib.call("fvx_qg_set_wall", self._h, int(bool(on)), C.byref(en))
+        return bool(en.value)
+
+    def set_chebyshev(self, lo: float | None = 0.1) -> bool:
+        """Chebyshev weights for the inner FP32 V-cycles' damped-Jacobi
+        sweeps (mixed precision; no reference counterpart): sweep k of an
+        m-sweep leg takes the reciprocal of the k-th root of the degree-m
+        Chebyshev polynomial on [lo, 2].  ``None`` restores the uniform
+        weight.  Returns whether it is on (hierarchies with a coarse level
+        below 64 x 64 cells, more than 3 sweeps per leg and the FP64 path
+        cannot take it).  The FP64 certification is unchanged.  A mixed
+        plan starts with it on where it can (lo = 0.25 with a land mask,
+        0.4 without); batched ensemble solves use the uniform weight."""
+        en = _i(0)
+        _lib.call("fvx_qg_set_chebyshev", self._h, 0.0 if lo is None else float(lo), C.byref(en))
         return bool(en.value)
 
     def set_warm_start(self, mode: str = "extrapolate"):

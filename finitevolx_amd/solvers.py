@@ -45,6 +45,7 @@ _sigs = {
                            C.POINTER(_vp)],
     "fvx_mg_plan_destroy": [_vp],
     "fvx_mg_plan_set_wall": [_vp, _i],
+    "fvx_mg_plan_set_chebyshev": [_vp, _d],
     "fvx_mg_plan_info": [_vp, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i)],
     "fvx_mg_mixed_solve": [_vp, _vp, _i64, _vp, _i64, _d, _d, _i, _i, _d,
                            C.POINTER(_CgInfoC), _vp],
@@ -331,13 +332,16 @@ def build_multigrid_solver(mask, dx: float, dy: float, lambda_: float = 0.0, coe
                            n_levels: int | None = None, n_pre: int = 6, n_post: int = 6,
                            n_coarse: int = 50, omega: float = 0.95, n_cycles: int = 5, *,
                            kind: str = "noflux", precision: str = "float64",
-                           wall: bool = False) -> MultigridSolver:
+                           wall: bool = False, chebyshev: float | None = None) -> MultigridSolver:
     """Offline hierarchy build (``multigrid.py:905-1062``).
 
     ``mask`` may be a ``Mask2D`` (its ``xy_corner_strict`` is used, as in the
     reference) or any 0/1 array.  Non-divisible grids raise ``ValueError``.
     ``wall`` (mixed-precision Dirichlet plans whose levels are all tiled; no
     reference counterpart): the coarse-level wall correction of the V-cycle.
+    ``chebyshev`` (same plans; no reference counterpart): the lower end ``lo``
+    of the smoothing interval [lo, 2] whose Chebyshev weights replace the
+    uniform ``omega`` in the V-cycle's sweeps.
     """
     if kind not in KINDS:
         raise ValueError(f"kind must be one of {list(KINDS)}, got {kind!r}")
@@ -368,6 +372,8 @@ def build_multigrid_solver(mask, dx: float, dy: float, lambda_: float = 0.0, coe
                              kind, precision)
     if wall:
         _lib.call("fvx_mg_plan_set_wall", h, 1)
+    if chebyshev is not None:
+        _lib.call("fvx_mg_plan_set_chebyshev", h, float(chebyshev))
     return solver
 
 

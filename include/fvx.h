@@ -225,6 +225,15 @@ int fvx_mg_plan_destroy(void* plan);
  * preconditioner only.  Refused (-1) unless every level runs on the tiled
  * kernels; the level operators of such a plan are solver-internal. */
 int fvx_mg_plan_set_wall(void* plan, int on);
+/* Chebyshev weights for the V-cycle's damped-Jacobi sweeps of a Dirichlet
+ * plan's FP32 hierarchy: sweep k of an m-sweep leg takes 1 / root_k of the
+ * degree-m Chebyshev polynomial on [lo, 2] (the part of D^-1 A's spectrum
+ * the smoother damps); lo <= 0 restores the uniform omega.  No reference
+ * counterpart (multigrid.py:408-461 uses one weight); changes the V-cycle
+ * preconditioner only.  Refused (-1) unless every level runs on the tiled
+ * kernels with 1-3 sweeps per leg and at the bottom; the plan's level
+ * smoothing entry point is then solver-internal. */
+int fvx_mg_plan_set_chebyshev(void* plan, double lo);
 /* Level count, level shapes (2 ints per level) and the first level solved by
  * the single-workgroup coarse kernel. */
 int fvx_mg_plan_info(void* plan, int* n_levels, int* shapes, int* coarse_level);
@@ -363,6 +372,12 @@ int fvx_qg_plan_destroy(void* plan);
  * hierarchy cannot take it or on = 0.  No reference counterpart; the FP64
  * certification is unchanged.  Invalidates captured graphs. */
 int fvx_qg_set_wall(void* plan, int on, int* enabled);
+/* Chebyshev smoothing (fvx_mg_plan_set_chebyshev) of the inner FP32 V-cycles
+ * of a mixed-precision plan (lo <= 0: uniform weight): *enabled = 1 when
+ * on.  A mixed plan starts with it on where the hierarchy takes it (lo =
+ * 0.25 with a land mask, 0.4 without).  No reference counterpart; the FP64
+ * certification is unchanged.  Invalidates captured graphs. */
+int fvx_qg_set_chebyshev(void* plan, double lo, int* enabled);
 /* nsteps rk3_ssp_step steps of q (full (ny+2, nx+2), state precision) in
  * place; psi (FP64 full array) is the warm start and the last solution.    */
 int fvx_qg_step(void* plan, void* q, double* psi, int nsteps, fvx_qg_info* info, void* stream);

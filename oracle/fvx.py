@@ -884,10 +884,21 @@ def apply_operator(u, lev):
 
 
 def weighted_jacobi(u, rhs, lev, omega, n):
-    """``solvers/multigrid.py:408-461``."""
-    for _ in range(n):
-        u = (u + omega * lev.inv_diag * (rhs - apply_operator(u, lev))) * lev.mask
+    """``solvers/multigrid.py:408-461``; ``omega`` may also be a sequence of
+    per-sweep weights (sweep k takes omega[k mod len])."""
+    ws = list(omega) if np.ndim(omega) else [omega]
+    for k in range(n):
+        u = (u + ws[k % len(ws)] * lev.inv_diag * (rhs - apply_operator(u, lev))) * lev.mask
     return u
+
+
+def chebyshev_weights(lo, m, hi=2.0):
+    """Per-sweep weights of an m-sweep polynomial (Chebyshev) smoother on
+    [lo, hi] of D^-1 A (no reference counterpart): the reciprocals of the
+    roots of the degree-m Chebyshev polynomial of that interval, ascending."""
+    k = np.arange(m)
+    roots = 0.5 * (hi + lo) + 0.5 * (hi - lo) * np.cos(np.pi * (2 * k + 1) / (2 * m))
+    return list(1.0 / roots)
 
 
 def restrict(v, mf, mc_):
@@ -945,22 +956,27 @@ def prolongate_dirichlet(vc, mc_, mf):
     return out * mf
 
 
-def v_cycle(levels, u, rhs, idx=0, n_pre=6, n_post=6, n_coarse=50, omega=0.95):
+def v_cycle(levels, u, rhs, idx=0, n_pre=6, n_post=6, n_coarse=50, omega=0.95, cheb=None):
     """``MultigridSolver.v_cycle`` (``solvers/multigrid.py:808-883``); the
     Dirichlet hierarchy transfers with ``restrict_dirichlet`` and
-    ``prolongate_dirichlet``."""
+    ``prolongate_dirichlet``.  ``cheb`` (no reference counterpart): the lower
+    end of the Chebyshev smoothing interval [cheb, 2]; pre-smoothing and the
+    bottom take the weights ascending, post-smoothing descending."""
     lev = levels[idx]
+    w_pre = omega if cheb is None else chebyshev_weights(cheb, n_pre)
+    w_post = omega if cheb is None else chebyshev_weights(cheb, n_post)[::-1]
+    w_bot = omega if cheb is None else chebyshev_weights(cheb, n_coarse)
     if idx == len(levels) - 1:
-        return weighted_jacobi(u, rhs, lev, omega, n_coarse)
-    u = weighted_jacobi(u, rhs, lev, omega, n_pre)
+        return weighted_jacobi(u, rhs, lev, w_bot, n_coarse)
+    u = weighted_jacobi(u, rhs, lev, w_pre, n_pre)
     r = (rhs - apply_operator(u, lev)) * lev.mask
     cl = levels[idx + 1]
     Rs = restrict_dirichlet if lev.kind == "dirichlet" else restrict
     rc = Rs(r, lev.mask, cl.mask)
-    ec = v_cycle(levels, np.zeros_like(rc), rc, idx + 1, n_pre, n_post, n_coarse, omega)
+    ec = v_cycle(levels, np.zeros_like(rc), rc, idx + 1, n_pre, n_post, n_coarse, omega, cheb)
     P = prolongate_dirichlet if lev.kind == "dirichlet" else prolongate
     u = (u + P(ec, cl.mask, lev.mask)) * lev.mask
-    return weighted_jacobi(u, rhs, lev, omega, n_post)
+    return weighted_jacobi(u, rhs, lev, w_post, n_post)
 
 
 def mg_solve(levels, rhs, n_cycles=5, **kw):

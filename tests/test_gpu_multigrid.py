@@ -130,6 +130,46 @@ def test_wall_corrected_fp32_v_cycle_matches_oracle(ny, nx, nl, masked):
         _lib.call("fvx_mg_plan_set_wall", deep._h, 1)
 
 
+@pytest.mark.parametrize("masked", [False, True])
+@pytest.mark.parametrize("ny,nx,nl,sweeps", [(512, 512, 3, (3, 3, 3)), (384, 640, 2, (2, 1, 3))])
+def test_chebyshev_fp32_v_cycle_matches_oracle(ny, nx, nl, sweeps, masked):
+    """Chebyshev weights on [0.1, 2] for the sweeps of the wave-tile V-cycle
+    (pre-smoothing and bottom ascending, post-smoothing descending) against
+    the oracle's V-cycle with the same weights, from zero and from a nonzero
+    initial guess; lo = 0 restores the uniform weight; plans that cannot take
+    it (FP64, more than 3 sweeps a leg) refuse it."""
+    need_gpu()
+    from finitevolx_amd import _lib
+
+    m = basin_mask(ny, nx, 31).astype(np.float64) if masked else np.ones((ny, nx))
+    dx, dy, lam = 1.0 / nx, 1.0 / ny, 40.0
+    kw = dict(n_pre=sweeps[0], n_post=sweeps[1], n_coarse=sweeps[2], omega=0.85)
+    levels = fvx.build_multigrid(m, dx, dy, lam, n_levels=nl, kind="dirichlet")
+    mg = S.build_multigrid_solver(m, dx, dy, lam, n_levels=nl, kind="dirichlet",
+                                  precision="float32", chebyshev=0.1, **kw)
+    rng = np.random.default_rng(23)
+    b = rng.standard_normal((ny, nx)) * m
+    u0 = rng.standard_normal((ny, nx)) * m
+    for u in (None, u0):
+        ref = fvx.v_cycle(levels, np.zeros_like(b) if u is None else u, b, cheb=0.1, **kw)
+        got = mg.v_cycle(None if u is None else torch.tensor(u, dtype=torch.float32),
+                         torch.tensor(b, dtype=torch.float32))
+        assert_close_scaled(to_np(got), ref, 1e-4, f"Chebyshev V-cycle u0={u is not None}")
+    plain = fvx.v_cycle(levels, np.zeros_like(b), b, **kw)
+    assert np.abs(plain - fvx.v_cycle(levels, np.zeros_like(b), b, cheb=0.1, **kw)).max() > \
+        1e-3 * np.abs(plain).max()
+    _lib.call("fvx_mg_plan_set_chebyshev", mg._h, 0.0)
+    assert_close_scaled(to_np(mg.v_cycle(None, torch.tensor(b, dtype=torch.float32))), plain,
+                        1e-4, "uniform weights restored")
+    with pytest.raises(ValueError):
+        S.build_multigrid_solver(m, dx, dy, lam, n_levels=nl, kind="dirichlet",
+                                 precision="float64", chebyshev=0.1, **kw)
+    with pytest.raises(ValueError):
+        S.build_multigrid_solver(m, dx, dy, lam, n_levels=nl, kind="dirichlet",
+                                 precision="float32", chebyshev=0.1, n_pre=6, n_post=6,
+                                 n_coarse=3)
+
+
 @pytest.mark.parametrize("ny,nx", [(256, 192), (200, 168), (96, 520)])
 @pytest.mark.parametrize("npre,npost", [(2, 2), (1, 3), (6, 6)])
 def test_all_wet_dirichlet_fp32_v_cycle_matches_oracle(ny, nx, npre, npost):

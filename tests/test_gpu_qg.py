@@ -194,6 +194,29 @@ def test_graph_retry_after_a_jump_in_q(precision):
     assert err <= (1e-10 if precision == "float64" else 1e-6), err
 
 
+@pytest.mark.parametrize("masked", [False, True])
+def test_chebyshev_smoothing_tracks_fp64(masked):
+    """Mixed-precision steps whose inner V-cycles take Chebyshev weights
+    (host-driven and replayed graphs) certify every solve and track the FP64
+    steps; the FP64 plan reports it off."""
+    need_gpu()
+    ny, nx = 256, 192
+    p = _params(nx, ny)
+    wet = _basin_mask(ny, nx, 7) if masked else None
+    q0 = inputs.qg_initial_pv(ny, nx, seed=4)
+    m64 = QGModel(nx, ny, p.Lx, p.Ly, precision="float64", mask=wet)
+    assert not m64.set_chebyshev(0.1)
+    q64, _, _ = m64.step(m64.new_state(q0), nsteps=6, graph=False)
+    for graph in (False, True):
+        mm = QGModel(nx, ny, p.Lx, p.Ly, precision="mixed", mask=wet)
+        assert mm.set_chebyshev(0.1)
+        qm, _, im = mm.step(mm.new_state(q0), nsteps=6, graph=graph)
+        print(f"[chebyshev] masked={masked} graph={graph}: {im}", flush=True)
+        assert im["converged"] and im["solves"] == 18 and im["max_rel_residual"] <= 1e-8, im
+        err = np.max(np.abs(to_np(qm) - to_np(q64))) / np.max(np.abs(to_np(q64)))
+        assert err <= 1e-6, err
+
+
 def test_config2_basin_mixed_within_pv_gate():
     """The 1024^2 closed-basin configuration: f0 = 9.375e-5, beta = 1.754e-11,
     Ld = 4e4 m, nu4 = 1e9, dt = 1200 s, k^-3 PV (rms 1e-5, seed 2) with the

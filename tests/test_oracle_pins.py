@@ -943,3 +943,75 @@ def test_wall_correction_planted_mistakes_are_caught():
     assert _wall_failures(lambda l: -fvx.wall_g(l))
     assert _wall_failures(lambda l: (1.0 - 2.0**l) / 2.0**l)
     assert _wall_failures(lambda l: 0.0)
+
+
+# ---------------------------------------------------------------------------
+# Chebyshev smoothing weights (no reference counterpart)
+# ---------------------------------------------------------------------------
+
+
+def _chebyshev_failures(weights):
+    """The error of m damped-Jacobi sweeps with weights w_k is
+    prod_k (1 - w_k D^-1 A) e: for the Chebyshev weights on [lo, 2] that
+    polynomial is T_m((2 + lo - 2x) / (2 - lo)) / T_m((2 + lo) / (2 - lo))
+    (numpy's Chebyshev basis) at every x of the interval."""
+    fails = []
+    for lo in (0.05, 0.1, 0.3):
+        for m in (1, 2, 3, 5):
+            w = np.asarray(weights(lo, m))
+            x = np.linspace(lo, 2.0, 41)
+            p = np.prod(1.0 - np.outer(x, w), axis=1) if len(w) else np.ones_like(x)
+            T = np.polynomial.chebyshev.Chebyshev.basis(m)
+            ref = T((2.0 + lo - 2.0 * x) / (2.0 - lo)) / T((2.0 + lo) / (2.0 - lo))
+            if len(w) != m or not np.allclose(p, ref, rtol=0.0, atol=1e-12):
+                fails.append((lo, m))
+    return fails
+
+
+def _sweep_sequence_failures(jacobi):
+    """weighted_jacobi with a weight sequence multiplies the error by
+    prod_k (I - w_k D^-1 A) (the level's matrix, land cells held at zero)."""
+    rng = np.random.default_rng(5)
+    m = (rng.random((7, 6)) > 0.2).astype(np.float64)
+    lev = fvx.build_multigrid(m, 0.7, 0.9, 1.5, n_levels=1, kind="dirichlet")[0]
+    A = fvx.level_matrix(lev).toarray()
+    wet = m.ravel() > 0
+    Aw = A[np.ix_(wet, wet)]
+    J = np.diag(1.0 / np.diag(Aw)) @ Aw
+    x_true = rng.standard_normal(m.shape) * m
+    b = fvx.apply_operator(x_true, lev)
+    u0 = rng.standard_normal(m.shape) * m
+    w = fvx.chebyshev_weights(0.1, 3)
+    E = np.eye(J.shape[0])
+    for wk in w:
+        E = (np.eye(J.shape[0]) - wk * J) @ E
+    e = (E @ (u0 - x_true).ravel()[wet])
+    got = (jacobi(u0, b, lev, w, 3) - x_true).ravel()[wet]
+    return [] if np.allclose(got, e, atol=1e-12 * np.abs(u0).max()) else ["sequence"]
+
+
+def test_chebyshev_weight_pins():
+    assert _chebyshev_failures(fvx.chebyshev_weights) == []
+    assert fvx.chebyshev_weights(0.1, 1) == pytest.approx([2.0 / 2.1])
+    assert _sweep_sequence_failures(fvx.weighted_jacobi) == []
+    # a V-cycle with lo given takes those weights; without, the uniform omega
+    m = np.ones((32, 32))
+    levs = fvx.build_multigrid(m, 1.0, 1.0, 0.5, n_levels=2, kind="dirichlet")
+    b = np.random.default_rng(6).standard_normal((32, 32))
+    kw = dict(n_pre=2, n_post=2, n_coarse=3, omega=0.85)
+    assert not np.allclose(fvx.v_cycle(levs, 0 * b, b, cheb=0.1, **kw),
+                           fvx.v_cycle(levs, 0 * b, b, **kw))
+
+
+def test_chebyshev_planted_mistakes_are_caught():
+    cw = fvx.chebyshev_weights
+    assert _chebyshev_failures(lambda lo, m: cw(lo, m, hi=2.2))
+    assert _chebyshev_failures(lambda lo, m: cw(0.0, m))
+    assert _chebyshev_failures(
+        lambda lo, m: list(1.0 / (0.5 * (2 + lo) + 0.5 * (2 - lo) * np.cos(np.pi * np.arange(m) / m))))
+    assert _chebyshev_failures(lambda lo, m: cw(lo, m)[:-1])
+
+    def first_weight_only(u, rhs, lev, omega, n):
+        return fvx.weighted_jacobi(u, rhs, lev, omega[0], n)
+
+    assert _sweep_sequence_failures(first_weight_only)
