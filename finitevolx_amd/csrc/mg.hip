@@ -1151,6 +1151,9 @@ __global__ __launch_bounds__(256) void mg_wave_tile_kernel(TileArgs<float> a) {
 // use) and writes the result to levels[lc].u.
 // ---------------------------------------------------------------------------
 constexpr int kCoarseThreads = 1024;
+// Per-sweep weights the coarse kernel carries for each leg (sweeps past the
+// last take the last weight).
+constexpr int kMaxW = 8;
 
 template <typename T>
 struct CoarseArgs {
@@ -1158,15 +1161,17 @@ struct CoarseArgs {
   int lc, nl, n_pre, n_post, n_coarse;
   int nonzero_init;  // level lc starts from lv[lc].u instead of zero
   T omega;
+  T w[3][kMaxW];  // sweep weights: pre-smoothing, post-smoothing, bottom
   const int* done;
 };
 
 template <typename T, int KIND>
-__device__ void cw_sweeps(const LevelDev<T>& L, int n, bool zero, T om) {
+__device__ void cw_sweeps(const LevelDev<T>& L, int n, bool zero, const T* w) {
   const int N = L.ny * L.nx;
   T* src = L.u;
   T* dst = L.t;
   for (int s = 0; s < n; ++s) {
+    const T om = w[s < kMaxW ? s : kMaxW - 1];
     for (int k = threadIdx.x; k < N; k += kCoarseThreads) {
       const int j = k / L.nx, i = k % L.nx;
       const T m = mval<T>(L.mask, k);
@@ -1216,7 +1221,6 @@ __global__ __launch_bounds__(kCoarseThreads) void mg_coarse_kernel(CoarseArgs<T>
       o += 3 * a.lv[l].ny * a.lv[l].nx;
     }
   }
-  const T om = a.omega;
   // Level lc's right-hand side (and guess) from global memory.
   if constexpr (LDS) {
     const LevelDev<T>& G = a.lv[a.lc];
@@ -1234,7 +1238,7 @@ __global__ __launch_bounds__(kCoarseThreads) void mg_coarse_kernel(CoarseArgs<T>
     const LevelDev<T> C = lds_level<T, LDS>(a, l + 1, sm, off[l + 1]);
     const bool zero = !(a.nonzero_init && l == a.lc);
     if (a.n_pre > 0) {
-      cw_sweeps<T, KIND>(L, a.n_pre, zero, om);
+      cw_sweeps<T, KIND>(L, a.n_pre, zero, a.w[0]);
     } else if (zero) {
       for (int k = threadIdx.x; k < L.ny * L.nx; k += kCoarseThreads) L.u[k] = T(0);
       __syncthreads();
@@ -1263,7 +1267,7 @@ __global__ __launch_bounds__(kCoarseThreads) void mg_coarse_kernel(CoarseArgs<T>
     const LevelDev<T> B = lds_level<T, LDS>(a, a.nl - 1, sm, off[a.nl - 1]);
     const bool zero = !(a.nonzero_init && a.lc == a.nl - 1);
     if (a.n_coarse > 0) {
-      cw_sweeps<T, KIND>(B, a.n_coarse, zero, om);
+      cw_sweeps<T, KIND>(B, a.n_coarse, zero, a.w[2]);
     } else if (zero) {
       for (int k = threadIdx.x; k < B.ny * B.nx; k += kCoarseThreads) B.u[k] = T(0);
       __syncthreads();
@@ -1279,7 +1283,7 @@ __global__ __launch_bounds__(kCoarseThreads) void mg_coarse_kernel(CoarseArgs<T>
       L.u[k] = (L.u[k] + prolong_at<T, KIND>(C, j, i) * m) * m;
     }
     __syncthreads();
-    cw_sweeps<T, KIND>(L, a.n_post, false, om);
+    cw_sweeps<T, KIND>(L, a.n_post, false, a.w[1]);
   }
   // Level lc's corrected iterate back to global memory.
   if constexpr (LDS) {
@@ -1324,10 +1328,11 @@ struct CoarseLds {
   }
   // n damped-Jacobi sweeps of level L (u at ou, scratch at ot, b at ob); the
   // result ends in ou.
-  __device__ void sweeps(const LevelDev<T>& L, int ou, int ot, int ob, int n, bool zero, T om,
-                         int tx, int ty) const {
+  __device__ void sweeps(const LevelDev<T>& L, int ou, int ot, int ob, int n, bool zero,
+                         const T* w, int tx, int ty) const {
     int src = ou, dst = ot;
     for (int sw = 0; sw < n; ++sw) {
+      const T om = w[sw < kMaxW ? sw : kMaxW - 1];
       for (int j = ty; j < L.ny; j += 32)
         for (int i = tx; i < L.nx; i += 32) {
           const int k = j * L.nx + i;
@@ -1372,7 +1377,6 @@ __device__ __forceinline__ void mg_coarse_lds_body(const CoarseArgs<T>& a,
     }
   }
   __syncthreads();
-  const T om = a.omega;
   // Level lc's right-hand side (and masked guess) from global memory.
   {
     const LevelDev<T>& G = a.lv[a.lc];
@@ -1391,7 +1395,7 @@ __device__ __forceinline__ void mg_coarse_lds_body(const CoarseArgs<T>& a,
     const LevelDev<T>& C = a.lv[l + 1];
     const bool zero = !(a.nonzero_init && l == a.lc);
     if (a.n_pre > 0) {
-      S.sweeps(L, ou[l], ot[l], ob[l], a.n_pre, zero, om, tx, ty);
+      S.sweeps(L, ou[l], ot[l], ob[l], a.n_pre, zero, a.w[0], tx, ty);
     } else if (zero) {
       for (int j = ty; j < L.ny; j += 32)
         for (int i = tx; i < L.nx; i += 32) sm[ou[l] + j * L.nx + i] = T(0);
@@ -1425,7 +1429,7 @@ __device__ __forceinline__ void mg_coarse_lds_body(const CoarseArgs<T>& a,
     const LevelDev<T>& B = a.lv[lb];
     const bool zero = !(a.nonzero_init && a.lc == lb);
     if (a.n_coarse > 0) {
-      S.sweeps(B, ou[lb], ot[lb], ob[lb], a.n_coarse, zero, om, tx, ty);
+      S.sweeps(B, ou[lb], ot[lb], ob[lb], a.n_coarse, zero, a.w[2], tx, ty);
     } else if (zero) {
       for (int j = ty; j < B.ny; j += 32)
         for (int i = tx; i < B.nx; i += 32) sm[ou[lb] + j * B.nx + i] = T(0);
@@ -1460,7 +1464,7 @@ __device__ __forceinline__ void mg_coarse_lds_body(const CoarseArgs<T>& a,
         sm[ou[l] + k] = (sm[ou[l] + k] + e * m) * m;
       }
     __syncthreads();
-    S.sweeps(L, ou[l], ot[l], ob[l], a.n_post, false, om, tx, ty);
+    S.sweeps(L, ou[l], ot[l], ob[l], a.n_post, false, a.w[1], tx, ty);
   }
   // Level lc's corrected iterate back to global memory.
   {
@@ -2099,7 +2103,7 @@ struct Plan {
   // Per-sweep weights of the wave-tile sweeps: [0] pre-smoothing (down
   // legs), [1] post-smoothing (up legs), [2] the bottom level; omega unless
   // Chebyshev smoothing is on (plan_set_chebyshev).
-  double omw[3][3];
+  double omw[3][kMaxW];
   bool cheb = false;
   int lc;  // first level handled by the single-workgroup coarse kernel
   bool has64 = false, has32 = false;
@@ -2467,6 +2471,15 @@ static T* smooth_chunks(const Plan& P, const LevelDev<T>& L, const T* u_in, bool
 // 808-883).  With ``az`` given (level 0 only), the last kernel also writes
 // az = A z and the PCG dots against rhs and w (kUpCG) when it can (two or
 // fewer post-sweeps on a tiled level 0); *fused says whether it did.
+// The coarse kernel's sweep weights: the plan's (Chebyshev) weights on the
+// FP32 hierarchy, omega on the FP64 one (as its tiled levels).
+template <typename T>
+static void coarse_weights(const Plan& P, CoarseArgs<T>& ca) {
+  for (int r = 0; r < 3; ++r)
+    for (int k = 0; k < kMaxW; ++k)
+      ca.w[r][k] = std::is_same<T, float>::value ? (T)P.omw[r][k] : (T)P.omega;
+}
+
 template <typename T, int KIND>
 static int v_cycle_impl(Plan& P, const T* u0, const T* rhs, T* out, T* az, const T* w,
                         PcgScalars* sc, hipStream_t s, bool* fused,
@@ -2501,6 +2514,7 @@ static int v_cycle_impl(Plan& P, const T* u0, const T* rhs, T* out, T* az, const
     ca.n_coarse = P.n_coarse;
     ca.nonzero_init = u0 != nullptr;
     ca.omega = (T)P.omega;
+    coarse_weights(P, ca);
     ca.done = sc ? &sc->done : nullptr;
     launch_coarse<T, KIND>(ca, s);
     FVX_HIP(hipMemcpyAsync(out, L.u, bytes, hipMemcpyDeviceToDevice, s));
@@ -2564,6 +2578,7 @@ static int v_cycle_impl(Plan& P, const T* u0, const T* rhs, T* out, T* az, const
     ca.n_post = P.n_post;
     ca.n_coarse = P.n_coarse;
     ca.omega = (T)P.omega;
+    coarse_weights(P, ca);
     ca.done = sc ? &sc->done : nullptr;
     launch_coarse<T, KIND>(ca, s);
   }
@@ -3237,32 +3252,41 @@ static void chebyshev_weights(double lo, int m, double* w) {
 int plan_set_chebyshev(Plan& P, double lo) {
   if (lo <= 0.0) {
     for (int r = 0; r < 3; ++r)
-      for (int k = 0; k < 3; ++k) P.omw[r][k] = P.omega;
+      for (int k = 0; k < kMaxW; ++k) P.omw[r][k] = P.omega;
     P.cheb = false;
     return kOk;
   }
-  if (!P.has32 || P.kind != kDirichlet || lo >= 2.0) {
-    set_error("Chebyshev smoothing needs an FP32 Dirichlet hierarchy and 0 < lo < 2");
+  if (!P.has32 || P.kind != kDirichlet || lo >= 2.0 || P.nl < 2) {
+    set_error("Chebyshev smoothing needs an FP32 Dirichlet hierarchy of 2 or more levels and "
+              "0 < lo < 2");
     return kInvalidArgument;
   }
   const Hier<float>& H = P.h32;
   const LevelDev<float>& B = H.lv[P.nl - 1];
-  const bool tiled = P.nl > 1 && P.lc == P.nl - 1 && (int64_t)B.ny * B.nx > kCoarseCells;
-  if (!(tiled && P.n_pre >= 1 && P.n_pre <= 3 && P.n_post >= 1 && P.n_post <= 3 &&
-        P.n_coarse >= 1 && P.n_coarse <= 3)) {
-    set_error("Chebyshev smoothing needs every level on the tiled kernels (bottom level over "
-              "64x64 cells) and 1-3 sweeps before, after and at the bottom");
+  // the bottom level on the tiled kernels (up to 3 sweeps in one launch) or
+  // in the coarse kernel with levels lc .. nl-1 (up to kMaxW weights)
+  const bool tiled_bottom = P.lc == P.nl - 1 && (int64_t)B.ny * B.nx > kCoarseCells;
+  bool ok = P.n_pre >= 1 && P.n_pre <= kMaxFuse && P.n_post >= 1 && P.n_post <= kMaxFuse &&
+            P.n_coarse >= 1 && P.n_coarse <= (tiled_bottom ? kMaxFuse : kMaxW);
+  for (int l = 0; l < P.lc && l < P.nl; ++l)
+    ok = ok && wave_tiled<float, kDirichlet>(H.lv[l], nullptr);
+  if (!ok) {
+    set_error("Chebyshev smoothing needs the tiled levels on the wave-tile kernels, 1-3 sweeps "
+              "before and after, and 1-3 bottom sweeps (1-8 in the coarse kernel)");
     return kInvalidArgument;
   }
-  // pre-smoothing in ascending weight order, post-smoothing descending (the
-  // same polynomial: the V-cycle stays symmetric), the bottom ascending
-  double w[3];
+  // pre-smoothing in ascending weight order, post-smoothing descending, the
+  // bottom ascending (in exact arithmetic the order does not matter: the
+  // sweeps' error factors commute)
+  double w[kMaxW];
+  for (int r = 0; r < 3; ++r)
+    for (int k = 0; k < kMaxW; ++k) P.omw[r][k] = P.omega;
   chebyshev_weights(lo, P.n_pre, w);
-  for (int k = 0; k < 3; ++k) P.omw[0][k] = k < P.n_pre ? w[k] : P.omega;
+  for (int k = 0; k < P.n_pre; ++k) P.omw[0][k] = w[k];
   chebyshev_weights(lo, P.n_post, w);
-  for (int k = 0; k < 3; ++k) P.omw[1][k] = k < P.n_post ? w[P.n_post - 1 - k] : P.omega;
+  for (int k = 0; k < P.n_post; ++k) P.omw[1][k] = w[P.n_post - 1 - k];
   chebyshev_weights(lo, P.n_coarse, w);
-  for (int k = 0; k < 3; ++k) P.omw[2][k] = k < P.n_coarse ? w[k] : P.omega;
+  for (int k = 0; k < P.n_coarse; ++k) P.omw[2][k] = w[k];
   P.cheb = true;
   return kOk;
 }
@@ -3367,7 +3391,7 @@ int fvx_mg_plan_create(const double* mask, const double* coeff, int ny, int nx, 
   P->n_coarse = n_coarse;
   P->omega = omega;
   for (int r = 0; r < 3; ++r)
-    for (int k = 0; k < 3; ++k) P->omw[r][k] = omega;
+    for (int k = 0; k < kMaxW; ++k) P->omw[r][k] = omega;
   P->lam = lambda_;
   P->dx = dx;
   P->dy = dy;
@@ -3898,7 +3922,6 @@ static int batch_build_slot(MgBatch& B, int s) {
       a.L = L;
       a.C = H.lv[l + 1];
       a.omega = (float)Pm.omega;
-      for (int k = 0; k < 3; ++k) a.omw[k] = (float)Pm.omega;  // batches: uniform weights
       a.partials = Pm.partials;
       a.sc = sc;
       a.tiles_x = cdiv(L.nx, kTx);
@@ -3907,6 +3930,7 @@ static int batch_build_slot(MgBatch& B, int s) {
       // restricted residual), iterate into L.u, restriction into C.b
       const bool wave = B.kind == kDirichlet && wave_tiled<float, kDirichlet>(L, nullptr);
       TileArgs<float> d = a;
+      for (int k = 0; k < 3; ++k) d.omw[k] = (float)Pm.omw[0][k];
       d.b = (l == 0) ? H.r : L.b;
       d.u_out = L.u;
       if (wave) {
@@ -3919,6 +3943,7 @@ static int batch_build_slot(MgBatch& B, int s) {
       // up: coarse correction from level l+1's corrected iterate (the coarse
       // kernel's output at lc, else that level's up output L.t), post-sweeps
       TileArgs<float> u = a;
+      for (int k = 0; k < 3; ++k) u.omw[k] = (float)Pm.omw[1][k];
       u.C.u = (l + 1 == Pm.lc) ? H.lv[l + 1].u : H.lv[l + 1].t;
       u.u_in = L.u;
       u.b = (l == 0) ? H.r : L.b;
@@ -3972,11 +3997,11 @@ static int batch_build_slot(MgBatch& B, int s) {
         a.L = L;
         a.C = H.lv[1];
         a.omega = (float)Pm.omega;
-        for (int k = 0; k < 3; ++k) a.omw[k] = (float)Pm.omega;
         a.partials = Pm.partials;
         a.sc = &Pm.sc[2 * s];
         // down leg after the step that takes (rc, qc) to (rn, qn)
         TileArgs<float> d = a;
+        for (int k = 0; k < 3; ++k) d.omw[k] = (float)Pm.omw[0][k];
         d.b = rc;
         d.u_out = L.u;
         d.upd = PcgUpdArgs<float>{H.z, H.p, H.x, (int64_t)L.nx, qc, qn, rc, rn};
@@ -3985,6 +4010,7 @@ static int batch_build_slot(MgBatch& B, int s) {
         // up leg with the dots on (rc, qc): z only (the next down kernel
         // forms A z itself)
         TileArgs<float> u = a;
+        for (int k = 0; k < 3; ++k) u.omw[k] = (float)Pm.omw[1][k];
         u.C.u = (1 == Pm.lc) ? H.lv[1].u : H.lv[1].t;
         u.u_in = L.u;
         u.b = rc;
@@ -4022,6 +4048,7 @@ static int batch_build_slot(MgBatch& B, int s) {
       c.n_coarse = Pm.n_coarse;
       c.nonzero_init = 0;
       c.omega = (float)Pm.omega;
+      coarse_weights(Pm, c);
       c.done = &sc->done;
       S.coarse_lds = std::max(S.coarse_lds, coarse_lds_bytes(c));
       ca.push_back(c);

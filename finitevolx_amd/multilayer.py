@@ -55,7 +55,11 @@ class MultiLayerQG(BudgetedSteps):
 
     ``q0``: initial layer PV, (nl, ny, nx) interior (layer 0 on top);
     ``c0``: initial tracer, (ny, nx) interior; ``mask``: optional (ny, nx)
-    wet mask of the basin interior (the outer ring is always wall).
+    wet mask of the basin interior (the outer ring is always wall);
+    ``chebyshev``: the lower end lo of [lo, 2] whose Chebyshev weights the
+    modal solves' V-cycle sweeps take (None: the uniform omega).  Measured
+    on 3 layers at 4096^2: 29.6 steps/s uniform, 39.8 / 39.0 / 38.1 at lo =
+    0.15 / 0.2 / 0.25 (5.60 -> 4.18 inner iterations per modal solve).
     """
 
     def __init__(self, nx: int, ny: int, Lx: float, Ly: float, H, g_prime, q0, c0=None, *,
@@ -63,7 +67,7 @@ class MultiLayerQG(BudgetedSteps):
                  F0: float = 2e-12, dt: float = 900.0, tracer_dtype=torch.float32,
                  tracer_method: str = "weno5", n_pre: int = 3, n_post: int = 3,
                  n_coarse: int = 4, omega: float = 0.85, rtol: float = 1e-8,
-                 inner_rtol: float = 1e-4):
+                 inner_rtol: float = 1e-4, chebyshev: float | None = 0.2):
         dev = _lib.require_gpu()
         self.nx, self.ny, self.Lx, self.Ly = nx, ny, Lx, Ly
         self.dx, self.dy = Lx / nx, Ly / ny
@@ -118,7 +122,7 @@ class MultiLayerQG(BudgetedSteps):
             wet, self.dx, self.dy, float(self.mode_lambda[k]),
             n_levels=hierarchy_depth(ny, nx, min(self.dx, self.dy), float(self.radii[k])),
             n_pre=n_pre, n_post=n_post, n_coarse=n_coarse, omega=omega, kind="dirichlet",
-            precision="mixed") for k in range(nl)]
+            precision="mixed", chebyshev=chebyshev) for k in range(nl)]
         self.batch = MultigridBatch(self.solvers)
         self._stage = {}
         self._init_budgets()

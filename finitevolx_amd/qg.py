@@ -143,11 +143,10 @@ class QGModel:
         sweeps (mixed precision; no reference counterpart): sweep k of an
         m-sweep leg takes the reciprocal of the k-th root of the degree-m
         Chebyshev polynomial on [lo, 2].  ``None`` restores the uniform
-        weight.  Returns whether it is on (hierarchies with a coarse level
-        below 64 x 64 cells, more than 3 sweeps per leg and the FP64 path
-        cannot take it).  The FP64 certification is unchanged.  A mixed
-        plan starts with it on where it can (lo = 0.25 with a land mask,
-        0.4 without); batched ensemble solves use the uniform weight."""
+        weight.  Returns whether it is on (more than 3 sweeps per leg and
+        the FP64 path cannot take it).  The FP64 certification is
+        unchanged.  A mixed plan starts with it on where it can (lo = 0.25
+        with a land mask, 0.4 without)."""
         en = _i(0)
         _lib.call("fvx_qg_set_chebyshev", self._h, 0.0 if lo is None else float(lo), C.byref(en))
         return bool(en.value)

@@ -131,13 +131,16 @@ def test_wall_corrected_fp32_v_cycle_matches_oracle(ny, nx, nl, masked):
 
 
 @pytest.mark.parametrize("masked", [False, True])
-@pytest.mark.parametrize("ny,nx,nl,sweeps", [(512, 512, 3, (3, 3, 3)), (384, 640, 2, (2, 1, 3))])
+@pytest.mark.parametrize("ny,nx,nl,sweeps", [(512, 512, 3, (3, 3, 3)), (384, 640, 2, (2, 1, 3)),
+                                             (256, 320, 5, (2, 3, 5))])
 def test_chebyshev_fp32_v_cycle_matches_oracle(ny, nx, nl, sweeps, masked):
-    """Chebyshev weights on [0.1, 2] for the sweeps of the wave-tile V-cycle
+    """Chebyshev weights on [0.1, 2] for the sweeps of the V-cycle
     (pre-smoothing and bottom ascending, post-smoothing descending) against
     the oracle's V-cycle with the same weights, from zero and from a nonzero
-    initial guess; lo = 0 restores the uniform weight; plans that cannot take
-    it (FP64, more than 3 sweeps a leg) refuse it."""
+    initial guess: every level on the wave-tile kernels, or (256 x 320, 5
+    levels) the last two in the single-workgroup coarse kernel with 5 bottom
+    sweeps; lo = 0 restores the uniform weight; plans that cannot take it
+    (FP64, more than 3 sweeps a leg) refuse it."""
     need_gpu()
     from finitevolx_amd import _lib
 
