@@ -52,6 +52,10 @@ struct LevelDev {
   // neighbour adds wall_g / h^2 to the diagonal (a ghost value wall_g u_c
   // instead of 0); 0 = off.
   T wall_g;
+  // The mask packed by columns (with mask): word (blk, i) = the mask of rows
+  // 32 blk .. 32 blk + 31 of column i as bits (rows beyond the level 0),
+  // blocks 0 .. ny / 32 + 1, so a 64-bit window from any row block is valid.
+  const uint32_t* mcol;
 };
 
 template <typename T>
@@ -796,12 +800,18 @@ __device__ __forceinline__ void mg_wave_tile_body(const TileArgs<float>& a, int 
   // (rows outside the level are cleared with the edge rows below).
   uint32_t mb = 0;
   if constexpr (MK) {
-    if (L.mask) {
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const int gjc = min(max(j0 - H + r, 0), L.ny - 1);
-        mb |= (L.mask[gjc * L.nx + (int)gic] ? 1u : 0u) << r;
-      }
+    static_assert(R <= 32, "one 32-bit mask word per lane");
+    if (L.mcol) {
+      // rows g .. g + 31 from two words of the column-packed mask (rows above
+      // the level, g0 < 0, take row 0's bits shifted in: they are cleared
+      // with the edge rows below, as are rows below the level)
+      const int g0 = j0 - H, g = max(g0, 0);
+      const int blk = g >> 5;
+      const uint64_t w0 = L.mcol[blk * L.nx + (int)gic];
+      const uint64_t w1 = L.mcol[(blk + 1) * L.nx + (int)gic];
+      const uint64_t win = ((w1 << 32) | w0) >> (g & 31);
+      mb = (uint32_t)(win << (g - g0));
+      if (R < 32) mb &= (1u << R) - 1u;
     } else {
       mb = (R >= 32) ? ~0u : ((1u << R) - 1u);
     }
@@ -1861,14 +1871,16 @@ __device__ __forceinline__ void refine_residual_body(
     const int ic = min(max(i, 0), nx - 1);
     // bit t: the mask of row j0 - 1 + t in this lane's column, one byte load
     // per row for both the masked x and the residual's mask
+    // (from two words of the column-packed mask; rows outside the level
+    // read as dry, and their x and b are zero anyway)
+    static_assert(R + 2 <= 32, "one 32-bit mask word per lane");
     uint32_t mb = ~0u;
-    if (L.mask) {
-      mb = 0;
-#pragma unroll
-      for (int t = 0; t < R + 2; ++t) {
-        const int jc = min(max(j0 - 1 + t, 0), ny - 1);
-        mb |= (L.mask[(int64_t)jc * nx + ic] ? 1u : 0u) << t;
-      }
+    if (L.mcol) {
+      const int g0 = j0 - 1, g = max(g0, 0);
+      const int64_t blk = g >> 5;
+      const uint64_t w0 = L.mcol[blk * nx + ic];
+      const uint64_t w1 = L.mcol[(blk + 1) * nx + ic];
+      mb = (uint32_t)((((w1 << 32) | w0) >> (g & 31)) << (g - g0));
     }
     double X[R + 2], B[R];
     float M[R];
@@ -2233,6 +2245,15 @@ static int upload_hier(Plan& P, Hier<T>& H, const std::vector<HostLevel>& hl) {
       if (rc) return rc;
       FVX_HIP(hipMemcpy(d, m.data(), N, hipMemcpyHostToDevice));
       L.mask = (const uint8_t*)d;
+      const size_t nb = (size_t)h.ny / 32 + 2;
+      std::vector<uint32_t> cb(nb * h.nx, 0u);
+      for (int j = 0; j < h.ny; ++j)
+        for (int i = 0; i < h.nx; ++i)
+          if (m[(size_t)j * h.nx + i]) cb[(size_t)(j >> 5) * h.nx + i] |= 1u << (j & 31);
+      rc = alloc(cb.size() * sizeof(uint32_t), &d);
+      if (rc) return rc;
+      FVX_HIP(hipMemcpy(d, cb.data(), cb.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+      L.mcol = (const uint32_t*)d;
     }
     if (P.kind == kNoflux) {
       std::vector<T> tmp(N);
