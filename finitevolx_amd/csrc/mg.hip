@@ -1063,6 +1063,19 @@ __device__ __forceinline__ void mg_wave_tile_body(const TileArgs<float>& a, int 
     // (gi even, gi + 1) through one lane shift.
     const LevelDev<T>& C = a.C;
     const bool even = lin && ((gi & 1) == 0);
+    // MK: bit k2 = the coarse mask of coarse row j0 / 2 + k2 in column I,
+    // from two words of the coarse level's column-packed mask
+    uint32_t cmb = ~0u;
+    if constexpr (MK) {
+      static_assert(WTR / 2 <= 32, "one 32-bit coarse mask word per lane");
+      if (C.mcol) {
+        const int cg = j0 >> 1, blk = cg >> 5;
+        const int Icl = min(max(I, 0), C.nx - 1);
+        const uint64_t w0 = C.mcol[blk * C.nx + Icl];
+        const uint64_t w1 = C.mcol[(blk + 1) * C.nx + Icl];
+        cmb = (uint32_t)(((w1 << 32) | w0) >> (cg & 31));
+      }
+    }
 #pragma unroll
     for (int k2 = 0; k2 < WTR / 2; ++k2) {
       const int r0 = H + 2 * k2;
@@ -1079,7 +1092,7 @@ __device__ __forceinline__ void mg_wave_tile_body(const TileArgs<float>& a, int 
         // restriction); the coarse level of an all-wet level is all wet
         T v = s4 * T(0.25);
         if constexpr (MK)
-          if (C.mask && !C.mask[J * C.nx + I]) v = T(0);
+          if (!((cmb >> k2) & 1u)) v = T(0);
         buf_store(buf_rsrc(C.b, C.ny * C.nx), (unsigned)I, J * C.nx, v);
       }
     }
