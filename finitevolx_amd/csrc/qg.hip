@@ -256,9 +256,8 @@ __device__ __forceinline__ void qg_stage_wave_body(
   }
   auto wet_t = [&](int t) -> bool { return MASKED ? ((mbits >> t) & 1u) != 0 : true; };
   // With CERT: psi * mask in FP64 on the interior, zero elsewhere (the
-  // solve's masked, zero-padded operand), and the mask of each row.
+  // solve's masked, zero-padded operand); the row masks stay as bits.
   double xm[R + 2];
-  double md[R + 2];
 #pragma unroll
   for (int t = 0; t < R + 2; ++t) {
     const int gj = j0 - 1 + t;
@@ -267,9 +266,7 @@ __device__ __forceinline__ void qg_stage_wave_body(
     f[t] = (colin && gj >= 0 && gj < Ny) ? (T)v : T(0);
     if constexpr (CERT) {
       const bool in = colin && gj >= 1 && gj <= Ny - 2 && gi >= 1 && gi <= Nx - 2;
-      const double m = wet_t(t) ? 1.0 : 0.0;
-      md[t] = m;
-      xm[t] = in ? (double)v * m : 0.0;
+      xm[t] = (in && wet_t(t)) ? (double)v : 0.0;
     }
   }
   // beta (y - y0) at cell centres y_j = (j - 1/2) dy for rows j0-1 .. j0+R and
@@ -316,7 +313,7 @@ __device__ __forceinline__ void qg_stage_wave_body(
       const double e = lane_e(c), w = lane_w(c);
       if (lout && ci && gj >= 1 && gj <= Ny - 2) {
         const double lap = (e + w - 2.0 * c) * cert.idx2 + (xm[y + 1] + xm[y - 1] - 2.0 * c) * cert.idy2;
-        const double m = md[y];
+        const double m = wet_t(y) ? 1.0 : 0.0;
         const double bv = (double)qc[t] * m;
         const double rv = (bv - (lap - cert.lam * c) * m) * m;
         bb += bv * bv;
