@@ -569,9 +569,9 @@ __device__ __forceinline__ void mg_tile_body(const TileArgs<T>& a, int bx) {
       return (gj >= 0 && gj < L.ny && gi >= 0 && gi < L.nx) ? T(1) : T(0);
     }
   };
-  const T om = a.omega;
   int cur = 0;
   auto sweep = [&](auto inner, int s) {
+    const T om = a.omw[(s - 1) % 3];  // this sweep's weight
     // Valid region shrinks by one cell per sweep.
     const int lo = s, hiy = Hh - s, hix = W - s;
     const int ny_ = hiy - lo, nx_ = hix - lo;
@@ -2505,13 +2505,11 @@ static T* smooth_chunks(const Plan& P, const LevelDev<T>& L, const T* u_in, bool
 // 808-883).  With ``az`` given (level 0 only), the last kernel also writes
 // az = A z and the PCG dots against rhs and w (kUpCG) when it can (two or
 // fewer post-sweeps on a tiled level 0); *fused says whether it did.
-// The coarse kernel's sweep weights: the plan's (Chebyshev) weights on the
-// FP32 hierarchy, omega on the FP64 one (as its tiled levels).
+// The coarse kernel's sweep weights (omega, or the plan's Chebyshev weights).
 template <typename T>
 static void coarse_weights(const Plan& P, CoarseArgs<T>& ca) {
   for (int r = 0; r < 3; ++r)
-    for (int k = 0; k < kMaxW; ++k)
-      ca.w[r][k] = std::is_same<T, float>::value ? (T)P.omw[r][k] : (T)P.omega;
+    for (int k = 0; k < kMaxW; ++k) ca.w[r][k] = (T)P.omw[r][k];
 }
 
 template <typename T, int KIND>
@@ -3290,23 +3288,21 @@ int plan_set_chebyshev(Plan& P, double lo) {
     P.cheb = false;
     return kOk;
   }
-  if (!P.has32 || P.kind != kDirichlet || lo >= 2.0 || P.nl < 2) {
-    set_error("Chebyshev smoothing needs an FP32 Dirichlet hierarchy of 2 or more levels and "
-              "0 < lo < 2");
+  if (lo >= 2.0 || P.nl < 2) {
+    set_error("Chebyshev smoothing needs a hierarchy of 2 or more levels and 0 < lo < 2");
     return kInvalidArgument;
   }
-  const Hier<float>& H = P.h32;
-  const LevelDev<float>& B = H.lv[P.nl - 1];
-  // the bottom level on the tiled kernels (up to 3 sweeps in one launch) or
-  // in the coarse kernel with levels lc .. nl-1 (up to kMaxW weights)
-  const bool tiled_bottom = P.lc == P.nl - 1 && (int64_t)B.ny * B.nx > kCoarseCells;
-  bool ok = P.n_pre >= 1 && P.n_pre <= kMaxFuse && P.n_post >= 1 && P.n_post <= kMaxFuse &&
-            P.n_coarse >= 1 && P.n_coarse <= (tiled_bottom ? kMaxFuse : kMaxW);
-  for (int l = 0; l < P.lc && l < P.nl; ++l)
-    ok = ok && wave_tiled<float, kDirichlet>(H.lv[l], nullptr);
+  // The tiled kernels fuse up to 3 sweeps per launch (more restart the
+  // weights); the bottom level runs on them (3 sweeps at most) or in the
+  // coarse kernel with levels lc .. nl-1 (up to kMaxW weights).
+  const int bny = P.has64 ? P.h64.lv[P.nl - 1].ny : P.h32.lv[P.nl - 1].ny;
+  const int bnx = P.has64 ? P.h64.lv[P.nl - 1].nx : P.h32.lv[P.nl - 1].nx;
+  const bool tiled_bottom = P.lc == P.nl - 1 && (int64_t)bny * bnx > kCoarseCells;
+  const bool ok = P.n_pre >= 1 && P.n_pre <= kMaxFuse && P.n_post >= 1 && P.n_post <= kMaxFuse &&
+                  P.n_coarse >= 1 && P.n_coarse <= (tiled_bottom ? kMaxFuse : kMaxW);
   if (!ok) {
-    set_error("Chebyshev smoothing needs the tiled levels on the wave-tile kernels, 1-3 sweeps "
-              "before and after, and 1-3 bottom sweeps (1-8 in the coarse kernel)");
+    set_error("Chebyshev smoothing needs 1-3 sweeps before and after, and 1-3 bottom sweeps "
+              "(1-8 in the coarse kernel)");
     return kInvalidArgument;
   }
   // pre-smoothing in ascending weight order, post-smoothing descending, the

@@ -143,8 +143,8 @@ class QGModel:
         sweeps (mixed precision; no reference counterpart): sweep k of an
         m-sweep leg takes the reciprocal of the k-th root of the degree-m
         Chebyshev polynomial on [lo, 2].  ``None`` restores the uniform
-        weight.  Returns whether it is on (more than 3 sweeps per leg and
-        the FP64 path cannot take it).  The FP64 certification is
+        weight.  Returns whether it is on (plans with more than 3 sweeps
+        per leg cannot take it; FP64 plans keep the uniform weight).  The FP64 certification is
         unchanged.  A mixed plan starts with it on where it can (lo = 0.25
         with a land mask, 0.4 without)."""
         en = _i(0)

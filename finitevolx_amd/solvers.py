@@ -339,9 +339,10 @@ def build_multigrid_solver(mask, dx: float, dy: float, lambda_: float = 0.0, coe
     reference) or any 0/1 array.  Non-divisible grids raise ``ValueError``.
     ``wall`` (mixed-precision Dirichlet plans whose levels are all tiled; no
     reference counterpart): the coarse-level wall correction of the V-cycle.
-    ``chebyshev`` (same plans; no reference counterpart): the lower end ``lo``
-    of the smoothing interval [lo, 2] whose Chebyshev weights replace the
-    uniform ``omega`` in the V-cycle's sweeps.
+    ``chebyshev`` (any plan with 1-3 sweeps per leg; no reference
+    counterpart): the lower end ``lo`` of the smoothing interval [lo, 2]
+    whose Chebyshev weights replace the uniform ``omega`` in the V-cycle's
+    sweeps.
     """
     if kind not in KINDS:
         raise ValueError(f"kind must be one of {list(KINDS)}, got {kind!r}")

@@ -225,16 +225,15 @@ int fvx_mg_plan_destroy(void* plan);
  * preconditioner only.  Refused (-1) unless every level runs on the tiled
  * kernels; the level operators of such a plan are solver-internal. */
 int fvx_mg_plan_set_wall(void* plan, int on);
-/* Chebyshev weights for the V-cycle's damped-Jacobi sweeps of a Dirichlet
- * plan's FP32 hierarchy: sweep k of an m-sweep leg takes 1 / root_k of the
+/* Chebyshev weights for the V-cycle's damped-Jacobi sweeps (both operators,
+ * both precisions): sweep k of an m-sweep leg takes 1 / root_k of the
  * degree-m Chebyshev polynomial on [lo, 2] (the part of D^-1 A's spectrum
  * the smoother damps); lo <= 0 restores the uniform omega.  No reference
  * counterpart (multigrid.py:408-461 uses one weight); changes the V-cycle
  * preconditioner only (batched solves take each member's weights).
- * Refused (-1) unless the tiled levels run on the wave-tile kernels with 1-3
- * sweeps per leg, and the bottom takes 1-3 sweeps on the tiled kernels or
- * 1-8 in the single-workgroup coarse kernel; the plan's level smoothing
- * entry point is then solver-internal. */
+ * Refused (-1) unless the plan has 1-3 sweeps per leg, and the bottom takes
+ * 1-3 sweeps on the tiled kernels or 1-8 in the single-workgroup coarse
+ * kernel; the plan's level smoothing entry point is then solver-internal. */
 int fvx_mg_plan_set_chebyshev(void* plan, double lo);
 /* Level count, level shapes (2 ints per level) and the first level solved by
  * the single-workgroup coarse kernel. */

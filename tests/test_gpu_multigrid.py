@@ -139,8 +139,8 @@ def test_chebyshev_fp32_v_cycle_matches_oracle(ny, nx, nl, sweeps, masked):
     the oracle's V-cycle with the same weights, from zero and from a nonzero
     initial guess: every level on the wave-tile kernels, or (256 x 320, 5
     levels) the last two in the single-workgroup coarse kernel with 5 bottom
-    sweeps; lo = 0 restores the uniform weight; plans that cannot take it
-    (FP64, more than 3 sweeps a leg) refuse it."""
+    sweeps; the FP64 plan (LDS tile kernels) to 1e-10; lo = 0 restores the
+    uniform weight; more than 3 sweeps a leg is refused."""
     need_gpu()
     from finitevolx_amd import _lib
 
@@ -164,13 +164,37 @@ def test_chebyshev_fp32_v_cycle_matches_oracle(ny, nx, nl, sweeps, masked):
     _lib.call("fvx_mg_plan_set_chebyshev", mg._h, 0.0)
     assert_close_scaled(to_np(mg.v_cycle(None, torch.tensor(b, dtype=torch.float32))), plain,
                         1e-4, "uniform weights restored")
-    with pytest.raises(ValueError):
-        S.build_multigrid_solver(m, dx, dy, lam, n_levels=nl, kind="dirichlet",
-                                 precision="float64", chebyshev=0.1, **kw)
+    mg64 = S.build_multigrid_solver(m, dx, dy, lam, n_levels=nl, kind="dirichlet",
+                                    precision="float64", chebyshev=0.1, **kw)
+    ref0 = fvx.v_cycle(levels, np.zeros_like(b), b, cheb=0.1, **kw)
+    assert_close_scaled(to_np(mg64.v_cycle(None, b)), ref0, 1e-10, "FP64 Chebyshev V-cycle")
     with pytest.raises(ValueError):
         S.build_multigrid_solver(m, dx, dy, lam, n_levels=nl, kind="dirichlet",
                                  precision="float32", chebyshev=0.1, n_pre=6, n_post=6,
                                  n_coarse=3)
+
+
+@pytest.mark.parametrize("ny,nx,masked", [(128, 96, True), (192, 256, False)])
+def test_chebyshev_noflux_v_cycle_matches_oracle(ny, nx, masked):
+    """Chebyshev weights on the no-flux variable-coefficient operator (LDS
+    tile kernels and the coarse kernel with 6 bottom sweeps) against the
+    oracle: FP64 to 1e-10, FP32 to 1e-4."""
+    need_gpu()
+    m, coeff, dx, dy, lam = _setup(ny, nx, "noflux", masked, True, seed=6)
+    levels = fvx.build_multigrid(m, dx, dy, lam, coeff, kind="noflux")
+    rng = np.random.default_rng(7)
+    b = rng.standard_normal((ny, nx)) * m
+    u0 = rng.standard_normal((ny, nx))
+    kw = dict(n_pre=2, n_post=2, n_coarse=6, omega=0.8)
+    for prec, tol in (("float64", 1e-10), ("float32", 1e-4)):
+        mg = S.build_multigrid_solver(m, dx, dy, lam, coeff, kind="noflux", precision=prec,
+                                      chebyshev=0.2, **kw)
+        dt = torch.float64 if prec == "float64" else torch.float32
+        for u in (None, u0):
+            ref = fvx.v_cycle(levels, np.zeros_like(b) if u is None else u, b, cheb=0.2, **kw)
+            got = mg.v_cycle(None if u is None else torch.tensor(u, dtype=dt),
+                             torch.tensor(b, dtype=dt))
+            assert_close_scaled(to_np(got), ref, tol, f"noflux Chebyshev {prec} u0={u is not None}")
 
 
 @pytest.mark.parametrize("ny,nx", [(256, 192), (200, 168), (96, 520)])
