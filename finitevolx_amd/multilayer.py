@@ -57,9 +57,11 @@ class MultiLayerQG(BudgetedSteps):
     ``c0``: initial tracer, (ny, nx) interior; ``mask``: optional (ny, nx)
     wet mask of the basin interior (the outer ring is always wall);
     ``chebyshev``: the lower end lo of [lo, 2] whose Chebyshev weights the
-    modal solves' V-cycle sweeps take (None: the uniform omega).  Measured
-    on 3 layers at 4096^2: 29.6 steps/s uniform, 39.8 / 39.0 / 38.1 at lo =
-    0.15 / 0.2 / 0.25 (5.60 -> 4.18 inner iterations per modal solve).
+    modal solves' V-cycle sweeps take, one value or one per vertical mode
+    (None: the uniform omega).  Measured on 3 layers at 4096^2: 29.6 steps/s
+    uniform, 39.8 / 39.0 / 38.1 at lo = 0.15 / 0.2 / 0.25 (5.60 -> 4.18 inner
+    iterations per modal solve); per mode, (0.15, 0.3, 0.3) 42.1 against
+    41.0 at 0.2 on one box (within 3%, so one value stays the default).
     """
 
     def __init__(self, nx: int, ny: int, Lx: float, Ly: float, H, g_prime, q0, c0=None, *,
@@ -118,11 +120,14 @@ class MultiLayerQG(BudgetedSteps):
                        if masked and tracer_method in _MASK_DISPATCHABLE else (None, None))
         self._mh = self.mask_h if masked else None
 
+        cheb = list(chebyshev) if np.ndim(chebyshev) else [chebyshev] * nl
+        if len(cheb) != nl:
+            raise ValueError(f"chebyshev: one value or one per vertical mode ({nl})")
         self.solvers = [build_multigrid_solver(
             wet, self.dx, self.dy, float(self.mode_lambda[k]),
             n_levels=hierarchy_depth(ny, nx, min(self.dx, self.dy), float(self.radii[k])),
             n_pre=n_pre, n_post=n_post, n_coarse=n_coarse, omega=omega, kind="dirichlet",
-            precision="mixed", chebyshev=chebyshev) for k in range(nl)]
+            precision="mixed", chebyshev=cheb[k]) for k in range(nl)]
         self.batch = MultigridBatch(self.solvers)
         self._stage = {}
         self._init_budgets()
