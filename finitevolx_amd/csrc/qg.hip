@@ -1321,18 +1321,17 @@ int fvx_qg_set_wall(void* plan, int on, int* enabled) {
   return kOk;
 }
 
-// Chebyshev smoothing weights (fvx_mg_plan_set_chebyshev) for the inner FP32
-// V-cycles of a mixed-precision plan; *enabled = 1 when on.  Invalidates
+// Chebyshev smoothing weights (fvx_mg_plan_set_chebyshev) for the V-cycles
+// of the plan's solves (the inner FP32 V-cycles of a mixed-precision plan,
+// the FP64 V-cycles of an FP64 one); *enabled = 1 when on.  Invalidates
 // captured graphs.
 int fvx_qg_set_chebyshev(void* plan, double lo, int* enabled) {
   using namespace fvx;
   FVX_CHECK_ARG(plan && enabled, "null argument");
   QGPlan* Q = (QGPlan*)plan;
   *enabled = 0;
-  if (Q->solve == kSolveMixed) {
-    if (lo > 0.0 && plan_set_chebyshev(*Q->mg, lo) == kOk) *enabled = 1;
-    else (void)plan_set_chebyshev(*Q->mg, 0.0);
-  }
+  if (lo > 0.0 && plan_set_chebyshev(*Q->mg, lo) == kOk) *enabled = 1;
+  else (void)plan_set_chebyshev(*Q->mg, 0.0);
   drop_graphs(*Q);
   Q->calibrated = false;
   Q->eager_steps = 0;

@@ -139,14 +139,14 @@ class QGModel:
         return bool(en.value)
 
     def set_chebyshev(self, lo: float | None = 0.1) -> bool:
-        """Chebyshev weights for the inner FP32 V-cycles' damped-Jacobi
-        sweeps (mixed precision; no reference counterpart): sweep k of an
-        m-sweep leg takes the reciprocal of the k-th root of the degree-m
-        Chebyshev polynomial on [lo, 2].  ``None`` restores the uniform
-        weight.  Returns whether it is on (plans with more than 3 sweeps
-        per leg cannot take it; FP64 plans keep the uniform weight).  The FP64 certification is
-        unchanged.  A mixed plan starts with it on where it can (lo = 0.25
-        with a land mask, 0.4 without)."""
+        """Chebyshev weights for the V-cycles' damped-Jacobi sweeps (no
+        reference counterpart): sweep k of an m-sweep leg takes the
+        reciprocal of the k-th root of the degree-m Chebyshev polynomial on
+        [lo, 2].  ``None`` restores the uniform weight.  Returns whether it
+        is on (plans with more than 3 sweeps per leg cannot take it).  The
+        FP64 certification is unchanged.  A mixed plan starts with it on
+        where it can (lo = 0.25 with a land mask, 0.4 without); an FP64 plan
+        starts with the uniform weight (the reference's smoother)."""
         en = _i(0)
         _lib.call("fvx_qg_set_chebyshev", self._h, 0.0 if lo is None else float(lo), C.byref(en))
         return bool(en.value)

@@ -373,11 +373,12 @@ int fvx_qg_plan_destroy(void* plan);
  * hierarchy cannot take it or on = 0.  No reference counterpart; the FP64
  * certification is unchanged.  Invalidates captured graphs. */
 int fvx_qg_set_wall(void* plan, int on, int* enabled);
-/* Chebyshev smoothing (fvx_mg_plan_set_chebyshev) of the inner FP32 V-cycles
- * of a mixed-precision plan (lo <= 0: uniform weight): *enabled = 1 when
- * on.  A mixed plan starts with it on where the hierarchy takes it (lo =
- * 0.25 with a land mask, 0.4 without).  No reference counterpart; the FP64
- * certification is unchanged.  Invalidates captured graphs. */
+/* Chebyshev smoothing (fvx_mg_plan_set_chebyshev) of the V-cycles of the
+ * plan's solves (lo <= 0: uniform weight): *enabled = 1 when on.  A mixed
+ * plan starts with it on where the hierarchy takes it (lo = 0.25 with a land
+ * mask, 0.4 without); an FP64 plan starts with the uniform weight (the
+ * reference's smoother).  No reference counterpart; the FP64 certification
+ * is unchanged.  Invalidates captured graphs. */
 int fvx_qg_set_chebyshev(void* plan, double lo, int* enabled);
 /* nsteps rk3_ssp_step steps of q (full (ny+2, nx+2), state precision) in
  * place; psi (FP64 full array) is the warm start and the last solution.    */

@@ -198,15 +198,19 @@ def test_graph_retry_after_a_jump_in_q(precision):
 def test_chebyshev_smoothing_tracks_fp64(masked):
     """Mixed-precision steps whose inner V-cycles take Chebyshev weights
     (host-driven and replayed graphs) certify every solve and track the FP64
-    steps; the FP64 plan reports it off."""
+    steps; so do FP64 steps with the weights (off by default there)."""
     need_gpu()
     ny, nx = 256, 192
     p = _params(nx, ny)
     wet = _basin_mask(ny, nx, 7) if masked else None
     q0 = inputs.qg_initial_pv(ny, nx, seed=4)
     m64 = QGModel(nx, ny, p.Lx, p.Ly, precision="float64", mask=wet)
-    assert not m64.set_chebyshev(0.1)
     q64, _, _ = m64.step(m64.new_state(q0), nsteps=6, graph=False)
+    c64 = QGModel(nx, ny, p.Lx, p.Ly, precision="float64", mask=wet)
+    assert c64.set_chebyshev(0.25)
+    qc, _, ic = c64.step(c64.new_state(q0), nsteps=6, graph=True)
+    assert ic["converged"] and ic["max_rel_residual"] <= 1e-8, ic
+    assert np.max(np.abs(to_np(qc) - to_np(q64))) <= 1e-6 * np.max(np.abs(to_np(q64)))
     for graph in (False, True):
         mm = QGModel(nx, ny, p.Lx, p.Ly, precision="mixed", mask=wet)
         assert mm.set_chebyshev(0.1)
