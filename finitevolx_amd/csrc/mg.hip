@@ -912,7 +912,7 @@ __device__ __forceinline__ void mg_wave_tile_body(const TileArgs<float>& a, int 
     }
   };
 
-  const T om = a.omega, idx2 = L.idx2, idy2 = L.idy2, lam = L.lam, invd = L.inv_d;
+  const T idx2 = L.idx2, idy2 = L.idy2, lam = L.lam, invd = L.inv_d;
   // WALL: per row of this lane's column, the diagonal correction
   // wall_g (dry x neighbours / dx^2 + dry y neighbours / dy^2) and the
   // cell's inverse diagonal (0 on dry or outside cells).  Neighbours outside
