@@ -585,19 +585,11 @@ int face_flux_call(int method, int lim, int dir, const T* h, const T* vel, T* ou
 // ---------------------------------------------------------------------------
 // One rk3_ssp_step (timestepping/explicit_rk.py:74-121) of a doubly periodic
 // tracer: per stage the periodic ghost fill of width ng (boundary.py:52-78,
-// as the periodic advection tests do before each tendency), the Advection2D
-// tendency and the Shu-Osher combination; h advances in place and leaves with
-// its ghosts filled.  Scratch: the two stage states y1, y2.
+// as the periodic advection tests do before each tendency) and the
+// Advection2D tendency with the Shu-Osher combination fused into its launch;
+// h advances in place and leaves with its ghosts filled.  Scratch: the two
+// stage states y1, y2.
 // ---------------------------------------------------------------------------
-
-// out = a x + b y + c z elementwise; out may alias x, y or z.
-template <typename T>
-__global__ __launch_bounds__(256) void rk_stage_kernel(int64_t n, T a, const T* x, T b,
-                                                       const T* y, T c, const T* z, T* out) {
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += stride)
-    out[k] = a * x[k] + b * y[k] + c * z[k];
-}
 
 }  // namespace fvx
 
@@ -611,16 +603,6 @@ static int periodic(float* f, int ny, int nx, int ng, hipStream_t s) {
 }
 static int periodic(double* f, int ny, int nx, int ng, hipStream_t s) {
   return fvx_enforce_periodic_f64(f, ny, nx, ng, s);
-}
-
-template <typename T>
-static int rk_stage(int64_t n, double a, const T* x, double b, const T* y, double c, const T* z,
-                    T* out, hipStream_t s) {
-  const int blocks = (int)std::min<int64_t>((n + 255) / 256, 4096);
-  hipLaunchKernelGGL(rk_stage_kernel<T>, dim3(blocks), dim3(256), 0, s, n, (T)a, x, (T)b, y,
-                     (T)c, z, out);
-  FVX_LAUNCH_CHECK();
-  return kOk;
 }
 
 // Argument checks shared by both entry points (no device work).
@@ -640,22 +622,24 @@ static int adv_rk3_enqueue(int method, int lim, T* h, const T* u, const T* v,
                            const uint8_t* tier_x, const uint8_t* tier_y, const uint8_t* mh,
                            int ny, int nx, int ng, double dx, double dy, double dt, T* y1, T* y2,
                            hipStream_t s) {
-  const int64_t n = (int64_t)ny * nx;
-  auto L = [&](const T* q, T* k) {
-    return advection<T>(method, lim, q, u, v, k, tier_x, tier_y, mh, ny, nx, dx, dy, s);
+  // Each stage is one advection launch with the Shu-Osher combination fused
+  // (out = a h + b q + cdt L(q) on every cell, q the stage's input).
+  auto stage = [&](const T* q, T* out, double a, double b, double cdt) {
+    StageCoef<T> sc;
+    sc.hn = h;
+    sc.a = a;
+    sc.b = b;
+    sc.cdt = cdt;
+    return advection<T>(method, lim, q, u, v, out, tier_x, tier_y, mh, ny, nx, dx, dy, s, sc);
   };
   int rc;
-  // k1 = L(h) into y1, y1 = h + dt k1
-  if ((rc = periodic(h, ny, nx, ng, s)) || (rc = L(h, y1)) ||
-      (rc = rk_stage<T>(n, 1.0, h, 0.0, h, dt, y1, y1, s)))
-    return rc;
-  // k2 = L(y1) into y2, y2 = 3/4 h + 1/4 y1 + 1/4 dt k2
-  if ((rc = periodic(y1, ny, nx, ng, s)) || (rc = L(y1, y2)) ||
-      (rc = rk_stage<T>(n, 0.75, h, 0.25, y1, 0.25 * dt, y2, y2, s)))
-    return rc;
-  // k3 = L(y2) into y1, h = 1/3 h + 2/3 y2 + 2/3 dt k3
-  if ((rc = periodic(y2, ny, nx, ng, s)) || (rc = L(y2, y1)) ||
-      (rc = rk_stage<T>(n, 1.0 / 3.0, h, 2.0 / 3.0, y2, 2.0 / 3.0 * dt, y1, h, s)))
+  // y1 = h + dt L(h)
+  if ((rc = periodic(h, ny, nx, ng, s)) || (rc = stage(h, y1, 0.0, 1.0, dt))) return rc;
+  // y2 = 3/4 h + 1/4 y1 + 1/4 dt L(y1)
+  if ((rc = periodic(y1, ny, nx, ng, s)) || (rc = stage(y1, y2, 0.75, 0.25, 0.25 * dt))) return rc;
+  // h = 1/3 h + 2/3 y2 + 2/3 dt L(y2) (the output may alias h: read per cell)
+  if ((rc = periodic(y2, ny, nx, ng, s)) ||
+      (rc = stage(y2, h, 1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0 * dt)))
     return rc;
   return periodic(h, ny, nx, ng, s);
 }
