@@ -348,36 +348,37 @@ int masked_laplacian(const T* psi, const T* mask, T* out, int ny, int nx, double
 }
 
 // ---------------------------------------------------------------------------
-// Periodic ghost fill of width ng (boundary.py:52-78), in place: rows first,
-// then columns (so corners take the diagonally opposite interior values).
+// Periodic ghost fill of width ng (boundary.py:52-78), in place, in one
+// launch: every ghost cell copies the interior cell its row and column wrap
+// to (the reference fills rows, then columns, so a corner takes the
+// diagonally opposite interior value: the same cell).  Reads touch interior
+// cells only and writes ghost cells only, so the threads do not race.
 // ---------------------------------------------------------------------------
 
-template <typename T>
-__global__ void periodic_rows_kernel(T* f, int ny, int nx, int ng) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int g = blockIdx.y;  // 0..2ng-1
-  if (i >= nx) return;
-  const int64_t ld = nx;
-  if (g < ng) {  // south ghost row g <- row ny - 2ng + g
-    f[(int64_t)g * ld + i] = f[(int64_t)(ny - 2 * ng + g) * ld + i];
-  } else {  // north ghost row ny - ng + k <- row ng + k
-    const int k = g - ng;
-    f[(int64_t)(ny - ng + k) * ld + i] = f[(int64_t)(ng + k) * ld + i];
-  }
+__device__ __forceinline__ int periodic_wrap(int x, int n, int ng) {
+  return x < ng ? x + n - 2 * ng : (x >= n - ng ? x - n + 2 * ng : x);
 }
 
 template <typename T>
-__global__ void periodic_cols_kernel(T* f, int ny, int nx, int ng) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int g = blockIdx.y;
-  if (j >= ny) return;
-  const int64_t r = (int64_t)j * nx;
-  if (g < ng) {
-    f[r + g] = f[r + nx - 2 * ng + g];
+__global__ __launch_bounds__(256) void periodic_fill_kernel(T* f, int ny, int nx, int ng) {
+  // cells 0 .. 2 ng nx - 1: the ghost rows (full width); then the ghost
+  // columns of the interior rows, 2 ng per row
+  const int64_t nrow = 2LL * ng * nx, ncol = 2LL * ng * (ny - 2 * ng);
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int j, i;
+  if (k < nrow) {
+    const int g = (int)(k / nx);
+    i = (int)(k - (int64_t)g * nx);
+    j = g < ng ? g : ny - 2 * ng + g;
+  } else if (k < nrow + ncol) {
+    const int64_t q = k - nrow;
+    const int r = (int)(q / (2 * ng)), g = (int)(q - (int64_t)r * 2 * ng);
+    j = ng + r;
+    i = g < ng ? g : nx - 2 * ng + g;
   } else {
-    const int k = g - ng;
-    f[r + nx - ng + k] = f[r + ng + k];
+    return;
   }
+  f[(int64_t)j * nx + i] = f[(int64_t)periodic_wrap(j, ny, ng) * nx + periodic_wrap(i, nx, ng)];
 }
 
 template <typename T>
@@ -385,11 +386,9 @@ int enforce_periodic(T* f, int ny, int nx, int ng, hipStream_t s) {
   FVX_CHECK_ARG(f, "null field pointer");
   FVX_CHECK_ARG(ng >= 1 && ny >= 3 * ng && nx >= 3 * ng,
                 "ghost width %d too large for %d x %d", ng, ny, nx);
-  hipLaunchKernelGGL(periodic_rows_kernel<T>, dim3(cdiv(nx, 256), 2 * ng), dim3(256), 0, s, f,
-                     ny, nx, ng);
-  FVX_LAUNCH_CHECK();
-  hipLaunchKernelGGL(periodic_cols_kernel<T>, dim3(cdiv(ny, 256), 2 * ng), dim3(256), 0, s, f,
-                     ny, nx, ng);
+  const int64_t cells = 2LL * ng * nx + 2LL * ng * (ny - 2 * ng);
+  hipLaunchKernelGGL(periodic_fill_kernel<T>, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0,
+                     s, f, ny, nx, ng);
   FVX_LAUNCH_CHECK();
   return kOk;
 }
